@@ -1,0 +1,80 @@
+"""ctypes binding of the C ABI declared in include/dmftgpu.h.
+
+The shared library is built in-tree (python -m dmftpack_amd.build) and loaded
+from this directory; loading fails loudly when it is missing -- there is no
+CPU fallback.
+"""
+import ctypes as C
+import os
+import re
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdmftgpu.so")
+HEADER = os.path.join(HERE, "..", "include", "dmftgpu.h")
+
+OK, ERR_ARG, ERR_STATE, ERR_HIP, ERR_NUMERIC, ERR_UNSUPPORTED = 0, 1, 2, 3, 4, 5
+PLAIN_F64, SCHUR_F64, SCHUR_F32R = 0, 1, 2
+
+
+class Lattice(C.Structure):
+    _fields_ = [("n", C.c_int32), ("n_corr", C.c_int32), ("n_cluster", C.c_int32), ("n_k", C.c_int32),
+                ("n_k_global", C.c_int32), ("n_freq", C.c_int32), ("high_freq", C.c_int32), ("beta", C.c_double)]
+
+
+class DmftGpuError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"dmftgpu error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+P = C.c_void_p
+D = C.c_double
+I = C.c_int
+PD = C.POINTER(C.c_double)
+PI = C.POINTER(C.c_int)
+
+_SIGS = {
+    "dmftgpu_create": ([C.POINTER(P), P], I),
+    "dmftgpu_destroy": ([P], I),
+    "dmftgpu_set_stream": ([P, P], I),
+    "dmftgpu_last_error": ([P], C.c_char_p),
+    "dmftgpu_version": ([], C.c_char_p),
+    "dmftgpu_prepare": ([P, C.POINTER(Lattice), P, P], I),
+    "dmftgpu_static_eig": ([P, P, P], I),
+    "dmftgpu_greenftn_w": ([P, I, P, D, P, P, P], I),
+    "dmftgpu_density_matrix": ([P, P, D, P], I),
+    "dmftgpu_density": ([P, I, P, D, PD, PD], I),
+    "dmftgpu_adjust_mu": ([P, I, P, D, D, D, PD, PI], I),
+    "dmftgpu_gloc_vjp": ([P, I, P, D, P, D, P, PD], I),
+    "dmftgpu_moments": ([P, I, I, D, P, I, P, PI], I),
+    "dmftgpu_ft_w2tau": ([P, I, I, I, D, P, P, P, PI], I),
+    "dmftgpu_ft_tau2w": ([P, I, I, I, D, P, P, PI], I),
+}
+
+
+def header_symbols():
+    """Function names declared in include/dmftgpu.h."""
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(dmftgpu_\w+)\s*\(", txt, re.M)))
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError(f"{LIB_PATH} is not built; run `python -m dmftpack_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (args, res) in _SIGS.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def check(handle, rc):
+    if rc != OK:
+        msg = lib().dmftgpu_last_error(handle)
+        raise DmftGpuError(rc, msg.decode() if msg else "")
+    return rc

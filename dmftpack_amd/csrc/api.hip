@@ -1,0 +1,501 @@
+// C ABI of the lattice Green's-function library (include/dmftgpu.h): host
+// logic around the HIP kernels.  Each entry point restates the control flow of
+// the reference routine it replaces; the per-(k, iw) work runs on the GPU.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/dmftgpu.h"
+#include "api_internal.hpp"
+#include "lattice_kernels.hpp"
+
+using dg::cd;
+
+const char* dmftgpu_version(void) { return "dmftgpu 0.1 (gfx950)"; }
+
+namespace dgi {
+
+int fail(dmftgpu_handle h, int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (h) h->err = buf;
+    return code;
+}
+
+void* Workspace::get(size_t bytes, hipError_t* e)
+{
+    *e = hipSuccess;
+    if (bytes <= cap) return ptr;
+    if (ptr) {
+        *e = hipFree(ptr);
+        if (*e != hipSuccess) return nullptr;
+        ptr = nullptr;
+        cap = 0;
+    }
+    *e = hipMalloc(&ptr, bytes);
+    if (*e != hipSuccess) {
+        ptr = nullptr;
+        return nullptr;
+    }
+    cap = bytes;
+    return ptr;
+}
+
+void Workspace::release()
+{
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+}
+
+// Carves consecutive 256-byte aligned regions out of one allocation.
+struct Carver {
+    size_t off = 0;
+    template <class T>
+    size_t take(size_t count)
+    {
+        size_t o = off;
+        off += (count * sizeof(T) + 255) / 256 * 256;
+        return o;
+    }
+};
+
+}  // namespace dgi
+
+using namespace dgi;
+
+#define HIPCHK(expr)                                                                           \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess) return fail(h, DMFTGPU_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+extern "C" int dmftgpu_create(dmftgpu_handle* out, void* stream)
+{
+    if (!out) return DMFTGPU_ERR_ARG;
+    auto* h = new dmftgpu_handle_s();
+    h->stream = (hipStream_t)stream;
+    if (hipGetDevice(&h->device) != hipSuccess) {
+        delete h;
+        return DMFTGPU_ERR_HIP;
+    }
+    *out = h;
+    return DMFTGPU_OK;
+}
+
+extern "C" int dmftgpu_destroy(dmftgpu_handle h)
+{
+    if (!h) return DMFTGPU_ERR_ARG;
+    (void)hipStreamSynchronize(h->stream);
+    h->ws.release();
+    h->prep.release();
+    delete h;
+    return DMFTGPU_OK;
+}
+
+extern "C" int dmftgpu_set_stream(dmftgpu_handle h, void* stream)
+{
+    if (!h) return DMFTGPU_ERR_ARG;
+    h->stream = (hipStream_t)stream;
+    return DMFTGPU_OK;
+}
+
+extern "C" const char* dmftgpu_last_error(dmftgpu_handle h) { return h ? h->err.c_str() : "null handle"; }
+
+static int check_lattice(dmftgpu_handle h, const dmftgpu_lattice* L)
+{
+    if (!L) return fail(h, DMFTGPU_ERR_ARG, "lattice descriptor is null");
+    if (L->n < 1 || L->n > 32) return fail(h, DMFTGPU_ERR_UNSUPPORTED, "n=%d outside 1..32", L->n);
+    if (L->n_corr < 1 || L->n_corr > L->n) return fail(h, DMFTGPU_ERR_ARG, "n_corr=%d outside 1..n", L->n_corr);
+    if (L->n_cluster < 1 || L->n_corr % L->n_cluster != 0)
+        return fail(h, DMFTGPU_ERR_ARG, "n_cluster=%d does not divide n_corr=%d", L->n_cluster, L->n_corr);
+    if (L->n_k < 1 || L->n_k_global < 1) return fail(h, DMFTGPU_ERR_ARG, "n_k=%d n_k_global=%d", L->n_k, L->n_k_global);
+    if (L->n_freq < 1 || L->high_freq < 1) return fail(h, DMFTGPU_ERR_ARG, "n_freq=%d high_freq=%d", L->n_freq, L->high_freq);
+    if (!(L->beta > 0.0) || !std::isfinite(L->beta)) return fail(h, DMFTGPU_ERR_ARG, "beta=%g", L->beta);
+    return DMFTGPU_OK;
+}
+
+extern "C" int dmftgpu_prepare(dmftgpu_handle h, const dmftgpu_lattice* L, const double* Hk, const double* moments)
+{
+    if (!h) return DMFTGPU_ERR_ARG;
+    int rc = check_lattice(h, L);
+    if (rc) return rc;
+    if (!Hk || !moments) return fail(h, DMFTGPU_ERR_ARG, "Hk or moments is null");
+    h->lat = *L;
+    h->Hk = (const cd*)Hk;
+    const int n = L->n, nc = L->n_corr, nk = L->n_k;
+    const int nd = n - nc;
+    Carver cv;
+    size_t o_mom = cv.take<cd>(4 * (size_t)nc * nc);
+    size_t o_ev = cv.take<double>((size_t)nk * n);
+    size_t o_U = cv.take<cd>((size_t)nk * n * n);
+    size_t o_ed = cv.take<double>((size_t)nk * (nd > 0 ? nd : 1));
+    size_t o_Vd = cv.take<cd>((size_t)nk * (nd > 0 ? nd * nd : 1));
+    size_t o_W = cv.take<cd>((size_t)nk * nc * (nd > 0 ? nd : 1));
+    hipError_t e;
+    char* base = (char*)h->prep.get(cv.off, &e);
+    if (!base) return fail(h, DMFTGPU_ERR_HIP, "prepare alloc %zu bytes: %s", cv.off, hipGetErrorString(e));
+    h->mom = (cd*)(base + o_mom);
+    h->ev = (double*)(base + o_ev);
+    h->U = (cd*)(base + o_U);
+    h->ed = (double*)(base + o_ed);
+    h->Vd = (cd*)(base + o_Vd);
+    h->W = (cd*)(base + o_W);
+    HIPCHK(hipMemcpyAsync(h->mom, moments, sizeof(cd) * 4 * nc * nc, hipMemcpyDefault, h->stream));
+    // static part: eig(H(k) + P m0 P^H)   (GreenFtn.cpp:197-215)
+    HIPCHK(dg::launch_herm_eig(h->Hk, n, 0, n, h->mom, nc, h->ev, h->U, nk, h->stream));
+    // Schur data: eig(H_dd(k)) and W = H_cd V_d
+    h->schur_ready = false;
+    if (nd > 0 && dg::schur_supported(n, nc)) {
+        HIPCHK(dg::launch_herm_eig(h->Hk, n, nc, nd, nullptr, 0, h->ed, h->Vd, nk, h->stream));
+        HIPCHK(dg::launch_schur_prep(h->Hk, h->Vd, n, nc, nk, h->W, h->stream));
+        h->schur_ready = true;
+    }
+    h->prepared = true;
+    return DMFTGPU_OK;
+}
+
+extern "C" int dmftgpu_static_eig(dmftgpu_handle h, double* eps, double* U)
+{
+    if (!h) return DMFTGPU_ERR_ARG;
+    if (!h->prepared) return fail(h, DMFTGPU_ERR_STATE, "dmftgpu_prepare has not been called");
+    const dmftgpu_lattice& L = h->lat;
+    if (eps) HIPCHK(hipMemcpyAsync(eps, h->ev, sizeof(double) * L.n_k * L.n, hipMemcpyDefault, h->stream));
+    if (U) HIPCHK(hipMemcpyAsync(U, h->U, sizeof(cd) * (size_t)L.n_k * L.n * L.n, hipMemcpyDefault, h->stream));
+    return DMFTGPU_OK;
+}
+
+namespace {
+
+struct Plan {
+    int n_chunks, k_per_chunk;
+};
+
+Plan make_plan(int nk, int nw_tot, int slots, int target_wg)
+{
+    const int n_wt = (nw_tot + slots - 1) / slots;
+    int chunks = (target_wg + n_wt - 1) / n_wt;
+    if (chunks < 1) chunks = 1;
+    if (chunks > nk) chunks = nk;
+    int kpc = (nk + chunks - 1) / chunks;
+    chunks = (nk + kpc - 1) / kpc;
+    return Plan{chunks, kpc};
+}
+
+// Static weights and their per-k reduction: fills st_out[m2] (complex) and
+// st_scalars[2] = {sum w, sum dN terms} (device), all unnormalised.
+int run_static(dmftgpu_handle h, double mu, cd* st_out, double* st_scalars, char* ws, size_t& off)
+{
+    const dmftgpu_lattice& L = h->lat;
+    const int n = L.n, nc = L.n_corr, nk = L.n_k, m2 = nc * nc;
+    const int nw_tot = L.n_freq * L.high_freq;
+    double* wt = (double*)(ws + off);
+    off += ((size_t)nk * n * sizeof(double) + 255) / 256 * 256;
+    double* dt = (double*)(ws + off);
+    off += ((size_t)nk * n * sizeof(double) + 255) / 256 * 256;
+    const int kpc = (nk + 255) / 256;
+    const int chunks = (nk + kpc - 1) / kpc;
+    cd* pNum = (cd*)(ws + off);
+    off += ((size_t)chunks * m2 * sizeof(cd) + 255) / 256 * 256;
+    double* pN = (double*)(ws + off);
+    off += ((size_t)chunks * 2 * sizeof(double) + 255) / 256 * 256;
+    HIPCHK(dg::launch_static_terms(h->ev, n, nk, nw_tot, L.beta, mu, wt, dt, h->stream));
+    HIPCHK(dg::launch_static_reduce(wt, dt, h->U, n, nc, nk, kpc, chunks, pNum, pN, pN + chunks, h->stream));
+    HIPCHK(dg::launch_reduce_chunks(pNum, chunks, m2, st_out, h->stream));
+    HIPCHK(dg::launch_reduce_chunks_d(pN, chunks, 1, st_scalars, h->stream));
+    HIPCHK(dg::launch_reduce_chunks_d(pN + chunks, chunks, 1, st_scalars + 1, h->stream));
+    return DMFTGPU_OK;
+}
+
+size_t static_ws_bytes(const dmftgpu_lattice& L)
+{
+    const int kpc = (L.n_k + 255) / 256;
+    const int chunks = (L.n_k + kpc - 1) / kpc;
+    return 2 * (((size_t)L.n_k * L.n * 8 + 255) / 256 * 256) + ((size_t)chunks * L.n_corr * L.n_corr * 16 + 255) / 256 * 256 +
+           ((size_t)chunks * 16 + 255) / 256 * 256 + 4096;
+}
+
+bool valid_method(int m) { return m == DMFTGPU_PLAIN_F64 || m == DMFTGPU_SCHUR_F64 || m == DMFTGPU_SCHUR_F32R; }
+
+// Runs the per-(k, iw) kernel of the chosen method in the given mode and
+// reduces the per-chunk partials.  Outputs (device, in ws):
+//   Gsum [nw_tot][m2] (FWD), trsum [nw_tot], tr2sum [nw_tot] (DENS/VJP),
+//   Sbsum [n_freq][m2] and musum [n_freq] (VJP).
+struct Sums {
+    cd *Gsum = nullptr, *tr = nullptr, *tr2 = nullptr, *Sb = nullptr, *mu = nullptr;
+};
+
+int run_lattice(dmftgpu_handle h, int method, int mode, const double* sigma, double mu, const double* gbar,
+                double nbar, char* ws, size_t& off, Sums& S)
+{
+    const dmftgpu_lattice& L = h->lat;
+    const int n = L.n, nc = L.n_corr, nk = L.n_k, m2 = nc * nc;
+    const int nw_tot = L.n_freq * L.high_freq;
+    int slots;
+    if (method == DMFTGPU_PLAIN_F64) slots = dg::plain_slots(n, mode);
+    else slots = dg::schur_slots(nc, mode);
+    Plan P = make_plan(nk, nw_tot, slots, h->target_wg);
+    dg::LatArgs a{};
+    a.Hk = h->Hk;
+    a.sigma = (const cd*)sigma;
+    a.mom = h->mom;
+    a.n = n;
+    a.nc = nc;
+    a.nk = nk;
+    a.n_freq = L.n_freq;
+    a.nw_tot = nw_tot;
+    a.beta = L.beta;
+    a.mu = mu;
+    a.k_per_chunk = P.k_per_chunk;
+    a.n_chunks = P.n_chunks;
+    a.gbar = (const cd*)gbar;
+    a.nbar = nbar;
+    a.ed = h->ed;
+    a.W = h->W;
+    a.refine = h->refine_steps;
+    auto take = [&](size_t bytes) {
+        char* p = ws + off;
+        off += (bytes + 255) / 256 * 256;
+        return p;
+    };
+    const size_t C = P.n_chunks;
+    a.pTr = (cd*)take(C * nw_tot * sizeof(cd));
+    if (mode == dg::MODE_FWD) a.pG = (cd*)take(C * nw_tot * m2 * sizeof(cd));
+    if (mode != dg::MODE_FWD) a.pTr2 = (cd*)take(C * nw_tot * sizeof(cd));
+    if (mode == dg::MODE_VJP) {
+        a.pSb = (cd*)take(C * L.n_freq * m2 * sizeof(cd));
+        a.pMu = (cd*)take(C * L.n_freq * sizeof(cd));
+    }
+    if (method == DMFTGPU_PLAIN_F64) {
+        HIPCHK(dg::launch_lattice_plain(a, mode, h->stream, nullptr));
+    } else {
+        HIPCHK(dg::launch_lattice_schur(a, mode, method == DMFTGPU_SCHUR_F32R, h->stream));
+    }
+    S.tr = (cd*)take(nw_tot * sizeof(cd));
+    HIPCHK(dg::launch_reduce_chunks(a.pTr, P.n_chunks, nw_tot, S.tr, h->stream));
+    if (mode == dg::MODE_FWD) {
+        S.Gsum = (cd*)take((size_t)nw_tot * m2 * sizeof(cd));
+        HIPCHK(dg::launch_reduce_chunks(a.pG, P.n_chunks, (size_t)nw_tot * m2, S.Gsum, h->stream));
+    } else {
+        S.tr2 = (cd*)take(nw_tot * sizeof(cd));
+        HIPCHK(dg::launch_reduce_chunks(a.pTr2, P.n_chunks, nw_tot, S.tr2, h->stream));
+    }
+    if (mode == dg::MODE_VJP) {
+        S.Sb = (cd*)take((size_t)L.n_freq * m2 * sizeof(cd));
+        S.mu = (cd*)take(L.n_freq * sizeof(cd));
+        HIPCHK(dg::launch_reduce_chunks(a.pSb, P.n_chunks, (size_t)L.n_freq * m2, S.Sb, h->stream));
+        HIPCHK(dg::launch_reduce_chunks(a.pMu, P.n_chunks, L.n_freq, S.mu, h->stream));
+    }
+    return DMFTGPU_OK;
+}
+
+size_t lattice_ws_bytes(dmftgpu_handle h, int method, int mode)
+{
+    const dmftgpu_lattice& L = h->lat;
+    const int nc = L.n_corr, m2 = nc * nc;
+    const size_t nw_tot = (size_t)L.n_freq * L.high_freq;
+    int slots = method == DMFTGPU_PLAIN_F64 ? dg::plain_slots(L.n, mode) : dg::schur_slots(nc, mode);
+    Plan P = make_plan(L.n_k, (int)nw_tot, slots, h->target_wg);
+    size_t C = P.n_chunks + 1;  // +1 for the reduced copy
+    size_t b = C * nw_tot * 16 * (2 + (size_t)m2) + C * L.n_freq * 16 * (m2 + 1) + 64 * 256;
+    return b;
+}
+
+}  // namespace
+
+// GreenFtn_w (GreenFtn.cpp:142-422)
+extern "C" int dmftgpu_greenftn_w(dmftgpu_handle h, int method, const double* sigma, double mu, double* gloc,
+                                  double* gloc_hf, double* num_matrix)
+{
+    if (!h) return DMFTGPU_ERR_ARG;
+    if (!h->prepared) return fail(h, DMFTGPU_ERR_STATE, "dmftgpu_prepare has not been called");
+    if (!valid_method(method)) return fail(h, DMFTGPU_ERR_ARG, "unknown method %d", method);
+    if (method != DMFTGPU_PLAIN_F64 && !h->schur_ready)
+        return fail(h, DMFTGPU_ERR_UNSUPPORTED, "Schur method needs 0 < n_corr < n (n=%d, n_corr=%d)", h->lat.n, h->lat.n_corr);
+    if (!sigma || !gloc || !num_matrix) return fail(h, DMFTGPU_ERR_ARG, "sigma, gloc and num_matrix are required");
+    if (!std::isfinite(mu)) return fail(h, DMFTGPU_ERR_ARG, "mu is not finite");
+    const dmftgpu_lattice& L = h->lat;
+    const int nc = L.n_corr, m2 = nc * nc;
+    const int nw_tot = L.n_freq * L.high_freq;
+    hipError_t e;
+    size_t need = lattice_ws_bytes(h, method, dg::MODE_FWD) + static_ws_bytes(L) + (size_t)(m2 + 8) * 32 + 8192;
+    char* ws = (char*)h->ws.get(need, &e);
+    if (!ws) return fail(h, DMFTGPU_ERR_HIP, "workspace %zu bytes: %s", need, hipGetErrorString(e));
+    size_t off = 0;
+    Sums S;
+    int rc = run_lattice(h, method, dg::MODE_FWD, sigma, mu, nullptr, 0.0, ws, off, S);
+    if (rc) return rc;
+    cd* tot = (cd*)(ws + off);
+    off += ((size_t)(m2 + 3) * sizeof(cd) + 255) / 256 * 256;
+    cd* st = (cd*)(ws + off);
+    off += ((size_t)m2 * sizeof(cd) + 255) / 256 * 256;
+    double* sts = (double*)(ws + off);
+    off += 256;
+    rc = run_static(h, mu, st, sts, ws, off);
+    if (rc) return rc;
+    HIPCHK(dg::launch_freq_totals(S.Gsum, m2, nc, S.tr, nullptr, nw_tot, nullptr, 0, tot, h->stream));
+    const double inv_nk = 1.0 / (double)L.n_k_global;
+    HIPCHK(dg::launch_gloc_finalize(S.Gsum, L.n_freq, nc, L.n_cluster, inv_nk, (cd*)gloc, h->stream));
+    if (gloc_hf)
+        HIPCHK(dg::launch_gloc_hf(h->ev, h->U, L.n, nc, L.n_k, L.n_freq, L.beta, mu, inv_nk, L.n_cluster, (cd*)gloc_hf,
+                                  h->stream));
+    // NumMatrix = [ (1/beta) sum_n sum_k (G_cc + G_cc^H) + sum_k (U w U^H)_cc ] / knum_mpiGlobal
+    std::vector<cd> htot(m2 + 3), hst(m2);
+    HIPCHK(hipMemcpyAsync(htot.data(), tot, sizeof(cd) * (m2 + 3), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(hst.data(), st, sizeof(cd) * m2, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    std::vector<cd> num(m2);
+    for (int q = 0; q < m2; ++q) {
+        num[q].x = (htot[q].x / L.beta + hst[q].x) * inv_nk;
+        num[q].y = (htot[q].y / L.beta + hst[q].y) * inv_nk;
+        if (!std::isfinite(num[q].x) || !std::isfinite(num[q].y))
+            return fail(h, DMFTGPU_ERR_NUMERIC, "NumMatrix is not finite");
+    }
+    HIPCHK(hipMemcpyAsync(num_matrix, num.data(), sizeof(cd) * m2, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return DMFTGPU_OK;
+}
+
+// N(mu), dN/dmu
+static int density_impl(dmftgpu_handle h, int method, const double* sigma, double mu, double* n_out, double* dn_out)
+{
+    const dmftgpu_lattice& L = h->lat;
+    const int nc = L.n_corr, m2 = nc * nc;
+    const int nw_tot = L.n_freq * L.high_freq;
+    hipError_t e;
+    size_t need = lattice_ws_bytes(h, method, dg::MODE_DENS) + static_ws_bytes(L) + (size_t)(m2 + 8) * 32 + 8192;
+    char* ws = (char*)h->ws.get(need, &e);
+    if (!ws) return fail(h, DMFTGPU_ERR_HIP, "workspace %zu bytes: %s", need, hipGetErrorString(e));
+    size_t off = 0;
+    Sums S;
+    int rc = run_lattice(h, method, dg::MODE_DENS, sigma, mu, nullptr, 0.0, ws, off, S);
+    if (rc) return rc;
+    cd* tot = (cd*)(ws + off);
+    off += ((size_t)(m2 + 3) * sizeof(cd) + 255) / 256 * 256;
+    cd* st = (cd*)(ws + off);
+    off += ((size_t)m2 * sizeof(cd) + 255) / 256 * 256;
+    double* sts = (double*)(ws + off);
+    off += 256;
+    rc = run_static(h, mu, st, sts, ws, off);
+    if (rc) return rc;
+    HIPCHK(dg::launch_freq_totals(nullptr, m2, nc, S.tr, S.tr2, nw_tot, nullptr, 0, tot, h->stream));
+    std::vector<cd> htot(m2 + 3);
+    double hsts[2];
+    HIPCHK(hipMemcpyAsync(htot.data(), tot, sizeof(cd) * (m2 + 3), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(hsts, sts, sizeof(double) * 2, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const double inv_nk = 1.0 / (double)L.n_k_global;
+    const double N = (htot[m2].x / L.beta + hsts[0]) * inv_nk;
+    const double dN = (-htot[m2 + 1].x / L.beta + hsts[1]) * inv_nk;
+    if (!std::isfinite(N) || !std::isfinite(dN)) return fail(h, DMFTGPU_ERR_NUMERIC, "N(mu) is not finite");
+    if (n_out) *n_out = N;
+    if (dn_out) *dn_out = dN;
+    return DMFTGPU_OK;
+}
+
+extern "C" int dmftgpu_density(dmftgpu_handle h, int method, const double* sigma, double mu, double* n_out,
+                               double* dn_dmu_out)
+{
+    if (!h) return DMFTGPU_ERR_ARG;
+    if (!h->prepared) return fail(h, DMFTGPU_ERR_STATE, "dmftgpu_prepare has not been called");
+    if (!valid_method(method)) return fail(h, DMFTGPU_ERR_ARG, "unknown method %d", method);
+    if (method != DMFTGPU_PLAIN_F64 && !h->schur_ready)
+        return fail(h, DMFTGPU_ERR_UNSUPPORTED, "Schur method needs 0 < n_corr < n");
+    if (!sigma) return fail(h, DMFTGPU_ERR_ARG, "sigma is null");
+    if (!std::isfinite(mu)) return fail(h, DMFTGPU_ERR_ARG, "mu is not finite");
+    return density_impl(h, method, sigma, mu, n_out, dn_dmu_out);
+}
+
+// GreenFtn_w_adjust_mu (GreenFtn.cpp:66-139).  The reference brackets with
+// steps of dmu and bisects until |dmu| <= 1e-5; here N(mu) is monotone
+// increasing with the analytic slope dN/dmu, so Newton steps are taken inside
+// a maintained bracket and replaced by bisection whenever they leave it.
+extern "C" int dmftgpu_adjust_mu(dmftgpu_handle h, int method, const double* sigma, double mu0, double n_target,
+                                 double tol, double* mu_out, int* n_eval)
+{
+    if (!h) return DMFTGPU_ERR_ARG;
+    if (!h->prepared) return fail(h, DMFTGPU_ERR_STATE, "dmftgpu_prepare has not been called");
+    if (!valid_method(method)) return fail(h, DMFTGPU_ERR_ARG, "unknown method %d", method);
+    if (!sigma || !mu_out) return fail(h, DMFTGPU_ERR_ARG, "sigma and mu_out are required");
+    if (!(tol > 0.0)) tol = 1e-5;
+    double mu = mu0, N, dN;
+    int evals = 0;
+    double lo = -INFINITY, hi = INFINITY;  // N(lo) < target < N(hi)
+    for (int it = 0; it < 200; ++it) {
+        int rc = density_impl(h, method, sigma, mu, &N, &dN);
+        ++evals;
+        if (rc) return rc;
+        const double r = N - n_target;
+        if (r == 0.0) break;
+        if (r < 0.0) lo = mu;
+        else hi = mu;
+        double step = (dN > 0.0) ? -r / dN : (r < 0.0 ? 1.0 : -1.0);
+        double next = mu + step;
+        if (std::isfinite(lo) && std::isfinite(hi)) {
+            if (!(next > lo && next < hi)) next = 0.5 * (lo + hi);
+        } else {
+            // no bracket yet: limit the Newton step (the reference steps by dmu = 1)
+            if (step > 1.0) next = mu + 1.0;
+            if (step < -1.0) next = mu - 1.0;
+        }
+        if (std::fabs(next) > 600.0) return fail(h, DMFTGPU_ERR_NUMERIC, "mu search diverged (|mu| > 600)");
+        const double dmu = next - mu;
+        mu = next;
+        if (std::fabs(dmu) <= tol) break;
+    }
+    *mu_out = mu;
+    if (n_eval) *n_eval = evals;
+    return DMFTGPU_OK;
+}
+
+extern "C" int dmftgpu_gloc_vjp(dmftgpu_handle h, int method, const double* sigma, double mu, const double* gloc_bar,
+                                double n_bar, double* sigma_bar, double* mu_bar)
+{
+    if (!h) return DMFTGPU_ERR_ARG;
+    if (!h->prepared) return fail(h, DMFTGPU_ERR_STATE, "dmftgpu_prepare has not been called");
+    if (!valid_method(method)) return fail(h, DMFTGPU_ERR_ARG, "unknown method %d", method);
+    if (method != DMFTGPU_PLAIN_F64 && !h->schur_ready)
+        return fail(h, DMFTGPU_ERR_UNSUPPORTED, "Schur method needs 0 < n_corr < n");
+    if (!sigma || !gloc_bar || !sigma_bar) return fail(h, DMFTGPU_ERR_ARG, "sigma, gloc_bar and sigma_bar are required");
+    const dmftgpu_lattice& L = h->lat;
+    const int nc = L.n_corr, m2 = nc * nc;
+    const int nw_tot = L.n_freq * L.high_freq;
+    hipError_t e;
+    size_t need = lattice_ws_bytes(h, method, dg::MODE_VJP) + static_ws_bytes(L) + (size_t)(m2 + 8) * 32 + 8192;
+    char* ws = (char*)h->ws.get(need, &e);
+    if (!ws) return fail(h, DMFTGPU_ERR_HIP, "workspace %zu bytes: %s", need, hipGetErrorString(e));
+    size_t off = 0;
+    Sums S;
+    int rc = run_lattice(h, method, dg::MODE_VJP, sigma, mu, gloc_bar, n_bar, ws, off, S);
+    if (rc) return rc;
+    cd* tot = (cd*)(ws + off);
+    off += ((size_t)(m2 + 3) * sizeof(cd) + 255) / 256 * 256;
+    cd* st = (cd*)(ws + off);
+    off += ((size_t)m2 * sizeof(cd) + 255) / 256 * 256;
+    double* sts = (double*)(ws + off);
+    off += 256;
+    rc = run_static(h, mu, st, sts, ws, off);
+    if (rc) return rc;
+    const double inv_nk = 1.0 / (double)L.n_k_global;
+    HIPCHK(dg::launch_scale(S.Sb, (size_t)L.n_freq * m2, inv_nk, (cd*)sigma_bar, h->stream));
+    HIPCHK(dg::launch_freq_totals(nullptr, m2, nc, S.tr, S.tr2, nw_tot, S.mu, L.n_freq, tot, h->stream));
+    std::vector<cd> htot(m2 + 3);
+    double hsts[2];
+    HIPCHK(hipMemcpyAsync(htot.data(), tot, sizeof(cd) * (m2 + 3), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(hsts, sts, sizeof(double) * 2, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const double dN = (-htot[m2 + 1].x / L.beta + hsts[1]) * inv_nk;
+    const double mub = -htot[m2 + 2].x * inv_nk + n_bar * dN;
+    if (!std::isfinite(mub)) return fail(h, DMFTGPU_ERR_NUMERIC, "mu_bar is not finite");
+    if (mu_bar) *mu_bar = mub;
+    return DMFTGPU_OK;
+}

@@ -1,0 +1,44 @@
+// Handle state shared by the C-ABI translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/dmftgpu.h"
+#include "dev_common.hpp"
+
+namespace dgi {
+
+// Grow-only device allocation owned by a handle.
+struct Workspace {
+    void* ptr = nullptr;
+    size_t cap = 0;
+    void* get(size_t bytes, hipError_t* e);
+    void release();
+};
+
+int fail(dmftgpu_handle h, int code, const char* fmt, ...);
+
+}  // namespace dgi
+
+struct dmftgpu_handle_s {
+    hipStream_t stream = nullptr;
+    int device = 0;
+    std::string err;
+    // lattice bound by dmftgpu_prepare
+    bool prepared = false;
+    bool schur_ready = false;
+    dmftgpu_lattice lat{};
+    const dg::cd* Hk = nullptr;
+    dg::cd* mom = nullptr;   // [4][nc][nc]
+    double* ev = nullptr;    // [nk][n]    static eigenvalues
+    dg::cd* U = nullptr;     // [nk][n][n] static eigenvectors
+    double* ed = nullptr;    // [nk][nd]   eig(H_dd)
+    dg::cd* Vd = nullptr;    // [nk][nd][nd]
+    dg::cd* W = nullptr;     // [nk][nc][nd] H_cd V_d
+    int target_wg = 2048;    // workgroups per lattice launch (k chunking)
+    int refine_steps = 2;    // FP64-residual refinement steps of the FP32 Schur solve
+    dgi::Workspace prep;     // prepare-time data
+    dgi::Workspace ws;       // per-call scratch
+    dgi::Workspace ft;       // Fourier-transform scratch
+};

@@ -1,0 +1,251 @@
+// Static-part densities, GwHF and the deterministic reductions of the lattice
+// path.
+//
+// Static part (GreenFtn.cpp:311-340): with H(k) + P m0 P^H = U diag(eps) U^H,
+// the G0 subtraction and the Fermi add-back of the density collapse per level
+// to the weight
+//     w_kl = f(eps_kl - mu) - (1/beta) sum_{n < high*N_freq} 2 Re 1/(iw_n + mu - eps_kl)
+// so that NumMatrix_static = sum_k (U diag(w) U^H)_cc and N_static = sum w;
+// dN/dmu gets beta f (1-f) + (1/beta) sum_n 2 Re 1/(iw_n+mu-eps)^2 per level.
+#include "lattice_kernels.hpp"
+
+namespace dg {
+
+__global__ __launch_bounds__(256) void k_static_terms(const double* __restrict__ ev, int total, int nw_tot,
+                                                      double beta, double mu, double* __restrict__ wt,
+                                                      double* __restrict__ dt)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const double e = ev[t];
+    const double x = mu - e;
+    const double x2 = x * x;
+    double s = 0.0, s2 = 0.0;
+    for (int nw = 0; nw < nw_tot; ++nw) {
+        const double w = matsubara(nw, beta);
+        const double den = 1.0 / (x2 + w * w);
+        s = fma(2.0 * x, den, s);              // 2 Re 1/(x + iw)
+        s2 = fma(2.0 * (x2 - w * w), den * den, s2);  // 2 Re 1/(x + iw)^2
+    }
+    const double xb = beta * (e - mu);
+    double f;
+    if (xb > 100.0) f = 0.0;  // clamp of GreenFtn.cpp:314-316
+    else if (xb < -100.0) f = 1.0;
+    else f = 1.0 / (1.0 + exp(xb));
+    wt[t] = f - s / beta;
+    dt[t] = beta * f * (1.0 - f) + s2 / beta;
+}
+
+hipError_t launch_static_terms(const double* ev, int n, int nk, int nw_tot, double beta, double mu, double* wt,
+                               double* dt, hipStream_t s)
+{
+    const int total = n * nk;
+    hipLaunchKernelGGL(k_static_terms, dim3((total + 255) / 256), dim3(256), 0, s, ev, total, nw_tot, beta, mu, wt,
+                       dt);
+    return hipGetLastError();
+}
+
+// One workgroup per k chunk.  Thread e < m2 accumulates element e of
+// sum_k U diag(w) U^H over the chunk; the level sums are reduced by the block.
+__global__ __launch_bounds__(256) void k_static_reduce(const double* __restrict__ wt, const double* __restrict__ dt,
+                                                       const cd* __restrict__ U, int n, int nc, int nk,
+                                                       int k_per_chunk, cd* __restrict__ pNum,
+                                                       double* __restrict__ pN, double* __restrict__ pdN)
+{
+    __shared__ double red[2][256];
+    const int ch = blockIdx.x;
+    const int k0 = ch * k_per_chunk, k1 = min(nk, k0 + k_per_chunk);
+    const int m2 = nc * nc;
+    for (int e = threadIdx.x; e < m2; e += blockDim.x) {
+        const int i = e % nc, j = e / nc;
+        cd acc = cd{0.0, 0.0};
+        for (int k = k0; k < k1; ++k) {
+            const cd* Uk = U + (size_t)k * n * n;
+            const double* w = wt + (size_t)k * n;
+            for (int l = 0; l < n; ++l) acc = cadd(acc, cscale(cmulc(Uk[i + l * n], Uk[j + l * n]), w[l]));
+        }
+        pNum[(size_t)ch * m2 + e] = acc;
+    }
+    double sN = 0.0, sD = 0.0;
+    for (int t = k0 * n + threadIdx.x; t < k1 * n; t += blockDim.x) {
+        sN += wt[t];
+        sD += dt[t];
+    }
+    red[0][threadIdx.x] = sN;
+    red[1][threadIdx.x] = sD;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + o];
+            red[1][threadIdx.x] += red[1][threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        pN[ch] = red[0][0];
+        pdN[ch] = red[1][0];
+    }
+}
+
+hipError_t launch_static_reduce(const double* wt, const double* dt, const cd* U, int n, int nc, int nk,
+                                int k_per_chunk, int n_chunks, cd* pNum, double* pN, double* pdN, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_static_reduce, dim3(n_chunks), dim3(256), 0, s, wt, dt, U, n, nc, nk, k_per_chunk, pNum, pN,
+                       pdN);
+    return hipGetLastError();
+}
+
+// GwHF: thread per (nw, e); loops over all k and levels.  Optional output,
+// written only to files by the reference (GreenFtn.cpp:394-407).
+__global__ __launch_bounds__(256) void k_gloc_hf(const double* __restrict__ ev, const cd* __restrict__ U, int n,
+                                                 int nc, int nk, int n_freq, double beta, double mu, double inv_nkg,
+                                                 int npc, cd* __restrict__ out)
+{
+    const int m2 = nc * nc;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_freq * m2) return;
+    const int nw = t / m2, e = t % m2, i = e % nc, j = e / nc;
+    if (i / npc != j / npc) {
+        out[t] = cd{0.0, 0.0};
+        return;
+    }
+    const double w = matsubara(nw, beta);
+    cd acc = cd{0.0, 0.0};
+    for (int k = 0; k < nk; ++k) {
+        const cd* Uk = U + (size_t)k * n * n;
+        const double* ek = ev + (size_t)k * n;
+        for (int l = 0; l < n; ++l) {
+            const cd g = crcp(cd{mu - ek[l], w});
+            acc = cfma(cmulc(Uk[i + l * n], Uk[j + l * n]), g, acc);
+        }
+    }
+    out[t] = cscale(acc, inv_nkg);
+}
+
+hipError_t launch_gloc_hf(const double* ev, const cd* U, int n, int nc, int nk, int n_freq, double beta, double mu,
+                          double inv_nkg, int n_cluster, cd* out, hipStream_t s)
+{
+    const int total = n_freq * nc * nc;
+    hipLaunchKernelGGL(k_gloc_hf, dim3((total + 255) / 256), dim3(256), 0, s, ev, U, n, nc, nk, n_freq, beta, mu,
+                       inv_nkg, nc / n_cluster, out);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_reduce_chunks(const cd* __restrict__ part, int n_chunks, size_t len,
+                                                       cd* __restrict__ out)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= len) return;
+    cd acc = cd{0.0, 0.0};
+    for (int c = 0; c < n_chunks; ++c) acc = cadd(acc, part[(size_t)c * len + i]);
+    out[i] = acc;
+}
+
+__global__ __launch_bounds__(256) void k_reduce_chunks_d(const double* __restrict__ part, int n_chunks, size_t len,
+                                                         double* __restrict__ out)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= len) return;
+    double acc = 0.0;
+    for (int c = 0; c < n_chunks; ++c) acc += part[(size_t)c * len + i];
+    out[i] = acc;
+}
+
+hipError_t launch_reduce_chunks(const cd* part, int n_chunks, size_t len, cd* out, hipStream_t s)
+{
+    if (len == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_reduce_chunks, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, part, n_chunks, len,
+                       out);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_chunks_d(const double* part, int n_chunks, size_t len, double* out, hipStream_t s)
+{
+    if (len == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_reduce_chunks_d, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, part, n_chunks, len,
+                       out);
+    return hipGetLastError();
+}
+
+// One workgroup per output quantity q: q < m2 -> element e=q of
+// sum_nw (G + G^H)[e]; q = m2 -> sum 2 Re tr; m2+1 -> sum 2 Re tr2;
+// m2+2 -> sum Re mu.  Fixed-order block reduction.
+__global__ __launch_bounds__(256) void k_freq_totals(const cd* __restrict__ G, int m2, int nc,
+                                                     const cd* __restrict__ tr, const cd* __restrict__ tr2, int nw_tot,
+                                                     const cd* __restrict__ mu, int n_mu, cd* __restrict__ out)
+{
+    __shared__ double rx[256], ry[256];
+    const int q = blockIdx.x;
+    double ax = 0.0, ay = 0.0;
+    if (q < m2) {
+        if (G) {
+            const int i = q % nc, j = q / nc, et = j + i * nc;
+            for (int nw = threadIdx.x; nw < nw_tot; nw += 256) {
+                const cd g = G[(size_t)nw * m2 + q], gt = G[(size_t)nw * m2 + et];
+                ax += g.x + gt.x;
+                ay += g.y - gt.y;
+            }
+        }
+    } else if (q == m2) {
+        for (int nw = threadIdx.x; nw < nw_tot; nw += 256) ax += 2.0 * tr[nw].x;
+    } else if (q == m2 + 1) {
+        if (tr2)
+            for (int nw = threadIdx.x; nw < nw_tot; nw += 256) ax += 2.0 * tr2[nw].x;
+    } else {
+        if (mu)
+            for (int nw = threadIdx.x; nw < n_mu; nw += 256) ax += mu[nw].x;
+    }
+    rx[threadIdx.x] = ax;
+    ry[threadIdx.x] = ay;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            rx[threadIdx.x] += rx[threadIdx.x + o];
+            ry[threadIdx.x] += ry[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[q] = cd{rx[0], ry[0]};
+}
+
+hipError_t launch_freq_totals(const cd* G, int m2, int nc, const cd* tr, const cd* tr2, int nw_tot, const cd* mu,
+                              int n_mu, cd* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_freq_totals, dim3(m2 + 3), dim3(256), 0, s, G, m2, nc, tr, tr2, nw_tot, mu, n_mu, out);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_gloc_finalize(const cd* __restrict__ Gsum, int n_freq, int nc, int npc,
+                                                       double scale, cd* __restrict__ gloc)
+{
+    const int m2 = nc * nc;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_freq * m2) return;
+    const int e = t % m2, i = e % nc, j = e / nc;
+    gloc[t] = (i / npc == j / npc) ? cscale(Gsum[t], scale) : cd{0.0, 0.0};
+}
+
+hipError_t launch_gloc_finalize(const cd* Gsum, int n_freq, int nc, int n_cluster, double scale, cd* gloc,
+                                hipStream_t s)
+{
+    const int total = n_freq * nc * nc;
+    hipLaunchKernelGGL(k_gloc_finalize, dim3((total + 255) / 256), dim3(256), 0, s, Gsum, n_freq, nc, nc / n_cluster,
+                       scale, gloc);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_scale(const cd* __restrict__ in, size_t len, double scale,
+                                               cd* __restrict__ out)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < len) out[i] = cscale(in[i], scale);
+}
+
+hipError_t launch_scale(const cd* in, size_t len, double scale, cd* out, hipStream_t s)
+{
+    if (len == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scale, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, in, len, scale, out);
+    return hipGetLastError();
+}
+
+}  // namespace dg

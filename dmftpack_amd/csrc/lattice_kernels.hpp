@@ -1,0 +1,81 @@
+// Host-visible launchers of the lattice kernels (lattice_plain.hip,
+// lattice_schur.hip, eig.hip).  All pointers are device pointers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "dev_common.hpp"
+
+namespace dg {
+
+enum LatMode { MODE_FWD = 0, MODE_DENS = 1, MODE_VJP = 2 };
+
+struct LatArgs {
+    const cd* Hk;     // [nk][n][n]
+    const cd* sigma;  // [n_freq][nc][nc]
+    const cd* mom;    // [4][nc][nc]
+    int n, nc, nk, n_freq, nw_tot;
+    double beta, mu;
+    int k_per_chunk, n_chunks;
+    // partial sums, [chunk][nw][...]
+    cd* pG;    // FWD: sum_k G_cc       [chunks][nw_tot][nc*nc]
+    cd* pTr;   // sum_k tr G            [chunks][nw_tot]
+    cd* pTr2;  // sum_k tr G^2          [chunks][nw_tot]   (DENS, VJP)
+    // VJP
+    const cd* gbar;  // [n_freq][nc][nc]
+    double nbar;
+    cd* pSb;  // [chunks][n_freq][nc*nc] sum_k G^H gbar G^H + (2 nbar/beta) ((G^2)_cc)^H
+    cd* pMu;  // [chunks][n_freq]         sum_k tr(gbar^H (G^2)_cc)
+    // Schur-complement data (per k), see lattice_schur.hip
+    const double* ed;  // [nk][nd]      eigenvalues of H_dd(k)
+    const cd* W;       // [nk][nc][nd]  H_cd(k) V_d(k), column-major nc x nd
+    int refine;        // FP32R: number of FP64-residual refinement steps
+};
+
+// plain FP64 full inverse; returns 0 or a HIP error
+hipError_t launch_lattice_plain(const LatArgs& a, int mode, hipStream_t s, int* wtile_out);
+int plain_slots(int n, int mode);
+
+// Schur complement (FP64 or FP32 + refinement)
+hipError_t launch_lattice_schur(const LatArgs& a, int mode, bool fp32, hipStream_t s);
+int schur_slots(int nc, int mode);
+bool schur_supported(int n, int nc);
+
+// W = H_cd V_d per k (Schur preparation)
+hipError_t launch_schur_prep(const cd* Hk, const cd* Vd, int n, int nc, int nk, cd* W, hipStream_t s);
+
+// Batched Hermitian eigen-decomposition of X_b[off:off+m, off:off+m] (+ add on
+// the leading nadd x nadd block).  w [batch][m] ascending, V [batch][m][m] (may be null).
+hipError_t launch_herm_eig(const cd* X, int ldx, int off, int m, const cd* add, int nadd, double* w, cd* V,
+                           int batch, hipStream_t s);
+
+// Static part of the densities: per (k,l) weights f - s and dN terms, then
+// per-chunk partials of sum_k (U diag(f-s) U^H)_cc, sum (f-s), sum dN.
+hipError_t launch_static_terms(const double* ev, int n, int nk, int nw_tot, double beta, double mu, double* wt,
+                               double* dt, hipStream_t s);
+hipError_t launch_static_reduce(const double* wt, const double* dt, const cd* U, int n, int nc, int nk,
+                                int k_per_chunk, int n_chunks, cd* pNum, double* pN, double* pdN, hipStream_t s);
+
+// GwHF (optional output): gloc_hf[nw][a][b] = (1/nkg) sum_k sum_l U_al conj(U_bl)/(iw+mu-e_kl)
+hipError_t launch_gloc_hf(const double* ev, const cd* U, int n, int nc, int nk, int n_freq, double beta, double mu,
+                          double inv_nkg, int n_cluster, cd* out, hipStream_t s);
+
+// Deterministic reductions over chunks: out[i] = sum_c part[c*len + i]
+hipError_t launch_reduce_chunks(const cd* part, int n_chunks, size_t len, cd* out, hipStream_t s);
+hipError_t launch_reduce_chunks_d(const double* part, int n_chunks, size_t len, double* out, hipStream_t s);
+
+// Final reductions over frequency: for FWD/DENS/VJP totals.
+//   sumG[e]  = sum_nw (G[nw][e] + conj(G[nw][e^T]))        (e < m2)   if G != null
+//   sumTr    = sum_nw 2 Re tr[nw]
+//   sumTr2   = sum_nw 2 Re tr2[nw]                          if tr2 != null
+//   sumMu    = sum_nw Re mu[nw] over n_mu entries           if mu != null
+// out layout: [m2 complex] [3 doubles as complex .x]
+hipError_t launch_freq_totals(const cd* G, int m2, int nc, const cd* tr, const cd* tr2, int nw_tot, const cd* mu,
+                              int n_mu, cd* out, hipStream_t s);
+
+// gloc[nw][e] = Gsum[nw][e] * scale for nw < n_freq, zero outside cluster blocks
+hipError_t launch_gloc_finalize(const cd* Gsum, int n_freq, int nc, int n_cluster, double scale, cd* gloc,
+                                hipStream_t s);
+// out[i] = in[i] * scale
+hipError_t launch_scale(const cd* in, size_t len, double scale, cd* out, hipStream_t s);
+
+}  // namespace dg

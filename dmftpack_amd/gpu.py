@@ -1,0 +1,188 @@
+"""Thin torch-memory wrapper over the C ABI (include/dmftgpu.h).
+
+Torch only provides device memory and the stream here; every computation runs
+in libdmftgpu.so.  Arrays are accepted in "math" layout (A[m, i, j] = A_m(i, j))
+as numpy arrays or torch tensors and converted to the column-major complex128
+device buffers the ABI expects.  Method names follow the reference routines
+(GreenFtn_w, GreenFtn_w_adjust_mu, getAsymto_moments, FourierTransform,
+FT_t_to_w).
+"""
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import capi
+
+PLAIN_F64, SCHUR_F64, SCHUR_F32R = capi.PLAIN_F64, capi.SCHUR_F64, capi.SCHUR_F32R
+
+
+def _require_gpu():
+    if not torch.cuda.is_available():
+        raise RuntimeError("dmftpack_amd needs a ROCm GPU: torch.cuda.is_available() is False")
+
+
+def to_dev_cm(a, device="cuda"):
+    """math-layout array -> contiguous column-major complex128 device tensor"""
+    if isinstance(a, np.ndarray):
+        t = torch.from_numpy(np.ascontiguousarray(np.swapaxes(np.asarray(a, np.complex128), -1, -2)))
+        return t.to(device)
+    t = a.to(device=device, dtype=torch.complex128)
+    return t.transpose(-1, -2).contiguous()
+
+
+def from_dev_cm(t):
+    """column-major device tensor -> math-layout numpy array"""
+    return np.ascontiguousarray(np.swapaxes(t.cpu().numpy(), -1, -2))
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _stream_ptr(stream):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+class Handle:
+    def __init__(self, stream=None):
+        _require_gpu()
+        self.lib = capi.lib()
+        self.h = C.c_void_p()
+        self.stream = stream
+        rc = self.lib.dmftgpu_create(C.byref(self.h), _stream_ptr(stream))
+        if rc != 0:
+            raise capi.DmftGpuError(rc, "dmftgpu_create failed")
+
+    def check(self, rc):
+        return capi.check(self.h, rc)
+
+    def close(self):
+        if self.h:
+            self.lib.dmftgpu_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- Matsubara tails and transforms ----------------------------------
+    def get_asymto_moments(self, fw, beta, green_asym=False, return_windows=False):
+        """getAsymto_moments (FourierTransform.cpp:184-465)."""
+        f = to_dev_cm(fw)
+        nf, d = f.shape[0], f.shape[1]
+        mom = torch.zeros((4, d, d), dtype=torch.complex128, device=f.device)
+        win = (C.c_int * 4)()
+        self.check(self.lib.dmftgpu_moments(self.h, d, nf, beta, _ptr(f), int(green_asym), _ptr(mom), win))
+        m = from_dev_cm(mom)
+        return (m, list(win)) if return_windows else m
+
+    def fourier_transform(self, fw, moments, beta, n_tau):
+        """FourierTransform_ (FourierTransform.cpp:126-178): G(iw) -> G(tau_j), j=0..n_tau."""
+        f, m = to_dev_cm(fw), to_dev_cm(moments)
+        nf, d = f.shape[0], f.shape[1]
+        out = torch.zeros((n_tau + 1, d, d), dtype=torch.complex128, device=f.device)
+        patched = C.c_int(0)
+        self.check(self.lib.dmftgpu_ft_w2tau(self.h, d, nf, n_tau, beta, _ptr(f), _ptr(m), _ptr(out),
+                                             C.byref(patched)))
+        return from_dev_cm(out), patched.value
+
+    def ft_t_to_w(self, gt, beta, n_freq):
+        """FT_t_to_w (FourierTransform.cpp:521-561)."""
+        g = to_dev_cm(gt)
+        d = g.shape[1]
+        out = torch.zeros((n_freq, d, d), dtype=torch.complex128, device=g.device)
+        viol = C.c_int(0)
+        self.check(self.lib.dmftgpu_ft_tau2w(self.h, d, n_freq, g.shape[0] - 1, beta, _ptr(g), _ptr(out),
+                                             C.byref(viol)))
+        return from_dev_cm(out), viol.value
+
+
+class LatticeGF(Handle):
+    """One lattice (H(k) + Sigma tail moments) bound to a library handle."""
+
+    def __init__(self, Hk, n_corr, n_freq, beta, moments, n_cluster=1, n_k_global=None, high_freq=2, stream=None):
+        super().__init__(stream)
+        self.Hk = Hk if isinstance(Hk, torch.Tensor) and Hk.is_cuda and Hk.dtype == torch.complex128 \
+            and getattr(Hk, "_dmft_colmajor", False) else to_dev_cm(Hk)
+        nk, n = self.Hk.shape[0], self.Hk.shape[1]
+        self.n, self.nc, self.nk = n, n_corr, nk
+        self.n_freq, self.beta = n_freq, beta
+        self.lat = capi.Lattice(n, n_corr, n_cluster, nk, n_k_global or nk, n_freq, high_freq, beta)
+        self.set_moments(moments)
+
+    def set_moments(self, moments):
+        self.mom = to_dev_cm(moments)
+        self.check(self.lib.dmftgpu_prepare(self.h, C.byref(self.lat), _ptr(self.Hk), _ptr(self.mom)))
+
+    def sigma_dev(self, sigma):
+        if isinstance(sigma, torch.Tensor) and getattr(sigma, "_dmft_colmajor", False):
+            return sigma
+        s = to_dev_cm(sigma)
+        s._dmft_colmajor = True
+        return s
+
+    def static_eig(self):
+        ev = torch.zeros((self.nk, self.n), dtype=torch.float64, device="cuda")
+        U = torch.zeros((self.nk, self.n, self.n), dtype=torch.complex128, device="cuda")
+        self.check(self.lib.dmftgpu_static_eig(self.h, _ptr(ev), _ptr(U)))
+        torch.cuda.synchronize()
+        return ev.cpu().numpy(), from_dev_cm(U)
+
+    def greenftn_w_dev(self, sigma_dev, mu, method=PLAIN_F64, gloc=None, num=None, gloc_hf=None):
+        nc = self.nc
+        if gloc is None:
+            gloc = torch.empty((self.n_freq, nc, nc), dtype=torch.complex128, device="cuda")
+        if num is None:
+            num = torch.empty((nc, nc), dtype=torch.complex128, device="cuda")
+        self.check(self.lib.dmftgpu_greenftn_w(self.h, method, _ptr(sigma_dev), mu, _ptr(gloc), _ptr(gloc_hf),
+                                               _ptr(num)))
+        return gloc, num
+
+    def greenftn_w(self, sigma, mu, method=PLAIN_F64, want_hf=False):
+        """GreenFtn_w (GreenFtn.cpp:142-422) -> dict(gloc, num_matrix[, gloc_hf]) in math layout."""
+        s = self.sigma_dev(sigma)
+        hf = torch.empty((self.n_freq, self.nc, self.nc), dtype=torch.complex128, device="cuda") if want_hf else None
+        gl, num = self.greenftn_w_dev(s, mu, method, gloc_hf=hf)
+        out = dict(gloc=from_dev_cm(gl), num_matrix=from_dev_cm(num))
+        if want_hf:
+            out["gloc_hf"] = from_dev_cm(hf)
+        return out
+
+    def density_matrix(self, sigma, mu):
+        s = self.sigma_dev(sigma)
+        out = torch.empty((self.nk, self.n, self.n), dtype=torch.complex128, device="cuda")
+        self.check(self.lib.dmftgpu_density_matrix(self.h, _ptr(s), mu, _ptr(out)))
+        return from_dev_cm(out)
+
+    def density(self, sigma, mu, method=PLAIN_F64):
+        """(N(mu), dN/dmu) -- GiwToNele (GreenFtn.cpp:513-635) plus its derivative."""
+        s = self.sigma_dev(sigma)
+        N, dN = C.c_double(0), C.c_double(0)
+        self.check(self.lib.dmftgpu_density(self.h, method, _ptr(s), mu, C.byref(N), C.byref(dN)))
+        return N.value, dN.value
+
+    def adjust_mu(self, sigma, mu0, n_target, method=PLAIN_F64, tol=1e-10):
+        """GreenFtn_w_adjust_mu (GreenFtn.cpp:66-139), Newton on the analytic dN/dmu."""
+        s = self.sigma_dev(sigma)
+        mu, ne = C.c_double(0), C.c_int(0)
+        self.check(self.lib.dmftgpu_adjust_mu(self.h, method, _ptr(s), mu0, n_target, tol, C.byref(mu), C.byref(ne)))
+        return mu.value, ne.value
+
+    def vjp_dev(self, sigma_dev, mu, gbar_dev, nbar, method=PLAIN_F64, out=None):
+        if out is None:
+            out = torch.empty_like(sigma_dev)
+        mb = C.c_double(0)
+        self.check(self.lib.dmftgpu_gloc_vjp(self.h, method, _ptr(sigma_dev), mu, _ptr(gbar_dev), nbar, _ptr(out),
+                                             C.byref(mb)))
+        return out, mb.value
+
+    def vjp(self, sigma, mu, gbar, nbar, method=PLAIN_F64):
+        """(sigma_bar, mu_bar) for cotangents (gbar of G_loc, nbar of N); dL/dRe + i dL/dIm."""
+        s = self.sigma_dev(sigma)
+        g = to_dev_cm(gbar)
+        sb, mb = self.vjp_dev(s, mu, g, nbar, method)
+        return from_dev_cm(sb), mb

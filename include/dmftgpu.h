@@ -1,0 +1,140 @@
+/*
+ * dmftgpu.h -- C ABI of the MI355X lattice Green's-function library.
+ *
+ * Replaces the k-space lattice Green's function path of DMFTpack
+ * (src/GreenFtn.cpp, src/FourierTransform.cpp).  Plain C: pointers, sizes and
+ * an opaque handle.  No HIP, torch or Eigen types cross this boundary; the HIP
+ * stream is passed as a void*.
+ *
+ * Data layout (all complex numbers are complex128 = two doubles, re then im):
+ *   every matrix is column-major, element (i,j) at offset i + j*rows, which is
+ *   Eigen's MatrixXcd::data() layout; a stack of matrices is contiguous,
+ *   matrix m starting at m*rows*cols.
+ *     Hk        [n_k][n][n]            H(k) in the model space, correlated
+ *                                      orbitals first (downfolding.cpp:69-79)
+ *     sigma     [n_freq][nc][nc]       SelfE_w(n), n < N_freq (ImgTimeFtn.h:32-98)
+ *     moments   [4][nc][nc]            m0..m3 of getAsymto_moments
+ *     gloc      [n_freq][nc][nc]       Gw (block-diagonal per cluster)
+ *   Array arguments are DEVICE pointers unless the comment says "host".
+ *   nc = n_corr = NumCluster * NumHartrOrbit_per_cluster.
+ *
+ * Errors: every function returns a status (0 = success); the message of the
+ * last failure is available from dmftgpu_last_error().  The reference asserts
+ * or calls exit(1) in the same situations (GreenFtn.cpp:85,113;
+ * FourierTransform.cpp:321,440,557); the C++ adapter in dmftgpu_shim.hpp
+ * maps a non-zero status back to that behaviour.
+ *
+ * Threading: a handle is not thread-safe; calls on one handle are ordered on
+ * its stream.  Functions that return host scalars synchronise that stream.
+ */
+#ifndef DMFTGPU_H
+#define DMFTGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DMFTGPU_OK 0
+#define DMFTGPU_ERR_ARG 1         /* invalid argument or size out of range */
+#define DMFTGPU_ERR_STATE 2       /* dmftgpu_prepare not called / stale */
+#define DMFTGPU_ERR_HIP 3         /* HIP runtime error */
+#define DMFTGPU_ERR_NUMERIC 4     /* non-finite result, failed bracketing */
+#define DMFTGPU_ERR_UNSUPPORTED 5 /* size or option this build does not handle */
+
+/* Methods for the per-(k, iw) solve. */
+#define DMFTGPU_PLAIN_F64 0  /* full n x n inverse per (k,iw) in FP64 (GreenFtn.cpp:261) */
+#define DMFTGPU_SCHUR_F64 1  /* Schur complement on the correlated block, FP64 */
+#define DMFTGPU_SCHUR_F32R 2 /* Schur complement, FP32 solve + FP64-residual refinement */
+
+typedef struct dmftgpu_lattice {
+    int32_t n;          /* NBAND[k] (TB.h:29), uniform over k, 1..32 */
+    int32_t n_corr;     /* NumCluster*NumHartrOrbit_per_cluster (tight_common.h:61-62) */
+    int32_t n_cluster;  /* NumCluster; gloc is block-diagonal over clusters */
+    int32_t n_k;        /* knum: k points held in Hk */
+    int32_t n_k_global; /* knum_mpiGlobal: normalisation of the k average */
+    int32_t n_freq;     /* N_freq (tight_common.h:67) */
+    int32_t high_freq;  /* frequencies summed for densities = high_freq*N_freq (highFreq2, GreenFtn.cpp:8) */
+    double beta;        /* inverse temperature (tight_common.h:64) */
+} dmftgpu_lattice;
+
+typedef struct dmftgpu_handle_s* dmftgpu_handle;
+
+/* ---- handle ---------------------------------------------------------------- */
+int dmftgpu_create(dmftgpu_handle* h, void* hip_stream);
+int dmftgpu_destroy(dmftgpu_handle h);
+int dmftgpu_set_stream(dmftgpu_handle h, void* hip_stream);
+const char* dmftgpu_last_error(dmftgpu_handle h);
+const char* dmftgpu_version(void);
+
+/* ---- lattice path ---------------------------------------------------------- */
+
+/* Binds H(k) and the Sigma tail moments to the handle and computes the static
+ * eigen-decomposition H(k) + P m0 P^H = U diag(eps) U^H for every k
+ * (GreenFtn.cpp:197-215).  Hk must stay valid until the next prepare.
+ * For the Schur methods it also diagonalises H_dd(k) once per k. */
+int dmftgpu_prepare(dmftgpu_handle h, const dmftgpu_lattice* lat, const double* Hk, const double* moments);
+
+/* Copies the static eigenpairs out: eps [n_k][n] ascending, U [n_k][n][n]. */
+int dmftgpu_static_eig(dmftgpu_handle h, double* eps, double* U);
+
+/* GreenFtn_w (GreenFtn.cpp:142-422) for one mu.
+ *   gloc        [n_freq][nc][nc]  required; inter-cluster blocks are zero (:371-383)
+ *   gloc_hf     [n_freq][nc][nc]  optional (NULL): GwHF, the static-part G0 (:277)
+ *   num_matrix  [nc][nc]          required; the global NumMatrix (:322-356)
+ * Method: DMFTGPU_PLAIN_F64 or DMFTGPU_SCHUR_*. */
+int dmftgpu_greenftn_w(dmftgpu_handle h, int method, const double* sigma, double mu, double* gloc, double* gloc_hf,
+                       double* num_matrix);
+
+/* Per-k density matrix densityMatDFT (GreenFtn.cpp:291-319): dens [n_k][n][n].
+ * Full n x n inverse, DMFTGPU_PLAIN_F64 only. */
+int dmftgpu_density_matrix(dmftgpu_handle h, const double* sigma, double mu, double* dens);
+
+/* Total electron number N(mu) of GiwToNele (GreenFtn.cpp:513-635, via the
+ * trace identity sum_l 1/(z+mu-lambda_l) = tr G) and its analytic dN/dmu.
+ * n_out, dn_dmu_out are HOST pointers (either may be NULL). */
+int dmftgpu_density(dmftgpu_handle h, int method, const double* sigma, double mu, double* n_out, double* dn_dmu_out);
+
+/* GreenFtn_w_adjust_mu (GreenFtn.cpp:66-139): mu with N(mu) = n_target.
+ * Safeguarded Newton on the analytic dN/dmu (bracket + bisection fallback),
+ * stopping when |dmu| <= tol.  mu0 initial guess; mu_out, n_eval HOST. */
+int dmftgpu_adjust_mu(dmftgpu_handle h, int method, const double* sigma, double mu0, double n_target, double tol,
+                      double* mu_out, int* n_eval);
+
+/* Vector-Jacobian product of (G_loc, N) with respect to (Sigma(iw_n), n<N_freq)
+ * and mu, moments held fixed; G is recomputed, never stored.
+ *   gloc_bar  [n_freq][nc][nc]   cotangent of the full nc x nc G_loc
+ *   n_bar     cotangent of N
+ *   sigma_bar [n_freq][nc][nc]   output, convention dL/dRe + i dL/dIm
+ *   mu_bar    HOST output
+ * No reference counterpart (the reference has no derivatives). */
+int dmftgpu_gloc_vjp(dmftgpu_handle h, int method, const double* sigma, double mu, const double* gloc_bar,
+                     double n_bar, double* sigma_bar, double* mu_bar);
+
+/* ---- Matsubara tails and Fourier transforms ---------------------------------- */
+
+/* getAsymto_moments (FourierTransform.cpp:184-465): fw [n_freq][d][d] ->
+ * moments [4][d][d].  green_asym != 0 selects the G-like branch (m0=0, m1=I).
+ * windows (HOST, optional, 4 ints) receives the selected window centres. */
+int dmftgpu_moments(dmftgpu_handle h, int d, int n_freq, double beta, const double* fw, int green_asym,
+                    double* moments, int* windows);
+
+/* FourierTransform_ (FourierTransform.cpp:126-178) with FourierTransform_tau
+ * (:16-118): G(tau_j), tau_j = j beta/n_tau, j = 0..n_tau, two-pole tail
+ * subtraction, Hermitisation and the causality patch.
+ *   gtau [n_tau+1][d][d]; patched HOST (optional) = the reference's CHECK flag. */
+int dmftgpu_ft_w2tau(dmftgpu_handle h, int d, int n_freq, int n_tau, double beta, const double* fw,
+                     const double* moments, double* gtau, int* patched);
+
+/* FT_t_to_w (FourierTransform.cpp:521-561): exact transform of the piecewise
+ * linear interpolant of G(tau).  violations HOST (optional) counts frequencies
+ * failing the causality test of :551-558 (the reference asserts). */
+int dmftgpu_ft_tau2w(dmftgpu_handle h, int d, int n_freq, int n_tau, double beta, const double* gtau, double* fw,
+                     int* violations);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DMFTGPU_H */

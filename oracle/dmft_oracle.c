@@ -1,0 +1,1001 @@
+/*
+ * dmft_oracle.c -- CPU restatement of DMFTpack's lattice Green's-function path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP library in
+ * dmftpack_amd/: only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it, and never as the thing measured or shipped.
+ * The product never links, loads or calls it.
+ *
+ * Every function restates the reference routine named in its comment
+ * (paths relative to the DMFTpack source tree).  The reference itself cannot
+ * be compiled here (it needs the un-vendored Eigen >= 3.3), so Eigen's
+ * dense kernels are restated too:
+ *   - MatrixXcd::inverse()           -> partial-pivot LU + two triangular solves
+ *   - SelfAdjointEigenSolver          -> cyclic complex Jacobi, eigenvalues ascending
+ *   - SelfAdjointEigenSolver::operatorSqrt -> V diag(sqrt(w)) V^H
+ *
+ * Layout: every matrix is complex128, column-major (Eigen's MatrixXcd::data()).
+ * A stack of matrices is [count][rows][cols], i.e. matrix m starts at m*rows*cols.
+ *
+ * Pinning (what checks each function; see tests/):
+ *   orc_lu_inverse       numpy.linalg.inv, A*Ainv = I
+ *   orc_herm_eig         numpy.linalg.eigh, residual |AV - VL|
+ *   orc_greenftn_w       golden fixtures from numpy brute force (tests/golden),
+ *                        closed form for orbital-diagonal H and diagonal Sigma,
+ *                        G(-iw)=G(iw)^H, Sigma=const => Ncc = Fermi sum
+ *   orc_giw_to_nele      eigenvalue formula of GreenFtn.cpp:600-634 evaluated
+ *                        with numpy.linalg.eigvals (two independent codings)
+ *   orc_adjust_mu        N(mu_out) = target within the bisection tolerance
+ *   orc_asymto_moments   exact recovery of polynomial tails, single pole
+ *   orc_ft_w2tau         two-pole closed form, G(0)+G(beta) = -m1
+ *   orc_ft_tau2w         exact transform of piecewise-linear G(tau)
+ *   orc_gloc_vjp         central finite differences of orc_density/orc_loss
+ */
+#include <complex.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef double complex cplx;
+
+/* dmft_common.h:43 */
+static const double PI_REF = 3.1415926535897932384626;
+
+typedef struct orc_lattice {
+    int n;          /* NBAND[k] (uniform) */
+    int n_corr;     /* NumCluster*NumHartrOrbit_per_cluster, correlated orbitals first */
+    int n_cluster;  /* NumCluster */
+    int n_k;        /* knum */
+    int n_k_global; /* knum_mpiGlobal */
+    int n_freq;     /* N_freq */
+    int high_freq;  /* highFreq2 (GreenFtn.cpp:8) */
+    double beta;
+} orc_lattice;
+
+#define IDX(i, j, ld) ((size_t)(i) + (size_t)(j) * (size_t)(ld))
+
+/* ------------------------------------------------------------------------- */
+/* Small dense kernels (Eigen restatements)                                   */
+/* ------------------------------------------------------------------------- */
+
+/* Eigen PartialPivLU + inverse(): pivot = largest modulus in the column. */
+int orc_lu_inverse(int n, const cplx *A, cplx *Ainv)
+{
+    cplx *lu = (cplx *)malloc(sizeof(cplx) * (size_t)n * n);
+    int *perm = (int *)malloc(sizeof(int) * (size_t)n);
+    int status = 0;
+    memcpy(lu, A, sizeof(cplx) * (size_t)n * n);
+    for (int i = 0; i < n; ++i) perm[i] = i;
+    for (int k = 0; k < n; ++k) {
+        int p = k;
+        double best = cabs(lu[IDX(k, k, n)]);
+        for (int i = k + 1; i < n; ++i) {
+            double s = cabs(lu[IDX(i, k, n)]);
+            if (s > best) { best = s; p = i; }
+        }
+        if (best == 0.0) status = 1;
+        if (p != k) {
+            for (int j = 0; j < n; ++j) {
+                cplx t = lu[IDX(k, j, n)];
+                lu[IDX(k, j, n)] = lu[IDX(p, j, n)];
+                lu[IDX(p, j, n)] = t;
+            }
+            int t = perm[k]; perm[k] = perm[p]; perm[p] = t;
+        }
+        if (best != 0.0) {
+            cplx inv = 1.0 / lu[IDX(k, k, n)];
+            for (int i = k + 1; i < n; ++i) lu[IDX(i, k, n)] *= inv;
+        }
+        for (int j = k + 1; j < n; ++j) {
+            cplx u = lu[IDX(k, j, n)];
+            for (int i = k + 1; i < n; ++i) lu[IDX(i, j, n)] -= lu[IDX(i, k, n)] * u;
+        }
+    }
+    /* solve LU X = P I column by column */
+    for (int c = 0; c < n; ++c) {
+        cplx *x = Ainv + (size_t)c * n;
+        for (int i = 0; i < n; ++i) x[i] = (perm[i] == c) ? 1.0 : 0.0;
+        for (int i = 0; i < n; ++i) {
+            cplx s = x[i];
+            for (int j = 0; j < i; ++j) s -= lu[IDX(i, j, n)] * x[j];
+            x[i] = s;
+        }
+        for (int i = n - 1; i >= 0; --i) {
+            cplx s = x[i];
+            for (int j = i + 1; j < n; ++j) s -= lu[IDX(i, j, n)] * x[j];
+            x[i] = s / lu[IDX(i, i, n)];
+        }
+    }
+    free(lu);
+    free(perm);
+    return status;
+}
+
+/* Hermitian eigen-decomposition, eigenvalues ascending (Eigen's
+ * SelfAdjointEigenSolver order).  Cyclic Jacobi with complex rotations.
+ * V (may be NULL) receives the eigenvectors as columns. */
+int orc_herm_eig(int n, const cplx *A, double *w, cplx *V)
+{
+    cplx *a = (cplx *)malloc(sizeof(cplx) * (size_t)n * n);
+    cplx *v = (cplx *)malloc(sizeof(cplx) * (size_t)n * n);
+    /* symmetrise from the lower triangle convention-free: use (A+A^H)/2 */
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+            a[IDX(i, j, n)] = 0.5 * (A[IDX(i, j, n)] + conj(A[IDX(j, i, n)]));
+            v[IDX(i, j, n)] = (i == j) ? 1.0 : 0.0;
+        }
+    double norm2 = 0.0;
+    for (size_t e = 0; e < (size_t)n * n; ++e) norm2 += creal(a[e] * conj(a[e]));
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off2 = 0.0;
+        for (int q = 1; q < n; ++q)
+            for (int p = 0; p < q; ++p) off2 += 2.0 * creal(a[IDX(p, q, n)] * conj(a[IDX(p, q, n)]));
+        if (off2 <= 1e-34 * norm2 || off2 == 0.0) break;
+        for (int p = 0; p < n - 1; ++p) {
+            for (int q = p + 1; q < n; ++q) {
+                cplx b = a[IDX(p, q, n)];
+                double ab = cabs(b);
+                if (ab == 0.0) continue;
+                double app = creal(a[IDX(p, p, n)]);
+                double aqq = creal(a[IDX(q, q, n)]);
+                cplx ph = b / ab; /* e^{i phi} */
+                double zeta = (aqq - app) / (2.0 * ab);
+                double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                double c = 1.0 / sqrt(1.0 + t * t);
+                double s = t * c;
+                cplx phc = conj(ph);
+                /* columns: col_p' = c col_p - s e^{-i phi} col_q ; col_q' = s col_p + c e^{-i phi} col_q */
+                for (int i = 0; i < n; ++i) {
+                    cplx xp = a[IDX(i, p, n)], xq = a[IDX(i, q, n)];
+                    a[IDX(i, p, n)] = c * xp - s * phc * xq;
+                    a[IDX(i, q, n)] = s * xp + c * phc * xq;
+                    cplx vp = v[IDX(i, p, n)], vq = v[IDX(i, q, n)];
+                    v[IDX(i, p, n)] = c * vp - s * phc * vq;
+                    v[IDX(i, q, n)] = s * vp + c * phc * vq;
+                }
+                /* rows: row_p' = c row_p - s e^{i phi} row_q ; row_q' = s row_p + c e^{i phi} row_q */
+                for (int j = 0; j < n; ++j) {
+                    cplx xp = a[IDX(p, j, n)], xq = a[IDX(q, j, n)];
+                    a[IDX(p, j, n)] = c * xp - s * ph * xq;
+                    a[IDX(q, j, n)] = s * xp + c * ph * xq;
+                }
+                a[IDX(p, q, n)] = 0.0;
+                a[IDX(q, p, n)] = 0.0;
+                a[IDX(p, p, n)] = creal(a[IDX(p, p, n)]);
+                a[IDX(q, q, n)] = creal(a[IDX(q, q, n)]);
+            }
+        }
+    }
+    /* sort ascending (selection sort keeps it simple and stable) */
+    int *ord = (int *)malloc(sizeof(int) * (size_t)n);
+    for (int i = 0; i < n; ++i) { ord[i] = i; w[i] = creal(a[IDX(i, i, n)]); }
+    for (int i = 0; i < n; ++i) {
+        int m = i;
+        for (int j = i + 1; j < n; ++j) if (w[ord[j]] < w[ord[m]]) m = j;
+        int t = ord[i]; ord[i] = ord[m]; ord[m] = t;
+    }
+    double *ws = (double *)malloc(sizeof(double) * (size_t)n);
+    for (int i = 0; i < n; ++i) ws[i] = w[ord[i]];
+    memcpy(w, ws, sizeof(double) * (size_t)n);
+    if (V)
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) V[IDX(i, j, n)] = v[IDX(i, ord[j], n)];
+    free(ws); free(ord); free(a); free(v);
+    return 0;
+}
+
+/* SelfAdjointEigenSolver::operatorSqrt(): V diag(sqrt(max(w,0))) V^H */
+static void herm_sqrt(int n, const cplx *A, cplx *S)
+{
+    double *w = (double *)malloc(sizeof(double) * (size_t)n);
+    cplx *V = (cplx *)malloc(sizeof(cplx) * (size_t)n * n);
+    orc_herm_eig(n, A, w, V);
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+            cplx s = 0;
+            for (int l = 0; l < n; ++l) s += V[IDX(i, l, n)] * sqrt(w[l] > 0 ? w[l] : 0.0) * conj(V[IDX(j, l, n)]);
+            S[IDX(i, j, n)] = s;
+        }
+    free(w); free(V);
+}
+
+static double herm_min_eig(int n, const cplx *A)
+{
+    double *w = (double *)malloc(sizeof(double) * (size_t)n);
+    orc_herm_eig(n, A, w, NULL);
+    double m = w[0];
+    free(w);
+    return m;
+}
+
+static double herm_max_eig(int n, const cplx *A)
+{
+    double *w = (double *)malloc(sizeof(double) * (size_t)n);
+    orc_herm_eig(n, A, w, NULL);
+    double m = w[n - 1];
+    free(w);
+    return m;
+}
+
+/* C = A * B, all n x n */
+static void matmul(int n, const cplx *A, const cplx *B, cplx *C)
+{
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+            cplx s = 0;
+            for (int l = 0; l < n; ++l) s += A[IDX(i, l, n)] * B[IDX(l, j, n)];
+            C[IDX(i, j, n)] = s;
+        }
+}
+
+/* C = A^H */
+static void adjoint(int n, const cplx *A, cplx *C)
+{
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) C[IDX(i, j, n)] = conj(A[IDX(j, i, n)]);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Self-energy at frequency index n: SelfE_w for n < N_freq, moment expansion */
+/* beyond (GreenFtn.cpp:224-240).                                              */
+/* ------------------------------------------------------------------------- */
+static void sigma_at(const orc_lattice *L, const cplx *sigma, const cplx *mom, int nw, cplx *out)
+{
+    const int nc = L->n_corr;
+    const size_t m2 = (size_t)nc * nc;
+    if (nw < L->n_freq) {
+        memcpy(out, sigma + (size_t)nw * m2, sizeof(cplx) * m2);
+        return;
+    }
+    cplx iw = I * PI_REF * (2.0 * nw + 1.0) / L->beta;
+    cplx iw2 = iw * iw, iw3 = iw * iw * iw;
+    for (size_t e = 0; e < m2; ++e) {
+        cplx v = mom[e];
+        v += mom[m2 + e] / iw;
+        v += mom[2 * m2 + e] / iw2;
+        v += mom[3 * m2 + e] / iw3;
+        out[e] = v;
+    }
+}
+
+/* Static part (GreenFtn.cpp:197-215): eig of H(k) + P m0 P^H for every k. */
+static void static_eig(const orc_lattice *L, const cplx *Hk, const cplx *mom, double *ev, cplx *U, int nthreads)
+{
+    const int n = L->n, nc = L->n_corr;
+    const size_t n2 = (size_t)n * n;
+    (void)nthreads;
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+    for (int k = 0; k < L->n_k; ++k) {
+        cplx *M = (cplx *)malloc(sizeof(cplx) * n2);
+        memcpy(M, Hk + k * n2, sizeof(cplx) * n2);
+        for (int j = 0; j < nc; ++j)
+            for (int i = 0; i < nc; ++i) M[IDX(i, j, n)] += mom[IDX(i, j, nc)];
+        orc_herm_eig(n, M, ev + (size_t)k * n, U + k * n2);
+        free(M);
+    }
+}
+
+/* Fermi factor with the clamp of GreenFtn.cpp:314-316 */
+static double fermi_clamped(double beta, double e, double mu)
+{
+    double x = beta * (e - mu);
+    if (x > 100.0) return 0.0;
+    if (x < -100.0) return 1.0;
+    return 1.0 / (1.0 + exp(x));
+}
+
+/* ------------------------------------------------------------------------- */
+/* GreenFtn_w (GreenFtn.cpp:142-422).                                          */
+/*   gloc       [n_freq][nc][nc]   block-diagonal per cluster (:371-383)       */
+/*   gloc_hf    [n_freq][nc][nc]   (optional, GwHF)                            */
+/*   num_matrix [nc][nc]           global NumMatrix (:322-356)                 */
+/*   density    [n_k][n][n]        densityMatDFT (optional)                    */
+/* moments [4][nc][nc] are the Sigma tail coefficients of getAsymto_moments.   */
+/* ------------------------------------------------------------------------- */
+int orc_greenftn_w(const orc_lattice *L, const cplx *Hk, const cplx *sigma, const cplx *mom, double mu,
+                   cplx *gloc, cplx *gloc_hf, cplx *num_matrix, cplx *density, int nthreads)
+{
+    const int n = L->n, nc = L->n_corr, ncl = L->n_cluster;
+    const int npc = nc / ncl; /* NumHartrOrbit_per_cluster */
+    const size_t n2 = (size_t)n * n, m2 = (size_t)nc * nc;
+    const int nw_tot = L->n_freq * L->high_freq;
+    const double beta = L->beta;
+    if (nthreads < 1) nthreads = 1;
+
+    double *ev = (double *)malloc(sizeof(double) * (size_t)L->n_k * n);
+    cplx *U = (cplx *)malloc(sizeof(cplx) * (size_t)L->n_k * n2);
+    static_eig(L, Hk, mom, ev, U, nthreads);
+
+    /* per-thread accumulators: Gw_loc, GwHF_loc (only intra-cluster blocks are
+     * used, stored as full nc x nc), NumMat_loc, NumMat_loc0 */
+    cplx *accG = (cplx *)calloc((size_t)nthreads * L->n_freq * m2, sizeof(cplx));
+    cplx *accH = (cplx *)calloc((size_t)nthreads * L->n_freq * m2, sizeof(cplx));
+    cplx *accN = (cplx *)calloc((size_t)nthreads * m2, sizeof(cplx));
+    cplx *accN0 = (cplx *)calloc((size_t)nthreads * m2, sizeof(cplx));
+    if (density) memset(density, 0, sizeof(cplx) * (size_t)L->n_k * n2);
+
+#pragma omp parallel num_threads(nthreads)
+    {
+        int tid = 0, nth = 1;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+        nth = omp_get_num_threads();
+#endif
+        /* contiguous k block per thread, as para_range splits k over ranks */
+        int kb = (int)((long)L->n_k * tid / nth), ke = (int)((long)L->n_k * (tid + 1) / nth);
+        cplx *A = (cplx *)malloc(sizeof(cplx) * n2);
+        cplx *G = (cplx *)malloc(sizeof(cplx) * n2);
+        cplx *G0 = (cplx *)malloc(sizeof(cplx) * n2);
+        cplx *sw = (cplx *)malloc(sizeof(cplx) * m2);
+        cplx *myG = accG + (size_t)tid * L->n_freq * m2;
+        cplx *myH = accH + (size_t)tid * L->n_freq * m2;
+        cplx *myN = accN + (size_t)tid * m2;
+        cplx *myN0 = accN0 + (size_t)tid * m2;
+        for (int nw = 0; nw < nw_tot; ++nw) {
+            cplx iw = I * PI_REF * (2.0 * nw + 1.0) / beta;
+            sigma_at(L, sigma, mom, nw, sw);
+            for (int k = kb; k < ke; ++k) {
+                const cplx *H = Hk + (size_t)k * n2;
+                const cplx *Uk = U + (size_t)k * n2;
+                const double *ek = ev + (size_t)k * n;
+                /* Gkw = -H - P Sigma P^H + (iw+mu) I   (:245-257) */
+                for (size_t e = 0; e < n2; ++e) A[e] = -H[e];
+                for (int j = 0; j < nc; ++j)
+                    for (int i = 0; i < nc; ++i) A[IDX(i, j, n)] -= sw[IDX(i, j, nc)];
+                for (int i = 0; i < n; ++i) A[IDX(i, i, n)] += iw + mu;
+                orc_lu_inverse(n, A, G); /* :261 */
+                /* Gkw0 = U diag(1/(iw+mu-e)) U^H   (:247,258-262) */
+                for (int j = 0; j < n; ++j)
+                    for (int i = 0; i < n; ++i) {
+                        cplx s = 0;
+                        for (int l = 0; l < n; ++l)
+                            s += Uk[IDX(i, l, n)] * (1.0 / (-ek[l] + iw + mu)) * conj(Uk[IDX(j, l, n)]);
+                        G0[IDX(i, j, n)] = s;
+                    }
+                if (nw < L->n_freq) { /* :264-281, intra-cluster blocks */
+                    for (int cl = 0; cl < ncl; ++cl)
+                        for (int i0 = 0; i0 < npc; ++i0)
+                            for (int m0 = 0; m0 < npc; ++m0) {
+                                int p = cl * npc + i0, q = cl * npc + m0;
+                                myG[(size_t)nw * m2 + IDX(p, q, nc)] += G[IDX(p, q, n)];
+                                myH[(size_t)nw * m2 + IDX(p, q, nc)] += G0[IDX(p, q, n)];
+                            }
+                }
+                /* NumMat (:283-290) */
+                for (int l2 = 0; l2 < nc; ++l2)
+                    for (int l1 = 0; l1 < nc; ++l1) {
+                        myN[IDX(l1, l2, nc)] += G[IDX(l1, l2, n)] + conj(G[IDX(l2, l1, n)]);
+                        myN0[IDX(l1, l2, nc)] += G0[IDX(l1, l2, n)] + conj(G0[IDX(l2, l1, n)]);
+                    }
+                if (density) { /* :291-296 */
+                    cplx *D = density + (size_t)k * n2;
+                    for (int l2 = 0; l2 < n; ++l2)
+                        for (int l1 = 0; l1 < n; ++l1) {
+                            D[IDX(l1, l2, n)] += G[IDX(l1, l2, n)] + conj(G[IDX(l2, l1, n)]);
+                            D[IDX(l1, l2, n)] -= G0[IDX(l1, l2, n)] + conj(G0[IDX(l2, l1, n)]);
+                        }
+                }
+            }
+        }
+        free(A); free(G); free(G0); free(sw);
+    }
+
+    /* reduce threads (the MPI_Allreduce of :356,368,369) */
+    for (int t = 1; t < nthreads; ++t) {
+        for (size_t e = 0; e < (size_t)L->n_freq * m2; ++e) {
+            accG[e] += accG[(size_t)t * L->n_freq * m2 + e];
+            accH[e] += accH[(size_t)t * L->n_freq * m2 + e];
+        }
+        for (size_t e = 0; e < m2; ++e) {
+            accN[e] += accN[(size_t)t * m2 + e];
+            accN0[e] += accN0[(size_t)t * m2 + e];
+        }
+    }
+    /* Gw_loc /= knum_mpiGlobal (:298-305) */
+    for (size_t e = 0; e < (size_t)L->n_freq * m2; ++e) {
+        accG[e] /= (double)L->n_k_global;
+        accH[e] /= (double)L->n_k_global;
+    }
+    /* densityMatDFT finalisation (:311-319) */
+    cplx *occ = (cplx *)malloc(sizeof(cplx) * n2);
+    double *f = (double *)malloc(sizeof(double) * (size_t)n);
+    for (size_t e = 0; e < m2; ++e) { accN[e] /= beta; accN0[e] /= beta; } /* :323-327 */
+    for (int l = 0; l < nc; ++l) { /* :328-329 */
+        accN[IDX(l, l, nc)] += 0.5 * L->n_k;
+        accN0[IDX(l, l, nc)] += 0.5 * L->n_k;
+    }
+    for (int k = 0; k < L->n_k; ++k) {
+        const cplx *Uk = U + (size_t)k * n2;
+        for (int l = 0; l < n; ++l) f[l] = fermi_clamped(beta, ev[(size_t)k * n + l], mu);
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                cplx s = 0;
+                for (int l = 0; l < n; ++l) s += Uk[IDX(i, l, n)] * f[l] * conj(Uk[IDX(j, l, n)]);
+                occ[IDX(i, j, n)] = s;
+            }
+        if (density) {
+            cplx *D = density + (size_t)k * n2;
+            for (size_t e = 0; e < n2; ++e) D[e] = D[e] / beta + occ[e];
+        }
+        for (int j = 0; j < nc; ++j) /* :331-340 */
+            for (int i = 0; i < nc; ++i) accN[IDX(i, j, nc)] += occ[IDX(i, j, n)];
+    }
+    for (size_t e = 0; e < m2; ++e) { /* :347-354 */
+        accN[e] /= (double)L->n_k_global;
+        accN0[e] /= (double)L->n_k_global;
+        num_matrix[e] = accN[e] - accN0[e];
+    }
+    /* assemble block-diagonal Gw (:371-383) */
+    memset(gloc, 0, sizeof(cplx) * (size_t)L->n_freq * m2);
+    if (gloc_hf) memset(gloc_hf, 0, sizeof(cplx) * (size_t)L->n_freq * m2);
+    for (int nw = 0; nw < L->n_freq; ++nw)
+        for (int cl = 0; cl < ncl; ++cl)
+            for (int i0 = 0; i0 < npc; ++i0)
+                for (int m0 = 0; m0 < npc; ++m0) {
+                    size_t e = (size_t)nw * m2 + IDX(cl * npc + i0, cl * npc + m0, nc);
+                    gloc[e] = accG[e];
+                    if (gloc_hf) gloc_hf[e] = accH[e];
+                }
+    free(occ); free(f); free(accG); free(accH); free(accN); free(accN0); free(ev); free(U);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* GiwToNele (GreenFtn.cpp:513-635) and valenceHAMILTONIAN (:9-56).            */
+/* The reference diagonalises H(k)+Sigma_n per (k,n) and sums                  */
+/*   Re[1/(w-lambda+mu) - 1/(w-lambda_inf+mu)]  (:604-613).                     */
+/* Here sum_l 1/(z+mu-lambda_l) is evaluated as tr (z+mu-H-Sigma)^{-1}, which  */
+/* equals it for any diagonalisable H+Sigma; lambda_inf are the (real)         */
+/* eigenvalues of the Hermitian H + P m0 P^H.  Sigma and m0 enter through      */
+/* their intra-cluster blocks only (:32-39).  The core-band term (:622-631)    */
+/* is zero for a model-space-only Hamiltonian and is omitted.                  */
+/* ------------------------------------------------------------------------- */
+static void cluster_mask(const orc_lattice *L, const cplx *in, cplx *out)
+{
+    const int nc = L->n_corr, npc = nc / L->n_cluster;
+    for (int j = 0; j < nc; ++j)
+        for (int i = 0; i < nc; ++i) out[IDX(i, j, nc)] = (i / npc == j / npc) ? in[IDX(i, j, nc)] : 0.0;
+}
+
+double orc_giw_to_nele(const orc_lattice *L, const cplx *Hk, const cplx *sigma, const cplx *mom, double mu, int nthreads)
+{
+    const int n = L->n, nc = L->n_corr;
+    const size_t n2 = (size_t)n * n, m2 = (size_t)nc * nc;
+    const int nw_tot = L->n_freq * L->high_freq;
+    const double beta = L->beta;
+    if (nthreads < 1) nthreads = 1;
+    cplx *m0 = (cplx *)malloc(sizeof(cplx) * 4 * m2);
+    for (int q = 0; q < 4; ++q) cluster_mask(L, mom + q * m2, m0 + q * m2);
+    double total = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : total) num_threads(nthreads)
+    for (int k = 0; k < L->n_k; ++k) {
+        cplx *A = (cplx *)malloc(sizeof(cplx) * n2);
+        cplx *G = (cplx *)malloc(sizeof(cplx) * n2);
+        cplx *sw = (cplx *)malloc(sizeof(cplx) * m2);
+        cplx *swm = (cplx *)malloc(sizeof(cplx) * m2);
+        double *linf = (double *)malloc(sizeof(double) * (size_t)n);
+        const cplx *H = Hk + (size_t)k * n2;
+        /* lambda_inf: n = N_freq*highFreq2 branch of valenceHAMILTONIAN (:41-54) */
+        memcpy(A, H, sizeof(cplx) * n2);
+        for (int j = 0; j < nc; ++j)
+            for (int i = 0; i < nc; ++i) A[IDX(i, j, n)] += m0[IDX(i, j, nc)];
+        orc_herm_eig(n, A, linf, NULL);
+        double acc = 0.0;
+        for (int nw = 0; nw < nw_tot; ++nw) {
+            cplx w = I * PI_REF / beta + nw * (I * (2 * PI_REF) / beta); /* w0 + n dw (:605) */
+            sigma_at(L, sigma, m0, nw, sw);
+            if (nw < L->n_freq) cluster_mask(L, sw, swm); else memcpy(swm, sw, sizeof(cplx) * m2);
+            for (size_t e = 0; e < n2; ++e) A[e] = -H[e];
+            for (int j = 0; j < nc; ++j)
+                for (int i = 0; i < nc; ++i) A[IDX(i, j, n)] -= swm[IDX(i, j, nc)];
+            for (int i = 0; i < n; ++i) A[IDX(i, i, n)] += w + mu;
+            orc_lu_inverse(n, A, G);
+            cplx tr = 0;
+            for (int i = 0; i < n; ++i) tr += G[IDX(i, i, n)];
+            double s0 = 0.0;
+            for (int l = 0; l < n; ++l) s0 += creal(1.0 / (w - linf[l] + mu));
+            acc += creal(tr) - s0;
+        }
+        acc *= 2.0 / beta; /* :614 */
+        for (int l = 0; l < n; ++l) acc += 1.0 / (1.0 + exp(beta * (linf[l] - mu))); /* :616-620 */
+        total += acc;
+        free(A); free(G); free(sw); free(swm); free(linf);
+    }
+    free(m0);
+    return total / (double)L->n_k_global; /* :632 */
+}
+
+/* GreenFtn_w_adjust_mu (GreenFtn.cpp:66-139): bracket by steps of dmu, then
+ * bisect until |dmu| <= 1e-5.  Returns mu; *n_eval counts N(mu) evaluations. */
+double orc_adjust_mu(const orc_lattice *L, const cplx *Hk, const cplx *sigma, const cplx *mom, double mu, double dmu,
+                     double n_target, int nthreads, int *n_eval)
+{
+    double nele[2];
+    int evals = 0;
+    nele[0] = orc_giw_to_nele(L, Hk, sigma, mom, mu, nthreads);
+    nele[1] = -1;
+    ++evals;
+    double muUB = 0, muLB = 0, goal = n_target;
+    int nearAns = 0;
+    if (nele[0] > goal) mu -= (dmu - dmu);
+    else if (nele[0] < goal) mu -= (dmu + dmu);
+    while (nele[0] != goal && fabs(dmu) > 1e-5) {
+        mu += dmu;
+        nele[1] = nele[0];
+        nele[0] = orc_giw_to_nele(L, Hk, sigma, mom, mu, nthreads);
+        ++evals;
+        if (nearAns == 0) {
+            if (nele[0] < goal) {
+                muLB = mu;
+                if (nele[1] > goal && nele[1] > 0) nearAns = 1;
+                else dmu = fabs(dmu);
+            }
+            if (nele[0] > goal) {
+                muUB = mu;
+                if (nele[1] < goal && nele[1] > 0) nearAns = 1;
+                else dmu = -1 * fabs(dmu);
+            }
+            if (nele[0] == goal) break;
+            if (fabs(mu) >= 600) break; /* assert at :113 */
+        } else {
+            if (nele[0] > goal) muUB = mu;
+            else if (nele[0] < goal) muLB = mu;
+            double mu_next = (muUB + muLB) / 2.;
+            dmu = mu_next - mu;
+        }
+    }
+    if (n_eval) *n_eval = evals;
+    return mu;
+}
+
+/* ------------------------------------------------------------------------- */
+/* getAsymto_moments (FourierTransform.cpp:184-465).                           */
+/* fw [n_freq][d][d]; moments out [4][d][d].                                   */
+/* Returns the selected window centre per moment in win_out[4] (may be NULL).  */
+/* ------------------------------------------------------------------------- */
+int orc_asymto_moments(int d, int n_freq, double beta, const cplx *fw, int green_asym, cplx *moments, int *win_out)
+{
+    const size_t m2 = (size_t)d * d;
+    const int nmom = green_asym ? 2 : 4;
+    const int w_end = green_asym ? n_freq - 2 : n_freq - 4;
+    const int range = w_end + 1; /* w_Asymto_start = 0 */
+    /* momentstest[w][alpha,beta][mom] */
+    cplx *mt = (cplx *)calloc((size_t)range * m2 * nmom, sizeof(cplx));
+    double *trace = (double *)calloc((size_t)range * nmom, sizeof(double));
+    for (int w = 0; w < range; ++w) {
+        const int len = n_freq - w;
+        /* KM (2len x nmom) real entries; KtK = KM^H KM; KM_solve = KtK^{-1} KM^H */
+        double *KM = (double *)calloc((size_t)2 * len * nmom, sizeof(double));
+        for (int jj = 0; jj < len; ++jj) {
+            int wn = w + jj;
+            double z = (2 * wn + 1) * PI_REF / beta;
+            if (!green_asym) {
+                KM[(2 * jj) + (size_t)0 * 2 * len] = 1.0;
+                KM[(2 * jj + 1) + (size_t)1 * 2 * len] = -1. / pow(z, 1);
+                KM[(2 * jj) + (size_t)2 * 2 * len] = -1. / pow(z, 2);
+                KM[(2 * jj + 1) + (size_t)3 * 2 * len] = 1. / pow(z, 3);
+            } else {
+                KM[(2 * jj) + (size_t)0 * 2 * len] = -1. / pow(z, 2);
+                KM[(2 * jj + 1) + (size_t)1 * 2 * len] = 1. / pow(z, 3);
+            }
+        }
+        cplx KtK[16], KtKinv[16];
+        for (int b = 0; b < nmom; ++b)
+            for (int a = 0; a < nmom; ++a) {
+                double s = 0;
+                for (int r = 0; r < 2 * len; ++r) s += KM[r + (size_t)a * 2 * len] * KM[r + (size_t)b * 2 * len];
+                KtK[a + b * nmom] = s;
+            }
+        orc_lu_inverse(nmom, KtK, KtKinv);
+        /* KM_solve (nmom x 2len) */
+        cplx *KS = (cplx *)malloc(sizeof(cplx) * (size_t)nmom * 2 * len);
+        for (int r = 0; r < 2 * len; ++r)
+            for (int a = 0; a < nmom; ++a) {
+                cplx s = 0;
+                for (int b = 0; b < nmom; ++b) s += KtKinv[a + b * nmom] * KM[r + (size_t)b * 2 * len];
+                KS[a + (size_t)r * nmom] = s;
+            }
+        for (int al = 0; al < d; ++al) {
+            for (int be = 0; be < d; ++be) {
+                cplx acc[4] = {0, 0, 0, 0};
+                for (int jj = 0; jj < len; ++jj) {
+                    int wn = w + jj;
+                    double z = (2 * wn + 1) * PI_REF / beta;
+                    cplx fab = fw[(size_t)wn * m2 + IDX(al, be, d)];
+                    cplx fba = fw[(size_t)wn * m2 + IDX(be, al, d)];
+                    cplx re = (fab + conj(fba)) / 2.0;
+                    cplx im = (fab - conj(fba)) / (2.0 * I);
+                    if (green_asym && al == be) im += 1.0 / z; /* :373 */
+                    for (int a = 0; a < nmom; ++a)
+                        acc[a] += KS[a + (size_t)(2 * jj) * nmom] * re + KS[a + (size_t)(2 * jj + 1) * nmom] * im;
+                }
+                for (int a = 0; a < nmom; ++a) mt[((size_t)w * m2 + IDX(al, be, d)) * nmom + a] = acc[a];
+            }
+            for (int a = 0; a < nmom; ++a) trace[(size_t)w * nmom + a] += creal(mt[((size_t)w * m2 + IDX(al, al, d)) * nmom + a]);
+        }
+        free(KS);
+        free(KM);
+    }
+    int Nv = n_freq / 20;
+    if (Nv == 0) Nv = 1;
+    const int ncent = range - 2 * Nv;
+    if (ncent <= 0) { free(mt); free(trace); return 1; }
+    /* window averages of the moment estimates (:255-283 / :391-409) */
+    cplx *avg = (cplx *)calloc((size_t)ncent * nmom * m2, sizeof(cplx));
+    for (int c = Nv; c < range - Nv; ++c)
+        for (int iw = c - Nv; iw <= c + Nv; ++iw)
+            for (size_t e = 0; e < m2; ++e)
+                for (int a = 0; a < nmom; ++a) avg[((size_t)(c - Nv) * nmom + a) * m2 + e] += mt[((size_t)iw * m2 + e) * nmom + a];
+    for (size_t e = 0; e < (size_t)ncent * nmom * m2; ++e) avg[e] /= (double)(2 * Nv + 1);
+    /* positivity tests per window (independent of the moment index) */
+    int *ok = (int *)malloc(sizeof(int) * (size_t)ncent);
+    cplx *t1 = (cplx *)malloc(sizeof(cplx) * m2), *t2 = (cplx *)malloc(sizeof(cplx) * m2), *t3 = (cplx *)malloc(sizeof(cplx) * m2);
+    cplx *sq = (cplx *)malloc(sizeof(cplx) * m2), *vinv = (cplx *)malloc(sizeof(cplx) * m2);
+    for (int i = 0; i < ncent; ++i) {
+        const cplx *a0 = avg + ((size_t)i * nmom + 0) * m2;
+        const cplx *a1 = avg + ((size_t)i * nmom + 1) * m2;
+        if (!green_asym) {
+            /* :298-313 */
+            double a = herm_min_eig(d, a1);
+            ok[i] = 0;
+            if (a > 0) {
+                const cplx *a2 = avg + ((size_t)i * nmom + 2) * m2;
+                const cplx *a3 = avg + ((size_t)i * nmom + 3) * m2;
+                herm_sqrt(d, a1, sq);
+                orc_lu_inverse(d, sq, vinv);
+                /* V_inv*m3*V_inv - (V_inv*m2*V_inv)^2 */
+                matmul(d, vinv, a3, t1); matmul(d, t1, vinv, t3);  /* t3 = Vi m3 Vi */
+                matmul(d, vinv, a2, t1); matmul(d, t1, vinv, t2);  /* t2 = Vi m2 Vi */
+                matmul(d, t2, t2, t1);
+                for (size_t e = 0; e < m2; ++e) t3[e] -= t1[e];
+                double b = herm_min_eig(d, t3);
+                ok[i] = (b > 0);
+            }
+        } else {
+            /* :423-432: a = min eig(m1avg - m0avg^2) > 1 */
+            matmul(d, a0, a0, t1);
+            for (size_t e = 0; e < m2; ++e) t2[e] = a1[e] - t1[e];
+            ok[i] = herm_min_eig(d, t2) > 1;
+        }
+    }
+    for (int mom = 0; mom < nmom; ++mom) {
+        double *var = (double *)malloc(sizeof(double) * (size_t)ncent);
+        for (int c = Nv; c < range - Nv; ++c) {
+            double mean = 0, ss = 0;
+            for (int j = c - Nv; j <= c + Nv; ++j) mean += trace[(size_t)j * nmom + mom];
+            mean /= (2 * Nv + 1);
+            for (int j = c - Nv; j <= c + Nv; ++j) {
+                double dv = trace[(size_t)j * nmom + mom] - mean;
+                ss += dv * dv;
+            }
+            var[c - Nv] = sqrt(ss / (2 * Nv + 1));
+        }
+        int minPos = ncent - 1;
+        for (int i = 0; i < ncent; ++i)
+            if (ok[i] && var[i] < var[minPos]) minPos = i;
+        int cen = minPos + Nv;
+        int target = green_asym ? mom + 2 : mom;
+        cplx *out = moments + (size_t)target * m2;
+        memset(out, 0, sizeof(cplx) * m2);
+        for (int w = cen - Nv; w <= cen + Nv; ++w)
+            for (int al = 0; al < d; ++al)
+                for (int be = 0; be < d; ++be)
+                    out[IDX(al, be, d)] += (mt[((size_t)w * m2 + IDX(al, be, d)) * nmom + mom] +
+                                            conj(mt[((size_t)w * m2 + IDX(be, al, d)) * nmom + mom])) / 2.0;
+        for (size_t e = 0; e < m2; ++e) out[e] /= (double)(2 * Nv + 1);
+        if (win_out) win_out[target] = cen;
+        free(var);
+    }
+    if (green_asym) {
+        memset(moments, 0, sizeof(cplx) * m2);
+        for (size_t e = 0; e < m2; ++e) moments[m2 + e] = 0;
+        for (int i = 0; i < d; ++i) moments[m2 + IDX(i, i, d)] = 1.0;
+        if (win_out) { win_out[0] = -1; win_out[1] = -1; }
+    }
+    for (int q = 0; q < 4; ++q) { /* :458-461 */
+        cplx *m = moments + (size_t)q * m2;
+        adjoint(d, m, t1);
+        for (size_t e = 0; e < m2; ++e) m[e] = (m[e] + t1[e]) / 2.0;
+    }
+    free(ok); free(t1); free(t2); free(t3); free(sq); free(vinv); free(avg); free(mt); free(trace);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* FourierTransform_tau (FourierTransform.cpp:16-118): one tau point.          */
+/* ------------------------------------------------------------------------- */
+int orc_ft_tau(int d, int n_freq, double beta, const cplx *fw, const cplx *mom, double tau, cplx *out)
+{
+    const size_t m2 = (size_t)d * d;
+    cplx *V = (cplx *)calloc(m2, sizeof(cplx)), *H1 = (cplx *)calloc(m2, sizeof(cplx)), *H2 = (cplx *)calloc(m2, sizeof(cplx));
+    cplx *t1 = (cplx *)malloc(sizeof(cplx) * m2), *t2 = (cplx *)malloc(sizeof(cplx) * m2), *t3 = (cplx *)malloc(sizeof(cplx) * m2);
+    cplx *Vi = (cplx *)malloc(sizeof(cplx) * m2), *ViH = (cplx *)malloc(sizeof(cplx) * m2);
+    cplx *M2 = (cplx *)malloc(sizeof(cplx) * m2), *M3 = (cplx *)malloc(sizeof(cplx) * m2);
+    cplx *acc = (cplx *)calloc(m2, sizeof(cplx));
+    const cplx *m1 = mom + m2, *mm2 = mom + 2 * m2, *mm3 = mom + 3 * m2;
+    double a = herm_min_eig(d, m1);
+    if (a > 0) { /* :28-39 */
+        herm_sqrt(d, m1, V);
+        orc_lu_inverse(d, V, Vi);
+        adjoint(d, V, t1);
+        orc_lu_inverse(d, t1, ViH); /* V^H^{-1} */
+        for (size_t e = 0; e < m2; ++e) t2[e] = 2.0 * mm2[e];
+        matmul(d, ViH, t2, t1); matmul(d, t1, Vi, M2);
+        for (size_t e = 0; e < m2; ++e) t2[e] = 2.0 * mm3[e];
+        matmul(d, ViH, t2, t1); matmul(d, t1, Vi, M3);
+        matmul(d, M2, M2, t1);
+        for (size_t e = 0; e < m2; ++e) t2[e] = M3[e] / 2.0 - t1[e] / 4.0;
+        if (herm_min_eig(d, t2) > 0) {
+            herm_sqrt(d, t2, t3);
+            for (size_t e = 0; e < m2; ++e) H1[e] = M2[e] / 2.0 + t3[e];
+        } else {
+            for (size_t e = 0; e < m2; ++e) H1[e] = M2[e] / 2.0;
+        }
+        for (size_t e = 0; e < m2; ++e) H2[e] = M2[e] - H1[e];
+    }
+    const double dwl = (2 * PI_REF) / beta, w0l = (PI_REF / beta);
+    cplx *VH = (cplx *)malloc(sizeof(cplx) * m2);
+    adjoint(d, V, VH);
+    cplx *R1 = (cplx *)malloc(sizeof(cplx) * m2), *R2 = (cplx *)malloc(sizeof(cplx) * m2), *tail = (cplx *)malloc(sizeof(cplx) * m2);
+    for (int i = 0; i < n_freq; ++i) { /* :55-61 */
+        double w = w0l + i * dwl;
+        for (size_t e = 0; e < m2; ++e) { t1[e] = -H1[e]; t2[e] = -H2[e]; }
+        for (int l = 0; l < d; ++l) { t1[IDX(l, l, d)] += I * w; t2[IDX(l, l, d)] += I * w; }
+        orc_lu_inverse(d, t1, R1);
+        orc_lu_inverse(d, t2, R2);
+        matmul(d, VH, R1, t1); matmul(d, t1, V, t3);
+        matmul(d, VH, R2, t1); matmul(d, t1, V, tail);
+        for (size_t e = 0; e < m2; ++e) tail[e] = (t3[e] + tail[e]) / 2.0;
+        cplx ph = cexp(-I * w * tau);
+        for (size_t e = 0; e < m2; ++e) acc[e] += (fw[(size_t)i * m2 + e] - tail[e]) * ph;
+    }
+    adjoint(d, acc, t1); /* :62-63 */
+    for (size_t e = 0; e < m2; ++e) acc[e] = (acc[e] + t1[e]) / beta;
+    /* analytic tail: G0H = -U diag(1/(e^{e tau} + e^{e (tau-beta)})) U^H (:66-87) */
+    double *ew = (double *)malloc(sizeof(double) * (size_t)d);
+    cplx *Ue = (cplx *)malloc(sizeof(cplx) * m2);
+    for (int h = 0; h < 2; ++h) {
+        orc_herm_eig(d, h == 0 ? H1 : H2, ew, Ue);
+        for (int j = 0; j < d; ++j)
+            for (int i = 0; i < d; ++i) {
+                cplx s = 0;
+                for (int l = 0; l < d; ++l)
+                    s += Ue[IDX(i, l, d)] * (-1.0 / (exp(ew[l] * tau) + exp(ew[l] * (tau - beta)))) * conj(Ue[IDX(j, l, d)]);
+                t3[IDX(i, j, d)] = s;
+            }
+        matmul(d, VH, t3, t1); matmul(d, t1, V, t2);
+        for (size_t e = 0; e < m2; ++e) acc[e] += t2[e] / 2.0;
+    }
+    adjoint(d, acc, t1); /* :89-90 */
+    for (size_t e = 0; e < m2; ++e) out[e] = (acc[e] + t1[e]) / 2.0;
+    free(ew); free(Ue); free(VH); free(R1); free(R2); free(tail);
+    free(V); free(H1); free(H2); free(t1); free(t2); free(t3); free(Vi); free(ViH); free(M2); free(M3); free(acc);
+    return 0;
+}
+
+/* FourierTransform_ (FourierTransform.cpp:126-178): all tau_j = j beta/N_tau,
+ * j = 0..N_tau, followed by the causality patch.  *patched = CHECK. */
+int orc_ft_w2tau(int d, int n_freq, int n_tau, double beta, const cplx *fw, const cplx *mom, cplx *gt, int *patched,
+                 int nthreads)
+{
+    const size_t m2 = (size_t)d * d;
+    if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads)
+    for (int j = 0; j <= n_tau; ++j) orc_ft_tau(d, n_freq, beta, fw, mom, j * beta / n_tau, gt + (size_t)j * m2);
+    int check = 0;
+    double mine11 = 0;
+    for (int t = 0; t <= n_tau; ++t) {
+        double tau = t * beta / n_tau;
+        double mine = herm_min_eig(d, gt + (size_t)t * m2);
+        mine11 = mine;
+        if (mine >= 0 && (t != 0 && t != n_tau)) {
+            check = 1;
+            for (int t1 = t + 1; t1 < n_tau + 1; ++t1) {
+                double tau1 = t1 * beta / n_tau, taup = (t - 1) * beta / n_tau;
+                for (size_t e = 0; e < m2; ++e)
+                    gt[(size_t)t * m2 + e] = (gt[(size_t)t1 * m2 + e] - gt[(size_t)(t - 1) * m2 + e]) / (tau1 - taup) * (tau - taup) +
+                                            gt[(size_t)(t - 1) * m2 + e];
+                mine11 = herm_min_eig(d, gt + (size_t)t * m2);
+                if (mine11 < 0) break;
+            }
+        }
+        if (mine11 >= 0) {
+            check = 1;
+            for (size_t e = 0; e < m2; ++e) gt[(size_t)t * m2 + e] = 0;
+            for (int l = 0; l < d; ++l) gt[(size_t)t * m2 + IDX(l, l, d)] = -1e-10;
+        }
+    }
+    if (patched) *patched = check;
+    return 0;
+}
+
+/* FT_t_to_w (FourierTransform.cpp:521-561): exact integral of the piecewise
+ * linear interpolant of G(tau).  violations counts frequencies whose
+ * anti-Hermitian part has a positive eigenvalue (the assert at :555-558). */
+int orc_ft_tau2w(int d, int n_freq, int n_tau, double beta, const cplx *gt, cplx *fw, int *violations)
+{
+    const size_t m2 = (size_t)d * d;
+    const int nseg = n_tau;
+    const double dt = beta / nseg;
+    cplx *at = (cplx *)malloc(sizeof(cplx) * (size_t)nseg * m2), *bt = (cplx *)malloc(sizeof(cplx) * (size_t)nseg * m2);
+    for (int t = 0; t < nseg; ++t) {
+        double tau0 = t * dt, tau1 = (t + 1) * dt;
+        for (size_t e = 0; e < m2; ++e) {
+            at[(size_t)t * m2 + e] = (gt[(size_t)(t + 1) * m2 + e] - gt[(size_t)t * m2 + e]) / (tau1 - tau0);
+            bt[(size_t)t * m2 + e] = -at[(size_t)t * m2 + e] * tau0 + gt[(size_t)t * m2 + e];
+        }
+    }
+    int viol = 0;
+    cplx *test = (cplx *)malloc(sizeof(cplx) * m2);
+    for (int n = 0; n < n_freq; ++n) {
+        double wn = (2 * n + 1) * PI_REF / beta;
+        cplx *out = fw + (size_t)n * m2;
+        memset(out, 0, sizeof(cplx) * m2);
+        for (int t = 0; t < nseg; ++t) {
+            double tau0 = t * dt, tau1 = (t + 1) * dt;
+            cplx pI = (cexp(I * tau1 * wn) - cexp(I * tau0 * wn));
+            cplx pI2 = (tau1 * cexp(I * tau1 * wn) - tau0 * cexp(I * tau0 * wn));
+            cplx ca = pI / (wn * wn) + pI2 / (I * wn);
+            cplx cb = pI / (I * wn);
+            for (size_t e = 0; e < m2; ++e) out[e] += at[(size_t)t * m2 + e] * ca + bt[(size_t)t * m2 + e] * cb;
+        }
+        for (int j = 0; j < d; ++j)
+            for (int i = 0; i < d; ++i) test[IDX(i, j, d)] = (out[IDX(i, j, d)] - conj(out[IDX(j, i, d)])) / (2.0 * I);
+        if (herm_max_eig(d, test) > 0) ++viol;
+    }
+    if (violations) *violations = viol;
+    free(at); free(bt); free(test);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Density functional used by the mu-search and the adjoint:                   */
+/*   N(mu)   = (1/Nk) sum_k [ (1/beta) sum_{n<2Nw} 2 Re tr(G - G0)            */
+/*                            + sum_l f(eps_kl - mu) ]                         */
+/*   dN/dmu  = (1/Nk) sum_k [ (1/beta) sum_n 2 Re tr(-G^2 + G0^2)              */
+/*                            + beta sum_l f(1-f) ]                            */
+/* with G built from the full nc x nc Sigma and G0 from eig(H + P m0 P^H), the  */
+/* trace of the densityMatDFT of GreenFtn.cpp:291-319 (Fermi unclamped).       */
+/* ------------------------------------------------------------------------- */
+int orc_density(const orc_lattice *L, const cplx *Hk, const cplx *sigma, const cplx *mom, double mu, double *n_out,
+                double *dn_out, int nthreads)
+{
+    const int n = L->n, nc = L->n_corr;
+    const size_t n2 = (size_t)n * n, m2 = (size_t)nc * nc;
+    const int nw_tot = L->n_freq * L->high_freq;
+    const double beta = L->beta;
+    if (nthreads < 1) nthreads = 1;
+    double *ev = (double *)malloc(sizeof(double) * (size_t)L->n_k * n);
+    cplx *U = (cplx *)malloc(sizeof(cplx) * (size_t)L->n_k * n2);
+    static_eig(L, Hk, mom, ev, U, nthreads);
+    double Nt = 0, dN = 0;
+#pragma omp parallel for schedule(static) reduction(+ : Nt, dN) num_threads(nthreads)
+    for (int k = 0; k < L->n_k; ++k) {
+        cplx *A = (cplx *)malloc(sizeof(cplx) * n2), *G = (cplx *)malloc(sizeof(cplx) * n2);
+        cplx *sw = (cplx *)malloc(sizeof(cplx) * m2);
+        const cplx *H = Hk + (size_t)k * n2;
+        const double *ek = ev + (size_t)k * n;
+        double sN = 0, sD = 0;
+        for (int nw = 0; nw < nw_tot; ++nw) {
+            cplx iw = I * PI_REF * (2.0 * nw + 1.0) / beta;
+            sigma_at(L, sigma, mom, nw, sw);
+            for (size_t e = 0; e < n2; ++e) A[e] = -H[e];
+            for (int j = 0; j < nc; ++j)
+                for (int i = 0; i < nc; ++i) A[IDX(i, j, n)] -= sw[IDX(i, j, nc)];
+            for (int i = 0; i < n; ++i) A[IDX(i, i, n)] += iw + mu;
+            orc_lu_inverse(n, A, G);
+            cplx tr = 0, tr2 = 0;
+            for (int i = 0; i < n; ++i) {
+                tr += G[IDX(i, i, n)];
+                for (int j = 0; j < n; ++j) tr2 += G[IDX(i, j, n)] * G[IDX(j, i, n)];
+            }
+            cplx t0 = 0, t02 = 0;
+            for (int l = 0; l < n; ++l) {
+                cplx g0 = 1.0 / (iw + mu - ek[l]);
+                t0 += g0;
+                t02 += g0 * g0;
+            }
+            sN += 2.0 * creal(tr - t0);
+            sD += 2.0 * creal(-tr2 + t02);
+        }
+        sN /= beta;
+        sD /= beta;
+        for (int l = 0; l < n; ++l) {
+            double x = beta * (ek[l] - mu);
+            double f = 1.0 / (1.0 + exp(x));
+            sN += f;
+            sD += beta * f * (1.0 - f);
+        }
+        Nt += sN;
+        dN += sD;
+        free(A); free(G); free(sw);
+    }
+    free(ev); free(U);
+    if (n_out) *n_out = Nt / L->n_k_global;
+    if (dn_out) *dn_out = dN / L->n_k_global;
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* FP64 reference vector-Jacobian product of (G_loc, N) with respect to       */
+/* (Sigma(iw_n), n < N_freq) and mu, moments held fixed.                       */
+/*   Gloc_n = (1/Nk) sum_k G_cc(k,n)   (full nc x nc, no cluster masking)       */
+/*   cotangents: gbar [n_freq][nc][nc] (dL = Re sum conj(gbar) dGloc), nbar.    */
+/*   sigma_bar_n = (1/Nk) sum_k [ G_cc^H gbar_n G_cc^H + (2 nbar/beta) ((G^2)_cc)^H ] */
+/*   mu_bar      = -(1/Nk) Re sum_{n<Nw} sum_k tr(gbar_n^H (G^2)_cc) + nbar dN/dmu */
+/* Gradients use the convention g = dL/dRe + i dL/dIm.                          */
+/* ------------------------------------------------------------------------- */
+int orc_gloc_vjp(const orc_lattice *L, const cplx *Hk, const cplx *sigma, const cplx *mom, double mu, const cplx *gbar,
+                 double nbar, cplx *sigma_bar, double *mu_bar, int nthreads)
+{
+    const int n = L->n, nc = L->n_corr;
+    const size_t n2 = (size_t)n * n, m2 = (size_t)nc * nc;
+    const double beta = L->beta;
+    if (nthreads < 1) nthreads = 1;
+    double dNdmu = 0;
+    orc_density(L, Hk, sigma, mom, mu, NULL, &dNdmu, nthreads);
+    memset(sigma_bar, 0, sizeof(cplx) * (size_t)L->n_freq * m2);
+    double mub = 0;
+#pragma omp parallel num_threads(nthreads)
+    {
+        cplx *A = (cplx *)malloc(sizeof(cplx) * n2), *G = (cplx *)malloc(sizeof(cplx) * n2);
+        cplx *sw = (cplx *)malloc(sizeof(cplx) * m2), *G2 = (cplx *)malloc(sizeof(cplx) * m2);
+        cplx *Gc = (cplx *)malloc(sizeof(cplx) * m2), *GcH = (cplx *)malloc(sizeof(cplx) * m2);
+        cplx *t1 = (cplx *)malloc(sizeof(cplx) * m2), *t2 = (cplx *)malloc(sizeof(cplx) * m2);
+        cplx *loc = (cplx *)calloc((size_t)L->n_freq * m2, sizeof(cplx));
+        double mloc = 0;
+#pragma omp for schedule(static)
+        for (int k = 0; k < L->n_k; ++k) {
+            const cplx *H = Hk + (size_t)k * n2;
+            for (int nw = 0; nw < L->n_freq; ++nw) {
+                cplx iw = I * PI_REF * (2.0 * nw + 1.0) / beta;
+                sigma_at(L, sigma, mom, nw, sw);
+                for (size_t e = 0; e < n2; ++e) A[e] = -H[e];
+                for (int j = 0; j < nc; ++j)
+                    for (int i = 0; i < nc; ++i) A[IDX(i, j, n)] -= sw[IDX(i, j, nc)];
+                for (int i = 0; i < n; ++i) A[IDX(i, i, n)] += iw + mu;
+                orc_lu_inverse(n, A, G);
+                for (int j = 0; j < nc; ++j)
+                    for (int i = 0; i < nc; ++i) {
+                        Gc[IDX(i, j, nc)] = G[IDX(i, j, n)];
+                        cplx s = 0;
+                        for (int m = 0; m < n; ++m) s += G[IDX(i, m, n)] * G[IDX(m, j, n)];
+                        G2[IDX(i, j, nc)] = s;
+                    }
+                adjoint(nc, Gc, GcH);
+                const cplx *gb = gbar + (size_t)nw * m2;
+                matmul(nc, GcH, gb, t1);
+                matmul(nc, t1, GcH, t2);
+                for (int j = 0; j < nc; ++j)
+                    for (int i = 0; i < nc; ++i)
+                        loc[(size_t)nw * m2 + IDX(i, j, nc)] += t2[IDX(i, j, nc)] + (2.0 * nbar / beta) * conj(G2[IDX(j, i, nc)]);
+                cplx tr = 0;
+                for (int j = 0; j < nc; ++j)
+                    for (int i = 0; i < nc; ++i) tr += conj(gb[IDX(i, j, nc)]) * G2[IDX(i, j, nc)];
+                mloc -= creal(tr);
+            }
+        }
+#pragma omp critical
+        {
+            for (size_t e = 0; e < (size_t)L->n_freq * m2; ++e) sigma_bar[e] += loc[e];
+            mub += mloc;
+        }
+        free(A); free(G); free(sw); free(G2); free(Gc); free(GcH); free(t1); free(t2); free(loc);
+    }
+    for (size_t e = 0; e < (size_t)L->n_freq * m2; ++e) sigma_bar[e] /= (double)L->n_k_global;
+    if (mu_bar) *mu_bar = mub / L->n_k_global + nbar * dNdmu;
+    return 0;
+}
+
+/* Tiny utility for the CPU-baseline timing: the GreenFtn_w loop over a k subset. */
+int orc_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
