@@ -155,9 +155,11 @@ extern "C" int dmftgpu_prepare(dmftgpu_handle h, const dmftgpu_lattice* L, const
     HIPCHK(dg::launch_herm_eig(h->Hk, n, 0, n, h->mom, nc, h->ev, h->U, nk, h->stream));
     // Schur data: eig(H_dd(k)) and W = H_cd V_d
     h->schur_ready = false;
-    if (nd > 0 && dg::schur_supported(n, nc)) {
-        HIPCHK(dg::launch_herm_eig(h->Hk, n, nc, nd, nullptr, 0, h->ed, h->Vd, nk, h->stream));
-        HIPCHK(dg::launch_schur_prep(h->Hk, h->Vd, n, nc, nk, h->W, h->stream));
+    if (dg::schur_supported(n, nc)) {
+        if (nd > 0) {
+            HIPCHK(dg::launch_herm_eig(h->Hk, n, nc, nd, nullptr, 0, h->ed, h->Vd, nk, h->stream));
+            HIPCHK(dg::launch_schur_prep(h->Hk, h->Vd, n, nc, nk, h->W, h->stream));
+        }
         h->schur_ready = true;
     }
     h->prepared = true;
@@ -275,6 +277,7 @@ int run_lattice(dmftgpu_handle h, int method, int mode, const double* sigma, dou
         a.pSb = (cd*)take(C * L.n_freq * m2 * sizeof(cd));
         a.pMu = (cd*)take(C * L.n_freq * sizeof(cd));
     }
+    if (method != DMFTGPU_PLAIN_F64) a.pSigAll = (cd*)take((size_t)nw_tot * m2 * sizeof(cd));
     if (method == DMFTGPU_PLAIN_F64) {
         HIPCHK(dg::launch_lattice_plain(a, mode, h->stream, nullptr));
     } else {
@@ -306,7 +309,7 @@ size_t lattice_ws_bytes(dmftgpu_handle h, int method, int mode)
     int slots = method == DMFTGPU_PLAIN_F64 ? dg::plain_slots(L.n, mode) : dg::schur_slots(nc, mode);
     Plan P = make_plan(L.n_k, (int)nw_tot, slots, h->target_wg);
     size_t C = P.n_chunks + 1;  // +1 for the reduced copy
-    size_t b = C * nw_tot * 16 * (2 + (size_t)m2) + C * L.n_freq * 16 * (m2 + 1) + 64 * 256;
+    size_t b = (C + 1) * nw_tot * 16 * (2 + (size_t)m2) + C * L.n_freq * 16 * (m2 + 1) + 64 * 256;
     return b;
 }
 
@@ -320,7 +323,7 @@ extern "C" int dmftgpu_greenftn_w(dmftgpu_handle h, int method, const double* si
     if (!h->prepared) return fail(h, DMFTGPU_ERR_STATE, "dmftgpu_prepare has not been called");
     if (!valid_method(method)) return fail(h, DMFTGPU_ERR_ARG, "unknown method %d", method);
     if (method != DMFTGPU_PLAIN_F64 && !h->schur_ready)
-        return fail(h, DMFTGPU_ERR_UNSUPPORTED, "Schur method needs 0 < n_corr < n (n=%d, n_corr=%d)", h->lat.n, h->lat.n_corr);
+        return fail(h, DMFTGPU_ERR_UNSUPPORTED, "Schur method needs n_corr <= 6 (n=%d, n_corr=%d)", h->lat.n, h->lat.n_corr);
     if (!sigma || !gloc || !num_matrix) return fail(h, DMFTGPU_ERR_ARG, "sigma, gloc and num_matrix are required");
     if (!std::isfinite(mu)) return fail(h, DMFTGPU_ERR_ARG, "mu is not finite");
     const dmftgpu_lattice& L = h->lat;
@@ -409,7 +412,7 @@ extern "C" int dmftgpu_density(dmftgpu_handle h, int method, const double* sigma
     if (!h->prepared) return fail(h, DMFTGPU_ERR_STATE, "dmftgpu_prepare has not been called");
     if (!valid_method(method)) return fail(h, DMFTGPU_ERR_ARG, "unknown method %d", method);
     if (method != DMFTGPU_PLAIN_F64 && !h->schur_ready)
-        return fail(h, DMFTGPU_ERR_UNSUPPORTED, "Schur method needs 0 < n_corr < n");
+        return fail(h, DMFTGPU_ERR_UNSUPPORTED, "Schur method needs n_corr <= 6");
     if (!sigma) return fail(h, DMFTGPU_ERR_ARG, "sigma is null");
     if (!std::isfinite(mu)) return fail(h, DMFTGPU_ERR_ARG, "mu is not finite");
     return density_impl(h, method, sigma, mu, n_out, dn_dmu_out);
@@ -464,7 +467,7 @@ extern "C" int dmftgpu_gloc_vjp(dmftgpu_handle h, int method, const double* sigm
     if (!h->prepared) return fail(h, DMFTGPU_ERR_STATE, "dmftgpu_prepare has not been called");
     if (!valid_method(method)) return fail(h, DMFTGPU_ERR_ARG, "unknown method %d", method);
     if (method != DMFTGPU_PLAIN_F64 && !h->schur_ready)
-        return fail(h, DMFTGPU_ERR_UNSUPPORTED, "Schur method needs 0 < n_corr < n");
+        return fail(h, DMFTGPU_ERR_UNSUPPORTED, "Schur method needs n_corr <= 6");
     if (!sigma || !gloc_bar || !sigma_bar) return fail(h, DMFTGPU_ERR_ARG, "sigma, gloc_bar and sigma_bar are required");
     const dmftgpu_lattice& L = h->lat;
     const int nc = L.n_corr, m2 = nc * nc;

@@ -29,6 +29,7 @@ struct LatArgs {
     const double* ed;  // [nk][nd]      eigenvalues of H_dd(k)
     const cd* W;       // [nk][nc][nd]  H_cd(k) V_d(k), column-major nc x nd
     int refine;        // FP32R: number of FP64-residual refinement steps
+    cd* pSigAll;       // Schur: Sigma expanded to all nw_tot frequencies [nw_tot][nc*nc]
 };
 
 // plain FP64 full inverse; returns 0 or a HIP error

@@ -1,9 +1,453 @@
-// Schur-complement lattice path (placeholder until the kernel lands).
+// Schur-complement lattice path: G_cc without the full n x n inverse.
+//
+// Same outputs as the plain path (the hot loop of GreenFtn_w,
+// GreenFtn.cpp:218-308, and the densities of GiwToNele :513-635), computed
+// from the block structure of A = (z+mu) I - H(k) - P Sigma P^H with the
+// correlated orbitals first (downfolding.cpp:69-79):
+//   H_dd(k) = V_d diag(e_d) V_d^H           (once per k, in dmftgpu_prepare)
+//   W(k)    = H_cd V_d,  P_d = w_d w_d^H    (w_d = column d of W)
+//   g_d     = 1/(z + mu - e_d)
+//   S       = (z+mu) I - H_cc - Sigma - sum_d g_d P_d        (Schur complement)
+//   G_cc    = S^{-1}
+//   tr G    = sum_d g_d + tr G_cc + sum_d g_d^2 M_d,   M_d = w_d^H G_cc w_d
+//   tr G^2  = sum_d g_d^2 + tr((G_cc S')^2) + 2 sum_d g_d^3 M_d,
+//             S' = I + X, X = sum_d g_d^2 P_d
+//   (G^2)_cc = G_cc S' G_cc
+// (tr G = d/dz log det A = d/dz[log det A_dd + log det S]; tr G^2 = -d/dz tr G.)
+// Per (k, iw) this costs ~ n_c^2 n_d + n_c^3 complex operations instead of n^3.
+//
+// Mapping: one (k, iw) matrix per lane, everything in registers (n_c <= 6):
+// a workgroup of one wavefront owns 64 consecutive frequencies and walks a chunk
+// of k points; the per-k data (H_cc, e_d, P_d for pairs a <= b) is staged in
+// LDS, double buffered, and read as wave-wide broadcasts.  The n_c x n_c
+// inversion is Gauss-Jordan without pivoting (the Hermitian part of -iS is
+// >= w_n I for causal Sigma, so elimination without pivoting is stable).
+//
+// FP32R: the Schur complement S is formed in FP64, inverted in FP32, and the
+// FP32 inverse X0 preconditions FP64 iterative refinement
+//   X <- X + X0 (I - S X),
+// one column at a time, `refine` times.
 #include "lattice_kernels.hpp"
 
 namespace dg {
-bool schur_supported(int, int) { return false; }
+
+template <int NC>
+struct Mat {
+    cd v[NC][NC];  // v[i][j] = element (i, j)
+};
+
+template <int NC>
+__device__ __forceinline__ void gj_inverse(cd (&S)[NC][NC])
+{
+#pragma unroll
+    for (int j = 0; j < NC; ++j) {
+        const cd p = crcp(S[j][j]);
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+            if (c != j) S[j][c] = cmul(S[j][c], p);
+#pragma unroll
+        for (int i = 0; i < NC; ++i) {
+            if (i == j) continue;
+            const cd f = S[i][j];
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+                if (c != j) S[i][c] = cfms(f, S[j][c], S[i][c]);
+            S[i][j] = cneg(cmul(f, p));
+        }
+        S[j][j] = p;
+    }
+}
+
+template <int NC>
+__device__ __forceinline__ void gj_inverse_f32(cf (&S)[NC][NC])
+{
+#pragma unroll
+    for (int j = 0; j < NC; ++j) {
+        const cf p = frcp(S[j][j]);
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+            if (c != j) S[j][c] = fmul(S[j][c], p);
+#pragma unroll
+        for (int i = 0; i < NC; ++i) {
+            if (i == j) continue;
+            const cf f = S[i][j];
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+                if (c != j) S[i][c] = ffms(f, S[j][c], S[i][c]);
+            const cf fp = fmul(f, p);
+            S[i][j] = cf{-fp.x, -fp.y};
+        }
+        S[j][j] = p;
+    }
+}
+
+// staging buffer of one k: [Hcc: NC*NC cd][P: nd*NP cd][e_d: nd doubles]
+template <int NC>
+__host__ __device__ __forceinline__ int stage_size(int nd)
+{
+    constexpr int NP = NC * (NC + 1) / 2;
+    return NC * NC + nd * NP + (nd + 1) / 2;
+}
+
+template <int NC, int WG>
+__device__ __forceinline__ void stage_k(const LatArgs& a, int k, cd* buf, int nd)
+{
+    constexpr int NP = NC * (NC + 1) / 2;
+    const int n = a.n;
+    const cd* Hk = a.Hk + (size_t)k * n * n;
+    const cd* W = a.W + (size_t)k * NC * nd;
+    const double* ed = a.ed + (size_t)k * nd;
+    for (int t = threadIdx.x; t < NC * NC; t += WG) buf[t] = Hk[(t % NC) + (t / NC) * n];
+    cd* P = buf + NC * NC;
+    for (int t = threadIdx.x; t < nd * NP; t += WG) {
+        const int d = t / NP, p = t % NP;
+        // p -> (ia, ib), ib-major: p = ib(ib+1)/2 + ia
+        int ib = 0;
+        while ((ib + 1) * (ib + 2) / 2 <= p) ++ib;
+        const int ia = p - ib * (ib + 1) / 2;
+        P[t] = cmulc(W[ia + d * NC], W[ib + d * NC]);  // W_ad conj(W_bd)
+    }
+    double* e = (double*)(P + nd * NP);
+    for (int t = threadIdx.x; t < nd; t += WG) e[t] = ed[t];
+}
+
+template <int NC, int MODE, bool F32R, int WG>
+__global__ __launch_bounds__(WG) void k_lattice_schur(LatArgs a, const cd* __restrict__ sig_all)
+{
+    constexpr int NP = NC * (NC + 1) / 2;
+    constexpr int M2 = NC * NC;
+    // accumulators that do not fit next to the working matrices live in LDS
+    constexpr bool ACC_LDS = (MODE == MODE_VJP) || (MODE == MODE_FWD && F32R);
+    extern __shared__ cd lds[];
+    const int nd = a.n - NC;
+    const int bs = stage_size<NC>(nd);
+    cd* sbuf = lds;           // [2][bs]
+    cd* sAcc = lds + 2 * bs;  // [M2][WG], lane-contiguous
+    const int lane = threadIdx.x;
+    const int nw = blockIdx.x * WG + lane;
+    const bool wvalid = nw < a.nw_tot;
+    const int nwc = wvalid ? nw : a.nw_tot - 1;  // clamp for loads
+    const int k0 = blockIdx.y * a.k_per_chunk;
+    const int k1 = min(a.nk, k0 + a.k_per_chunk);
+    const double w = matsubara(nwc, a.beta);
+    const double mu = a.mu;
+    const bool vjp_here = (MODE == MODE_VJP) && wvalid && nw < a.n_freq;
+    const cd* sig = sig_all + (size_t)nwc * M2;
+    const cd* gb = (MODE == MODE_VJP) ? a.gbar + (size_t)(nw < a.n_freq ? nw : 0) * M2 : nullptr;
+    const double cn = 2.0 * a.nbar / a.beta;
+
+    cd acc[ACC_LDS ? 1 : NC][ACC_LDS ? 1 : NC];
+    if (ACC_LDS) {
+        for (int e = 0; e < M2; ++e) sAcc[e * WG + lane] = cd{0.0, 0.0};
+    } else {
+#pragma unroll
+        for (int i = 0; i < (ACC_LDS ? 1 : NC); ++i)
+#pragma unroll
+            for (int j = 0; j < (ACC_LDS ? 1 : NC); ++j) acc[i][j] = cd{0.0, 0.0};
+    }
+    if (k0 < k1) stage_k<NC, WG>(a, k0, sbuf, nd);
+    cd trA = cd{0.0, 0.0}, tr2A = cd{0.0, 0.0}, muA = cd{0.0, 0.0};
+    __syncthreads();
+
+    for (int k = k0; k < k1; ++k) {
+        const cd* B = sbuf + ((k - k0) & 1) * bs;
+        const cd* Hcc = B;
+        const cd* P = B + M2;
+        const double* ed = (const double*)(P + nd * NP);
+        // ---- Schur complement S = (z+mu) I - H_cc - Sigma - sum_d g_d P_d
+        cd G[NC][NC];
+#pragma unroll
+        for (int j = 0; j < NC; ++j)
+#pragma unroll
+            for (int i = 0; i < NC; ++i) {
+                cd v = csub(cneg(Hcc[i + j * NC]), sig[i + j * NC]);
+                if (i == j) v = cadd(v, cd{mu, w});
+                G[i][j] = v;
+            }
+        for (int d = 0; d < nd; ++d) {
+            const double x = mu - ed[d];
+            const double r = 1.0 / fma(x, x, w * w);
+            const cd g = cd{x * r, -w * r};
+            const cd* Pd = P + d * NP;
+#pragma unroll
+            for (int ib = 0; ib < NC; ++ib)
+#pragma unroll
+                for (int ia = 0; ia <= ib; ++ia) {
+                    const cd p = Pd[ib * (ib + 1) / 2 + ia];
+                    G[ia][ib] = cfms(p, g, G[ia][ib]);
+                    if (ia != ib) G[ib][ia] = cfms(cconj(p), g, G[ib][ia]);
+                }
+        }
+        // ---- G_cc = S^{-1}
+        if (!F32R) {
+            gj_inverse<NC>(G);
+        } else {
+            // G holds S (FP64); X0 = S^{-1} in FP32
+            cf X0[NC][NC];
+#pragma unroll
+            for (int i = 0; i < NC; ++i)
+#pragma unroll
+                for (int j = 0; j < NC; ++j) X0[i][j] = cf{(float)G[i][j].x, (float)G[i][j].y};
+            gj_inverse_f32<NC>(X0);
+            cd X[NC][NC];
+#pragma unroll
+            for (int i = 0; i < NC; ++i)
+#pragma unroll
+                for (int j = 0; j < NC; ++j) X[i][j] = cd{(double)X0[i][j].x, (double)X0[i][j].y};
+            for (int it = 0; it < a.refine; ++it) {
+#pragma unroll
+                for (int c = 0; c < NC; ++c) {
+                    cd rc[NC];  // r = e_c - S x_c  (FP64 residual)
+#pragma unroll
+                    for (int i = 0; i < NC; ++i) {
+                        cd s = cd{i == c ? 1.0 : 0.0, 0.0};
+#pragma unroll
+                        for (int l = 0; l < NC; ++l) s = cfms(G[i][l], X[l][c], s);
+                        rc[i] = s;
+                    }
+#pragma unroll
+                    for (int i = 0; i < NC; ++i) {  // x_c += X0 r
+                        cd s = X[i][c];
+#pragma unroll
+                        for (int l = 0; l < NC; ++l) s = cfma(cd{(double)X0[i][l].x, (double)X0[i][l].y}, rc[l], s);
+                        X[i][c] = s;
+                    }
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < NC; ++i)
+#pragma unroll
+                for (int j = 0; j < NC; ++j) G[i][j] = X[i][j];
+        }
+
+        if (MODE == MODE_FWD) {
+            if (ACC_LDS) {
+#pragma unroll
+                for (int j = 0; j < NC; ++j)
+#pragma unroll
+                    for (int i = 0; i < NC; ++i) {
+                        cd* p = &sAcc[(i + j * NC) * WG + lane];
+                        *p = cadd(*p, G[i][j]);
+                    }
+            } else {
+#pragma unroll
+                for (int i = 0; i < (ACC_LDS ? 1 : NC); ++i)
+#pragma unroll
+                    for (int j = 0; j < (ACC_LDS ? 1 : NC); ++j) acc[i][j] = cadd(acc[i][j], G[i][j]);
+            }
+        } else {
+            // ---- Y = sum_d g_d^2 P_d (X of the header), trace terms via M_d = w_d^H G w_d
+            cd Y[NC][NC];
+#pragma unroll
+            for (int i = 0; i < NC; ++i)
+#pragma unroll
+                for (int j = 0; j < NC; ++j) Y[i][j] = cd{0.0, 0.0};
+            cd sg = cd{0.0, 0.0}, sg2 = cd{0.0, 0.0}, sgM2 = cd{0.0, 0.0}, sgM3 = cd{0.0, 0.0};
+            for (int d = 0; d < nd; ++d) {
+                const double x = mu - ed[d];
+                const double r = 1.0 / fma(x, x, w * w);
+                const cd g = cd{x * r, -w * r};
+                const cd g2 = cmul(g, g);
+                const cd* Pd = P + d * NP;
+                cd Md = cd{0.0, 0.0};
+#pragma unroll
+                for (int ib = 0; ib < NC; ++ib)
+#pragma unroll
+                    for (int ia = 0; ia <= ib; ++ia) {
+                        const cd p = Pd[ib * (ib + 1) / 2 + ia];  // P_{ia ib} = w_a conj(w_b)
+                        Y[ia][ib] = cfma(p, g2, Y[ia][ib]);
+                        if (ia != ib) Y[ib][ia] = cfma(cconj(p), g2, Y[ib][ia]);
+                        // M_d = sum_ab conj(w_a) G_ab w_b = sum_ab G_ab P_{ba}
+                        Md = cfma(G[ib][ia], p, Md);
+                        if (ia != ib) Md = cfma(G[ia][ib], cconj(p), Md);
+                    }
+                sg = cadd(sg, g);
+                sg2 = cadd(sg2, g2);
+                sgM2 = cfma(g2, Md, sgM2);
+                sgM3 = cfma(cmul(g2, g), Md, sgM3);
+            }
+            cd trG = cadd(sg, sgM2);
+#pragma unroll
+            for (int i = 0; i < NC; ++i) trG = cadd(trG, G[i][i]);
+            // Y <- G Y (column by column, in place)
+#pragma unroll
+            for (int j = 0; j < NC; ++j) {
+                cd t[NC];
+#pragma unroll
+                for (int i = 0; i < NC; ++i) {
+                    cd s = cd{0.0, 0.0};
+#pragma unroll
+                    for (int l = 0; l < NC; ++l) s = cfma(G[i][l], Y[l][j], s);
+                    t[i] = s;
+                }
+#pragma unroll
+                for (int i = 0; i < NC; ++i) Y[i][j] = t[i];
+            }
+            // tr((G S')^2) = tr(G^2) + 2 tr(G Y) + tr(Y^2),  G S' = G + Y
+            cd tz = cadd(sg2, cscale(sgM3, 2.0));
+#pragma unroll
+            for (int i = 0; i < NC; ++i)
+#pragma unroll
+                for (int j = 0; j < NC; ++j) {
+                    tz = cfma(G[i][j], G[j][i], tz);
+                    tz = cfma(cscale(G[i][j], 2.0), Y[j][i], tz);
+                    tz = cfma(Y[i][j], Y[j][i], tz);
+                }
+            if (wvalid) {
+                trA = cadd(trA, trG);
+                tr2A = cadd(tr2A, tz);
+            }
+            if (MODE == MODE_VJP && vjp_here) {
+                // (G^2)_cc = (G + Y) G, column j: q.  mu term += tr(gbar^H q-column),
+                // sigma_bar += cn * ((G^2)_cc)^H  -> row j of sigma_bar gets cn conj(q)
+                cd ms = cd{0.0, 0.0};
+#pragma unroll
+                for (int j = 0; j < NC; ++j) {
+#pragma unroll
+                    for (int i = 0; i < NC; ++i) {
+                        cd q = cd{0.0, 0.0};
+#pragma unroll
+                        for (int l = 0; l < NC; ++l) q = cfma(cadd(G[i][l], Y[i][l]), G[l][j], q);
+                        ms = cfmac(gb[i + j * NC], q, ms);
+                        cd* p = &sAcc[(j + i * NC) * WG + lane];
+                        *p = cadd(*p, cscale(cconj(q), cn));
+                    }
+                }
+                muA = cadd(muA, ms);
+                // sigma_bar += G^H gbar G^H ; gbar staged in Y
+#pragma unroll
+                for (int j = 0; j < NC; ++j)
+#pragma unroll
+                    for (int i = 0; i < NC; ++i) Y[i][j] = gb[i + j * NC];
+#pragma unroll
+                for (int b = 0; b < NC; ++b) {
+                    cd t[NC];  // t = gbar conj(row b of G)
+#pragma unroll
+                    for (int i = 0; i < NC; ++i) {
+                        cd s = cd{0.0, 0.0};
+#pragma unroll
+                        for (int l = 0; l < NC; ++l) s = cfma(Y[i][l], cconj(G[b][l]), s);
+                        t[i] = s;
+                    }
+#pragma unroll
+                    for (int aa = 0; aa < NC; ++aa) {
+                        cd s = cd{0.0, 0.0};
+#pragma unroll
+                        for (int i = 0; i < NC; ++i) s = cfmac(G[i][aa], t[i], s);
+                        cd* p = &sAcc[(aa + b * NC) * WG + lane];
+                        *p = cadd(*p, s);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (k + 1 < k1) stage_k<NC, WG>(a, k + 1, sbuf + (((k + 1 - k0) & 1) * bs), nd);
+        __syncthreads();
+    }
+
+    if (!wvalid) return;
+    const size_t ch = blockIdx.y;
+    if (MODE == MODE_FWD) {
+        a.pTr[ch * a.nw_tot + nw] = cd{0.0, 0.0};
+        if (ACC_LDS) {
+            for (int e = 0; e < M2; ++e) a.pG[(ch * a.nw_tot + nw) * M2 + e] = sAcc[e * WG + lane];
+        } else {
+#pragma unroll
+            for (int i = 0; i < (ACC_LDS ? 1 : NC); ++i)
+#pragma unroll
+                for (int j = 0; j < (ACC_LDS ? 1 : NC); ++j) a.pG[(ch * a.nw_tot + nw) * M2 + i + j * NC] = acc[i][j];
+        }
+    } else {
+        a.pTr[ch * a.nw_tot + nw] = trA;
+        a.pTr2[ch * a.nw_tot + nw] = tr2A;
+        if (MODE == MODE_VJP && vjp_here) {
+            a.pMu[ch * a.n_freq + nw] = muA;
+            for (int e = 0; e < M2; ++e) a.pSb[(ch * a.n_freq + nw) * M2 + e] = sAcc[e * WG + lane];
+        }
+    }
+}
+
+// -------------------------------------------------------------------------
+bool schur_supported(int n, int nc) { return nc >= 1 && nc <= 6 && n >= nc; }
 int schur_slots(int, int) { return 64; }
-hipError_t launch_lattice_schur(const LatArgs&, int, bool, hipStream_t) { return hipErrorNotSupported; }
-hipError_t launch_schur_prep(const cd*, const cd*, int, int, int, cd*, hipStream_t) { return hipErrorNotSupported; }
+
+template <int NC, int MODE, bool F32R>
+static hipError_t launch_nc(const LatArgs& a, const cd* sig_all, hipStream_t s)
+{
+    constexpr int WG = 64;
+    const int nd = a.n - NC;
+    constexpr bool ACC_LDS = (MODE == MODE_VJP) || (MODE == MODE_FWD && F32R);
+    size_t shm = sizeof(cd) * (2 * (size_t)stage_size<NC>(nd) + (ACC_LDS ? (size_t)NC * NC * WG : 0));
+    dim3 grid((a.nw_tot + WG - 1) / WG, a.n_chunks);
+    hipError_t e = hipFuncSetAttribute((const void*)k_lattice_schur<NC, MODE, F32R, WG>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_lattice_schur<NC, MODE, F32R, WG>), grid, dim3(WG), shm, s, a, sig_all);
+    return hipGetLastError();
+}
+
+template <int MODE, bool F32R>
+static hipError_t dispatch_nc(const LatArgs& a, const cd* sig_all, hipStream_t s)
+{
+    switch (a.nc) {
+        case 1: return launch_nc<1, MODE, F32R>(a, sig_all, s);
+        case 2: return launch_nc<2, MODE, F32R>(a, sig_all, s);
+        case 3: return launch_nc<3, MODE, F32R>(a, sig_all, s);
+        case 4: return launch_nc<4, MODE, F32R>(a, sig_all, s);
+        case 5: return launch_nc<5, MODE, F32R>(a, sig_all, s);
+        case 6: return launch_nc<6, MODE, F32R>(a, sig_all, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+// Sigma at every frequency of the density sum (moment tail beyond n_freq)
+__global__ void k_sigma_all(const cd* __restrict__ sigma, const cd* __restrict__ mom, int nc, int n_freq, int nw_tot,
+                            double beta, cd* __restrict__ out)
+{
+    const int m2 = nc * nc;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nw_tot * m2) return;
+    const int nw = t / m2, e = t % m2;
+    out[t] = sigma_elem(sigma, mom, nc, n_freq, beta, nw, e % nc, e / nc);
+}
+
+hipError_t launch_lattice_schur(const LatArgs& a, int mode, bool fp32, hipStream_t s)
+{
+    // the expanded Sigma table lives right after the caller's partial buffers
+    cd* sig_all = a.pSigAll;
+    const int m2 = a.nc * a.nc;
+    hipLaunchKernelGGL(k_sigma_all, dim3((a.nw_tot * m2 + 255) / 256), dim3(256), 0, s, a.sigma, a.mom, a.nc,
+                       a.n_freq, a.nw_tot, a.beta, sig_all);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (mode == MODE_FWD) return fp32 ? dispatch_nc<MODE_FWD, true>(a, sig_all, s) : dispatch_nc<MODE_FWD, false>(a, sig_all, s);
+    if (mode == MODE_DENS) return fp32 ? dispatch_nc<MODE_DENS, true>(a, sig_all, s) : dispatch_nc<MODE_DENS, false>(a, sig_all, s);
+    return fp32 ? dispatch_nc<MODE_VJP, true>(a, sig_all, s) : dispatch_nc<MODE_VJP, false>(a, sig_all, s);
+}
+
+// W(k) = H_cd(k) V_d(k): W_ad = sum_e H(a, nc+e) V_d(e, d)
+__global__ void k_schur_prep(const cd* __restrict__ Hk, const cd* __restrict__ Vd, int n, int nc, int nk,
+                             cd* __restrict__ W)
+{
+    const int nd = n - nc;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)nk * nc * nd) return;
+    const int k = (int)(t / ((size_t)nc * nd));
+    const int r = (int)(t % ((size_t)nc * nd));
+    const int aa = r % nc, d = r / nc;
+    const cd* H = Hk + (size_t)k * n * n;
+    const cd* V = Vd + (size_t)k * nd * nd;
+    cd s = cd{0.0, 0.0};
+    for (int e = 0; e < nd; ++e) s = cfma(H[aa + (size_t)(nc + e) * n], V[e + (size_t)d * nd], s);
+    W[t] = s;
+}
+
+hipError_t launch_schur_prep(const cd* Hk, const cd* Vd, int n, int nc, int nk, cd* W, hipStream_t s)
+{
+    const size_t total = (size_t)nk * nc * (n - nc);
+    if (total == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_schur_prep, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, Hk, Vd, n, nc, nk, W);
+    return hipGetLastError();
+}
+
 }  // namespace dg

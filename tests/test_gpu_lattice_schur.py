@@ -1,0 +1,98 @@
+"""Schur-complement paths against the FP64 oracle.
+
+SCHUR_F64: same bar as the plain path, 1e-12 normwise relative.
+SCHUR_F32R (FP32 solve + FP64-residual refinement): max |dG_loc| <= 1e-9 and
+relative 1e-9 on NumMatrix, N, dN/dmu; gradients relative 1e-6.
+"""
+import numpy as np
+import pytest
+
+from dmftpack_amd import lattice as L
+from oracle import oracle as O
+
+
+def rel(x, ref):
+    x, ref = np.asarray(x), np.asarray(ref)
+    return float(np.max(np.abs(x - ref)) / max(np.max(np.abs(ref)), 1e-300))
+
+
+def case(name, seed=2):
+    if name == "dp14":
+        w = L.workload("dp14", seed=seed, nk_side=4, n_freq=40)
+        return w["Hk"], w["sigma"], w["beta"], 0.3
+    n, nc = (int(v) for v in name.split("_")[1:])
+    Hk = L.random_tb_hk(n, 3, seed)
+    sig, _ = L.pole_sigma(nc, 36, 30.0, seed)
+    return Hk, sig, 30.0, 0.2
+
+
+CASES = ["dp14", "c_6_3", "c_9_4", "c_20_5", "c_8_6", "c_4_1", "c_3_3", "c_5_2"]
+METHODS = [("f64", 1, 1e-12), ("f32r", 2, 1e-9)]
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    from dmftpack_amd import gpu as G
+    return G
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("mname,method,tol", METHODS)
+def test_greenftn_w_schur(gpu, name, mname, method, tol):
+    Hk, sig, beta, mu = case(name)
+    mom = O.asymto_moments(sig, beta)
+    print(f"case {name} {mname}: n={Hk.shape[1]} nc={sig.shape[1]}", flush=True)
+    ref = O.greenftn_w(Hk, sig, mom, mu, beta)
+    lat = gpu.LatticeGF(Hk, sig.shape[1], sig.shape[0], beta, mom)
+    out = lat.greenftn_w(sig, mu, method=method)
+    assert np.max(np.abs(out["gloc"] - ref["gloc"])) <= tol
+    assert rel(out["gloc"], ref["gloc"]) <= tol
+    assert rel(out["num_matrix"], ref["num_matrix"]) <= tol
+    lat.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["dp14", "c_9_4", "c_3_3", "c_20_5"])
+@pytest.mark.parametrize("mname,method,tol", METHODS)
+def test_density_schur(gpu, name, mname, method, tol):
+    Hk, sig, beta, mu = case(name)
+    mom = O.asymto_moments(sig, beta)
+    N_ref, dN_ref = O.density(Hk, sig, mom, mu, beta)
+    lat = gpu.LatticeGF(Hk, sig.shape[1], sig.shape[0], beta, mom)
+    N, dN = lat.density(sig, mu, method=method)
+    assert abs(N - N_ref) <= tol * max(abs(N_ref), 1.0)
+    assert abs(dN - dN_ref) <= tol * max(abs(dN_ref), 1.0)
+    lat.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["dp14", "c_9_4", "c_20_5"])
+@pytest.mark.parametrize("mname,method,tol", [("f64", 1, 1e-12), ("f32r", 2, 1e-8)])
+def test_vjp_schur(gpu, name, mname, method, tol):
+    Hk, sig, beta, mu = case(name)
+    mom = O.asymto_moments(sig, beta)
+    rng = np.random.default_rng(5)
+    gbar = rng.normal(size=sig.shape) + 1j * rng.normal(size=sig.shape)
+    nbar = -0.4
+    sb_ref, mb_ref = O.gloc_vjp(Hk, sig, mom, mu, beta, gbar, nbar)
+    lat = gpu.LatticeGF(Hk, sig.shape[1], sig.shape[0], beta, mom)
+    sb, mb = lat.vjp(sig, mu, gbar, nbar, method=method)
+    assert rel(sb, sb_ref) <= tol
+    assert abs(mb - mb_ref) <= tol * max(abs(mb_ref), 1.0)
+    lat.close()
+
+
+@pytest.mark.gpu
+def test_schur_matches_plain_at_headline_shape(gpu):
+    """n=14, n_c=5 at a mid-size mesh: Schur F64 vs plain F64, F32R vs plain."""
+    w = L.workload("dp14", seed=3, nk_side=8, n_freq=256)
+    Hk, sig, beta = w["Hk"], w["sigma"], w["beta"]
+    lat = gpu.LatticeGF(Hk, 5, sig.shape[0], beta, w["exact_moments"])
+    p = lat.greenftn_w(sig, 0.25, method=gpu.PLAIN_F64)
+    s = lat.greenftn_w(sig, 0.25, method=gpu.SCHUR_F64)
+    r = lat.greenftn_w(sig, 0.25, method=gpu.SCHUR_F32R)
+    assert rel(s["gloc"], p["gloc"]) <= 1e-12
+    assert np.max(np.abs(r["gloc"] - p["gloc"])) <= 1e-9
+    assert rel(r["num_matrix"], p["num_matrix"]) <= 1e-9
+    lat.close()
