@@ -14,6 +14,8 @@ HEADER = os.path.join(HERE, "..", "include", "dmftgpu.h")
 
 OK, ERR_ARG, ERR_STATE, ERR_HIP, ERR_NUMERIC, ERR_UNSUPPORTED = 0, 1, 2, 3, 4, 5
 PLAIN_F64, SCHUR_F64, SCHUR_F32R = 0, 1, 2
+OPT_REFINE_STEPS, OPT_TARGET_WORKGROUPS = 1, 2
+MODE_FWD, MODE_DENS, MODE_VJP = 0, 1, 2
 
 
 class Lattice(C.Structure):
@@ -40,9 +42,12 @@ _SIGS = {
     "dmftgpu_set_stream": ([P, P], I),
     "dmftgpu_last_error": ([P], C.c_char_p),
     "dmftgpu_version": ([], C.c_char_p),
+    "dmftgpu_set_option": ([P, I, I], I),
+    "dmftgpu_enable_timing": ([P, I], I),
+    "dmftgpu_kernel_time": ([P, I, PD], I),
     "dmftgpu_prepare": ([P, C.POINTER(Lattice), P, P], I),
     "dmftgpu_static_eig": ([P, P, P], I),
-    "dmftgpu_greenftn_w": ([P, I, P, D, P, P, P], I),
+    "dmftgpu_greenftn_w": ([P, I, P, D, P, P, P, PD], I),
     "dmftgpu_density_matrix": ([P, P, D, P], I),
     "dmftgpu_density": ([P, I, P, D, PD, PD], I),
     "dmftgpu_adjust_mu": ([P, I, P, D, D, D, PD, PI], I),

@@ -96,6 +96,11 @@ extern "C" int dmftgpu_destroy(dmftgpu_handle h)
 {
     if (!h) return DMFTGPU_ERR_ARG;
     (void)hipStreamSynchronize(h->stream);
+    for (int m = 0; m < 3; ++m) {
+        if (h->ev0[m]) (void)hipEventDestroy(h->ev0[m]);
+        if (h->ev1[m]) (void)hipEventDestroy(h->ev1[m]);
+    }
+    h->ft.release();
     h->ws.release();
     h->prep.release();
     delete h;
@@ -107,6 +112,45 @@ extern "C" int dmftgpu_set_stream(dmftgpu_handle h, void* stream)
     if (!h) return DMFTGPU_ERR_ARG;
     h->stream = (hipStream_t)stream;
     return DMFTGPU_OK;
+}
+
+extern "C" int dmftgpu_enable_timing(dmftgpu_handle h, int enable)
+{
+    if (!h) return DMFTGPU_ERR_ARG;
+    if (enable)
+        for (int m = 0; m < 3; ++m) {
+            if (!h->ev0[m]) HIPCHK(hipEventCreate(&h->ev0[m]));
+            if (!h->ev1[m]) HIPCHK(hipEventCreate(&h->ev1[m]));
+        }
+    h->timing = enable != 0;
+    return DMFTGPU_OK;
+}
+
+extern "C" int dmftgpu_kernel_time(dmftgpu_handle h, int mode, double* ms)
+{
+    if (!h || !ms || mode < 0 || mode > 2) return DMFTGPU_ERR_ARG;
+    if (!h->ev_used[mode]) return fail(h, DMFTGPU_ERR_STATE, "no timed kernel of mode %d", mode);
+    HIPCHK(hipEventSynchronize(h->ev1[mode]));
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, h->ev0[mode], h->ev1[mode]));
+    *ms = t;
+    return DMFTGPU_OK;
+}
+
+extern "C" int dmftgpu_set_option(dmftgpu_handle h, int option, int value)
+{
+    if (!h) return DMFTGPU_ERR_ARG;
+    switch (option) {
+        case DMFTGPU_OPT_REFINE_STEPS:
+            if (value < 0 || value > 4) return fail(h, DMFTGPU_ERR_ARG, "refine steps %d outside 0..4", value);
+            h->refine_steps = value;
+            return DMFTGPU_OK;
+        case DMFTGPU_OPT_TARGET_WORKGROUPS:
+            if (value < 1) return fail(h, DMFTGPU_ERR_ARG, "target workgroups %d < 1", value);
+            h->target_wg = value;
+            return DMFTGPU_OK;
+        default: return fail(h, DMFTGPU_ERR_ARG, "unknown option %d", option);
+    }
 }
 
 extern "C" const char* dmftgpu_last_error(dmftgpu_handle h) { return h ? h->err.c_str() : "null handle"; }
@@ -237,7 +281,7 @@ struct Sums {
 };
 
 int run_lattice(dmftgpu_handle h, int method, int mode, const double* sigma, double mu, const double* gbar,
-                double nbar, char* ws, size_t& off, Sums& S)
+                double nbar, char* ws, size_t& off, Sums& S, bool want_trace = false)
 {
     const dmftgpu_lattice& L = h->lat;
     const int n = L.n, nc = L.n_corr, nk = L.n_k, m2 = nc * nc;
@@ -264,6 +308,7 @@ int run_lattice(dmftgpu_handle h, int method, int mode, const double* sigma, dou
     a.ed = h->ed;
     a.W = h->W;
     a.refine = h->refine_steps;
+    a.want_trace = want_trace ? 1 : 0;
     auto take = [&](size_t bytes) {
         char* p = ws + off;
         off += (bytes + 255) / 256 * 256;
@@ -278,10 +323,15 @@ int run_lattice(dmftgpu_handle h, int method, int mode, const double* sigma, dou
         a.pMu = (cd*)take(C * L.n_freq * sizeof(cd));
     }
     if (method != DMFTGPU_PLAIN_F64) a.pSigAll = (cd*)take((size_t)nw_tot * m2 * sizeof(cd));
+    if (h->timing) HIPCHK(hipEventRecord(h->ev0[mode], h->stream));
     if (method == DMFTGPU_PLAIN_F64) {
         HIPCHK(dg::launch_lattice_plain(a, mode, h->stream, nullptr));
     } else {
         HIPCHK(dg::launch_lattice_schur(a, mode, method == DMFTGPU_SCHUR_F32R, h->stream));
+    }
+    if (h->timing) {
+        HIPCHK(hipEventRecord(h->ev1[mode], h->stream));
+        h->ev_used[mode] = true;
     }
     S.tr = (cd*)take(nw_tot * sizeof(cd));
     HIPCHK(dg::launch_reduce_chunks(a.pTr, P.n_chunks, nw_tot, S.tr, h->stream));
@@ -317,7 +367,7 @@ size_t lattice_ws_bytes(dmftgpu_handle h, int method, int mode)
 
 // GreenFtn_w (GreenFtn.cpp:142-422)
 extern "C" int dmftgpu_greenftn_w(dmftgpu_handle h, int method, const double* sigma, double mu, double* gloc,
-                                  double* gloc_hf, double* num_matrix)
+                                  double* gloc_hf, double* num_matrix, double* n_total)
 {
     if (!h) return DMFTGPU_ERR_ARG;
     if (!h->prepared) return fail(h, DMFTGPU_ERR_STATE, "dmftgpu_prepare has not been called");
@@ -335,7 +385,7 @@ extern "C" int dmftgpu_greenftn_w(dmftgpu_handle h, int method, const double* si
     if (!ws) return fail(h, DMFTGPU_ERR_HIP, "workspace %zu bytes: %s", need, hipGetErrorString(e));
     size_t off = 0;
     Sums S;
-    int rc = run_lattice(h, method, dg::MODE_FWD, sigma, mu, nullptr, 0.0, ws, off, S);
+    int rc = run_lattice(h, method, dg::MODE_FWD, sigma, mu, nullptr, 0.0, ws, off, S, n_total != nullptr);
     if (rc) return rc;
     cd* tot = (cd*)(ws + off);
     off += ((size_t)(m2 + 3) * sizeof(cd) + 255) / 256 * 256;
@@ -353,9 +403,15 @@ extern "C" int dmftgpu_greenftn_w(dmftgpu_handle h, int method, const double* si
                                   h->stream));
     // NumMatrix = [ (1/beta) sum_n sum_k (G_cc + G_cc^H) + sum_k (U w U^H)_cc ] / knum_mpiGlobal
     std::vector<cd> htot(m2 + 3), hst(m2);
+    double hsts[2];
     HIPCHK(hipMemcpyAsync(htot.data(), tot, sizeof(cd) * (m2 + 3), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipMemcpyAsync(hst.data(), st, sizeof(cd) * m2, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(hsts, sts, sizeof(double) * 2, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
+    if (n_total) {
+        *n_total = (htot[m2].x / L.beta + hsts[0]) / (double)L.n_k_global;
+        if (!std::isfinite(*n_total)) return fail(h, DMFTGPU_ERR_NUMERIC, "N(mu) is not finite");
+    }
     std::vector<cd> num(m2);
     for (int q = 0; q < m2; ++q) {
         num[q].x = (htot[q].x / L.beta + hst[q].x) * inv_nk;

@@ -41,4 +41,9 @@ struct dmftgpu_handle_s {
     dgi::Workspace prep;     // prepare-time data
     dgi::Workspace ws;       // per-call scratch
     dgi::Workspace ft;       // Fourier-transform scratch
+    // optional kernel timing (dmftgpu_enable_timing): events around the main
+    // lattice kernel of each mode (FWD, DENS, VJP)
+    bool timing = false;
+    hipEvent_t ev0[3] = {nullptr, nullptr, nullptr}, ev1[3] = {nullptr, nullptr, nullptr};
+    bool ev_used[3] = {false, false, false};
 };

@@ -30,6 +30,7 @@ struct LatArgs {
     const cd* W;       // [nk][nc][nd]  H_cd(k) V_d(k), column-major nc x nd
     int refine;        // FP32R: number of FP64-residual refinement steps
     cd* pSigAll;       // Schur: Sigma expanded to all nw_tot frequencies [nw_tot][nc*nc]
+    int want_trace;    // FWD: also accumulate tr G (total N)
 };
 
 // plain FP64 full inverse; returns 0 or a HIP error

@@ -111,7 +111,7 @@ __device__ __forceinline__ void stage_k(const LatArgs& a, int k, cd* buf, int nd
     for (int t = threadIdx.x; t < nd; t += WG) e[t] = ed[t];
 }
 
-template <int NC, int MODE, bool F32R, int WG>
+template <int NC, int MODE, bool F32R, int WG, bool TRACE>
 __global__ __launch_bounds__(WG) void k_lattice_schur(LatArgs a, const cd* __restrict__ sig_all)
 {
     constexpr int NP = NC * (NC + 1) / 2;
@@ -235,6 +235,28 @@ __global__ __launch_bounds__(WG) void k_lattice_schur(LatArgs a, const cd* __res
 #pragma unroll
                     for (int j = 0; j < (ACC_LDS ? 1 : NC); ++j) acc[i][j] = cadd(acc[i][j], G[i][j]);
             }
+            if (TRACE) {  // tr G = sum_d g_d + tr G_cc + sum_d g_d^2 M_d
+                cd t = cd{0.0, 0.0};
+#pragma unroll
+                for (int i = 0; i < NC; ++i) t = cadd(t, G[i][i]);
+                for (int d = 0; d < nd; ++d) {
+                    const double x = mu - ed[d];
+                    const double r = 1.0 / fma(x, x, w * w);
+                    const cd g = cd{x * r, -w * r};
+                    const cd* Pd = P + d * NP;
+                    cd Md = cd{0.0, 0.0};
+#pragma unroll
+                    for (int ib = 0; ib < NC; ++ib)
+#pragma unroll
+                        for (int ia = 0; ia <= ib; ++ia) {
+                            const cd p = Pd[ib * (ib + 1) / 2 + ia];
+                            Md = cfma(G[ib][ia], p, Md);
+                            if (ia != ib) Md = cfma(G[ia][ib], cconj(p), Md);
+                        }
+                    t = cadd(t, cadd(g, cmul(cmul(g, g), Md)));
+                }
+                if (wvalid) trA = cadd(trA, t);
+            }
         } else {
             // ---- Y = sum_d g_d^2 P_d (X of the header), trace terms via M_d = w_d^H G w_d
             cd Y[NC][NC];
@@ -348,7 +370,7 @@ __global__ __launch_bounds__(WG) void k_lattice_schur(LatArgs a, const cd* __res
     if (!wvalid) return;
     const size_t ch = blockIdx.y;
     if (MODE == MODE_FWD) {
-        a.pTr[ch * a.nw_tot + nw] = cd{0.0, 0.0};
+        a.pTr[ch * a.nw_tot + nw] = trA;
         if (ACC_LDS) {
             for (int e = 0; e < M2; ++e) a.pG[(ch * a.nw_tot + nw) * M2 + e] = sAcc[e * WG + lane];
         } else {
@@ -371,7 +393,7 @@ __global__ __launch_bounds__(WG) void k_lattice_schur(LatArgs a, const cd* __res
 bool schur_supported(int n, int nc) { return nc >= 1 && nc <= 6 && n >= nc; }
 int schur_slots(int, int) { return 64; }
 
-template <int NC, int MODE, bool F32R>
+template <int NC, int MODE, bool F32R, bool TRACE>
 static hipError_t launch_nc(const LatArgs& a, const cd* sig_all, hipStream_t s)
 {
     constexpr int WG = 64;
@@ -379,23 +401,23 @@ static hipError_t launch_nc(const LatArgs& a, const cd* sig_all, hipStream_t s)
     constexpr bool ACC_LDS = (MODE == MODE_VJP) || (MODE == MODE_FWD && F32R);
     size_t shm = sizeof(cd) * (2 * (size_t)stage_size<NC>(nd) + (ACC_LDS ? (size_t)NC * NC * WG : 0));
     dim3 grid((a.nw_tot + WG - 1) / WG, a.n_chunks);
-    hipError_t e = hipFuncSetAttribute((const void*)k_lattice_schur<NC, MODE, F32R, WG>,
+    hipError_t e = hipFuncSetAttribute((const void*)k_lattice_schur<NC, MODE, F32R, WG, TRACE>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_lattice_schur<NC, MODE, F32R, WG>), grid, dim3(WG), shm, s, a, sig_all);
+    hipLaunchKernelGGL((k_lattice_schur<NC, MODE, F32R, WG, TRACE>), grid, dim3(WG), shm, s, a, sig_all);
     return hipGetLastError();
 }
 
-template <int MODE, bool F32R>
+template <int MODE, bool F32R, bool TRACE>
 static hipError_t dispatch_nc(const LatArgs& a, const cd* sig_all, hipStream_t s)
 {
     switch (a.nc) {
-        case 1: return launch_nc<1, MODE, F32R>(a, sig_all, s);
-        case 2: return launch_nc<2, MODE, F32R>(a, sig_all, s);
-        case 3: return launch_nc<3, MODE, F32R>(a, sig_all, s);
-        case 4: return launch_nc<4, MODE, F32R>(a, sig_all, s);
-        case 5: return launch_nc<5, MODE, F32R>(a, sig_all, s);
-        case 6: return launch_nc<6, MODE, F32R>(a, sig_all, s);
+        case 1: return launch_nc<1, MODE, F32R, TRACE>(a, sig_all, s);
+        case 2: return launch_nc<2, MODE, F32R, TRACE>(a, sig_all, s);
+        case 3: return launch_nc<3, MODE, F32R, TRACE>(a, sig_all, s);
+        case 4: return launch_nc<4, MODE, F32R, TRACE>(a, sig_all, s);
+        case 5: return launch_nc<5, MODE, F32R, TRACE>(a, sig_all, s);
+        case 6: return launch_nc<6, MODE, F32R, TRACE>(a, sig_all, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -420,9 +442,14 @@ hipError_t launch_lattice_schur(const LatArgs& a, int mode, bool fp32, hipStream
                        a.n_freq, a.nw_tot, a.beta, sig_all);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    if (mode == MODE_FWD) return fp32 ? dispatch_nc<MODE_FWD, true>(a, sig_all, s) : dispatch_nc<MODE_FWD, false>(a, sig_all, s);
-    if (mode == MODE_DENS) return fp32 ? dispatch_nc<MODE_DENS, true>(a, sig_all, s) : dispatch_nc<MODE_DENS, false>(a, sig_all, s);
-    return fp32 ? dispatch_nc<MODE_VJP, true>(a, sig_all, s) : dispatch_nc<MODE_VJP, false>(a, sig_all, s);
+    if (mode == MODE_FWD) {
+        if (a.want_trace)
+            return fp32 ? dispatch_nc<MODE_FWD, true, true>(a, sig_all, s) : dispatch_nc<MODE_FWD, false, true>(a, sig_all, s);
+        return fp32 ? dispatch_nc<MODE_FWD, true, false>(a, sig_all, s) : dispatch_nc<MODE_FWD, false, false>(a, sig_all, s);
+    }
+    if (mode == MODE_DENS)
+        return fp32 ? dispatch_nc<MODE_DENS, true, false>(a, sig_all, s) : dispatch_nc<MODE_DENS, false, false>(a, sig_all, s);
+    return fp32 ? dispatch_nc<MODE_VJP, true, false>(a, sig_all, s) : dispatch_nc<MODE_VJP, false, false>(a, sig_all, s);
 }
 
 // W(k) = H_cd(k) V_d(k): W_ad = sum_e H(a, nc+e) V_d(e, d)

@@ -58,6 +58,17 @@ class Handle:
     def check(self, rc):
         return capi.check(self.h, rc)
 
+    def set_option(self, option, value):
+        self.check(self.lib.dmftgpu_set_option(self.h, option, value))
+
+    def enable_timing(self, on=True):
+        self.check(self.lib.dmftgpu_enable_timing(self.h, int(on)))
+
+    def kernel_ms(self, mode):
+        ms = C.c_double(0)
+        self.check(self.lib.dmftgpu_kernel_time(self.h, mode, C.byref(ms)))
+        return ms.value
+
     def close(self):
         if self.h:
             self.lib.dmftgpu_destroy(self.h)
@@ -132,22 +143,25 @@ class LatticeGF(Handle):
         torch.cuda.synchronize()
         return ev.cpu().numpy(), from_dev_cm(U)
 
-    def greenftn_w_dev(self, sigma_dev, mu, method=PLAIN_F64, gloc=None, num=None, gloc_hf=None):
+    def greenftn_w_dev(self, sigma_dev, mu, method=PLAIN_F64, gloc=None, num=None, gloc_hf=None, want_n=False):
         nc = self.nc
         if gloc is None:
             gloc = torch.empty((self.n_freq, nc, nc), dtype=torch.complex128, device="cuda")
         if num is None:
             num = torch.empty((nc, nc), dtype=torch.complex128, device="cuda")
+        N = C.c_double(0)
         self.check(self.lib.dmftgpu_greenftn_w(self.h, method, _ptr(sigma_dev), mu, _ptr(gloc), _ptr(gloc_hf),
-                                               _ptr(num)))
+                                               _ptr(num), C.byref(N) if want_n else None))
+        if want_n:
+            return gloc, num, N.value
         return gloc, num
 
     def greenftn_w(self, sigma, mu, method=PLAIN_F64, want_hf=False):
-        """GreenFtn_w (GreenFtn.cpp:142-422) -> dict(gloc, num_matrix[, gloc_hf]) in math layout."""
+        """GreenFtn_w (GreenFtn.cpp:142-422) -> dict(gloc, num_matrix, n_total[, gloc_hf]) in math layout."""
         s = self.sigma_dev(sigma)
         hf = torch.empty((self.n_freq, self.nc, self.nc), dtype=torch.complex128, device="cuda") if want_hf else None
-        gl, num = self.greenftn_w_dev(s, mu, method, gloc_hf=hf)
-        out = dict(gloc=from_dev_cm(gl), num_matrix=from_dev_cm(num))
+        gl, num, N = self.greenftn_w_dev(s, mu, method, gloc_hf=hf, want_n=True)
+        out = dict(gloc=from_dev_cm(gl), num_matrix=from_dev_cm(num), n_total=N)
         if want_hf:
             out["gloc_hf"] = from_dev_cm(hf)
         return out

@@ -45,8 +45,10 @@ def normal(seed, stream, count):
     return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * PI * u2)
 
 
-def kmesh_phases(nx, ny, nz):
+def kmesh_phases(nx, ny=None, nz=None):
     """[Nk, 3] phases k_i * a on the reference's shifted uniform mesh."""
+    ny = ny or nx
+    nz = nz or nx
     k = np.arange(nx * ny * nz)
     jx, jy, jz = k % nx, (k % (nx * ny)) // nx, k // (nx * ny)
     th = np.empty((k.size, 3))

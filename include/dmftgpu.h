@@ -69,6 +69,18 @@ int dmftgpu_set_stream(dmftgpu_handle h, void* hip_stream);
 const char* dmftgpu_last_error(dmftgpu_handle h);
 const char* dmftgpu_version(void);
 
+/* Tuning options (no reference counterpart). */
+#define DMFTGPU_OPT_REFINE_STEPS 1      /* FP64-residual refinement steps of SCHUR_F32R (0..4, default 2) */
+#define DMFTGPU_OPT_TARGET_WORKGROUPS 2 /* workgroups per lattice launch, sets the k chunking (default 2048) */
+int dmftgpu_set_option(dmftgpu_handle h, int option, int value);
+
+/* Instrumentation (no reference counterpart): with timing enabled, HIP events
+ * are recorded on the handle's stream around the main lattice kernel of each
+ * mode (0 forward, 1 density, 2 VJP); dmftgpu_kernel_time returns the last
+ * such kernel's duration in milliseconds (HOST) and waits for it. */
+int dmftgpu_enable_timing(dmftgpu_handle h, int enable);
+int dmftgpu_kernel_time(dmftgpu_handle h, int mode, double* ms);
+
 /* ---- lattice path ---------------------------------------------------------- */
 
 /* Binds H(k) and the Sigma tail moments to the handle and computes the static
@@ -84,9 +96,11 @@ int dmftgpu_static_eig(dmftgpu_handle h, double* eps, double* U);
  *   gloc        [n_freq][nc][nc]  required; inter-cluster blocks are zero (:371-383)
  *   gloc_hf     [n_freq][nc][nc]  optional (NULL): GwHF, the static-part G0 (:277)
  *   num_matrix  [nc][nc]          required; the global NumMatrix (:322-356)
+ *   n_total     HOST, optional (NULL): total N(mu) of the same pass, the
+ *               quantity GiwToNele returns (GreenFtn.cpp:600-634)
  * Method: DMFTGPU_PLAIN_F64 or DMFTGPU_SCHUR_*. */
 int dmftgpu_greenftn_w(dmftgpu_handle h, int method, const double* sigma, double mu, double* gloc, double* gloc_hf,
-                       double* num_matrix);
+                       double* num_matrix, double* n_total);
 
 /* Per-k density matrix densityMatDFT (GreenFtn.cpp:291-319): dens [n_k][n][n].
  * Full n x n inverse, DMFTGPU_PLAIN_F64 only. */
