@@ -103,6 +103,8 @@ extern "C" int dmftgpu_destroy(dmftgpu_handle h)
     h->ft.release();
     h->ws.release();
     h->prep.release();
+    h->stat.release();
+    h->static_cache.release();
     delete h;
     return DMFTGPU_OK;
 }
@@ -207,6 +209,7 @@ extern "C" int dmftgpu_prepare(dmftgpu_handle h, const dmftgpu_lattice* L, const
         h->schur_ready = true;
     }
     h->prepared = true;
+    ++h->prep_gen;
     return DMFTGPU_OK;
 }
 
@@ -238,27 +241,37 @@ Plan make_plan(int nk, int nw_tot, int slots, int target_wg)
 }
 
 // Static weights and their per-k reduction: fills st_out[m2] (complex) and
-// st_scalars[2] = {sum w, sum dN terms} (device), all unnormalised.
+// st_scalars[2] = {sum w, sum dN terms} (device), all unnormalised.  The
+// result depends only on (mu, prepared lattice) and is cached per handle.
 int run_static(dmftgpu_handle h, double mu, cd* st_out, double* st_scalars, char* ws, size_t& off)
 {
     const dmftgpu_lattice& L = h->lat;
     const int n = L.n, nc = L.n_corr, nk = L.n_k, m2 = nc * nc;
     const int nw_tot = L.n_freq * L.high_freq;
-    double* wt = (double*)(ws + off);
-    off += ((size_t)nk * n * sizeof(double) + 255) / 256 * 256;
-    double* dt = (double*)(ws + off);
-    off += ((size_t)nk * n * sizeof(double) + 255) / 256 * 256;
-    const int kpc = (nk + 255) / 256;
-    const int chunks = (nk + kpc - 1) / kpc;
-    cd* pNum = (cd*)(ws + off);
-    off += ((size_t)chunks * m2 * sizeof(cd) + 255) / 256 * 256;
-    double* pN = (double*)(ws + off);
-    off += ((size_t)chunks * 2 * sizeof(double) + 255) / 256 * 256;
-    HIPCHK(dg::launch_static_terms(h->ev, n, nk, nw_tot, L.beta, mu, wt, dt, h->stream));
-    HIPCHK(dg::launch_static_reduce(wt, dt, h->U, n, nc, nk, kpc, chunks, pNum, pN, pN + chunks, h->stream));
-    HIPCHK(dg::launch_reduce_chunks(pNum, chunks, m2, st_out, h->stream));
-    HIPCHK(dg::launch_reduce_chunks_d(pN, chunks, 1, st_scalars, h->stream));
-    HIPCHK(dg::launch_reduce_chunks_d(pN + chunks, chunks, 1, st_scalars + 1, h->stream));
+    hipError_t e;
+    char* cache = (char*)h->static_cache.get((size_t)m2 * sizeof(cd) + 256, &e);
+    if (!cache) return fail(h, DMFTGPU_ERR_HIP, "static cache: %s", hipGetErrorString(e));
+    char* wbuf = (char*)h->stat.get(2 * (((size_t)nk * n * sizeof(double) + 255) / 256 * 256), &e);
+    if (!wbuf) return fail(h, DMFTGPU_ERR_HIP, "static weights: %s", hipGetErrorString(e));
+    double* wt = (double*)wbuf;
+    double* dt = (double*)(wbuf + ((size_t)nk * n * sizeof(double) + 255) / 256 * 256);
+    if (!(h->static_gen == h->prep_gen && h->static_mu == mu)) {
+        const int kpc = (nk + 255) / 256;
+        const int chunks = (nk + kpc - 1) / kpc;
+        cd* pNum = (cd*)(ws + off);
+        off += ((size_t)chunks * m2 * sizeof(cd) + 255) / 256 * 256;
+        double* pN = (double*)(ws + off);
+        off += ((size_t)chunks * 2 * sizeof(double) + 255) / 256 * 256;
+        HIPCHK(dg::launch_static_terms(h->ev, n, nk, nw_tot, L.beta, mu, wt, dt, h->stream));
+        HIPCHK(dg::launch_static_reduce(wt, dt, h->U, n, nc, nk, kpc, chunks, pNum, pN, pN + chunks, h->stream));
+        HIPCHK(dg::launch_reduce_chunks(pNum, chunks, m2, (cd*)cache, h->stream));
+        HIPCHK(dg::launch_reduce_chunks_d(pN, chunks, 1, (double*)(cache + (size_t)m2 * sizeof(cd)), h->stream));
+        HIPCHK(dg::launch_reduce_chunks_d(pN + chunks, chunks, 1, (double*)(cache + (size_t)m2 * sizeof(cd)) + 1, h->stream));
+        h->static_gen = h->prep_gen;
+        h->static_mu = mu;
+    }
+    HIPCHK(hipMemcpyAsync(st_out, cache, sizeof(cd) * m2, hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(st_scalars, cache + (size_t)m2 * sizeof(cd), 2 * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
     return DMFTGPU_OK;
 }
 
@@ -556,5 +569,42 @@ extern "C" int dmftgpu_gloc_vjp(dmftgpu_handle h, int method, const double* sigm
     const double mub = -htot[m2 + 2].x * inv_nk + n_bar * dN;
     if (!std::isfinite(mub)) return fail(h, DMFTGPU_ERR_NUMERIC, "mu_bar is not finite");
     if (mu_bar) *mu_bar = mub;
+    return DMFTGPU_OK;
+}
+
+// densityMatDFT of GreenFtn_w (GreenFtn.cpp:291-319), full n x n per k.
+extern "C" int dmftgpu_density_matrix(dmftgpu_handle h, const double* sigma, double mu, double* dens)
+{
+    if (!h) return DMFTGPU_ERR_ARG;
+    if (!h->prepared) return fail(h, DMFTGPU_ERR_STATE, "dmftgpu_prepare has not been called");
+    if (!sigma || !dens) return fail(h, DMFTGPU_ERR_ARG, "sigma and dens are required");
+    if (!std::isfinite(mu)) return fail(h, DMFTGPU_ERR_ARG, "mu is not finite");
+    const dmftgpu_lattice& L = h->lat;
+    const int m2 = L.n_corr * L.n_corr;
+    hipError_t e;
+    size_t need = static_ws_bytes(L) + (size_t)(m2 + 8) * 32 + 8192;
+    char* ws = (char*)h->ws.get(need, &e);
+    if (!ws) return fail(h, DMFTGPU_ERR_HIP, "workspace %zu bytes: %s", need, hipGetErrorString(e));
+    size_t off = 0;
+    cd* st = (cd*)(ws + off);
+    off += ((size_t)m2 * sizeof(cd) + 255) / 256 * 256;
+    double* sts = (double*)(ws + off);
+    off += 256;
+    int rc = run_static(h, mu, st, sts, ws, off);  // fills the static weights for this mu
+    if (rc) return rc;
+    dg::LatArgs a{};
+    a.Hk = h->Hk;
+    a.sigma = (const cd*)sigma;
+    a.mom = h->mom;
+    a.n = L.n;
+    a.nc = L.n_corr;
+    a.nk = L.n_k;
+    a.n_freq = L.n_freq;
+    a.nw_tot = L.n_freq * L.high_freq;
+    a.beta = L.beta;
+    a.mu = mu;
+    const double* wt = (const double*)h->stat.ptr;
+    HIPCHK(dg::launch_density_matrix(a, h->U, wt, (cd*)dens, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
     return DMFTGPU_OK;
 }

@@ -37,10 +37,15 @@ struct dmftgpu_handle_s {
     dg::cd* Vd = nullptr;    // [nk][nd][nd]
     dg::cd* W = nullptr;     // [nk][nc][nd] H_cd V_d
     int target_wg = 2048;    // workgroups per lattice launch (k chunking)
+    // static-part totals for the last mu (valid until the next prepare)
+    unsigned prep_gen = 0, static_gen = ~0u;
+    double static_mu = 0.0;
+    dgi::Workspace static_cache;  // [m2 cd][2 doubles]
     int refine_steps = 2;    // FP64-residual refinement steps of the FP32 Schur solve
     dgi::Workspace prep;     // prepare-time data
     dgi::Workspace ws;       // per-call scratch
     dgi::Workspace ft;       // Fourier-transform scratch
+    dgi::Workspace stat;     // static weights wt/dt [nk][n] of the cached mu
     // optional kernel timing (dmftgpu_enable_timing): events around the main
     // lattice kernel of each mode (FWD, DENS, VJP)
     bool timing = false;

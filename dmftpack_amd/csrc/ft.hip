@@ -1,19 +1,717 @@
-// Matsubara tails and Fourier transforms (placeholder until the kernels land).
-#include "api_internal.hpp"
+// Matsubara tail moments and the iw <-> tau transforms.
+//
+//   dmftgpu_moments   getAsymto_moments   (FourierTransform.cpp:184-465)
+//   dmftgpu_ft_w2tau  FourierTransform_ + FourierTransform_tau (:126-178, :16-118)
+//   dmftgpu_ft_tau2w  FT_t_to_w           (:521-561)
+//
+// GPU work: the sliding-window least-squares fits (one thread per window start
+// and matrix element, summing in the reference's ascending order), the window
+// averages, the batched positivity tests, and both transforms as tiled complex
+// matrix products over frequency (tau) with the twiddles computed from the
+// same FP64 phase w*tau the reference uses.  The tail of FourierTransform_tau
+// does not depend on tau, so it is subtracted once per frequency, not once
+// per (tau, frequency) as in the reference loop (:55-61).
+// Host work: the small d x d set-up (square roots, inverses, eigenbases) and
+// the sequential selection / patch logic, restated from the reference.
+#include <hip/hip_runtime.h>
 
-extern "C" int dmftgpu_moments(dmftgpu_handle h, int, int, double, const double*, int, double*, int*)
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/dmftgpu.h"
+#include "api_internal.hpp"
+#include "host_linalg.hpp"
+#include "lattice_kernels.hpp"
+
+using dg::cd;
+using dgi::fail;
+
+namespace {
+
+using namespace dg;
+
+// ---------------------------------------------------------------------------
+// getAsymto_moments kernels
+// ---------------------------------------------------------------------------
+__device__ void lu_inverse4(const double (&A)[16], int n, double (&X)[16])
 {
-    return dgi::fail(h, DMFTGPU_ERR_UNSUPPORTED, "moments not built yet");
+    double lu[16];
+    int perm[4];
+    for (int e = 0; e < 16; ++e) lu[e] = A[e];
+    for (int i = 0; i < n; ++i) perm[i] = i;
+    for (int k = 0; k < n; ++k) {
+        int p = k;
+        double best = fabs(lu[k + k * 4]);
+        for (int i = k + 1; i < n; ++i)
+            if (fabs(lu[i + k * 4]) > best) {
+                best = fabs(lu[i + k * 4]);
+                p = i;
+            }
+        if (p != k) {
+            for (int j = 0; j < n; ++j) {
+                double t = lu[k + j * 4];
+                lu[k + j * 4] = lu[p + j * 4];
+                lu[p + j * 4] = t;
+            }
+            int t = perm[k];
+            perm[k] = perm[p];
+            perm[p] = t;
+        }
+        if (best != 0.0) {
+            const double inv = 1.0 / lu[k + k * 4];
+            for (int i = k + 1; i < n; ++i) lu[i + k * 4] *= inv;
+        }
+        for (int j = k + 1; j < n; ++j) {
+            const double u = lu[k + j * 4];
+            for (int i = k + 1; i < n; ++i) lu[i + j * 4] -= lu[i + k * 4] * u;
+        }
+    }
+    for (int c = 0; c < n; ++c) {
+        double x[4];
+        for (int i = 0; i < n; ++i) x[i] = (perm[i] == c) ? 1.0 : 0.0;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < i; ++j) x[i] -= lu[i + j * 4] * x[j];
+        for (int i = n - 1; i >= 0; --i) {
+            for (int j = i + 1; j < n; ++j) x[i] -= lu[i + j * 4] * x[j];
+            x[i] /= lu[i + i * 4];
+        }
+        for (int i = 0; i < n; ++i) X[i + c * 4] = x[i];
+    }
 }
-extern "C" int dmftgpu_ft_w2tau(dmftgpu_handle h, int, int, int, double, const double*, const double*, double*, int*)
+
+// columns of the design matrix for row pair jj (Re row 2jj, Im row 2jj+1)
+// non-G: Re: [1, 0, -1/z^2, 0]   Im: [0, -1/z, 0, 1/z^3]   (:227-230)
+// G    : Re: [-1/z^2, 0]          Im: [0, 1/z^3]             (:366-367)
+__global__ void k_mom_ktk(int n_freq, int range, double beta, int green, double* __restrict__ kinv)
 {
-    return dgi::fail(h, DMFTGPU_ERR_UNSUPPORTED, "ft_w2tau not built yet");
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= range) return;
+    const int nm = green ? 2 : 4;
+    double K[16];
+    for (int e = 0; e < 16; ++e) K[e] = 0.0;
+    for (int wn = w; wn < n_freq; ++wn) {
+        const double z = (2 * wn + 1) * kPi / beta;
+        double re[4] = {0, 0, 0, 0}, im[4] = {0, 0, 0, 0};
+        if (!green) {
+            re[0] = 1.0;
+            im[1] = -1. / pow(z, 1);
+            re[2] = -1. / pow(z, 2);
+            im[3] = 1. / pow(z, 3);
+        } else {
+            re[0] = -1. / pow(z, 2);
+            im[1] = 1. / pow(z, 3);
+        }
+        for (int b = 0; b < nm; ++b)
+            for (int a = 0; a < nm; ++a) K[a + b * 4] += re[a] * re[b] + im[a] * im[b];
+    }
+    double X[16];
+    lu_inverse4(K, nm, X);
+    for (int e = 0; e < 16; ++e) kinv[(size_t)w * 16 + e] = X[e];
 }
-extern "C" int dmftgpu_ft_tau2w(dmftgpu_handle h, int, int, int, double, const double*, double*, int*)
+
+// mt[a][w][e] = sum_b Kinv[w][a][b] (K^T g_e)_b     (:239-243)
+__global__ void k_mom_fit(const cd* __restrict__ fw, int d, int n_freq, int range, double beta, int green,
+                          const double* __restrict__ kinv, cd* __restrict__ mt)
 {
-    return dgi::fail(h, DMFTGPU_ERR_UNSUPPORTED, "ft_tau2w not built yet");
+    const int m2 = d * d;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)range * m2) return;
+    const int w = (int)(t / m2), e = (int)(t % m2);
+    const int al = e % d, be = e / d;
+    const int nm = green ? 2 : 4;
+    cd s[4] = {cd{0, 0}, cd{0, 0}, cd{0, 0}, cd{0, 0}};
+    for (int wn = w; wn < n_freq; ++wn) {
+        const double z = (2 * wn + 1) * kPi / beta;
+        const cd fab = fw[(size_t)wn * m2 + al + be * d];
+        const cd fba = fw[(size_t)wn * m2 + be + al * d];
+        // Re = (F_ab + conj(F_ba))/2 ; Im = (F_ab - conj(F_ba))/(2i)
+        const cd re = cd{0.5 * (fab.x + fba.x), 0.5 * (fab.y - fba.y)};
+        cd im = cd{0.5 * (fab.y + fba.y), -0.5 * (fab.x - fba.x)};
+        if (!green) {
+            s[0] = cadd(s[0], re);
+            s[1] = cadd(s[1], cscale(im, -1. / pow(z, 1)));
+            s[2] = cadd(s[2], cscale(re, -1. / pow(z, 2)));
+            s[3] = cadd(s[3], cscale(im, 1. / pow(z, 3)));
+        } else {
+            if (al == be) im.x += 1.0 / z;  // (:373)
+            s[0] = cadd(s[0], cscale(re, -1. / pow(z, 2)));
+            s[1] = cadd(s[1], cscale(im, 1. / pow(z, 3)));
+        }
+    }
+    const double* X = kinv + (size_t)w * 16;
+    for (int a = 0; a < nm; ++a) {
+        cd m = cd{0, 0};
+        for (int b = 0; b < nm; ++b) m = cadd(m, cscale(s[b], X[a + b * 4]));
+        mt[((size_t)a * range + w) * m2 + e] = m;
+    }
 }
-extern "C" int dmftgpu_density_matrix(dmftgpu_handle h, const double*, double, double*)
+
+// trace[a][w] = sum_alpha Re mt[a][w][alpha alpha]     (:244-247)
+__global__ void k_mom_trace(const cd* __restrict__ mt, int d, int range, int nm, double* __restrict__ trace)
 {
-    return dgi::fail(h, DMFTGPU_ERR_UNSUPPORTED, "density_matrix not built yet");
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= range * nm) return;
+    const int a = t / range, w = t % range;
+    double s = 0.0;
+    for (int al = 0; al < d; ++al) s += mt[((size_t)a * range + w) * d * d + al + al * d].x;
+    trace[t] = s;
 }
+
+// avg[a][c][e] = mean_{iw=c..c+2Nv} mt[a][iw][e]     (:259-283)
+__global__ void k_mom_avg(const cd* __restrict__ mt, int m2, int range, int nm, int Nv, cd* __restrict__ avg)
+{
+    const int ncent = range - 2 * Nv;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)nm * ncent * m2) return;
+    const int e = (int)(t % m2);
+    const int c = (int)((t / m2) % ncent);
+    const int a = (int)(t / ((size_t)m2 * ncent));
+    cd s = cd{0, 0};
+    for (int iw = c; iw <= c + 2 * Nv; ++iw) s = cadd(s, mt[((size_t)a * range + iw) * m2 + e]);
+    avg[t] = cscale(s, 1.0 / (2 * Nv + 1));
+}
+
+// positivity matrix per window centre:
+//   non-G: T = Vi m3 Vi - (Vi m2 Vi)^2, Vi = (m1)^{-1/2}   (:298-308)
+//   G    : T = m1 - m0^2                                   (:423-424)
+// (w1, U1) = eig(m1avg) for the non-G branch.  scratch: 3 d*d per centre.
+__global__ void k_mom_posmat(const cd* __restrict__ avg, int d, int ncent, int green, const double* __restrict__ w1,
+                             const cd* __restrict__ U1, cd* __restrict__ T, cd* __restrict__ scr)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncent) return;
+    const int m2 = d * d;
+    auto A = [&](int a) { return avg + ((size_t)a * ncent + c) * m2; };
+    cd* out = T + (size_t)c * m2;
+    if (green) {
+        const cd* m0 = A(0);
+        const cd* m1 = A(1);
+        for (int j = 0; j < d; ++j)
+            for (int i = 0; i < d; ++i) {
+                cd s = m1[i + j * d];
+                for (int l = 0; l < d; ++l) s = cfms(m0[i + l * d], m0[l + j * d], s);
+                out[i + j * d] = s;
+            }
+        return;
+    }
+    const double* w = w1 + (size_t)c * d;
+    const cd* U = U1 + (size_t)c * m2;
+    if (!(w[0] > 0.0)) {  // m1 not positive: the window is rejected anyway
+        for (int e = 0; e < m2; ++e) out[e] = cd{-1.0, 0.0};
+        return;
+    }
+    cd* Vi = scr + (size_t)c * 3 * m2;
+    cd* P2 = Vi + m2;
+    cd* tmp = P2 + m2;
+    for (int j = 0; j < d; ++j)
+        for (int i = 0; i < d; ++i) {
+            cd s = cd{0, 0};
+            for (int l = 0; l < d; ++l) s = cadd(s, cscale(cmulc(U[i + l * d], U[j + l * d]), 1.0 / sqrt(w[l])));
+            Vi[i + j * d] = s;
+        }
+    const cd* m2a = A(2);
+    const cd* m3a = A(3);
+    // P2 = Vi m2 Vi
+    for (int j = 0; j < d; ++j)
+        for (int i = 0; i < d; ++i) {
+            cd s = cd{0, 0};
+            for (int l = 0; l < d; ++l) s = cfma(Vi[i + l * d], m2a[l + j * d], s);
+            tmp[i + j * d] = s;
+        }
+    for (int j = 0; j < d; ++j)
+        for (int i = 0; i < d; ++i) {
+            cd s = cd{0, 0};
+            for (int l = 0; l < d; ++l) s = cfma(tmp[i + l * d], Vi[l + j * d], s);
+            P2[i + j * d] = s;
+        }
+    // out = Vi m3 Vi
+    for (int j = 0; j < d; ++j)
+        for (int i = 0; i < d; ++i) {
+            cd s = cd{0, 0};
+            for (int l = 0; l < d; ++l) s = cfma(Vi[i + l * d], m3a[l + j * d], s);
+            tmp[i + j * d] = s;
+        }
+    for (int j = 0; j < d; ++j)
+        for (int i = 0; i < d; ++i) {
+            cd s = cd{0, 0};
+            for (int l = 0; l < d; ++l) s = cfma(tmp[i + l * d], Vi[l + j * d], s);
+            for (int l = 0; l < d; ++l) s = cfms(P2[i + l * d], P2[l + j * d], s);
+            out[i + j * d] = s;
+        }
+}
+
+// final moment: mean over the selected window of (mt_ab + conj(mt_ba))/2, then Hermitised (:319-330, :458-461)
+__global__ void k_mom_final(const cd* __restrict__ mt, int d, int range, int mom_idx, int cen, int Nv,
+                            cd* __restrict__ out)
+{
+    const int m2 = d * d;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m2) return;
+    const int al = e % d, be = e / d;
+    cd s = cd{0, 0}, st = cd{0, 0};
+    for (int w = cen - Nv; w <= cen + Nv; ++w) {
+        const cd* m = mt + ((size_t)mom_idx * range + w) * m2;
+        s = cadd(s, cscale(cadd(m[al + be * d], cconj(m[be + al * d])), 0.5));
+        st = cadd(st, cscale(cadd(m[be + al * d], cconj(m[al + be * d])), 0.5));
+    }
+    s = cscale(s, 1.0 / (2 * Nv + 1));
+    st = cscale(st, 1.0 / (2 * Nv + 1));
+    out[e] = cscale(cadd(s, cconj(st)), 0.5);
+}
+
+// ---------------------------------------------------------------------------
+// iw -> tau
+// ---------------------------------------------------------------------------
+// D_i = F_i - tail_i,  tail_i = 1/2 sum_h A_h diag(1/(i w_i - e_h)) A_h^H   (:55-61)
+__global__ void k_ft_subtract_tail(const cd* __restrict__ fw, int d, int n_freq, double beta,
+                                   const cd* __restrict__ A1, const double* __restrict__ e1,
+                                   const cd* __restrict__ A2, const double* __restrict__ e2, cd* __restrict__ D)
+{
+    const int m2 = d * d;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)n_freq * m2) return;
+    const int i = (int)(t / m2), e = (int)(t % m2), a = e % d, b = e / d;
+    const double w = (kPi / beta) + i * ((2 * kPi) / beta);
+    cd tail = cd{0, 0};
+    for (int l = 0; l < d; ++l) {
+        const cd g1 = crcp(cd{-e1[l], w});
+        const cd g2 = crcp(cd{-e2[l], w});
+        tail = cfma(cmulc(A1[a + l * d], A1[b + l * d]), g1, tail);
+        tail = cfma(cmulc(A2[a + l * d], A2[b + l * d]), g2, tail);
+    }
+    D[t] = csub(fw[t], cscale(tail, 0.5));
+}
+
+// out[j][e] = sum_i coef(j,i) in[i][e]  (NIN = 1)  or  sum_i ca in0 + cb in1 (NIN = 2)
+// workgroup: TJ consecutive rows j, all elements; inputs staged in LDS TI rows at a time.
+struct CoefW2Tau {
+    double beta, w0, dw, tstep;
+    __device__ void operator()(int j, int i, cd* c) const
+    {
+        const double w = w0 + i * dw;
+        const double tau = j * tstep;
+        double s, co;
+        sincos(w * tau, &s, &co);
+        c[0] = cd{co, -s};  // exp(-i w tau)
+    }
+};
+
+struct CoefTau2W {
+    double beta, dt;
+    __device__ void operator()(int n, int t, cd* c) const
+    {
+        const double wn = (2 * n + 1) * kPi / beta;
+        const double tau0 = t * dt, tau1 = (t + 1) * dt;
+        double s0, c0, s1, c1;
+        sincos(tau0 * wn, &s0, &c0);
+        sincos(tau1 * wn, &s1, &c1);
+        const cd pI = cd{c1 - c0, s1 - s0};
+        const cd pI2 = cd{tau1 * c1 - tau0 * c0, tau1 * s1 - tau0 * s0};
+        // ca = pI/w^2 + pI2/(i w) ; cb = pI/(i w);  1/(i w) = -i/w
+        const double r = 1.0 / wn;
+        const cd a1 = cscale(pI, r * r);
+        const cd a2 = cd{pI2.y * r, -pI2.x * r};
+        c[0] = cadd(a1, a2);
+        c[1] = cd{pI.y * r, -pI.x * r};
+    }
+};
+
+template <class Coef, int NIN, int OPT>
+__global__ __launch_bounds__(256) void k_dft(int nj, int ni, int m2, int TJ, int TI, const cd* __restrict__ in0,
+                                             const cd* __restrict__ in1, cd* __restrict__ out, Coef coef)
+{
+    extern __shared__ cd sm[];
+    cd* sc = sm;                        // [NIN][TJ][TI]
+    cd* sx = sm + NIN * TJ * TI;        // [NIN][TI][m2]
+    const int j0 = blockIdx.x * TJ;
+    const int nout = TJ * m2;
+    cd acc[OPT];
+#pragma unroll
+    for (int q = 0; q < OPT; ++q) acc[q] = cd{0, 0};
+    for (int i0 = 0; i0 < ni; i0 += TI) {
+        __syncthreads();
+        for (int t = threadIdx.x; t < TJ * TI; t += 256) {
+            const int jj = t / TI, ii = t % TI;
+            cd c[2] = {cd{0, 0}, cd{0, 0}};
+            if (j0 + jj < nj && i0 + ii < ni) coef(j0 + jj, i0 + ii, c);
+            for (int q = 0; q < NIN; ++q) sc[(q * TJ + jj) * TI + ii] = c[q];
+        }
+        for (int t = threadIdx.x; t < TI * m2; t += 256) {
+            const int ii = t / m2, e = t % m2;
+            const bool ok = i0 + ii < ni;
+            sx[ii * m2 + e] = ok ? in0[(size_t)(i0 + ii) * m2 + e] : cd{0, 0};
+            if (NIN == 2) sx[(TI + ii) * m2 + e] = ok ? in1[(size_t)(i0 + ii) * m2 + e] : cd{0, 0};
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < OPT; ++q) {
+            const int o = threadIdx.x + q * 256;
+            if (o >= nout) break;
+            const int jj = o / m2, e = o % m2;
+            cd s = acc[q];
+            for (int ii = 0; ii < TI; ++ii) {
+                s = cfma(sc[jj * TI + ii], sx[ii * m2 + e], s);
+                if (NIN == 2) s = cfma(sc[(TJ + jj) * TI + ii], sx[(TI + ii) * m2 + e], s);
+            }
+            acc[q] = s;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < OPT; ++q) {
+        const int o = threadIdx.x + q * 256;
+        if (o >= nout) break;
+        const int jj = o / m2, e = o % m2;
+        if (j0 + jj < nj) out[(size_t)(j0 + jj) * m2 + e] = acc[q];
+    }
+}
+
+// G(tau_j) = Herm[ (X + X^H)/beta + 1/2 sum_h A_h diag(-1/(e^{e tau} + e^{e (tau-beta)})) A_h^H ]   (:62-90)
+__global__ void k_ft_finish(const cd* __restrict__ X, int d, int ntau1, double beta, double tstep,
+                            const cd* __restrict__ A1, const double* __restrict__ e1, const cd* __restrict__ A2,
+                            const double* __restrict__ e2, cd* __restrict__ out)
+{
+    const int m2 = d * d;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)ntau1 * m2) return;
+    const int j = (int)(t / m2), e = (int)(t % m2), a = e % d, b = e / d;
+    const double tau = j * tstep;
+    const cd* Xj = X + (size_t)j * m2;
+    auto elem = [&](int p, int q) {
+        cd v = cadd(Xj[p + q * d], cconj(Xj[q + p * d]));
+        v = cscale(v, 1.0 / beta);
+        cd s = cd{0, 0};
+        for (int l = 0; l < d; ++l) {
+            const double g1 = -1.0 / (exp(e1[l] * tau) + exp(e1[l] * (tau - beta)));
+            const double g2 = -1.0 / (exp(e2[l] * tau) + exp(e2[l] * (tau - beta)));
+            s = cadd(s, cscale(cmulc(A1[p + l * d], A1[q + l * d]), g1));
+            s = cadd(s, cscale(cmulc(A2[p + l * d], A2[q + l * d]), g2));
+        }
+        return cadd(v, cscale(s, 0.5));
+    };
+    const cd gab = elem(a, b), gba = elem(b, a);
+    out[t] = cscale(cadd(gab, cconj(gba)), 0.5);
+}
+
+// FT_t_to_w segment slopes (:531-536)
+__global__ void k_ft_segments(const cd* __restrict__ gt, int m2, int nseg, double dt, cd* __restrict__ at,
+                              cd* __restrict__ bt)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)nseg * m2) return;
+    const int s = (int)(t / m2);
+    const double tau0 = s * dt, tau1 = (s + 1) * dt;
+    const cd a = cscale(csub(gt[t + m2], gt[t]), 1.0 / (tau1 - tau0));
+    at[t] = a;
+    bt[t] = cadd(cscale(cneg(a), tau0), gt[t]);
+}
+
+// (F - F^H)/(2i), for the causality test of FT_t_to_w (:552)
+__global__ void k_antiherm(const cd* __restrict__ f, int d, int count, cd* __restrict__ out)
+{
+    const int m2 = d * d;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)count * m2) return;
+    const int e = (int)(t % m2), a = e % d, b = e / d;
+    const size_t base = t - e;
+    const cd x = f[t], y = f[base + b + a * d];
+    const cd dlt = cd{x.x - y.x, x.y + y.y};  // x - conj(y)
+    out[t] = cd{0.5 * dlt.y, -0.5 * dlt.x};  // dlt / (2i)
+}
+
+template <class Coef, int NIN>
+hipError_t launch_dft(int nj, int ni, int m2, const cd* in0, const cd* in1, cd* out, Coef coef, hipStream_t s)
+{
+    int TJ = 2048 / m2;
+    if (TJ > 32) TJ = 32;
+    if (TJ < 1) TJ = 1;
+    int TI = 4096 / (m2 * NIN);
+    if (TI > 32) TI = 32;
+    if (TI < 1) TI = 1;
+    const int opt = (TJ * m2 + 255) / 256;
+    size_t shm = sizeof(cd) * ((size_t)NIN * TJ * TI + (size_t)NIN * TI * m2);
+    dim3 grid((nj + TJ - 1) / TJ);
+#define L_(O)                                                                                                     \
+    {                                                                                                             \
+        hipError_t e = hipFuncSetAttribute((const void*)k_dft<Coef, NIN, O>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                           (int)shm);                                                             \
+        if (e != hipSuccess) return e;                                                                            \
+        hipLaunchKernelGGL((k_dft<Coef, NIN, O>), grid, dim3(256), shm, s, nj, ni, m2, TJ, TI, in0, in1, out, coef); \
+        return hipGetLastError();                                                                                 \
+    }
+    if (opt <= 1) L_(1)
+    if (opt <= 2) L_(2)
+    if (opt <= 4) L_(4)
+    L_(8)
+#undef L_
+}
+
+unsigned nblk(size_t n) { return (unsigned)((n + 255) / 256); }
+
+}  // namespace
+
+#define HIPCHK(expr)                                                                                    \
+    do {                                                                                                \
+        hipError_t e_ = (expr);                                                                         \
+        if (e_ != hipSuccess) return fail(h, DMFTGPU_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+static char* ft_ws(dmftgpu_handle h, size_t bytes, int* rc)
+{
+    hipError_t e;
+    char* p = (char*)h->ft.get(bytes, &e);
+    if (!p) *rc = fail(h, DMFTGPU_ERR_HIP, "ft workspace %zu bytes: %s", bytes, hipGetErrorString(e));
+    return p;
+}
+
+struct Carve {
+    char* base;
+    size_t off = 0;
+    template <class T>
+    T* take(size_t count)
+    {
+        T* p = (T*)(base + off);
+        off += (count * sizeof(T) + 255) / 256 * 256;
+        return p;
+    }
+};
+
+extern "C" int dmftgpu_moments(dmftgpu_handle h, int d, int n_freq, double beta, const double* fw, int green_asym,
+                               double* moments, int* windows)
+{
+    if (!h) return DMFTGPU_ERR_ARG;
+    if (d < 1 || d > 32) return fail(h, DMFTGPU_ERR_UNSUPPORTED, "d=%d outside 1..32", d);
+    if (!fw || !moments) return fail(h, DMFTGPU_ERR_ARG, "fw and moments are required");
+    if (!(beta > 0.0)) return fail(h, DMFTGPU_ERR_ARG, "beta=%g", beta);
+    const int nm = green_asym ? 2 : 4;
+    const int range = (green_asym ? n_freq - 2 : n_freq - 4) + 1;
+    int Nv = n_freq / 20;
+    if (Nv == 0) Nv = 1;
+    const int ncent = range - 2 * Nv;
+    // the reference asserts on the window bounds (:321, :440)
+    if (range < 1 || ncent < 1) return fail(h, DMFTGPU_ERR_ARG, "n_freq=%d too small for the window search", n_freq);
+    const int m2 = d * d;
+    size_t need = (size_t)range * 16 * 8 + (size_t)nm * range * m2 * 16 + (size_t)nm * range * 8 +
+                  (size_t)nm * ncent * m2 * 16 + (size_t)ncent * m2 * 16 * 6 + (size_t)ncent * d * 8 * 2 + 64 * 256;
+    int rc = 0;
+    char* base = ft_ws(h, need, &rc);
+    if (!base) return rc;
+    Carve cv{base};
+    double* kinv = cv.take<double>((size_t)range * 16);
+    cd* mt = cv.take<cd>((size_t)nm * range * m2);
+    double* trace = cv.take<double>((size_t)nm * range);
+    cd* avg = cv.take<cd>((size_t)nm * ncent * m2);
+    double* w1 = cv.take<double>((size_t)ncent * d);
+    cd* U1 = cv.take<cd>((size_t)ncent * m2);
+    cd* T = cv.take<cd>((size_t)ncent * m2);
+    cd* scr = cv.take<cd>((size_t)ncent * m2 * 3);
+    double* wT = cv.take<double>((size_t)ncent * d);
+    hipStream_t s = h->stream;
+    hipLaunchKernelGGL(k_mom_ktk, dim3(nblk(range)), dim3(256), 0, s, n_freq, range, beta, green_asym, kinv);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_mom_fit, dim3(nblk((size_t)range * m2)), dim3(256), 0, s, (const cd*)fw, d, n_freq, range, beta,
+                       green_asym, kinv, mt);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_mom_trace, dim3(nblk((size_t)range * nm)), dim3(256), 0, s, mt, d, range, nm, trace);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_mom_avg, dim3(nblk((size_t)nm * ncent * m2)), dim3(256), 0, s, mt, m2, range, nm, Nv, avg);
+    HIPCHK(hipGetLastError());
+    if (!green_asym) HIPCHK(dg::launch_herm_eig(avg + (size_t)1 * ncent * m2, d, 0, d, nullptr, 0, w1, U1, ncent, s));
+    hipLaunchKernelGGL(k_mom_posmat, dim3(nblk(ncent)), dim3(256), 0, s, avg, d, ncent, green_asym, w1, U1, T, scr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(dg::launch_herm_eig(T, d, 0, d, nullptr, 0, wT, nullptr, ncent, s));
+    std::vector<double> htr((size_t)nm * range), hw1(green_asym ? 0 : (size_t)ncent * d), hwT((size_t)ncent * d);
+    HIPCHK(hipMemcpyAsync(htr.data(), trace, sizeof(double) * htr.size(), hipMemcpyDeviceToHost, s));
+    if (!green_asym) HIPCHK(hipMemcpyAsync(hw1.data(), w1, sizeof(double) * hw1.size(), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(hwT.data(), wT, sizeof(double) * hwT.size(), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    std::vector<char> ok(ncent);
+    for (int i = 0; i < ncent; ++i) {
+        if (!green_asym) ok[i] = hw1[(size_t)i * d] > 0 && hwT[(size_t)i * d] > 0;  // (:301-312)
+        else ok[i] = hwT[(size_t)i * d] > 1;                                         // (:432)
+    }
+    cd* dm = (cd*)moments;
+    HIPCHK(hipMemsetAsync(dm, 0, sizeof(cd) * 4 * m2, s));
+    int win[4] = {-1, -1, -1, -1};
+    for (int mom = 0; mom < nm; ++mom) {
+        std::vector<double> var(ncent);
+        for (int c = Nv; c < range - Nv; ++c) {  // (:289-293)
+            const double* v = &htr[(size_t)mom * range + c - Nv];
+            double mean = 0.0;
+            for (int j = 0; j < 2 * Nv + 1; ++j) mean += v[j];
+            mean /= (2 * Nv + 1);
+            double ss = 0.0;
+            for (int j = 0; j < 2 * Nv + 1; ++j) ss += (v[j] - mean) * (v[j] - mean);
+            var[c - Nv] = std::sqrt(ss / (2 * Nv + 1));
+        }
+        int minPos = ncent - 1;  // (:295-315)
+        for (int i = 0; i < ncent; ++i)
+            if (ok[i] && var[i] < var[minPos]) minPos = i;
+        const int cen = minPos + Nv;
+        const int target = green_asym ? mom + 2 : mom;
+        win[target] = cen;
+        hipLaunchKernelGGL(k_mom_final, dim3(nblk(m2)), dim3(256), 0, s, mt, d, range, mom, cen, Nv,
+                           dm + (size_t)target * m2);
+        HIPCHK(hipGetLastError());
+    }
+    if (green_asym) {  // m0 = 0, m1 = I (:451-452)
+        std::vector<cd> I1(m2, cd{0, 0});
+        for (int i = 0; i < d; ++i) I1[i + i * d] = cd{1.0, 0.0};
+        HIPCHK(hipMemcpyAsync(dm + m2, I1.data(), sizeof(cd) * m2, hipMemcpyHostToDevice, s));
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    if (windows) std::memcpy(windows, win, sizeof(win));
+    return DMFTGPU_OK;
+}
+
+// Two-pole tail set-up of FourierTransform_tau (:22-44): returns V^H U_h and eig values e_h.
+static void tail_setup(int d, const std::vector<hl::cx>& m1, const std::vector<hl::cx>& m2, const std::vector<hl::cx>& m3,
+                       std::vector<hl::cx>& A1, std::vector<double>& e1, std::vector<hl::cx>& A2, std::vector<double>& e2)
+{
+    using namespace hl;
+    Mat V = zeros(d), H1 = zeros(d), H2 = zeros(d);
+    if (min_eig(d, m1) > 0) {
+        V = herm_func(d, m1, [](double x) { return std::sqrt(x > 0 ? x : 0.0); });
+        Mat Vi, ViH;
+        inverse(d, V, Vi);
+        inverse(d, adj(d, V), ViH);
+        Mat M2 = mul(d, mul(d, ViH, lin(m2, 2.0, m2, 0.0)), Vi);
+        Mat M3 = mul(d, mul(d, ViH, lin(m3, 2.0, m3, 0.0)), Vi);
+        Mat X = lin(M3, 0.5, mul(d, M2, M2), -0.25);
+        if (min_eig(d, X) > 0) {
+            Mat sq = herm_func(d, X, [](double x) { return std::sqrt(x > 0 ? x : 0.0); });
+            H1 = lin(M2, 0.5, sq, 1.0);
+        } else {
+            H1 = lin(M2, 0.5, M2, 0.0);
+        }
+        H2 = lin(M2, 1.0, H1, -1.0);
+    }
+    Mat U;
+    Mat VH = adj(d, V);
+    eigh(d, H1, e1, U);
+    A1 = mul(d, VH, U);
+    eigh(d, H2, e2, U);
+    A2 = mul(d, VH, U);
+}
+
+extern "C" int dmftgpu_ft_w2tau(dmftgpu_handle h, int d, int n_freq, int n_tau, double beta, const double* fw,
+                                const double* moments, double* gtau, int* patched)
+{
+    if (!h) return DMFTGPU_ERR_ARG;
+    if (d < 1 || d > 32) return fail(h, DMFTGPU_ERR_UNSUPPORTED, "d=%d outside 1..32", d);
+    if (!fw || !moments || !gtau) return fail(h, DMFTGPU_ERR_ARG, "fw, moments and gtau are required");
+    if (n_freq < 1 || n_tau < 1 || !(beta > 0.0)) return fail(h, DMFTGPU_ERR_ARG, "n_freq=%d n_tau=%d beta=%g", n_freq, n_tau, beta);
+    const int m2 = d * d, ntau1 = n_tau + 1;
+    hipStream_t s = h->stream;
+    std::vector<hl::cx> hm(4 * (size_t)m2);
+    HIPCHK(hipMemcpyAsync(hm.data(), moments, sizeof(cd) * 4 * m2, hipMemcpyDefault, s));
+    HIPCHK(hipStreamSynchronize(s));
+    std::vector<hl::cx> m1(hm.begin() + m2, hm.begin() + 2 * m2), mm2(hm.begin() + 2 * m2, hm.begin() + 3 * m2),
+        m3(hm.begin() + 3 * m2, hm.begin() + 4 * m2);
+    std::vector<hl::cx> A1, A2;
+    std::vector<double> e1, e2;
+    tail_setup(d, m1, mm2, m3, A1, e1, A2, e2);
+    size_t need = (size_t)n_freq * m2 * 16 + (size_t)ntau1 * m2 * 16 + 4 * (size_t)m2 * 16 + 4 * (size_t)d * 8 +
+                  (size_t)ntau1 * d * 8 + 16 * 256;
+    int rc = 0;
+    char* base = ft_ws(h, need, &rc);
+    if (!base) return rc;
+    Carve cv{base};
+    cd* D = cv.take<cd>((size_t)n_freq * m2);
+    cd* X = cv.take<cd>((size_t)ntau1 * m2);
+    cd* dA1 = cv.take<cd>(m2);
+    cd* dA2 = cv.take<cd>(m2);
+    double* de1 = cv.take<double>(d);
+    double* de2 = cv.take<double>(d);
+    double* wmin = cv.take<double>((size_t)ntau1 * d);
+    HIPCHK(hipMemcpyAsync(dA1, A1.data(), sizeof(cd) * m2, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dA2, A2.data(), sizeof(cd) * m2, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(de1, e1.data(), sizeof(double) * d, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(de2, e2.data(), sizeof(double) * d, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_ft_subtract_tail, dim3(nblk((size_t)n_freq * m2)), dim3(256), 0, s, (const cd*)fw, d, n_freq,
+                       beta, dA1, de1, dA2, de2, D);
+    HIPCHK(hipGetLastError());
+    CoefW2Tau cw{beta, kPi / beta, (2 * kPi) / beta, beta / n_tau};
+    HIPCHK((launch_dft<CoefW2Tau, 1>(ntau1, n_freq, m2, D, nullptr, X, cw, s)));
+    cd* out = (cd*)gtau;
+    hipLaunchKernelGGL(k_ft_finish, dim3(nblk((size_t)ntau1 * m2)), dim3(256), 0, s, X, d, ntau1, beta, beta / n_tau,
+                       dA1, de1, dA2, de2, out);
+    HIPCHK(hipGetLastError());
+    // causality patch (:145-175): min eigenvalue of every G(tau_j)
+    HIPCHK(dg::launch_herm_eig(out, d, 0, d, nullptr, 0, wmin, nullptr, ntau1, s));
+    std::vector<double> hw((size_t)ntau1 * d);
+    HIPCHK(hipMemcpyAsync(hw.data(), wmin, sizeof(double) * hw.size(), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    bool any = false;
+    for (int t = 0; t <= n_tau; ++t) any |= hw[(size_t)t * d] >= 0;
+    int check = 0;
+    if (any) {  // rare path: restate the reference loop on the host
+        std::vector<hl::cx> g((size_t)ntau1 * m2);
+        HIPCHK(hipMemcpyAsync(g.data(), out, sizeof(cd) * g.size(), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        double mine11 = 0.0;
+        for (int t = 0; t < ntau1; ++t) {
+            const double tau = t * beta / n_tau;
+            const double mine = hw[(size_t)t * d];
+            mine11 = mine;
+            if (mine >= 0 && (t != 0 && t != n_tau)) {
+                check = 1;
+                for (int t1 = t + 1; t1 < ntau1; ++t1) {
+                    const double tau1 = t1 * beta / n_tau, taup = (t - 1) * beta / n_tau;
+                    for (int e = 0; e < m2; ++e)
+                        g[(size_t)t * m2 + e] = (g[(size_t)t1 * m2 + e] - g[(size_t)(t - 1) * m2 + e]) / (tau1 - taup) *
+                                                    (tau - taup) + g[(size_t)(t - 1) * m2 + e];
+                    hl::Mat gm(g.begin() + (size_t)t * m2, g.begin() + (size_t)(t + 1) * m2);
+                    mine11 = hl::min_eig(d, gm);
+                    if (mine11 < 0) break;
+                }
+            }
+            if (mine11 >= 0) {
+                check = 1;
+                for (int e = 0; e < m2; ++e) g[(size_t)t * m2 + e] = 0.0;
+                for (int l = 0; l < d; ++l) g[(size_t)t * m2 + l + (size_t)l * d] = -1e-10;
+            }
+        }
+        HIPCHK(hipMemcpyAsync(out, g.data(), sizeof(cd) * g.size(), hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    if (patched) *patched = check;
+    return DMFTGPU_OK;
+}
+
+extern "C" int dmftgpu_ft_tau2w(dmftgpu_handle h, int d, int n_freq, int n_tau, double beta, const double* gtau,
+                                double* fw, int* violations)
+{
+    if (!h) return DMFTGPU_ERR_ARG;
+    if (d < 1 || d > 32) return fail(h, DMFTGPU_ERR_UNSUPPORTED, "d=%d outside 1..32", d);
+    if (!gtau || !fw) return fail(h, DMFTGPU_ERR_ARG, "gtau and fw are required");
+    if (n_freq < 1 || n_tau < 1 || !(beta > 0.0)) return fail(h, DMFTGPU_ERR_ARG, "n_freq=%d n_tau=%d beta=%g", n_freq, n_tau, beta);
+    const int m2 = d * d;
+    const double dt = beta / n_tau;
+    hipStream_t s = h->stream;
+    size_t need = 2 * (size_t)n_tau * m2 * 16 + (size_t)n_freq * m2 * 16 + (size_t)n_freq * d * 8 + 8 * 256;
+    int rc = 0;
+    char* base = ft_ws(h, need, &rc);
+    if (!base) return rc;
+    Carve cv{base};
+    cd* at = cv.take<cd>((size_t)n_tau * m2);
+    cd* bt = cv.take<cd>((size_t)n_tau * m2);
+    cd* ah = cv.take<cd>((size_t)n_freq * m2);
+    double* wmax = cv.take<double>((size_t)n_freq * d);
+    hipLaunchKernelGGL(k_ft_segments, dim3(nblk((size_t)n_tau * m2)), dim3(256), 0, s, (const cd*)gtau, m2, n_tau, dt,
+                       at, bt);
+    HIPCHK(hipGetLastError());
+    CoefTau2W ct{beta, dt};
+    HIPCHK((launch_dft<CoefTau2W, 2>(n_freq, n_tau, m2, at, bt, (cd*)fw, ct, s)));
+    hipLaunchKernelGGL(k_antiherm, dim3(nblk((size_t)n_freq * m2)), dim3(256), 0, s, (const cd*)fw, d, n_freq, ah);
+    HIPCHK(hipGetLastError());
+    HIPCHK(dg::launch_herm_eig(ah, d, 0, d, nullptr, 0, wmax, nullptr, n_freq, s));
+    std::vector<double> hw((size_t)n_freq * d);
+    HIPCHK(hipMemcpyAsync(hw.data(), wmax, sizeof(double) * hw.size(), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    int viol = 0;
+    for (int n = 0; n < n_freq; ++n) viol += hw[(size_t)n * d + d - 1] > 0;
+    if (violations) *violations = viol;
+    return DMFTGPU_OK;
+}
+

@@ -36,6 +36,8 @@ struct LatArgs {
 // plain FP64 full inverse; returns 0 or a HIP error
 hipError_t launch_lattice_plain(const LatArgs& a, int mode, hipStream_t s, int* wtile_out);
 int plain_slots(int n, int mode);
+// densityMatDFT: dens [nk][n][n] from U, static weights wt [nk][n]
+hipError_t launch_density_matrix(const LatArgs& a, const cd* U, const double* wt, cd* dens, hipStream_t s);
 
 // Schur complement (FP64 or FP32 + refinement)
 hipError_t launch_lattice_schur(const LatArgs& a, int mode, bool fp32, hipStream_t s);

@@ -30,6 +30,40 @@
 
 namespace dg {
 
+// Column-per-lane Gauss-Jordan inverse without pivoting (see the header
+// comment): lane c of an L-lane slot holds column c of A in col[]; bc is the
+// slot's LDS broadcast buffer (2*N entries).  On return col[] holds column c
+// of A^{-1}.  Steps j >= n (padding of a bucketed size) are skipped.
+template <int N>
+__device__ __forceinline__ void gj_columns(cd (&col)[N], cd* bc, int c, int n)
+{
+    cd scl = cd{1.0, 0.0};
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        if (j < n) {
+            cd* buf = bc + (j & 1) * N;
+            if (c == j) {
+#pragma unroll
+                for (int i = 0; i < N; ++i) buf[i] = col[i];
+            }
+            wave_sync();
+            cd b[N];
+#pragma unroll
+            for (int i = 0; i < N; ++i) b[i] = buf[i];
+            const cd p = crcp(b[j]);
+            const bool isj = (c == j);
+            const cd t = isj ? cd{0.0, 0.0} : cmul(col[j], p);
+#pragma unroll
+            for (int i = 0; i < N; ++i)
+                if (i != j) col[i] = cfms(b[i], t, col[i]);
+            col[j] = isj ? cd{-1.0, 0.0} : t;
+            scl = isj ? cneg(p) : scl;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) col[i] = cmul(col[i], scl);
+}
+
 template <int N, int L, int MODE, int WG>
 __global__ __launch_bounds__(WG) void k_lattice_plain(LatArgs a)
 {
@@ -93,32 +127,7 @@ __global__ __launch_bounds__(WG) void k_lattice_plain(LatArgs a)
             for (int i = 0; i < N; ++i)
                 if (i < nc) col[i] = csub(col[i], sSig[slot * m2 + i + c * nc]);
         }
-        // Gauss-Jordan, deferred pivot-column scaling
-        cd scl = cd{1.0, 0.0};
-#pragma unroll
-        for (int j = 0; j < N; ++j) {
-            if (j < n) {
-                cd* buf = bc + (j & 1) * N;
-                if (c == j) {
-#pragma unroll
-                    for (int i = 0; i < N; ++i) buf[i] = col[i];
-                }
-                wave_sync();
-                cd b[N];
-#pragma unroll
-                for (int i = 0; i < N; ++i) b[i] = buf[i];
-                const cd p = crcp(b[j]);
-                const bool isj = (c == j);
-                const cd t = isj ? cd{0.0, 0.0} : cmul(col[j], p);
-#pragma unroll
-                for (int i = 0; i < N; ++i)
-                    if (i != j) col[i] = cfms(b[i], t, col[i]);
-                col[j] = isj ? cd{-1.0, 0.0} : t;
-                scl = isj ? cneg(p) : scl;
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < N; ++i) col[i] = cmul(col[i], scl);
+        gj_columns<N>(col, bc, c, n);
         // lane c < n now holds column c of G
         cd gcc = cd{0.0, 0.0};
 #pragma unroll
@@ -210,6 +219,110 @@ __global__ __launch_bounds__(WG) void k_lattice_plain(LatArgs a)
     } else if (MODE == MODE_VJP && nw < a.n_freq) {
         for (int e = c; e < m2; e += L) a.pSb[(ch * a.n_freq + nw) * m2 + e] = sAcc[slot * m2 + e];
     }
+}
+
+
+// densityMatDFT (GreenFtn.cpp:291-319): one workgroup per k walks all
+// high*N_freq frequencies, SLOTS at a time, summing G(k, iw) column-wise in
+// registers; then
+//   D(k) = (S + S^H)/beta + U diag(w) U^H,   S = sum_n G(k, iw_n),
+// where w_l = f(eps_l - mu) - (1/beta) sum_n 2 Re 1/(iw_n + mu - eps_l) folds
+// the G0 subtraction and the Fermi add-back (the static weights).
+template <int N, int L>
+__global__ __launch_bounds__(256) void k_density_plain(LatArgs a, const cd* __restrict__ U,
+                                                       const double* __restrict__ wt, cd* __restrict__ dens)
+{
+    constexpr int WG = 256;
+    constexpr int SLOTS = WG / L;
+    constexpr int NN = N * N;
+    extern __shared__ cd lds[];
+    const int n = a.n, nc = a.nc, m2 = nc * nc;
+    cd* sH = lds;                   // [N*N]
+    cd* sB = sH + NN;               // [SLOTS][2N]
+    cd* sS = sB + SLOTS * 2 * N;    // [SLOTS][N*N] slot sums
+    const int tid = threadIdx.x, slot = tid / L, c = tid % L;
+    const int k = blockIdx.x;
+    for (int e = tid; e < NN; e += WG) sH[e] = cd{0.0, 0.0};
+    __syncthreads();
+    for (int e = tid; e < n * n; e += WG) sH[(e % n) + (e / n) * N] = a.Hk[(size_t)k * n * n + e];
+    __syncthreads();
+    cd acc[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) acc[i] = cd{0.0, 0.0};
+    cd* bc = sB + slot * 2 * N;
+    const cd* Hc = sH + (c < N ? c : 0) * N;
+    for (int nw0 = 0; nw0 < a.nw_tot; nw0 += SLOTS) {
+        const int nw = nw0 + slot;
+        const bool wvalid = nw < a.nw_tot;
+        const int nwc = wvalid ? nw : a.nw_tot - 1;
+        const cd zmu = cd{a.mu, matsubara(nwc, a.beta)};
+        cd col[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            col[i] = (c < n) ? cneg(Hc[i]) : cd{0.0, 0.0};
+            if (i == c) col[i] = (c < n) ? cadd(col[i], zmu) : cd{1.0, 0.0};
+        }
+        if (c < nc) {
+#pragma unroll
+            for (int i = 0; i < N; ++i)
+                if (i < nc) col[i] = csub(col[i], sigma_elem(a.sigma, a.mom, nc, a.n_freq, a.beta, nwc, i, c));
+        }
+        gj_columns<N>(col, bc, c, n);
+        if (wvalid && c < n) {
+#pragma unroll
+            for (int i = 0; i < N; ++i) acc[i] = cadd(acc[i], col[i]);
+        }
+        wave_sync();
+    }
+    if (c < N) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) sS[slot * NN + i + c * N] = acc[i];
+    }
+    __syncthreads();
+    const cd* Uk = U + (size_t)k * n * n;
+    const double* w = wt + (size_t)k * n;
+    for (int e = tid; e < n * n; e += WG) {
+        const int i = e % n, j = e / n;
+        cd sij = cd{0.0, 0.0}, sji = cd{0.0, 0.0};
+        for (int q = 0; q < SLOTS; ++q) {
+            sij = cadd(sij, sS[q * NN + i + j * N]);
+            sji = cadd(sji, sS[q * NN + j + i * N]);
+        }
+        cd v = cscale(cadd(sij, cconj(sji)), 1.0 / a.beta);
+        for (int l = 0; l < n; ++l) v = cadd(v, cscale(cmulc(Uk[i + l * n], Uk[j + l * n]), w[l]));
+        dens[(size_t)k * n * n + e] = v;
+    }
+    (void)m2;
+}
+
+template <int N>
+static hipError_t launch_dens_n(const LatArgs& a, const cd* U, const double* wt, cd* dens, hipStream_t s)
+{
+    constexpr int L = N <= 4 ? 4 : N <= 8 ? 8 : N <= 16 ? 16 : 32;
+    constexpr int SLOTS = 256 / L;
+    size_t shm = sizeof(cd) * ((size_t)N * N + (size_t)SLOTS * 2 * N + (size_t)SLOTS * N * N);
+    hipError_t e = hipFuncSetAttribute((const void*)k_density_plain<N, L>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)shm);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_density_plain<N, L>), dim3(a.nk), dim3(256), shm, s, a, U, wt, dens);
+    return hipGetLastError();
+}
+
+hipError_t launch_density_matrix(const LatArgs& a, const cd* U, const double* wt, cd* dens, hipStream_t s)
+{
+    switch (a.n) {
+#define CASE(NV) \
+    case NV: return launch_dens_n<NV>(a, U, wt, dens, s);
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10) CASE(11) CASE(12) CASE(13)
+        CASE(14) CASE(15) CASE(16)
+#undef CASE
+        default: break;
+    }
+    if (a.n <= 20) return launch_dens_n<20>(a, U, wt, dens, s);
+    if (a.n <= 24) return launch_dens_n<24>(a, U, wt, dens, s);
+    if (a.n <= 28) return launch_dens_n<28>(a, U, wt, dens, s);
+    if (a.n <= 32) return launch_dens_n<32>(a, U, wt, dens, s);
+    return hipErrorInvalidValue;
 }
 
 static int lanes_for(int N) { return N <= 4 ? 4 : N <= 8 ? 8 : N <= 16 ? 16 : 32; }

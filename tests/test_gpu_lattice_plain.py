@@ -132,3 +132,19 @@ def test_errors_are_reported(gpu):
     with pytest.raises(gpu.capi.DmftGpuError):
         lat.density(sig, float("nan"))
     lat.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["t2g", "dp14", "n_9", "large32"])
+def test_density_matrix(gpu, name):
+    """densityMatDFT (GreenFtn.cpp:291-319), full n x n per k."""
+    Hk, sig, beta, ncl, mu = small_case(name)
+    mom = O.asymto_moments(sig, beta)
+    ref = O.greenftn_w(Hk, sig, mom, mu, beta, n_cluster=ncl, density=True)
+    lat = gpu.LatticeGF(Hk, sig.shape[1], sig.shape[0], beta, mom, n_cluster=ncl)
+    dm = lat.density_matrix(sig, mu)
+    assert rel(dm, ref["density"]) <= TOL
+    # its trace average is N(mu)
+    N, _ = lat.density(sig, mu)
+    assert abs(np.mean(np.trace(dm, axis1=1, axis2=2)).real - N) < 1e-11
+    lat.close()
