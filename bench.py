@@ -60,8 +60,8 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py measures one GPU: the lattice path runs on a single MI355X here "
-              f"(requested --gpus {args.gpus}, WORLD_SIZE {world}); multi-GPU k-sharding is described in "
-              f"DESIGN.md but not run.", file=sys.stderr)
+              f"(requested --gpus {args.gpus}, WORLD_SIZE {world}); sharding the k-mesh over GPUs "
+              f"(one rank per GPU, all-reduce of G_loc) is not part of this build.", file=sys.stderr)
         sys.exit(2)
 
     import torch

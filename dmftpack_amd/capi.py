@@ -14,13 +14,23 @@ HEADER = os.path.join(HERE, "..", "include", "dmftgpu.h")
 
 OK, ERR_ARG, ERR_STATE, ERR_HIP, ERR_NUMERIC, ERR_UNSUPPORTED = 0, 1, 2, 3, 4, 5
 PLAIN_F64, SCHUR_F64, SCHUR_F32R = 0, 1, 2
-OPT_REFINE_STEPS, OPT_TARGET_WORKGROUPS = 1, 2
+OPT_REFINE_STEPS, OPT_TARGET_WORKGROUPS, OPT_SCHUR_WG = 1, 2, 3
 MODE_FWD, MODE_DENS, MODE_VJP = 0, 1, 2
 
 
 class Lattice(C.Structure):
     _fields_ = [("n", C.c_int32), ("n_corr", C.c_int32), ("n_cluster", C.c_int32), ("n_k", C.c_int32),
                 ("n_k_global", C.c_int32), ("n_freq", C.c_int32), ("high_freq", C.c_int32), ("beta", C.c_double)]
+
+
+class System(C.Structure):
+    _fields_ = [("lat", Lattice), ("Hk", C.c_void_p), ("sigma", C.c_void_p), ("moments", C.c_void_p),
+                ("mu", C.c_double), ("gloc", C.c_void_p), ("num_matrix", C.c_void_p), ("n_total", C.c_void_p),
+                ("gloc_target", C.c_void_p), ("n_target", C.c_double), ("sigma_bar", C.c_void_p),
+                ("mu_bar", C.c_void_p)]
+
+
+BATCH_TUNE, BATCH_GRAPH, BATCH_PREPARE = 1, 2, 4
 
 
 class DmftGpuError(RuntimeError):
@@ -53,8 +63,13 @@ _SIGS = {
     "dmftgpu_adjust_mu": ([P, I, P, D, D, D, PD, PI], I),
     "dmftgpu_gloc_vjp": ([P, I, P, D, P, D, P, PD], I),
     "dmftgpu_moments": ([P, I, I, D, P, I, P, PI], I),
+    "dmftgpu_batch_create": ([P, I, P, I, I, C.POINTER(P)], I),
+    "dmftgpu_batch_run": ([P], I),
+    "dmftgpu_batch_tuning": ([P, I, PI, PI, PI, PD, PD], I),
+    "dmftgpu_batch_destroy": ([P], I),
     "dmftgpu_ft_w2tau": ([P, I, I, I, D, P, P, P, PI], I),
     "dmftgpu_ft_tau2w": ([P, I, I, I, D, P, P, PI], I),
+    "dmftgpu_ft_w2tau_points": ([P, I, I, D, P, P, I, PD, P], I),
 }
 
 

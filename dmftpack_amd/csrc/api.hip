@@ -81,6 +81,7 @@ using namespace dgi;
 
 extern "C" int dmftgpu_create(dmftgpu_handle* out, void* stream)
 {
+    (void)hipGetLastError();  // drop an error left by an earlier failed call
     if (!out) return DMFTGPU_ERR_ARG;
     auto* h = new dmftgpu_handle_s();
     h->stream = (hipStream_t)stream;
@@ -94,6 +95,7 @@ extern "C" int dmftgpu_create(dmftgpu_handle* out, void* stream)
 
 extern "C" int dmftgpu_destroy(dmftgpu_handle h)
 {
+    (void)hipGetLastError();  // drop an error left by an earlier failed call
     if (!h) return DMFTGPU_ERR_ARG;
     (void)hipStreamSynchronize(h->stream);
     for (int m = 0; m < 3; ++m) {
@@ -111,6 +113,7 @@ extern "C" int dmftgpu_destroy(dmftgpu_handle h)
 
 extern "C" int dmftgpu_set_stream(dmftgpu_handle h, void* stream)
 {
+    (void)hipGetLastError();  // drop an error left by an earlier failed call
     if (!h) return DMFTGPU_ERR_ARG;
     h->stream = (hipStream_t)stream;
     return DMFTGPU_OK;
@@ -118,6 +121,7 @@ extern "C" int dmftgpu_set_stream(dmftgpu_handle h, void* stream)
 
 extern "C" int dmftgpu_enable_timing(dmftgpu_handle h, int enable)
 {
+    (void)hipGetLastError();  // drop an error left by an earlier failed call
     if (!h) return DMFTGPU_ERR_ARG;
     if (enable)
         for (int m = 0; m < 3; ++m) {
@@ -130,6 +134,7 @@ extern "C" int dmftgpu_enable_timing(dmftgpu_handle h, int enable)
 
 extern "C" int dmftgpu_kernel_time(dmftgpu_handle h, int mode, double* ms)
 {
+    (void)hipGetLastError();  // drop an error left by an earlier failed call
     if (!h || !ms || mode < 0 || mode > 2) return DMFTGPU_ERR_ARG;
     if (!h->ev_used[mode]) return fail(h, DMFTGPU_ERR_STATE, "no timed kernel of mode %d", mode);
     HIPCHK(hipEventSynchronize(h->ev1[mode]));
@@ -141,15 +146,20 @@ extern "C" int dmftgpu_kernel_time(dmftgpu_handle h, int mode, double* ms)
 
 extern "C" int dmftgpu_set_option(dmftgpu_handle h, int option, int value)
 {
+    (void)hipGetLastError();  // drop an error left by an earlier failed call
     if (!h) return DMFTGPU_ERR_ARG;
     switch (option) {
         case DMFTGPU_OPT_REFINE_STEPS:
-            if (value < 0 || value > 4) return fail(h, DMFTGPU_ERR_ARG, "refine steps %d outside 0..4", value);
+            if (value < 1 || value > 2) return fail(h, DMFTGPU_ERR_ARG, "refine steps %d outside 1..2", value);
             h->refine_steps = value;
             return DMFTGPU_OK;
         case DMFTGPU_OPT_TARGET_WORKGROUPS:
             if (value < 1) return fail(h, DMFTGPU_ERR_ARG, "target workgroups %d < 1", value);
             h->target_wg = value;
+            return DMFTGPU_OK;
+        case DMFTGPU_OPT_SCHUR_WG:
+            if (value != 64 && value != 128) return fail(h, DMFTGPU_ERR_ARG, "Schur workgroup width %d not 64/128", value);
+            h->schur_wg = value;
             return DMFTGPU_OK;
         default: return fail(h, DMFTGPU_ERR_ARG, "unknown option %d", option);
     }
@@ -172,6 +182,7 @@ static int check_lattice(dmftgpu_handle h, const dmftgpu_lattice* L)
 
 extern "C" int dmftgpu_prepare(dmftgpu_handle h, const dmftgpu_lattice* L, const double* Hk, const double* moments)
 {
+    (void)hipGetLastError();  // drop an error left by an earlier failed call
     if (!h) return DMFTGPU_ERR_ARG;
     int rc = check_lattice(h, L);
     if (rc) return rc;
@@ -215,6 +226,7 @@ extern "C" int dmftgpu_prepare(dmftgpu_handle h, const dmftgpu_lattice* L, const
 
 extern "C" int dmftgpu_static_eig(dmftgpu_handle h, double* eps, double* U)
 {
+    (void)hipGetLastError();  // drop an error left by an earlier failed call
     if (!h) return DMFTGPU_ERR_ARG;
     if (!h->prepared) return fail(h, DMFTGPU_ERR_STATE, "dmftgpu_prepare has not been called");
     const dmftgpu_lattice& L = h->lat;
@@ -255,7 +267,7 @@ int run_static(dmftgpu_handle h, double mu, cd* st_out, double* st_scalars, char
     if (!wbuf) return fail(h, DMFTGPU_ERR_HIP, "static weights: %s", hipGetErrorString(e));
     double* wt = (double*)wbuf;
     double* dt = (double*)(wbuf + ((size_t)nk * n * sizeof(double) + 255) / 256 * 256);
-    if (!(h->static_gen == h->prep_gen && h->static_mu == mu)) {
+    if (!(h->static_cache_on && h->static_gen == h->prep_gen && h->static_mu == mu)) {
         const int kpc = (nk + 255) / 256;
         const int chunks = (nk + kpc - 1) / kpc;
         cd* pNum = (cd*)(ws + off);
@@ -340,7 +352,7 @@ int run_lattice(dmftgpu_handle h, int method, int mode, const double* sigma, dou
     if (method == DMFTGPU_PLAIN_F64) {
         HIPCHK(dg::launch_lattice_plain(a, mode, h->stream, nullptr));
     } else {
-        HIPCHK(dg::launch_lattice_schur(a, mode, method == DMFTGPU_SCHUR_F32R, h->stream));
+        HIPCHK(dg::launch_lattice_schur(a, mode, method == DMFTGPU_SCHUR_F32R, h->stream, h->schur_wg));
     }
     if (h->timing) {
         HIPCHK(hipEventRecord(h->ev1[mode], h->stream));
@@ -382,6 +394,7 @@ size_t lattice_ws_bytes(dmftgpu_handle h, int method, int mode)
 extern "C" int dmftgpu_greenftn_w(dmftgpu_handle h, int method, const double* sigma, double mu, double* gloc,
                                   double* gloc_hf, double* num_matrix, double* n_total)
 {
+    (void)hipGetLastError();  // drop an error left by an earlier failed call
     if (!h) return DMFTGPU_ERR_ARG;
     if (!h->prepared) return fail(h, DMFTGPU_ERR_STATE, "dmftgpu_prepare has not been called");
     if (!valid_method(method)) return fail(h, DMFTGPU_ERR_ARG, "unknown method %d", method);
@@ -477,6 +490,7 @@ static int density_impl(dmftgpu_handle h, int method, const double* sigma, doubl
 extern "C" int dmftgpu_density(dmftgpu_handle h, int method, const double* sigma, double mu, double* n_out,
                                double* dn_dmu_out)
 {
+    (void)hipGetLastError();  // drop an error left by an earlier failed call
     if (!h) return DMFTGPU_ERR_ARG;
     if (!h->prepared) return fail(h, DMFTGPU_ERR_STATE, "dmftgpu_prepare has not been called");
     if (!valid_method(method)) return fail(h, DMFTGPU_ERR_ARG, "unknown method %d", method);
@@ -494,6 +508,7 @@ extern "C" int dmftgpu_density(dmftgpu_handle h, int method, const double* sigma
 extern "C" int dmftgpu_adjust_mu(dmftgpu_handle h, int method, const double* sigma, double mu0, double n_target,
                                  double tol, double* mu_out, int* n_eval)
 {
+    (void)hipGetLastError();  // drop an error left by an earlier failed call
     if (!h) return DMFTGPU_ERR_ARG;
     if (!h->prepared) return fail(h, DMFTGPU_ERR_STATE, "dmftgpu_prepare has not been called");
     if (!valid_method(method)) return fail(h, DMFTGPU_ERR_ARG, "unknown method %d", method);
@@ -532,6 +547,7 @@ extern "C" int dmftgpu_adjust_mu(dmftgpu_handle h, int method, const double* sig
 extern "C" int dmftgpu_gloc_vjp(dmftgpu_handle h, int method, const double* sigma, double mu, const double* gloc_bar,
                                 double n_bar, double* sigma_bar, double* mu_bar)
 {
+    (void)hipGetLastError();  // drop an error left by an earlier failed call
     if (!h) return DMFTGPU_ERR_ARG;
     if (!h->prepared) return fail(h, DMFTGPU_ERR_STATE, "dmftgpu_prepare has not been called");
     if (!valid_method(method)) return fail(h, DMFTGPU_ERR_ARG, "unknown method %d", method);
@@ -575,6 +591,7 @@ extern "C" int dmftgpu_gloc_vjp(dmftgpu_handle h, int method, const double* sigm
 // densityMatDFT of GreenFtn_w (GreenFtn.cpp:291-319), full n x n per k.
 extern "C" int dmftgpu_density_matrix(dmftgpu_handle h, const double* sigma, double mu, double* dens)
 {
+    (void)hipGetLastError();  // drop an error left by an earlier failed call
     if (!h) return DMFTGPU_ERR_ARG;
     if (!h->prepared) return fail(h, DMFTGPU_ERR_STATE, "dmftgpu_prepare has not been called");
     if (!sigma || !dens) return fail(h, DMFTGPU_ERR_ARG, "sigma and dens are required");

@@ -42,6 +42,8 @@ struct dmftgpu_handle_s {
     double static_mu = 0.0;
     dgi::Workspace static_cache;  // [m2 cd][2 doubles]
     int refine_steps = 2;    // FP64-residual refinement steps of the FP32 Schur solve
+    int schur_wg = 64;       // workgroup width of the Schur kernels (64 or 128)
+    bool static_cache_on = true;  // reuse static weights for a repeated mu
     dgi::Workspace prep;     // prepare-time data
     dgi::Workspace ws;       // per-call scratch
     dgi::Workspace ft;       // Fourier-transform scratch

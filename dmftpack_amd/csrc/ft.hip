@@ -287,10 +287,11 @@ __global__ void k_ft_subtract_tail(const cd* __restrict__ fw, int d, int n_freq,
 // workgroup: TJ consecutive rows j, all elements; inputs staged in LDS TI rows at a time.
 struct CoefW2Tau {
     double beta, w0, dw, tstep;
+    const double* taus;  // explicit tau values (FourierTransform_tau), or null for tau_j = j*tstep
     __device__ void operator()(int j, int i, cd* c) const
     {
         const double w = w0 + i * dw;
-        const double tau = j * tstep;
+        const double tau = taus ? taus[j] : j * tstep;
         double s, co;
         sincos(w * tau, &s, &co);
         c[0] = cd{co, -s};  // exp(-i w tau)
@@ -368,14 +369,15 @@ __global__ __launch_bounds__(256) void k_dft(int nj, int ni, int m2, int TJ, int
 
 // G(tau_j) = Herm[ (X + X^H)/beta + 1/2 sum_h A_h diag(-1/(e^{e tau} + e^{e (tau-beta)})) A_h^H ]   (:62-90)
 __global__ void k_ft_finish(const cd* __restrict__ X, int d, int ntau1, double beta, double tstep,
-                            const cd* __restrict__ A1, const double* __restrict__ e1, const cd* __restrict__ A2,
-                            const double* __restrict__ e2, cd* __restrict__ out)
+                            const double* __restrict__ taus, const cd* __restrict__ A1,
+                            const double* __restrict__ e1, const cd* __restrict__ A2, const double* __restrict__ e2,
+                            cd* __restrict__ out)
 {
     const int m2 = d * d;
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (size_t)ntau1 * m2) return;
     const int j = (int)(t / m2), e = (int)(t % m2), a = e % d, b = e / d;
-    const double tau = j * tstep;
+    const double tau = taus ? taus[j] : j * tstep;
     const cd* Xj = X + (size_t)j * m2;
     auto elem = [&](int p, int q) {
         cd v = cadd(Xj[p + q * d], cconj(Xj[q + p * d]));
@@ -479,6 +481,7 @@ struct Carve {
 extern "C" int dmftgpu_moments(dmftgpu_handle h, int d, int n_freq, double beta, const double* fw, int green_asym,
                                double* moments, int* windows)
 {
+    (void)hipGetLastError();  // drop an error left by an earlier failed call
     if (!h) return DMFTGPU_ERR_ARG;
     if (d < 1 || d > 32) return fail(h, DMFTGPU_ERR_UNSUPPORTED, "d=%d outside 1..32", d);
     if (!fw || !moments) return fail(h, DMFTGPU_ERR_ARG, "fw and moments are required");
@@ -597,6 +600,7 @@ static void tail_setup(int d, const std::vector<hl::cx>& m1, const std::vector<h
 extern "C" int dmftgpu_ft_w2tau(dmftgpu_handle h, int d, int n_freq, int n_tau, double beta, const double* fw,
                                 const double* moments, double* gtau, int* patched)
 {
+    (void)hipGetLastError();  // drop an error left by an earlier failed call
     if (!h) return DMFTGPU_ERR_ARG;
     if (d < 1 || d > 32) return fail(h, DMFTGPU_ERR_UNSUPPORTED, "d=%d outside 1..32", d);
     if (!fw || !moments || !gtau) return fail(h, DMFTGPU_ERR_ARG, "fw, moments and gtau are required");
@@ -631,11 +635,11 @@ extern "C" int dmftgpu_ft_w2tau(dmftgpu_handle h, int d, int n_freq, int n_tau, 
     hipLaunchKernelGGL(k_ft_subtract_tail, dim3(nblk((size_t)n_freq * m2)), dim3(256), 0, s, (const cd*)fw, d, n_freq,
                        beta, dA1, de1, dA2, de2, D);
     HIPCHK(hipGetLastError());
-    CoefW2Tau cw{beta, kPi / beta, (2 * kPi) / beta, beta / n_tau};
+    CoefW2Tau cw{beta, kPi / beta, (2 * kPi) / beta, beta / n_tau, nullptr};
     HIPCHK((launch_dft<CoefW2Tau, 1>(ntau1, n_freq, m2, D, nullptr, X, cw, s)));
     cd* out = (cd*)gtau;
     hipLaunchKernelGGL(k_ft_finish, dim3(nblk((size_t)ntau1 * m2)), dim3(256), 0, s, X, d, ntau1, beta, beta / n_tau,
-                       dA1, de1, dA2, de2, out);
+                       (const double*)nullptr, dA1, de1, dA2, de2, out);
     HIPCHK(hipGetLastError());
     // causality patch (:145-175): min eigenvalue of every G(tau_j)
     HIPCHK(dg::launch_herm_eig(out, d, 0, d, nullptr, 0, wmin, nullptr, ntau1, s));
@@ -682,6 +686,7 @@ extern "C" int dmftgpu_ft_w2tau(dmftgpu_handle h, int d, int n_freq, int n_tau, 
 extern "C" int dmftgpu_ft_tau2w(dmftgpu_handle h, int d, int n_freq, int n_tau, double beta, const double* gtau,
                                 double* fw, int* violations)
 {
+    (void)hipGetLastError();  // drop an error left by an earlier failed call
     if (!h) return DMFTGPU_ERR_ARG;
     if (d < 1 || d > 32) return fail(h, DMFTGPU_ERR_UNSUPPORTED, "d=%d outside 1..32", d);
     if (!gtau || !fw) return fail(h, DMFTGPU_ERR_ARG, "gtau and fw are required");
@@ -715,3 +720,53 @@ extern "C" int dmftgpu_ft_tau2w(dmftgpu_handle h, int d, int n_freq, int n_tau, 
     return DMFTGPU_OK;
 }
 
+
+// FourierTransform_tau (FourierTransform.cpp:16-118) at arbitrary tau values
+// (no causality patch: the single-tau overloads of FourierTransform have none).
+extern "C" int dmftgpu_ft_w2tau_points(dmftgpu_handle h, int d, int n_freq, double beta, const double* fw,
+                                       const double* moments, int n_points, const double* taus_host, double* out)
+{
+    (void)hipGetLastError();  // drop an error left by an earlier failed call
+    if (!h) return DMFTGPU_ERR_ARG;
+    if (d < 1 || d > 32) return fail(h, DMFTGPU_ERR_UNSUPPORTED, "d=%d outside 1..32", d);
+    if (!fw || !moments || !out || !taus_host || n_points < 1 || n_freq < 1 || !(beta > 0.0))
+        return fail(h, DMFTGPU_ERR_ARG, "bad arguments to ft_w2tau_points");
+    const int m2 = d * d;
+    hipStream_t s = h->stream;
+    std::vector<hl::cx> hm(4 * (size_t)m2);
+    HIPCHK(hipMemcpyAsync(hm.data(), moments, sizeof(cd) * 4 * m2, hipMemcpyDefault, s));
+    HIPCHK(hipStreamSynchronize(s));
+    std::vector<hl::cx> m1(hm.begin() + m2, hm.begin() + 2 * m2), mm2(hm.begin() + 2 * m2, hm.begin() + 3 * m2),
+        m3(hm.begin() + 3 * m2, hm.begin() + 4 * m2);
+    std::vector<hl::cx> A1, A2;
+    std::vector<double> e1, e2;
+    tail_setup(d, m1, mm2, m3, A1, e1, A2, e2);
+    size_t need = (size_t)n_freq * m2 * 16 + (size_t)n_points * m2 * 16 + 4 * (size_t)m2 * 16 + 4 * (size_t)d * 8 +
+                  (size_t)n_points * 8 + 16 * 256;
+    int rc = 0;
+    char* base = ft_ws(h, need, &rc);
+    if (!base) return rc;
+    Carve cv{base};
+    cd* D = cv.take<cd>((size_t)n_freq * m2);
+    cd* X = cv.take<cd>((size_t)n_points * m2);
+    cd* dA1 = cv.take<cd>(m2);
+    cd* dA2 = cv.take<cd>(m2);
+    double* de1 = cv.take<double>(d);
+    double* de2 = cv.take<double>(d);
+    double* dtau = cv.take<double>(n_points);
+    HIPCHK(hipMemcpyAsync(dA1, A1.data(), sizeof(cd) * m2, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dA2, A2.data(), sizeof(cd) * m2, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(de1, e1.data(), sizeof(double) * d, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(de2, e2.data(), sizeof(double) * d, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dtau, taus_host, sizeof(double) * n_points, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_ft_subtract_tail, dim3(nblk((size_t)n_freq * m2)), dim3(256), 0, s, (const cd*)fw, d, n_freq,
+                       beta, dA1, de1, dA2, de2, D);
+    HIPCHK(hipGetLastError());
+    CoefW2Tau cw{beta, kPi / beta, (2 * kPi) / beta, 0.0, dtau};
+    HIPCHK((launch_dft<CoefW2Tau, 1>(n_points, n_freq, m2, D, nullptr, X, cw, s)));
+    hipLaunchKernelGGL(k_ft_finish, dim3(nblk((size_t)n_points * m2)), dim3(256), 0, s, X, d, n_points, beta, 0.0,
+                       (const double*)dtau, dA1, de1, dA2, de2, (cd*)out);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));
+    return DMFTGPU_OK;
+}

@@ -31,6 +31,7 @@ struct LatArgs {
     int refine;        // FP32R: number of FP64-residual refinement steps
     cd* pSigAll;       // Schur: Sigma expanded to all nw_tot frequencies [nw_tot][nc*nc]
     int want_trace;    // FWD: also accumulate tr G (total N)
+    const double* nbar_dev;  // VJP: device copy of nbar (overrides nbar when set)
 };
 
 // plain FP64 full inverse; returns 0 or a HIP error
@@ -39,8 +40,23 @@ int plain_slots(int n, int mode);
 // densityMatDFT: dens [nk][n][n] from U, static weights wt [nk][n]
 hipError_t launch_density_matrix(const LatArgs& a, const cd* U, const double* wt, cd* dens, hipStream_t s);
 
-// Schur complement (FP64 or FP32 + refinement)
-hipError_t launch_lattice_schur(const LatArgs& a, int mode, bool fp32, hipStream_t s);
+// Schur complement (FP64 or FP32 + refinement); wg = workgroup width (64 or 128)
+hipError_t launch_lattice_schur(const LatArgs& a, int mode, bool fp32, hipStream_t s, int wg = 64);
+// Sigma expanded over all nw_tot frequencies into a.pSigAll
+hipError_t launch_sigma_all(const LatArgs& a, hipStream_t s);
+// Persistent launch over a work list of several lattices sharing n_c (and mode,
+// precision, refinement): args [systems] in device memory, items {system,
+// wtile, chunk, 0}; proto carries nc / refine / want_trace.
+struct SchurMulti {
+    const LatArgs* args;
+    const int4* items;
+    int n_items;
+    int grid;
+    int max_nd;
+};
+hipError_t launch_lattice_schur_multi(const LatArgs& proto, int mode, bool fp32, hipStream_t s, int wg,
+                                      const SchurMulti& m);
+size_t schur_lds_bytes(int nc, int nd, int mode, bool fp32, int wg);
 int schur_slots(int nc, int mode);
 bool schur_supported(int n, int nc);
 

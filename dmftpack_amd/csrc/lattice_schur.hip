@@ -111,23 +111,23 @@ __device__ __forceinline__ void stage_k(const LatArgs& a, int k, cd* buf, int nd
     for (int t = threadIdx.x; t < nd; t += WG) e[t] = ed[t];
 }
 
-template <int NC, int MODE, bool F32R, int WG, bool TRACE>
-__global__ __launch_bounds__(WG) void k_lattice_schur(LatArgs a, const cd* __restrict__ sig_all)
+template <int NC, int MODE, bool F32R, int WG, bool TRACE, int REFINE>
+__device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restrict__ sig_all, int wtile, int chunk,
+                                           cd* lds)
 {
     constexpr int NP = NC * (NC + 1) / 2;
     constexpr int M2 = NC * NC;
     // accumulators that do not fit next to the working matrices live in LDS
     constexpr bool ACC_LDS = (MODE == MODE_VJP) || (MODE == MODE_FWD && F32R);
-    extern __shared__ cd lds[];
     const int nd = a.n - NC;
     const int bs = stage_size<NC>(nd);
-    cd* sbuf = lds;           // [2][bs]
-    cd* sAcc = lds + 2 * bs;  // [M2][WG], lane-contiguous
+    cd* sAcc = lds;            // [M2][WG], lane-contiguous
+    cd* sbuf = lds + (ACC_LDS ? M2 * WG : 0);  // [2][bs]
     const int lane = threadIdx.x;
-    const int nw = blockIdx.x * WG + lane;
+    const int nw = wtile * WG + lane;
     const bool wvalid = nw < a.nw_tot;
     const int nwc = wvalid ? nw : a.nw_tot - 1;  // clamp for loads
-    const int k0 = blockIdx.y * a.k_per_chunk;
+    const int k0 = chunk * a.k_per_chunk;
     const int k1 = min(a.nk, k0 + a.k_per_chunk);
     const double w = matsubara(nwc, a.beta);
     const double mu = a.mu;
@@ -194,23 +194,24 @@ __global__ __launch_bounds__(WG) void k_lattice_schur(LatArgs a, const cd* __res
             for (int i = 0; i < NC; ++i)
 #pragma unroll
                 for (int j = 0; j < NC; ++j) X[i][j] = cd{(double)X0[i][j].x, (double)X0[i][j].y};
-            for (int it = 0; it < a.refine; ++it) {
+            for (int it = 0; it < REFINE; ++it) {
 #pragma unroll
                 for (int c = 0; c < NC; ++c) {
-                    cd rc[NC];  // r = e_c - S x_c  (FP64 residual)
+                    cf rc[NC];  // r = e_c - S x_c, formed in FP64, stored in FP32
 #pragma unroll
                     for (int i = 0; i < NC; ++i) {
                         cd s = cd{i == c ? 1.0 : 0.0, 0.0};
 #pragma unroll
                         for (int l = 0; l < NC; ++l) s = cfms(G[i][l], X[l][c], s);
-                        rc[i] = s;
+                        rc[i] = cf{(float)s.x, (float)s.y};
                     }
+                    // x_c += X0 r: the correction is ~kappa*eps32 of |x|, so FP32 suffices
 #pragma unroll
-                    for (int i = 0; i < NC; ++i) {  // x_c += X0 r
-                        cd s = X[i][c];
+                    for (int i = 0; i < NC; ++i) {
+                        cf s = cf{0.f, 0.f};
 #pragma unroll
-                        for (int l = 0; l < NC; ++l) s = cfma(cd{(double)X0[i][l].x, (double)X0[i][l].y}, rc[l], s);
-                        X[i][c] = s;
+                        for (int l = 0; l < NC; ++l) s = ffms(cf{-X0[i][l].x, -X0[i][l].y}, rc[l], s);
+                        X[i][c] = cadd(X[i][c], cd{(double)s.x, (double)s.y});
                     }
                 }
             }
@@ -368,7 +369,7 @@ __global__ __launch_bounds__(WG) void k_lattice_schur(LatArgs a, const cd* __res
     }
 
     if (!wvalid) return;
-    const size_t ch = blockIdx.y;
+    const size_t ch = chunk;
     if (MODE == MODE_FWD) {
         a.pTr[ch * a.nw_tot + nw] = trA;
         if (ACC_LDS) {
@@ -389,37 +390,109 @@ __global__ __launch_bounds__(WG) void k_lattice_schur(LatArgs a, const cd* __res
     }
 }
 
+template <int NC, int MODE, bool F32R, int WG, bool TRACE, int REFINE>
+__global__ __launch_bounds__(WG) void k_lattice_schur(LatArgs a, const cd* __restrict__ sig_all)
+{
+    extern __shared__ cd lds[];
+    schur_item<NC, MODE, F32R, WG, TRACE, REFINE>(a, sig_all, blockIdx.x, blockIdx.y, lds);
+}
+
+// Persistent form over a work list spanning several lattices of the same n_c:
+// workgroup b takes items b, b + gridDim.x, ... (static striding, no atomics,
+// so every replay does the same work in the same order).
+template <int NC, int MODE, bool F32R, int WG, bool TRACE, int REFINE>
+__global__ __launch_bounds__(WG) void k_lattice_schur_multi(const LatArgs* __restrict__ args,
+                                                            const int4* __restrict__ items, int n_items)
+{
+    extern __shared__ cd lds[];
+    for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+        const int4 item = items[it];  // {system, wtile, chunk, -}
+        const LatArgs a = args[item.x];
+        schur_item<NC, MODE, F32R, WG, TRACE, REFINE>(a, a.pSigAll, item.y, item.z, lds);
+        __syncthreads();
+    }
+}
+
 // -------------------------------------------------------------------------
 bool schur_supported(int n, int nc) { return nc >= 1 && nc <= 6 && n >= nc; }
 int schur_slots(int, int) { return 64; }
 
-template <int NC, int MODE, bool F32R, bool TRACE>
-static hipError_t launch_nc(const LatArgs& a, const cd* sig_all, hipStream_t s)
+template <int NC, int MODE, bool F32R, int WG>
+size_t schur_shm(int nd)
 {
-    constexpr int WG = 64;
-    const int nd = a.n - NC;
     constexpr bool ACC_LDS = (MODE == MODE_VJP) || (MODE == MODE_FWD && F32R);
-    size_t shm = sizeof(cd) * (2 * (size_t)stage_size<NC>(nd) + (ACC_LDS ? (size_t)NC * NC * WG : 0));
+    return sizeof(cd) * (2 * (size_t)stage_size<NC>(nd) + (ACC_LDS ? (size_t)NC * NC * WG : 0));
+}
+
+template <int NC, int MODE, bool F32R, bool TRACE, int REFINE, int WG>
+static hipError_t launch_nc_w(const LatArgs& a, const cd* sig_all, hipStream_t s, const SchurMulti* multi)
+{
+    if (multi) {
+        size_t shm = schur_shm<NC, MODE, F32R, WG>(multi->max_nd);
+        hipError_t e = hipFuncSetAttribute((const void*)k_lattice_schur_multi<NC, MODE, F32R, WG, TRACE, REFINE>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((k_lattice_schur_multi<NC, MODE, F32R, WG, TRACE, REFINE>), dim3(multi->grid), dim3(WG), shm,
+                           s, multi->args, multi->items, multi->n_items);
+        return hipGetLastError();
+    }
+    const int nd = a.n - NC;
+    size_t shm = schur_shm<NC, MODE, F32R, WG>(nd);
     dim3 grid((a.nw_tot + WG - 1) / WG, a.n_chunks);
-    hipError_t e = hipFuncSetAttribute((const void*)k_lattice_schur<NC, MODE, F32R, WG, TRACE>,
+    hipError_t e = hipFuncSetAttribute((const void*)k_lattice_schur<NC, MODE, F32R, WG, TRACE, REFINE>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_lattice_schur<NC, MODE, F32R, WG, TRACE>), grid, dim3(WG), shm, s, a, sig_all);
+    hipLaunchKernelGGL((k_lattice_schur<NC, MODE, F32R, WG, TRACE, REFINE>), grid, dim3(WG), shm, s, a, sig_all);
     return hipGetLastError();
 }
 
+template <int NC, int MODE, bool F32R, bool TRACE, int REFINE>
+static hipError_t launch_nc_r(const LatArgs& a, const cd* sig_all, hipStream_t s, int wg, const SchurMulti* m)
+{
+    if (wg == 128) return launch_nc_w<NC, MODE, F32R, TRACE, REFINE, 128>(a, sig_all, s, m);
+    return launch_nc_w<NC, MODE, F32R, TRACE, REFINE, 64>(a, sig_all, s, m);
+}
+
+template <int NC, int MODE, bool F32R, bool TRACE>
+static hipError_t launch_nc(const LatArgs& a, const cd* sig_all, hipStream_t s, int wg, const SchurMulti* m)
+{
+    if constexpr (!F32R) {
+        return launch_nc_r<NC, MODE, false, TRACE, 0>(a, sig_all, s, wg, m);
+    } else {
+        if (a.refine <= 1) return launch_nc_r<NC, MODE, true, TRACE, 1>(a, sig_all, s, wg, m);
+        return launch_nc_r<NC, MODE, true, TRACE, 2>(a, sig_all, s, wg, m);
+    }
+}
+
 template <int MODE, bool F32R, bool TRACE>
-static hipError_t dispatch_nc(const LatArgs& a, const cd* sig_all, hipStream_t s)
+static hipError_t dispatch_nc(const LatArgs& a, const cd* sig_all, hipStream_t s, int wg, const SchurMulti* m)
 {
     switch (a.nc) {
-        case 1: return launch_nc<1, MODE, F32R, TRACE>(a, sig_all, s);
-        case 2: return launch_nc<2, MODE, F32R, TRACE>(a, sig_all, s);
-        case 3: return launch_nc<3, MODE, F32R, TRACE>(a, sig_all, s);
-        case 4: return launch_nc<4, MODE, F32R, TRACE>(a, sig_all, s);
-        case 5: return launch_nc<5, MODE, F32R, TRACE>(a, sig_all, s);
-        case 6: return launch_nc<6, MODE, F32R, TRACE>(a, sig_all, s);
+        case 1: return launch_nc<1, MODE, F32R, TRACE>(a, sig_all, s, wg, m);
+        case 2: return launch_nc<2, MODE, F32R, TRACE>(a, sig_all, s, wg, m);
+        case 3: return launch_nc<3, MODE, F32R, TRACE>(a, sig_all, s, wg, m);
+        case 4: return launch_nc<4, MODE, F32R, TRACE>(a, sig_all, s, wg, m);
+        case 5: return launch_nc<5, MODE, F32R, TRACE>(a, sig_all, s, wg, m);
+        case 6: return launch_nc<6, MODE, F32R, TRACE>(a, sig_all, s, wg, m);
         default: return hipErrorInvalidValue;
     }
+}
+
+static hipError_t dispatch_all(const LatArgs& a, int mode, bool fp32, const cd* sig_all, hipStream_t s, int wg,
+                               const SchurMulti* m)
+{
+    if (mode == MODE_FWD) {
+        if (a.want_trace)
+            return fp32 ? dispatch_nc<MODE_FWD, true, true>(a, sig_all, s, wg, m)
+                        : dispatch_nc<MODE_FWD, false, true>(a, sig_all, s, wg, m);
+        return fp32 ? dispatch_nc<MODE_FWD, true, false>(a, sig_all, s, wg, m)
+                    : dispatch_nc<MODE_FWD, false, false>(a, sig_all, s, wg, m);
+    }
+    if (mode == MODE_DENS)
+        return fp32 ? dispatch_nc<MODE_DENS, true, false>(a, sig_all, s, wg, m)
+                    : dispatch_nc<MODE_DENS, false, false>(a, sig_all, s, wg, m);
+    return fp32 ? dispatch_nc<MODE_VJP, true, false>(a, sig_all, s, wg, m)
+                : dispatch_nc<MODE_VJP, false, false>(a, sig_all, s, wg, m);
 }
 
 // Sigma at every frequency of the density sum (moment tail beyond n_freq)
@@ -433,23 +506,32 @@ __global__ void k_sigma_all(const cd* __restrict__ sigma, const cd* __restrict__
     out[t] = sigma_elem(sigma, mom, nc, n_freq, beta, nw, e % nc, e / nc);
 }
 
-hipError_t launch_lattice_schur(const LatArgs& a, int mode, bool fp32, hipStream_t s)
+hipError_t launch_sigma_all(const LatArgs& a, hipStream_t s)
 {
-    // the expanded Sigma table lives right after the caller's partial buffers
-    cd* sig_all = a.pSigAll;
     const int m2 = a.nc * a.nc;
-    hipLaunchKernelGGL(k_sigma_all, dim3((a.nw_tot * m2 + 255) / 256), dim3(256), 0, s, a.sigma, a.mom, a.nc,
-                       a.n_freq, a.nw_tot, a.beta, sig_all);
-    hipError_t e = hipGetLastError();
+    hipLaunchKernelGGL(k_sigma_all, dim3((a.nw_tot * m2 + 255) / 256), dim3(256), 0, s, a.sigma, a.mom, a.nc, a.n_freq,
+                       a.nw_tot, a.beta, a.pSigAll);
+    return hipGetLastError();
+}
+
+hipError_t launch_lattice_schur(const LatArgs& a, int mode, bool fp32, hipStream_t s, int wg)
+{
+    hipError_t e = launch_sigma_all(a, s);
     if (e != hipSuccess) return e;
-    if (mode == MODE_FWD) {
-        if (a.want_trace)
-            return fp32 ? dispatch_nc<MODE_FWD, true, true>(a, sig_all, s) : dispatch_nc<MODE_FWD, false, true>(a, sig_all, s);
-        return fp32 ? dispatch_nc<MODE_FWD, true, false>(a, sig_all, s) : dispatch_nc<MODE_FWD, false, false>(a, sig_all, s);
-    }
-    if (mode == MODE_DENS)
-        return fp32 ? dispatch_nc<MODE_DENS, true, false>(a, sig_all, s) : dispatch_nc<MODE_DENS, false, false>(a, sig_all, s);
-    return fp32 ? dispatch_nc<MODE_VJP, true, false>(a, sig_all, s) : dispatch_nc<MODE_VJP, false, false>(a, sig_all, s);
+    return dispatch_all(a, mode, fp32, a.pSigAll, s, wg, nullptr);
+}
+
+hipError_t launch_lattice_schur_multi(const LatArgs& proto, int mode, bool fp32, hipStream_t s, int wg,
+                                      const SchurMulti& m)
+{
+    return dispatch_all(proto, mode, fp32, nullptr, s, wg, &m);
+}
+
+size_t schur_lds_bytes(int nc, int nd, int mode, bool fp32, int wg)
+{
+    const bool acc_lds = (mode == MODE_VJP) || (mode == MODE_FWD && fp32);
+    const int np = nc * (nc + 1) / 2;
+    return sizeof(cd) * (2 * (size_t)(nc * nc + nd * np + (nd + 1) / 2) + (acc_lds ? (size_t)nc * nc * wg : 0));
 }
 
 // W(k) = H_cd(k) V_d(k): W_ad = sum_e H(a, nc+e) V_d(e, d)

@@ -101,6 +101,16 @@ class Handle:
                                              C.byref(patched)))
         return from_dev_cm(out), patched.value
 
+    def fourier_transform_tau(self, fw, moments, beta, taus):
+        """FourierTransform_tau (FourierTransform.cpp:16-118) at the given tau values."""
+        f, m = to_dev_cm(fw), to_dev_cm(moments)
+        nf, d = f.shape[0], f.shape[1]
+        taus = np.ascontiguousarray(np.asarray(taus, dtype=np.float64))
+        out = torch.zeros((taus.size, d, d), dtype=torch.complex128, device=f.device)
+        self.check(self.lib.dmftgpu_ft_w2tau_points(self.h, d, nf, beta, _ptr(f), _ptr(m), taus.size,
+                                                    taus.ctypes.data_as(C.POINTER(C.c_double)), _ptr(out)))
+        return from_dev_cm(out)
+
     def ft_t_to_w(self, gt, beta, n_freq):
         """FT_t_to_w (FourierTransform.cpp:521-561)."""
         g = to_dev_cm(gt)
@@ -200,3 +210,86 @@ class LatticeGF(Handle):
         g = to_dev_cm(gbar)
         sb, mb = self.vjp_dev(s, mu, g, nbar, method)
         return from_dev_cm(sb), mb
+
+
+class Batch:
+    """Many lattices in one replayable step (dmftgpu_batch_*).
+
+    systems: list of dicts with Hk, sigma, moments (math layout), beta, mu,
+    n_cluster (optional) and, for the loss + gradient step, gloc_target and
+    n_target.  Outputs after run(): gloc, num_matrix, n_total and (with
+    targets) sigma_bar, mu_bar per system."""
+
+    def __init__(self, handle, systems, method=SCHUR_F64, tune=True, graph=True, prepare=False):
+        self.handle = handle
+        self.keep = []
+        descs = []
+        self.out = []
+        for sd in systems:
+            Hk = to_dev_cm(sd["Hk"])
+            sig = to_dev_cm(sd["sigma"])
+            mom = to_dev_cm(sd["moments"])
+            nk, n = Hk.shape[0], Hk.shape[1]
+            nf, nc = sig.shape[0], sig.shape[1]
+            o = dict(gloc=torch.zeros((nf, nc, nc), dtype=torch.complex128, device="cuda"),
+                     num=torch.zeros((nc, nc), dtype=torch.complex128, device="cuda"),
+                     N=torch.zeros(1, dtype=torch.float64, device="cuda"))
+            tgt = sd.get("gloc_target")
+            if tgt is not None:
+                o["tgt"] = to_dev_cm(tgt)
+                o["sb"] = torch.zeros_like(sig)
+                o["mb"] = torch.zeros(1, dtype=torch.float64, device="cuda")
+            lat = capi.Lattice(n, nc, sd.get("n_cluster", 1), nk, nk, nf, 2, sd["beta"])
+            d = capi.System(lat, Hk.data_ptr(), sig.data_ptr(), mom.data_ptr(), sd["mu"], o["gloc"].data_ptr(),
+                            o["num"].data_ptr(), o["N"].data_ptr(),
+                            o["tgt"].data_ptr() if tgt is not None else None, sd.get("n_target", 0.0),
+                            o["sb"].data_ptr() if tgt is not None else None,
+                            o["mb"].data_ptr() if tgt is not None else None)
+            descs.append(d)
+            self.keep += [Hk, sig, mom]
+            self.out.append(o)
+        arr = (capi.System * len(descs))(*descs)
+        flags = (capi.BATCH_TUNE if tune else 0) | (capi.BATCH_GRAPH if graph else 0) | \
+            (capi.BATCH_PREPARE if prepare else 0)
+        self.b = C.c_void_p()
+        handle.check(handle.lib.dmftgpu_batch_create(handle.h, len(descs), arr, method, flags, C.byref(self.b)))
+
+    def run(self):
+        self.handle.check(self.handle.lib.dmftgpu_batch_run(self.b))
+
+    def results(self):
+        torch.cuda.synchronize()
+        res = []
+        for o in self.out:
+            r = dict(gloc=from_dev_cm(o["gloc"]), num_matrix=from_dev_cm(o["num"]), n_total=float(o["N"].item()))
+            if "sb" in o:
+                r["sigma_bar"] = from_dev_cm(o["sb"])
+                r["mu_bar"] = float(o["mb"].item())
+            res.append(r)
+        return res
+
+    def tuning(self):
+        out = []
+        g = 0
+        while True:
+            nc, ns = C.c_int(0), C.c_int(0)
+            wg = (C.c_int * 2)()
+            tg = (C.c_double * 2)()
+            ms = (C.c_double * 2)()
+            rc = self.handle.lib.dmftgpu_batch_tuning(self.b, g, C.byref(nc), C.byref(ns), wg, tg, ms)
+            if rc != 0:
+                break
+            out.append(dict(n_corr=nc.value, systems=ns.value, wg=list(wg), target=list(tg), ms=list(ms)))
+            g += 1
+        return out
+
+    def close(self):
+        if self.b:
+            self.handle.lib.dmftgpu_batch_destroy(self.b)
+            self.b = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -70,8 +70,9 @@ const char* dmftgpu_last_error(dmftgpu_handle h);
 const char* dmftgpu_version(void);
 
 /* Tuning options (no reference counterpart). */
-#define DMFTGPU_OPT_REFINE_STEPS 1      /* FP64-residual refinement steps of SCHUR_F32R (0..4, default 2) */
+#define DMFTGPU_OPT_REFINE_STEPS 1      /* FP64-residual refinement steps of SCHUR_F32R (1 or 2, default 2) */
 #define DMFTGPU_OPT_TARGET_WORKGROUPS 2 /* workgroups per lattice launch, sets the k chunking (default 2048) */
+#define DMFTGPU_OPT_SCHUR_WG 3          /* frequencies per workgroup of the Schur kernels (64 or 128) */
 int dmftgpu_set_option(dmftgpu_handle h, int option, int value);
 
 /* Instrumentation (no reference counterpart): with timing enabled, HIP events
@@ -127,6 +128,41 @@ int dmftgpu_adjust_mu(dmftgpu_handle h, int method, const double* sigma, double 
 int dmftgpu_gloc_vjp(dmftgpu_handle h, int method, const double* sigma, double mu, const double* gloc_bar,
                      double n_bar, double* sigma_bar, double* mu_bar);
 
+/* ---- batched lattices ----------------------------------------------------------
+ * No reference counterpart: DMFTpack handles one lattice per call.  Many
+ * independent lattices (uneven n, n_corr <= 6, n_k) run as one replayable
+ * step: per system GreenFtn_w (G_loc, NumMatrix, N) and, when a target is
+ * given, the VJP of L = sum_w ||G_loc - gloc_target||^2 + (N - n_target)^2.
+ * All pointers in dmftgpu_system are DEVICE pointers; n_total and mu_bar
+ * point to one double each.  The batch owns a stream (default priority) and
+ * orders itself after / before the caller's handle stream. */
+typedef struct dmftgpu_system {
+    dmftgpu_lattice lat;
+    const double* Hk;          /* [n_k][n][n] */
+    const double* sigma;       /* [n_freq][nc][nc] */
+    const double* moments;     /* [4][nc][nc] */
+    double mu;
+    double* gloc;              /* out [n_freq][nc][nc] */
+    double* num_matrix;        /* out [nc][nc] */
+    double* n_total;           /* out, 1 double */
+    const double* gloc_target; /* optional [n_freq][nc][nc]: enables the VJP */
+    double n_target;
+    double* sigma_bar;         /* out [n_freq][nc][nc] (with a target) */
+    double* mu_bar;            /* out, 1 double (with a target) */
+} dmftgpu_system;
+
+typedef struct dmftgpu_batch_s* dmftgpu_batch;
+#define DMFTGPU_BATCH_TUNE 1    /* autotune workgroup width and k chunking per n_corr group */
+#define DMFTGPU_BATCH_GRAPH 2   /* capture the step into a hipGraph replayed by dmftgpu_batch_run */
+#define DMFTGPU_BATCH_PREPARE 4 /* recompute the static eigen-decompositions in every step */
+int dmftgpu_batch_create(dmftgpu_handle h, int count, const dmftgpu_system* systems, int method, int flags,
+                         dmftgpu_batch* out);
+int dmftgpu_batch_run(dmftgpu_batch b);
+/* Tuning result of n_corr group `group` (returns the number of groups when out of range):
+ * wg[2], target[2], ms[2] per mode (forward, adjoint). */
+int dmftgpu_batch_tuning(dmftgpu_batch b, int group, int* nc, int* systems, int* wg, double* target, double* ms);
+int dmftgpu_batch_destroy(dmftgpu_batch b);
+
 /* ---- Matsubara tails and Fourier transforms ---------------------------------- */
 
 /* getAsymto_moments (FourierTransform.cpp:184-465): fw [n_freq][d][d] ->
@@ -141,6 +177,12 @@ int dmftgpu_moments(dmftgpu_handle h, int d, int n_freq, double beta, const doub
  *   gtau [n_tau+1][d][d]; patched HOST (optional) = the reference's CHECK flag. */
 int dmftgpu_ft_w2tau(dmftgpu_handle h, int d, int n_freq, int n_tau, double beta, const double* fw,
                      const double* moments, double* gtau, int* patched);
+
+/* FourierTransform_tau (FourierTransform.cpp:16-118) at n_points arbitrary
+ * tau values (taus_host, HOST); out [n_points][d][d].  No causality patch,
+ * as in the single-tau FourierTransform overloads (:494-512). */
+int dmftgpu_ft_w2tau_points(dmftgpu_handle h, int d, int n_freq, double beta, const double* fw,
+                            const double* moments, int n_points, const double* taus_host, double* out);
 
 /* FT_t_to_w (FourierTransform.cpp:521-561): exact transform of the piecewise
  * linear interpolant of G(tau).  violations HOST (optional) counts frequencies

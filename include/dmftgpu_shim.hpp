@@ -1,0 +1,325 @@
+// dmftgpu_shim.hpp -- drop-in C++ adapter keeping DMFTpack's call surface.
+//
+// Header-only, templated on the matrix type so it compiles against Eigen's
+// MatrixXcd inside DMFTpack and against any type with the same small API
+// (rows(), cols(), data() column-major complex<double>, setZero(r, c),
+// operator()(i, j)).  The self-energy container needs getMatrix(n) (the
+// ImgFreqFtn accessor, ImgTimeFtn.h:56).
+//
+// The reference reads its run parameters from globals (tight_common.h:61-79,
+// TB.h:24, model.h:4); here they travel in dmftgpu_shim::Globals, which the
+// integrator fills from those globals once per call site.  Every function
+// gathers host data into device buffers, calls the C ABI of dmftgpu.h, and
+// copies the results back in the reference's layout.  A non-zero status is
+// turned into the reference's failure behaviour: message + abort (the
+// reference asserts, GreenFtn.cpp:85, FourierTransform.cpp:321,440,557).
+//
+// MPI: with k split over ranks (knum < knum_mpiGlobal) the outputs are this
+// rank's partial sums, exactly the operands of the reference's MPI_Allreduce
+// calls (GreenFtn.cpp:356,368-369,633); Globals::allreduce, when set, is
+// applied to them (in the same places) before they are returned.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <complex>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <vector>
+
+#include "dmftgpu.h"
+
+namespace dmftgpu_shim {
+
+using cplx = std::complex<double>;
+
+struct Globals {
+    int N_freq = 0;                   // tight_common.h:67
+    int N_tau = 0;                    // tight_common.h:67
+    double beta = 0.0;                // tight_common.h:64
+    int knum = 0;                     // TB.h:24 (this rank's k points)
+    int knum_mpiGlobal = 0;           // TB.h:24
+    int highFreq2 = 2;                // GreenFtn.cpp:8
+    double NumberOfElectron = 0.0;    // tight_common.h:64
+    int method = DMFTGPU_PLAIN_F64;   // per-(k, iw) solver
+    double mu_tolerance = 1e-5;       // |dmu| stop of GreenFtn_w_adjust_mu (GreenFtn.cpp:96)
+    // optional cross-rank SUM of a complex128 buffer (identity when unset)
+    std::function<void(cplx*, size_t)> allreduce;
+};
+
+inline void die(dmftgpu_handle h, int rc, const char* where)
+{
+    if (rc == DMFTGPU_OK) return;
+    std::fprintf(stderr, "[dmftgpu] %s failed (%d): %s\n", where, rc, dmftgpu_last_error(h));
+    std::abort();
+}
+
+inline void hip_ok(hipError_t e, const char* where)
+{
+    if (e == hipSuccess) return;
+    std::fprintf(stderr, "[dmftgpu] %s: %s\n", where, hipGetErrorString(e));
+    std::abort();
+}
+
+// A device buffer of complex128 that owns its memory.
+struct DevBuf {
+    cplx* p = nullptr;
+    size_t n = 0;
+    explicit DevBuf(size_t count) : n(count) { hip_ok(hipMalloc((void**)&p, sizeof(cplx) * (count ? count : 1)), "hipMalloc"); }
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    void upload(const cplx* h) { hip_ok(hipMemcpy(p, h, sizeof(cplx) * n, hipMemcpyHostToDevice), "upload"); }
+    void download(cplx* h) const { hip_ok(hipMemcpy(h, p, sizeof(cplx) * n, hipMemcpyDeviceToHost), "download"); }
+};
+
+template <class Matrix>
+std::vector<cplx> pack(const std::vector<Matrix>& mats, size_t count, int rows)
+{
+    std::vector<cplx> out(count * rows * rows);
+    for (size_t m = 0; m < count; ++m) std::memcpy(&out[m * rows * rows], mats[m].data(), sizeof(cplx) * rows * rows);
+    return out;
+}
+
+template <class SelfEnergy>
+std::vector<cplx> pack_sigma(SelfEnergy& S, int n_freq, int nc)
+{
+    std::vector<cplx> out((size_t)n_freq * nc * nc);
+    for (int n = 0; n < n_freq; ++n) {
+        auto M = S.getMatrix(n);
+        std::memcpy(&out[(size_t)n * nc * nc], M.data(), sizeof(cplx) * nc * nc);
+    }
+    return out;
+}
+
+// Keeps only the intra-cluster blocks (valenceHAMILTONIAN, GreenFtn.cpp:32-39).
+inline void cluster_mask(cplx* m, int nc, int ncl, size_t count)
+{
+    const int npc = nc / ncl;
+    for (size_t c = 0; c < count; ++c)
+        for (int j = 0; j < nc; ++j)
+            for (int i = 0; i < nc; ++i)
+                if (i / npc != j / npc) m[c * nc * nc + i + (size_t)j * nc] = 0.0;
+}
+
+inline dmftgpu_lattice lattice_of(const Globals& g, int n, int nc, int ncl)
+{
+    dmftgpu_lattice L;
+    L.n = n;
+    L.n_corr = nc;
+    L.n_cluster = ncl;
+    L.n_k = g.knum;
+    L.n_k_global = g.knum_mpiGlobal;
+    L.n_freq = g.N_freq;
+    L.high_freq = g.highFreq2;
+    L.beta = g.beta;
+    return L;
+}
+
+// getAsymto_moments (FourierTransform.cpp:184-465) on a device array.
+inline std::vector<cplx> moments_dev(dmftgpu_handle h, int d, int n_freq, double beta, const DevBuf& fw, bool green)
+{
+    DevBuf mom(4 * (size_t)d * d);
+    die(h, dmftgpu_moments(h, d, n_freq, beta, (const double*)fw.p, green ? 1 : 0, (double*)mom.p, nullptr),
+        "getAsymto_moments");
+    std::vector<cplx> out(mom.n);
+    mom.download(out.data());
+    return out;
+}
+
+// void GreenFtn_w(int NumCluster, int NumHartrOrbit_per_cluster, vector<MatrixXcd>& H_k,
+//                 ImgFreqFtn& SelfE_w, vector<MatrixXcd>& Gw, double mu, vector<MatrixXcd>& densityMatDFT)
+// (GreenFtn.cpp:142).  NumMatrix (the global written at :356) is returned through num_matrix.
+// densityMatDFT is filled when it is non-empty on entry (the caller pre-sizes it, TB.cpp:296-297).
+template <class Matrix, class SelfEnergy>
+void GreenFtn_w(dmftgpu_handle h, const Globals& g, int NumCluster, int NumHartrOrbit_per_cluster,
+                std::vector<Matrix>& H_k, SelfEnergy& SelfE_w, std::vector<Matrix>& Gw, double mu,
+                std::vector<Matrix>& densityMatDFT, Matrix& num_matrix)
+{
+    const int nc = NumCluster * NumHartrOrbit_per_cluster;
+    const int n = (int)H_k[0].rows();
+    const dmftgpu_lattice L = lattice_of(g, n, nc, NumCluster);
+    DevBuf hk((size_t)g.knum * n * n), sig((size_t)g.N_freq * nc * nc), gl((size_t)g.N_freq * nc * nc), nm((size_t)nc * nc);
+    hk.upload(pack(H_k, g.knum, n).data());
+    sig.upload(pack_sigma(SelfE_w, g.N_freq, nc).data());
+    std::vector<cplx> mom = moments_dev(h, nc, g.N_freq, g.beta, sig, false);  // :189-195
+    DevBuf dmom(mom.size());
+    dmom.upload(mom.data());
+    die(h, dmftgpu_prepare(h, &L, (const double*)hk.p, (const double*)dmom.p), "prepare");
+    die(h, dmftgpu_greenftn_w(h, g.method, (const double*)sig.p, mu, (double*)gl.p, nullptr, (double*)nm.p, nullptr),
+        "GreenFtn_w");
+    std::vector<cplx> hgl(gl.n), hnm(nm.n);
+    gl.download(hgl.data());
+    nm.download(hnm.data());
+    if (g.allreduce) {  // :356, :368
+        g.allreduce(hnm.data(), hnm.size());
+        g.allreduce(hgl.data(), hgl.size());
+    }
+    Gw.resize(g.N_freq);
+    for (int w = 0; w < g.N_freq; ++w) {
+        Gw[w].setZero(nc, nc);
+        std::memcpy(Gw[w].data(), &hgl[(size_t)w * nc * nc], sizeof(cplx) * nc * nc);
+    }
+    num_matrix.setZero(nc, nc);
+    std::memcpy(num_matrix.data(), hnm.data(), sizeof(cplx) * nc * nc);
+    if (!densityMatDFT.empty()) {  // :291-319
+        DevBuf dm((size_t)g.knum * n * n);
+        die(h, dmftgpu_density_matrix(h, (const double*)sig.p, mu, (double*)dm.p), "densityMatDFT");
+        std::vector<cplx> hdm(dm.n);
+        dm.download(hdm.data());
+        for (int k = 0; k < g.knum; ++k) {
+            densityMatDFT[k].setZero(n, n);
+            std::memcpy(densityMatDFT[k].data(), &hdm[(size_t)k * n * n], sizeof(cplx) * n * n);
+        }
+    }
+}
+
+// double GreenFtn_w_adjust_mu(vector<MatrixXcd>& H_k, ImgFreqFtn& SelfE_w, double mu, double dmu, int mu_adjust)
+// (GreenFtn.cpp:66).  Sigma and m0 enter through their intra-cluster blocks, as in
+// valenceHAMILTONIAN.  Newton on the analytic dN/dmu replaces the bracketing of :93-121.
+template <class Matrix, class SelfEnergy>
+double GreenFtn_w_adjust_mu(dmftgpu_handle h, const Globals& g, int NumCluster, int NumHartrOrbit_per_cluster,
+                            std::vector<Matrix>& H_k, SelfEnergy& SelfE_w, double mu, double /*dmu*/,
+                            int /*mu_adjust*/)
+{
+    const int nc = NumCluster * NumHartrOrbit_per_cluster;
+    const int n = (int)H_k[0].rows();
+    dmftgpu_lattice L = lattice_of(g, n, nc, NumCluster);
+    DevBuf hk((size_t)g.knum * n * n), sig((size_t)g.N_freq * nc * nc);
+    hk.upload(pack(H_k, g.knum, n).data());
+    std::vector<cplx> hs = pack_sigma(SelfE_w, g.N_freq, nc);
+    sig.upload(hs.data());
+    std::vector<cplx> mom = moments_dev(h, nc, g.N_freq, g.beta, sig, false);  // GreenFtn.cpp:523
+    cluster_mask(hs.data(), nc, NumCluster, g.N_freq);
+    cluster_mask(mom.data(), nc, NumCluster, 4);
+    sig.upload(hs.data());
+    DevBuf dmom(mom.size());
+    dmom.upload(mom.data());
+    die(h, dmftgpu_prepare(h, &L, (const double*)hk.p, (const double*)dmom.p), "prepare");
+    double mu_out = mu;
+    int ne = 0;
+    die(h, dmftgpu_adjust_mu(h, g.method, (const double*)sig.p, mu, g.NumberOfElectron, g.mu_tolerance, &mu_out, &ne),
+        "GreenFtn_w_adjust_mu");
+    return mu_out;
+}
+
+// void getAsymto_moments(vector<MatrixXcd>& moments, MatrixXcd* delta_w, bool greenFtnAsym) (tight_common.h:141)
+template <class Matrix>
+void getAsymto_moments(dmftgpu_handle h, const Globals& g, std::vector<Matrix>& moments, Matrix* delta_w,
+                       bool greenFtnAsym = false)
+{
+    const int d = (int)delta_w[0].rows();
+    DevBuf fw((size_t)g.N_freq * d * d);
+    std::vector<cplx> hf((size_t)g.N_freq * d * d);
+    for (int w = 0; w < g.N_freq; ++w) std::memcpy(&hf[(size_t)w * d * d], delta_w[w].data(), sizeof(cplx) * d * d);
+    fw.upload(hf.data());
+    std::vector<cplx> m = moments_dev(h, d, g.N_freq, g.beta, fw, greenFtnAsym);
+    moments.resize(4);
+    for (int q = 0; q < 4; ++q) {
+        moments[q].setZero(d, d);
+        std::memcpy(moments[q].data(), &m[(size_t)q * d * d], sizeof(cplx) * d * d);
+    }
+}
+
+namespace detail {
+template <class Matrix>
+void ft_grid(dmftgpu_handle h, const Globals& g, Matrix* delta_w, Matrix* delta_t, const Matrix* moments1, bool green)
+{
+    const int d = (int)delta_w[0].rows();
+    DevBuf fw((size_t)g.N_freq * d * d), gt((size_t)(g.N_tau + 1) * d * d);
+    std::vector<cplx> hf((size_t)g.N_freq * d * d);
+    for (int w = 0; w < g.N_freq; ++w) std::memcpy(&hf[(size_t)w * d * d], delta_w[w].data(), sizeof(cplx) * d * d);
+    fw.upload(hf.data());
+    std::vector<cplx> m = moments_dev(h, d, g.N_freq, g.beta, fw, green);
+    if (moments1) {  // FourierTransform.cpp:485-486
+        std::fill(m.begin(), m.begin() + (size_t)d * d, cplx(0.0));
+        std::memcpy(&m[(size_t)d * d], moments1->data(), sizeof(cplx) * d * d);
+    }
+    DevBuf dm(m.size());
+    dm.upload(m.data());
+    int patched = 0;
+    die(h, dmftgpu_ft_w2tau(h, d, g.N_freq, g.N_tau, g.beta, (const double*)fw.p, (const double*)dm.p,
+                            (double*)gt.p, &patched), "FourierTransform");
+    std::vector<cplx> ht(gt.n);
+    gt.download(ht.data());
+    for (int t = 0; t <= g.N_tau; ++t) {
+        delta_t[t].setZero(d, d);
+        std::memcpy(delta_t[t].data(), &ht[(size_t)t * d * d], sizeof(cplx) * d * d);
+    }
+    if (patched) std::fprintf(stderr, " Warning : positive Gt\n");  // :175
+}
+
+template <class Matrix>
+void ft_point(dmftgpu_handle h, const Globals& g, Matrix* delta_w, Matrix& delta_t, double tau, const Matrix* moments1,
+              bool green)
+{
+    const int d = (int)delta_w[0].rows();
+    DevBuf fw((size_t)g.N_freq * d * d), out((size_t)d * d);
+    std::vector<cplx> hf((size_t)g.N_freq * d * d);
+    for (int w = 0; w < g.N_freq; ++w) std::memcpy(&hf[(size_t)w * d * d], delta_w[w].data(), sizeof(cplx) * d * d);
+    fw.upload(hf.data());
+    std::vector<cplx> m = moments_dev(h, d, g.N_freq, g.beta, fw, green);
+    if (moments1) {
+        std::fill(m.begin(), m.begin() + (size_t)d * d, cplx(0.0));
+        std::memcpy(&m[(size_t)d * d], moments1->data(), sizeof(cplx) * d * d);
+    }
+    DevBuf dm(m.size());
+    dm.upload(m.data());
+    die(h, dmftgpu_ft_w2tau_points(h, d, g.N_freq, g.beta, (const double*)fw.p, (const double*)dm.p, 1, &tau,
+                                   (double*)out.p), "FourierTransform(tau)");
+    delta_t.setZero(d, d);
+    out.download((cplx*)delta_t.data());
+}
+}  // namespace detail
+
+// The four FourierTransform overloads (tight_common.h:142-145, FourierTransform.cpp:470-512).
+template <class Matrix>
+void FourierTransform(dmftgpu_handle h, const Globals& g, Matrix* delta_w, Matrix* delta_t, bool greenFtnAsym = false)
+{
+    detail::ft_grid(h, g, delta_w, delta_t, (const Matrix*)nullptr, greenFtnAsym);
+}
+template <class Matrix>
+void FourierTransform(dmftgpu_handle h, const Globals& g, Matrix* delta_w, Matrix* delta_t, const Matrix& moments1,
+                      bool greenFtnAsym = false)
+{
+    detail::ft_grid(h, g, delta_w, delta_t, &moments1, greenFtnAsym);
+}
+template <class Matrix>
+void FourierTransform(dmftgpu_handle h, const Globals& g, Matrix* delta_w, Matrix& delta_t, double tau,
+                      bool greenFtnAsym = false)
+{
+    detail::ft_point(h, g, delta_w, delta_t, tau, (const Matrix*)nullptr, greenFtnAsym);
+}
+template <class Matrix>
+void FourierTransform(dmftgpu_handle h, const Globals& g, Matrix* delta_w, Matrix& delta_t, double tau,
+                      const Matrix& moments1, bool greenFtnAsym = false)
+{
+    detail::ft_point(h, g, delta_w, delta_t, tau, &moments1, greenFtnAsym);
+}
+
+// void FT_t_to_w(MatrixXcd* delta_w, MatrixXcd* delta_t, int N_freq) (tight_common.h:146)
+template <class Matrix>
+void FT_t_to_w(dmftgpu_handle h, const Globals& g, Matrix* delta_w, Matrix* delta_t, int N_freq)
+{
+    const int d = (int)delta_t[0].rows();
+    DevBuf gt((size_t)(g.N_tau + 1) * d * d), fw((size_t)N_freq * d * d);
+    std::vector<cplx> ht(gt.n);
+    for (int t = 0; t <= g.N_tau; ++t) std::memcpy(&ht[(size_t)t * d * d], delta_t[t].data(), sizeof(cplx) * d * d);
+    gt.upload(ht.data());
+    int viol = 0;
+    die(h, dmftgpu_ft_tau2w(h, d, N_freq, g.N_tau, g.beta, (const double*)gt.p, (double*)fw.p, &viol), "FT_t_to_w");
+    if (viol) {  // the reference asserts here (:557)
+        std::fprintf(stderr, "[dmftgpu] FT_t_to_w: %d frequencies violate causality\n", viol);
+        std::abort();
+    }
+    std::vector<cplx> hf(fw.n);
+    fw.download(hf.data());
+    for (int n = 0; n < N_freq; ++n) {
+        delta_w[n].setZero(d, d);
+        std::memcpy(delta_w[n].data(), &hf[(size_t)n * d * d], sizeof(cplx) * d * d);
+    }
+}
+
+}  // namespace dmftgpu_shim

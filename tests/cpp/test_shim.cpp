@@ -1,0 +1,151 @@
+// Drives include/dmftgpu_shim.hpp the way DMFTpack would (reference call
+// surface, a column-major matrix type standing in for Eigen::MatrixXcd) and
+// checks the results against the CPU oracle linked in as the checker.
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <vector>
+
+#include "dmftgpu.h"
+#include "dmftgpu_shim.hpp"
+
+using cplx = std::complex<double>;
+
+struct Mat {  // the subset of Eigen::MatrixXcd the adapter uses
+    int r = 0, c = 0;
+    std::vector<cplx> v;
+    int rows() const { return r; }
+    int cols() const { return c; }
+    cplx* data() { return v.data(); }
+    const cplx* data() const { return v.data(); }
+    void setZero(int rr, int cc) { r = rr; c = cc; v.assign((size_t)rr * cc, cplx(0.0)); }
+    cplx& operator()(int i, int j) { return v[i + (size_t)j * r]; }
+    const cplx& operator()(int i, int j) const { return v[i + (size_t)j * r]; }
+};
+
+struct Sigma {  // the subset of ImgFreqFtn the adapter uses
+    std::vector<Mat> w;
+    Mat getMatrix(int n) { return w[n]; }
+};
+
+extern "C" {
+struct orc_lattice { int n, n_corr, n_cluster, n_k, n_k_global, n_freq, high_freq; double beta; };
+int orc_greenftn_w(const orc_lattice*, const cplx*, const cplx*, const cplx*, double, cplx*, cplx*, cplx*, cplx*, int);
+int orc_asymto_moments(int, int, double, const cplx*, int, cplx*, int*);
+int orc_ft_w2tau(int, int, int, double, const cplx*, const cplx*, cplx*, int*, int);
+int orc_ft_tau2w(int, int, int, double, const cplx*, cplx*, int*);
+}
+
+static unsigned long long rng_state = 12345;
+static double urand()
+{
+    rng_state = rng_state * 6364136223846793005ULL + 1442695040888963407ULL;
+    return ((rng_state >> 11) * (1.0 / 9007199254740992.0)) - 0.5;
+}
+
+static int failures = 0;
+static void check(const char* what, double err, double tol)
+{
+    std::printf("%-28s err %.3e (tol %.0e) %s\n", what, err, tol, err <= tol ? "ok" : "FAIL");
+    if (!(err <= tol)) ++failures;
+}
+
+static double relerr(const cplx* a, const cplx* b, size_t n)
+{
+    double num = 0, den = 0;
+    for (size_t i = 0; i < n; ++i) {
+        num = std::fmax(num, std::abs(a[i] - b[i]));
+        den = std::fmax(den, std::abs(b[i]));
+    }
+    return num / (den > 0 ? den : 1.0);
+}
+
+int main()
+{
+    const int n = 6, ncl = 2, npc = 2, nc = ncl * npc, nk = 8, nf = 48, ntau = 64;
+    const double beta = 10.0, mu = 0.15, PI = 3.1415926535897932384626;
+    dmftgpu_shim::Globals g;
+    g.N_freq = nf;
+    g.N_tau = ntau;
+    g.beta = beta;
+    g.knum = nk;
+    g.knum_mpiGlobal = nk;
+    g.NumberOfElectron = 2.5;
+    // H(k): random Hermitian; Sigma: causal poles, block-diagonal per cluster
+    std::vector<Mat> Hk(nk);
+    for (auto& H : Hk) {
+        H.setZero(n, n);
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i <= j; ++i) {
+                cplx v(urand(), i == j ? 0.0 : urand());
+                H(i, j) = v;
+                H(j, i) = std::conj(v);
+            }
+    }
+    Sigma S;
+    S.w.resize(nf);
+    std::vector<double> amp(nc), pole(nc);
+    for (int a = 0; a < nc; ++a) { amp[a] = 0.5 + urand(); pole[a] = 2 * urand(); }
+    for (int w = 0; w < nf; ++w) {
+        S.w[w].setZero(nc, nc);
+        cplx z(0.0, PI * (2 * w + 1) / beta);
+        for (int a = 0; a < nc; ++a) S.w[w](a, a) = 0.2 + amp[a] / (z - pole[a]);
+        S.w[w](0, 1) = S.w[w](1, 0) = 0.1 / (z - 0.3);
+    }
+    dmftgpu_handle h;
+    if (dmftgpu_create(&h, nullptr) != DMFTGPU_OK) { std::printf("no device\n"); return 2; }
+
+    std::vector<Mat> Gw, dens(nk);
+    Mat num;
+    dmftgpu_shim::GreenFtn_w(h, g, ncl, npc, Hk, S, Gw, mu, dens, num);
+
+    // oracle with the moments the adapter computed
+    std::vector<Mat> mom;
+    dmftgpu_shim::getAsymto_moments(h, g, mom, S.w.data(), false);
+    std::vector<cplx> hk((size_t)nk * n * n), sg((size_t)nf * nc * nc), mm(4 * (size_t)nc * nc);
+    for (int k = 0; k < nk; ++k) std::copy(Hk[k].v.begin(), Hk[k].v.end(), hk.begin() + (size_t)k * n * n);
+    for (int w = 0; w < nf; ++w) std::copy(S.w[w].v.begin(), S.w[w].v.end(), sg.begin() + (size_t)w * nc * nc);
+    for (int q = 0; q < 4; ++q) std::copy(mom[q].v.begin(), mom[q].v.end(), mm.begin() + (size_t)q * nc * nc);
+    orc_lattice L{n, nc, ncl, nk, nk, nf, 2, beta};
+    std::vector<cplx> gl((size_t)nf * nc * nc), ghf(gl.size()), nm((size_t)nc * nc), dm((size_t)nk * n * n);
+    orc_greenftn_w(&L, hk.data(), sg.data(), mm.data(), mu, gl.data(), ghf.data(), nm.data(), dm.data(), 1);
+    std::vector<cplx> got((size_t)nf * nc * nc), gotd((size_t)nk * n * n);
+    for (int w = 0; w < nf; ++w) std::copy(Gw[w].v.begin(), Gw[w].v.end(), got.begin() + (size_t)w * nc * nc);
+    for (int k = 0; k < nk; ++k) std::copy(dens[k].v.begin(), dens[k].v.end(), gotd.begin() + (size_t)k * n * n);
+    check("GreenFtn_w Gw", relerr(got.data(), gl.data(), got.size()), 1e-12);
+    check("GreenFtn_w NumMatrix", relerr(num.v.data(), nm.data(), nm.size()), 1e-12);
+    check("GreenFtn_w densityMatDFT", relerr(gotd.data(), dm.data(), dm.size()), 1e-12);
+    // the moments against the oracle's own fit
+    std::vector<cplx> om(mm.size());
+    orc_asymto_moments(nc, nf, beta, sg.data(), 0, om.data(), nullptr);
+    check("getAsymto_moments", relerr(mm.data(), om.data(), om.size()), 1e-8);
+
+    // mu search: N(mu) = NumberOfElectron
+    double mu_new = dmftgpu_shim::GreenFtn_w_adjust_mu(h, g, ncl, npc, Hk, S, 0.0, 1.0, 1);
+    std::printf("adjusted mu %.10f\n", mu_new);
+    if (!std::isfinite(mu_new)) ++failures;
+
+    // FourierTransform (G-like input) and FT_t_to_w round trip
+    std::vector<Mat> gt(ntau + 1), gw2(nf);
+    dmftgpu_shim::FourierTransform(h, g, Gw.data(), gt.data(), true);
+    std::vector<cplx> ogt((size_t)(ntau + 1) * nc * nc), gm(4 * (size_t)nc * nc);
+    orc_asymto_moments(nc, nf, beta, got.data(), 1, gm.data(), nullptr);
+    int patched = 0;
+    orc_ft_w2tau(nc, nf, ntau, beta, got.data(), gm.data(), ogt.data(), &patched, 1);
+    std::vector<cplx> ggt(ogt.size());
+    for (int t = 0; t <= ntau; ++t) std::copy(gt[t].v.begin(), gt[t].v.end(), ggt.begin() + (size_t)t * nc * nc);
+    check("FourierTransform", relerr(ggt.data(), ogt.data(), ogt.size()), 1e-8);
+    Mat one;
+    dmftgpu_shim::FourierTransform(h, g, Gw.data(), one, 3 * beta / ntau, true);
+    check("FourierTransform(tau)", relerr(one.v.data(), &ggt[(size_t)3 * nc * nc], (size_t)nc * nc), 1e-12);
+    dmftgpu_shim::FT_t_to_w(h, g, gw2.data(), gt.data(), nf);
+    std::vector<cplx> ow((size_t)nf * nc * nc), gw((size_t)nf * nc * nc);
+    int viol = 0;
+    orc_ft_tau2w(nc, nf, ntau, beta, ggt.data(), ow.data(), &viol);
+    for (int w = 0; w < nf; ++w) std::copy(gw2[w].v.begin(), gw2[w].v.end(), gw.begin() + (size_t)w * nc * nc);
+    check("FT_t_to_w", relerr(gw.data(), ow.data(), ow.size()), 1e-12);
+
+    dmftgpu_destroy(h);
+    std::printf(failures ? "SHIM FAILED\n" : "SHIM OK\n");
+    return failures ? 1 : 0;
+}
