@@ -52,6 +52,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
+                    help="after the run, copy MIB MiB device-to-device 3 times (a plain copy kernel of known size "
+                         "that calibrates HBM byte counters)")
     return ap.parse_args()
 
 
@@ -220,6 +223,13 @@ def main():
     out.update(extra)
     print(json.dumps(out), flush=True)
     lat.close()
+    if args.calib_copy > 0:
+        src = torch.ones(args.calib_copy * (1 << 20) // 8, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"calibration copy: {args.calib_copy} MiB read + {args.calib_copy} MiB written per launch", flush=True)
 
 
 if __name__ == "__main__":

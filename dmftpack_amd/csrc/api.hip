@@ -313,7 +313,7 @@ int run_lattice(dmftgpu_handle h, int method, int mode, const double* sigma, dou
     const int nw_tot = L.n_freq * L.high_freq;
     int slots;
     if (method == DMFTGPU_PLAIN_F64) slots = dg::plain_slots(n, mode);
-    else slots = dg::schur_slots(nc, mode);
+    else slots = dg::schur_slots(nc, mode, method == DMFTGPU_SCHUR_F32R);
     Plan P = make_plan(nk, nw_tot, slots, h->target_wg);
     dg::LatArgs a{};
     a.Hk = h->Hk;
@@ -381,7 +381,8 @@ size_t lattice_ws_bytes(dmftgpu_handle h, int method, int mode)
     const dmftgpu_lattice& L = h->lat;
     const int nc = L.n_corr, m2 = nc * nc;
     const size_t nw_tot = (size_t)L.n_freq * L.high_freq;
-    int slots = method == DMFTGPU_PLAIN_F64 ? dg::plain_slots(L.n, mode) : dg::schur_slots(nc, mode);
+    int slots = method == DMFTGPU_PLAIN_F64 ? dg::plain_slots(L.n, mode)
+                                            : dg::schur_slots(nc, mode, method == DMFTGPU_SCHUR_F32R);
     Plan P = make_plan(L.n_k, (int)nw_tot, slots, h->target_wg);
     size_t C = P.n_chunks + 1;  // +1 for the reduced copy
     size_t b = (C + 1) * nw_tot * 16 * (2 + (size_t)m2) + C * L.n_freq * 16 * (m2 + 1) + 64 * 256;

@@ -400,7 +400,8 @@ static int batch_create_impl(dmftgpu_batch_s* b, int count, const dmftgpu_system
         Sys& s = b->sys[i];
         s.d = systems[i];
         const dmftgpu_lattice& L = s.d.lat;
-        if (L.n < 1 || L.n > 32 || L.n_corr < 1 || !dg::schur_supported(L.n, L.n_corr) || L.n_k < 1 || L.n_freq < 1 ||
+        if (L.n < 1 || L.n > 32 || L.n_corr < 1 || !dg::schur_supported(L.n, L.n_corr) ||
+            !dg::schur_multi_supported(L.n_corr) || L.n_k < 1 || L.n_freq < 1 ||
             L.n_cluster < 1 || L.n_corr % L.n_cluster || !(L.beta > 0) || !s.d.Hk || !s.d.sigma || !s.d.moments ||
             !s.d.gloc || !s.d.num_matrix || !s.d.n_total) {
             return fail(h, DMFTGPU_ERR_ARG, "system %d: invalid descriptor or unsupported size (n_corr <= 6)", i);

@@ -55,6 +55,11 @@ __device__ __forceinline__ cf ffms(cf a, cf b, cf acc)
 {
     return cf{fmaf(-a.x, b.x, fmaf(a.y, b.y, acc.x)), fmaf(-a.x, b.y, fmaf(-a.y, b.x, acc.y))};
 }
+// acc + a*b
+__device__ __forceinline__ cf ffma(cf a, cf b, cf acc)
+{
+    return cf{fmaf(a.x, b.x, fmaf(-a.y, b.y, acc.x)), fmaf(a.x, b.y, fmaf(a.y, b.x, acc.y))};
+}
 __device__ __forceinline__ cf frcp(cf a)
 {
     float d = __frcp_rn(fmaf(a.x, a.x, a.y * a.y));

@@ -57,8 +57,16 @@ struct SchurMulti {
 hipError_t launch_lattice_schur_multi(const LatArgs& proto, int mode, bool fp32, hipStream_t s, int wg,
                                       const SchurMulti& m);
 size_t schur_lds_bytes(int nc, int nd, int mode, bool fp32, int wg);
-int schur_slots(int nc, int mode);
+// frequencies per workgroup of the Schur kernel for n_c (grid planning)
+int schur_slots(int nc, int mode, bool fp32);
 bool schur_supported(int n, int nc);
+// the persistent multi-lattice launch covers the one-matrix-per-lane kernels
+bool schur_multi_supported(int nc);
+// column-per-lane Schur kernels for 7 <= n_c <= 16 (lattice_cols{8,10,12,16}.hip)
+hipError_t launch_cols_8(const LatArgs& a, int mode, bool fp32, const cd* sig_all, hipStream_t s);
+hipError_t launch_cols_10(const LatArgs& a, int mode, bool fp32, const cd* sig_all, hipStream_t s);
+hipError_t launch_cols_12(const LatArgs& a, int mode, bool fp32, const cd* sig_all, hipStream_t s);
+hipError_t launch_cols_16(const LatArgs& a, int mode, bool fp32, const cd* sig_all, hipStream_t s);
 
 // W = H_cd V_d per k (Schur preparation)
 hipError_t launch_schur_prep(const cd* Hk, const cd* Vd, int n, int nc, int nk, cd* W, hipStream_t s);

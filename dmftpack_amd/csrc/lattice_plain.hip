@@ -26,43 +26,10 @@
 // Modes: FWD   G_cc and tr G               (G_loc, NumMatrix, N)
 //        DENS  tr G and tr G^2             (N and dN/dmu)
 //        VJP   sigma_bar, mu_bar terms and tr G^2
+#include "gj.hpp"
 #include "lattice_kernels.hpp"
 
 namespace dg {
-
-// Column-per-lane Gauss-Jordan inverse without pivoting (see the header
-// comment): lane c of an L-lane slot holds column c of A in col[]; bc is the
-// slot's LDS broadcast buffer (2*N entries).  On return col[] holds column c
-// of A^{-1}.  Steps j >= n (padding of a bucketed size) are skipped.
-template <int N>
-__device__ __forceinline__ void gj_columns(cd (&col)[N], cd* bc, int c, int n)
-{
-    cd scl = cd{1.0, 0.0};
-#pragma unroll
-    for (int j = 0; j < N; ++j) {
-        if (j < n) {
-            cd* buf = bc + (j & 1) * N;
-            if (c == j) {
-#pragma unroll
-                for (int i = 0; i < N; ++i) buf[i] = col[i];
-            }
-            wave_sync();
-            cd b[N];
-#pragma unroll
-            for (int i = 0; i < N; ++i) b[i] = buf[i];
-            const cd p = crcp(b[j]);
-            const bool isj = (c == j);
-            const cd t = isj ? cd{0.0, 0.0} : cmul(col[j], p);
-#pragma unroll
-            for (int i = 0; i < N; ++i)
-                if (i != j) col[i] = cfms(b[i], t, col[i]);
-            col[j] = isj ? cd{-1.0, 0.0} : t;
-            scl = isj ? cneg(p) : scl;
-        }
-    }
-#pragma unroll
-    for (int i = 0; i < N; ++i) col[i] = cmul(col[i], scl);
-}
 
 template <int N, int L, int MODE, int WG>
 __global__ __launch_bounds__(WG) void k_lattice_plain(LatArgs a)

@@ -134,7 +134,8 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
     const bool vjp_here = (MODE == MODE_VJP) && wvalid && nw < a.n_freq;
     const cd* sig = sig_all + (size_t)nwc * M2;
     const cd* gb = (MODE == MODE_VJP) ? a.gbar + (size_t)(nw < a.n_freq ? nw : 0) * M2 : nullptr;
-    const double cn = 2.0 * a.nbar / a.beta;
+    // nbar: host value, or (batched adjoint) a device scalar written by an earlier kernel
+    const double cn = 2.0 * (a.nbar_dev ? *a.nbar_dev : a.nbar) / a.beta;
 
     cd acc[ACC_LDS ? 1 : NC][ACC_LDS ? 1 : NC];
     if (ACC_LDS) {
@@ -414,8 +415,18 @@ __global__ __launch_bounds__(WG) void k_lattice_schur_multi(const LatArgs* __res
 }
 
 // -------------------------------------------------------------------------
-bool schur_supported(int n, int nc) { return nc >= 1 && nc <= 6 && n >= nc; }
-int schur_slots(int, int) { return 64; }
+// n_c <= 6: one matrix per lane (this file); 7..16: column per lane
+// (lattice_schur_cols.hpp, buckets 8/10/12/16)
+bool schur_supported(int n, int nc) { return nc >= 1 && nc <= 16 && n >= nc && n <= 32; }
+bool schur_multi_supported(int nc) { return nc >= 1 && nc <= 6; }
+static int cols_bucket(int nc) { return nc <= 8 ? 8 : nc <= 10 ? 10 : nc <= 12 ? 12 : 16; }
+int schur_slots(int nc, int mode, bool fp32)
+{
+    if (nc <= 6) return 64;
+    const int N = cols_bucket(nc), prec = fp32 ? 1 : 0;
+    const int wg = (mode == MODE_FWD && prec == 0) ? 256 : 64;
+    return wg / 64 * (64 / N);
+}
 
 template <int NC, int MODE, bool F32R, int WG>
 size_t schur_shm(int nd)
@@ -518,6 +529,14 @@ hipError_t launch_lattice_schur(const LatArgs& a, int mode, bool fp32, hipStream
 {
     hipError_t e = launch_sigma_all(a, s);
     if (e != hipSuccess) return e;
+    if (a.nc > 6) {
+        switch (cols_bucket(a.nc)) {
+            case 8: return launch_cols_8(a, mode, fp32, a.pSigAll, s);
+            case 10: return launch_cols_10(a, mode, fp32, a.pSigAll, s);
+            case 12: return launch_cols_12(a, mode, fp32, a.pSigAll, s);
+            default: return launch_cols_16(a, mode, fp32, a.pSigAll, s);
+        }
+    }
     return dispatch_all(a, mode, fp32, a.pSigAll, s, wg, nullptr);
 }
 

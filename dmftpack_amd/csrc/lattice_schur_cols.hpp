@@ -1,0 +1,386 @@
+// Schur-complement lattice path for larger correlated blocks (7 <= n_c <= 16),
+// column per lane.
+//
+// Same mathematics as lattice_schur.hip (the hot loop of GreenFtn_w,
+// GreenFtn.cpp:218-308, and the densities of GiwToNele :513-635, from the
+// block structure of A = (z+mu) I - H(k) - P Sigma P^H with the correlated
+// orbitals first, downfolding.cpp:69-79):
+//   S     = (z+mu) I - H_cc - Sigma - sum_d g_d w_d w_d^H,  g_d = 1/(z+mu-e_d)
+//   G_cc  = S^{-1}
+//   tr G  = sum_d g_d + tr G_cc + sum_d g_d^2 M_d,        M_d = w_d^H G_cc w_d
+//   tr G^2 = sum_d g_d^2 + tr(Z^2) + 2 sum_d g_d^3 M_d,   Z = G_cc (I + Y),
+//            Y = sum_d g_d^2 w_d w_d^H
+//   (G^2)_cc = Z G_cc
+// An n_c x n_c complex matrix no longer fits one lane's registers, so a slot
+// of N lanes (N = n_c rounded up to 8/10/12/16) owns one (k, iw) matrix and
+// lane c holds column c of S, then of G_cc:
+//   * S column c:  S_ic = E_ic - H_cc,ic - sum_d W_id (g_d conj(W_cd)),
+//     E = (z+mu) I - Sigma(iw) kept in registers across the k loop;
+//     the g_d of the slot's frequency are computed once per slot and shared
+//     through LDS;
+//   * inversion: column-per-lane Gauss-Jordan (gj.hpp); FP32R inverts in FP32
+//     and refines each column against the FP64 S held in LDS,
+//     x_c <- x_c + X0 (e_c - S x_c);
+//   * FWD accumulates column c of G_cc over the k chunk in registers; the
+//     trace terms are per-lane partials summed over the slot at the end;
+//   * DENS/VJP exchange G_cc and Z through per-slot LDS matrices for the
+//     products Z = G + G Y, tr(Z^2), (G^2)_cc = Z G and G^H gbar G^H; lane c
+//     accumulates row c of sigma_bar.
+// The per-k data (H_cc, W = H_cd V_d, e_d) is staged in LDS, double buffered,
+// shared by all slots of the workgroup.
+#pragma once
+#include "gj.hpp"
+#include "lattice_kernels.hpp"
+
+namespace dg {
+
+// FWD FP64 needs no per-slot matrices: 4 wavefronts share each staged k.
+__host__ __device__ constexpr int cols_wg(int mode, int prec) { return (mode == MODE_FWD && prec == 0) ? 256 : 64; }
+
+// per-k staging: [H_cc nc*nc][W nc*nd][e_d nd doubles]
+__host__ __device__ inline int cols_stage(int nc, int nd) { return nc * nc + nc * nd + (nd + 1) / 2; }
+
+// per-slot LDS (cd units): bc 2N (GJ broadcast), gsh 32 (g_d), MA N*N, MB N*N, GB N*N
+__host__ __device__ constexpr int cols_slot_size(int N, int mode, int prec)
+{
+    return 2 * N + 32 + ((prec > 0 || mode != MODE_FWD) ? 2 * N * N : 0) + (mode == MODE_VJP ? N * N : 0);
+}
+
+__host__ __device__ constexpr int cols_slots(int N, int mode, int prec) { return cols_wg(mode, prec) / 64 * (64 / N); }
+
+inline size_t cols_lds_bytes(int N, int mode, int prec, int nc, int nd)
+{
+    const int nwave = cols_wg(mode, prec) / 64;
+    const int spw = 64 / N;
+    return sizeof(cd) * (2 * (size_t)cols_stage(nc, nd) + (size_t)nwave * (spw + 1) * cols_slot_size(N, mode, prec));
+}
+
+template <int WG>
+__device__ __forceinline__ void cols_stage_k(const LatArgs& a, int k, cd* buf, int nc, int nd)
+{
+    const int n = a.n;
+    const cd* Hk = a.Hk + (size_t)k * n * n;
+    for (int t = threadIdx.x; t < nc * nc; t += WG) buf[t] = Hk[(t % nc) + (size_t)(t / nc) * n];
+    const cd* W = a.W + (size_t)k * nc * nd;
+    for (int t = threadIdx.x; t < nc * nd; t += WG) buf[nc * nc + t] = W[t];
+    double* e = (double*)(buf + nc * nc + nc * nd);
+    for (int t = threadIdx.x; t < nd; t += WG) e[t] = a.ed[(size_t)k * nd + t];
+}
+
+template <int N, int MODE, int PREC, bool TRACE>
+__global__ __launch_bounds__(cols_wg(MODE, PREC)) void k_lattice_schur_cols(LatArgs a, const cd* __restrict__ sig_all)
+{
+    constexpr int WG = cols_wg(MODE, PREC);
+    constexpr int SPW = 64 / N;
+    constexpr int SLOTS = cols_slots(N, MODE, PREC);
+    constexpr int SS = cols_slot_size(N, MODE, PREC);
+    constexpr bool MATS = (PREC > 0 || MODE != MODE_FWD);
+    extern __shared__ cd lds[];
+    const int nc = a.nc, nd = a.n - nc, m2 = nc * nc;
+    const int bs = cols_stage(nc, nd);
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int sl = lane / N;  // == SPW for the spare lanes of a wavefront
+    const int c = lane - sl * N;
+    const bool active = sl < SPW;
+    const bool live = active && c < nc;  // lanes c >= nc hold identity padding
+    const int cc = live ? c : 0;         // clamped column index for loads
+    cd* my = lds + 2 * bs + (size_t)(wave * (SPW + 1) + sl) * SS;  // spare lanes get a scratch slot
+    cd* bc = my;
+    cd* gsh = my + 2 * N;
+    cd* MA = gsh + 32;
+    cd* MB = MA + N * N;
+    cd* GB = MB + N * N;
+    const int nw = blockIdx.x * SLOTS + wave * SPW + (active ? sl : 0);
+    const bool wvalid = active && nw < a.nw_tot;
+    const int nwc = nw < a.nw_tot ? nw : a.nw_tot - 1;
+    const int k0 = blockIdx.y * a.k_per_chunk;
+    const int k1 = min(a.nk, k0 + a.k_per_chunk);
+    const double w = matsubara(nwc, a.beta);
+    const double mu = a.mu;
+    const bool vjp_here = (MODE == MODE_VJP) && wvalid && nw < a.n_freq;
+    const double cn = (MODE == MODE_VJP) ? 2.0 * (a.nbar_dev ? *a.nbar_dev : a.nbar) / a.beta : 0.0;
+
+    // E = (z+mu) I - Sigma(iw), column c
+    cd E[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        cd v = cd{0.0, 0.0};
+        if (live && i < nc) v = cneg(sig_all[(size_t)nwc * m2 + i + cc * nc]);
+        if (live && i == c) v = cadd(v, cd{mu, w});
+        E[i] = v;
+    }
+    if (MODE == MODE_VJP) {
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+            GB[i + c * N] = (vjp_here && live && i < nc) ? a.gbar[(size_t)nw * m2 + i + cc * nc] : cd{0.0, 0.0};
+    }
+    cd acc[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) acc[i] = cd{0.0, 0.0};
+    cd trA = cd{0.0, 0.0}, tr2A = cd{0.0, 0.0}, muA = cd{0.0, 0.0};
+    if (k0 < k1) cols_stage_k<WG>(a, k0, lds, nc, nd);
+    __syncthreads();
+
+    for (int k = k0; k < k1; ++k) {
+        const cd* B = lds + ((k - k0) & 1) * bs;
+        const cd* Hcc = B;
+        const cd* Wk = B + m2;
+        const double* ed = (const double*)(Wk + nc * nd);
+        // g_d = 1/(iw + mu - e_d) of this slot's frequency, shared by its lanes
+        for (int d = c; d < nd; d += N) {
+            const double x = mu - ed[d];
+            const double r = 1.0 / fma(x, x, w * w);
+            gsh[d] = cd{x * r, -w * r};
+        }
+        wave_sync();
+
+        // ---- column c of S (identity column for padding lanes)
+        cd col[N];
+        cd y[MODE == MODE_FWD ? 1 : N];  // column c of Y = sum_d g_d^2 w_d w_d^H
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            cd v = (i == c) ? cd{1.0, 0.0} : cd{0.0, 0.0};
+            if (live) v = (i < nc) ? csub(E[i], Hcc[i + cc * nc]) : cd{0.0, 0.0};
+            col[i] = v;
+        }
+        if constexpr (MODE != MODE_FWD) {
+#pragma unroll
+            for (int i = 0; i < N; ++i) y[i] = cd{0.0, 0.0};
+        }
+        for (int d = 0; d < nd; ++d) {
+            const cd g = gsh[d];
+            const cd* Wd = Wk + d * nc;
+            const cd u = live ? cmulc(g, Wd[cc]) : cd{0.0, 0.0};  // g_d conj(W_cd)
+#pragma unroll
+            for (int i = 0; i < N; ++i)
+                if (i < nc) col[i] = cfms(Wd[i], u, col[i]);
+            if constexpr (MODE != MODE_FWD) {
+                const cd u2 = cmul(g, u);
+#pragma unroll
+                for (int i = 0; i < N; ++i)
+                    if (i < nc) y[i] = cfma(Wd[i], u2, y[i]);
+            }
+        }
+
+        // ---- G_cc = S^{-1}, column c
+        if constexpr (PREC == 0) {
+            gj_columns<N>(col, bc, c, nc);
+        } else {
+            cf* X0 = (cf*)MB;
+#pragma unroll
+            for (int i = 0; i < N; ++i) MA[i + c * N] = col[i];
+            cf x0[N];
+#pragma unroll
+            for (int i = 0; i < N; ++i) x0[i] = cf{(float)col[i].x, (float)col[i].y};
+            gj_columns_f32<N>(x0, (cf*)bc, c, nc);
+#pragma unroll
+            for (int i = 0; i < N; ++i) X0[i + c * N] = x0[i];
+            wave_sync();
+#pragma unroll
+            for (int i = 0; i < N; ++i) col[i] = cd{(double)x0[i].x, (double)x0[i].y};
+            const int steps = a.refine < 1 ? 1 : (a.refine > 2 ? 2 : a.refine);
+#pragma unroll 1
+            for (int it = 0; it < steps; ++it) {
+                wave_sync();  // keeps the S and X0 reads inside the step (no hoisting into registers)
+                cf r[N];  // e_c - S x_c in FP64, stored in FP32
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+                    cd s = (i == c) ? cd{1.0, 0.0} : cd{0.0, 0.0};
+#pragma unroll
+                    for (int l = 0; l < N; ++l) s = cfms(MA[i + l * N], col[l], s);
+                    r[i] = cf{(float)s.x, (float)s.y};
+                }
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+                    cf t = cf{0.f, 0.f};
+#pragma unroll
+                    for (int l = 0; l < N; ++l) t = ffma(X0[i + l * N], r[l], t);
+                    col[i] = cadd(col[i], cd{(double)t.x, (double)t.y});
+                }
+            }
+            wave_sync();
+        }
+        cd gdiag = cd{0.0, 0.0};  // G_cc(c, c)
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+            if (i == c) gdiag = col[i];
+
+        if constexpr (MODE == MODE_FWD) {
+#pragma unroll
+            for (int i = 0; i < N; ++i) acc[i] = cadd(acc[i], col[i]);
+            if constexpr (TRACE) {
+                // lane share of tr G: G_cc + sum_d g_d^2 W_cd (sum_i conj(W_id) G_ic)
+                cd t = live ? gdiag : cd{0.0, 0.0};
+                for (int d = 0; d < nd; ++d) {
+                    const cd g = gsh[d];
+                    const cd* Wd = Wk + d * nc;
+                    cd v = cd{0.0, 0.0};
+#pragma unroll
+                    for (int i = 0; i < N; ++i)
+                        if (i < nc) v = cfmac(Wd[i], col[i], v);
+                    const cd m = live ? cmul(Wd[cc], v) : cd{0.0, 0.0};
+                    t = cfma(cmul(g, g), m, t);
+                    if (c == 0) t = cadd(t, g);
+                }
+                if (wvalid) trA = cadd(trA, t);
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < N; ++i) MA[i + c * N] = col[i];
+            wave_sync();
+            // Z = G (I + Y), column c
+            cd z[N];
+#pragma unroll
+            for (int i = 0; i < N; ++i) z[i] = col[i];
+#pragma unroll
+            for (int l = 0; l < N; ++l) {
+                if (l < nc) {
+                    const cd yl = y[l];
+#pragma unroll
+                    for (int i = 0; i < N; ++i) z[i] = cfma(MA[i + l * N], yl, z[i]);
+                }
+            }
+            // trace shares: tr G and tr G^2 = sum g^2 + tr(Z^2) + 2 sum g^3 M_d
+            cd tg = live ? gdiag : cd{0.0, 0.0};
+            cd t2 = cd{0.0, 0.0};
+            for (int d = 0; d < nd; ++d) {
+                const cd g = gsh[d];
+                const cd g2 = cmul(g, g);
+                const cd* Wd = Wk + d * nc;
+                cd v = cd{0.0, 0.0};
+#pragma unroll
+                for (int i = 0; i < N; ++i)
+                    if (i < nc) v = cfmac(Wd[i], col[i], v);
+                const cd m = live ? cmul(Wd[cc], v) : cd{0.0, 0.0};
+                tg = cfma(g2, m, tg);
+                t2 = cfma(cscale(cmul(g2, g), 2.0), m, t2);
+                if (c == 0) {
+                    tg = cadd(tg, g);
+                    t2 = cadd(t2, g2);
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < N; ++i) MB[i + c * N] = z[i];
+            wave_sync();
+            if (live) {
+#pragma unroll
+                for (int i = 0; i < N; ++i)
+                    if (i < nc) t2 = cfma(z[i], MB[c + i * N], t2);
+            }
+            if (wvalid) {
+                trA = cadd(trA, tg);
+                tr2A = cadd(tr2A, t2);
+            }
+            if constexpr (MODE == MODE_VJP) {
+                if (vjp_here && live) {
+                    // q = column c of (G^2)_cc = Z G
+                    cd q[N];
+#pragma unroll
+                    for (int i = 0; i < N; ++i) q[i] = cd{0.0, 0.0};
+#pragma unroll
+                    for (int l = 0; l < N; ++l) {
+                        if (l < nc) {
+                            const cd gl = col[l];
+#pragma unroll
+                            for (int i = 0; i < N; ++i) q[i] = cfma(MB[i + l * N], gl, q[i]);
+                        }
+                    }
+                    cd ms = cd{0.0, 0.0};
+#pragma unroll
+                    for (int i = 0; i < N; ++i) {
+                        if (i < nc) {
+                            ms = cfmac(GB[i + c * N], q[i], ms);
+                            acc[i] = cadd(acc[i], cscale(cconj(q[i]), cn));  // sigma_bar(c, i) += cn conj((G^2)_ic)
+                        }
+                    }
+                    muA = cadd(muA, ms);
+                    // sigma_bar(c, b) += sum_j t_j conj(G_bj),  t_j = sum_i conj(G_ic) gbar_ij
+#pragma unroll
+                    for (int j = 0; j < N; ++j) {
+                        if (j < nc) {
+                            cd t = cd{0.0, 0.0};
+#pragma unroll
+                            for (int i = 0; i < N; ++i)
+                                if (i < nc) t = cfmac(col[i], GB[i + j * N], t);
+#pragma unroll
+                            for (int b = 0; b < N; ++b)
+                                if (b < nc) acc[b] = cfma(t, cconj(MA[b + j * N]), acc[b]);
+                        }
+                    }
+                }
+            }
+            wave_sync();
+        }
+        __syncthreads();
+        if (k + 1 < k1) cols_stage_k<WG>(a, k + 1, lds + ((k + 1 - k0) & 1) * bs, nc, nd);
+        __syncthreads();
+    }
+
+    // ---- sum the trace shares over the lanes of each slot
+    my[c] = trA;
+    my[N + c] = tr2A;
+    my[2 * N + c] = muA;
+    wave_sync();
+    if (!wvalid) return;
+    const size_t ch = blockIdx.y;
+    if (c == 0) {
+        cd s1 = cd{0.0, 0.0}, s2 = cd{0.0, 0.0}, s3 = cd{0.0, 0.0};
+        for (int l = 0; l < N; ++l) {
+            s1 = cadd(s1, my[l]);
+            s2 = cadd(s2, my[N + l]);
+            s3 = cadd(s3, my[2 * N + l]);
+        }
+        a.pTr[ch * a.nw_tot + nw] = s1;
+        if (MODE != MODE_FWD) a.pTr2[ch * a.nw_tot + nw] = s2;
+        if (MODE == MODE_VJP && nw < a.n_freq) a.pMu[ch * a.n_freq + nw] = s3;
+    }
+    if (!live) return;
+    if constexpr (MODE == MODE_FWD) {
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+            if (i < nc) a.pG[(ch * a.nw_tot + nw) * m2 + i + c * nc] = acc[i];
+    } else if constexpr (MODE == MODE_VJP) {
+        if (nw < a.n_freq) {
+#pragma unroll
+            for (int b = 0; b < N; ++b)
+                if (b < nc) a.pSb[(ch * a.n_freq + nw) * m2 + c + b * nc] = acc[b];
+        }
+    }
+    (void)MATS;
+}
+
+template <int N, int MODE, int PREC, bool TRACE>
+hipError_t launch_cols_k(const LatArgs& a, const cd* sig_all, hipStream_t s)
+{
+    const int nd = a.n - a.nc;
+    const size_t shm = cols_lds_bytes(N, MODE, PREC, a.nc, nd);
+    constexpr int SLOTS = cols_slots(N, MODE, PREC);
+    hipError_t e = hipFuncSetAttribute((const void*)k_lattice_schur_cols<N, MODE, PREC, TRACE>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+    if (e != hipSuccess) return e;
+    dim3 grid((a.nw_tot + SLOTS - 1) / SLOTS, a.n_chunks);
+    hipLaunchKernelGGL((k_lattice_schur_cols<N, MODE, PREC, TRACE>), grid, dim3(cols_wg(MODE, PREC)), shm, s, a,
+                       sig_all);
+    return hipGetLastError();
+}
+
+template <int N, int PREC>
+hipError_t launch_cols_p(const LatArgs& a, int mode, const cd* sig_all, hipStream_t s)
+{
+    if (mode == MODE_FWD)
+        return a.want_trace ? launch_cols_k<N, MODE_FWD, PREC, true>(a, sig_all, s)
+                            : launch_cols_k<N, MODE_FWD, PREC, false>(a, sig_all, s);
+    if (mode == MODE_DENS) return launch_cols_k<N, MODE_DENS, PREC, false>(a, sig_all, s);
+    return launch_cols_k<N, MODE_VJP, PREC, false>(a, sig_all, s);
+}
+
+// entry of one bucket (one translation unit each); PREC 0 = FP64, 1 = FP32 +
+// a.refine (1 or 2) FP64-residual steps
+template <int N>
+hipError_t launch_cols_bucket(const LatArgs& a, int mode, bool fp32, const cd* sig_all, hipStream_t s)
+{
+    if (!fp32) return launch_cols_p<N, 0>(a, mode, sig_all, s);
+    return launch_cols_p<N, 1>(a, mode, sig_all, s);
+}
+
+}  // namespace dg

@@ -1,0 +1,134 @@
+"""Summaries of rocprofv3 output for the lattice kernels.
+
+  python -m dmftpack_amd.rocprof_summary trace <results.db> "<command>"
+      per-kernel table (calls, total, average) and kernel resources from a
+      --kernel-trace --stats SQLite result.
+
+  python -m dmftpack_amd.rocprof_summary pmc --fetch <dir> --write <dir> --copy-mib M --method NAME
+          [--json profiles/pmc_traffic.json] [--command "..."]
+      HBM bytes per launch from two --pmc runs (FETCH_SIZE, then WRITE_SIZE;
+      SQLite or csv output), calibrated by a plain copy of known size:
+      the copy kernel (__amd_rocclr_copyBuffer, its largest dispatch) moves M MiB
+      each way, so factor = M MiB / (counter KiB * 1024) for each counter, and a
+      kernel's traffic per launch is the mean over its dispatches of
+      FETCH * factor_fetch + WRITE * factor_write.
+"""
+import argparse
+import csv
+import glob
+import json
+import os
+import re
+import sqlite3
+import sys
+from collections import defaultdict
+
+
+# ---------------------------------------------------------------------------
+def trace(db_path, command):
+    db = sqlite3.connect(db_path)
+    rows = db.execute("select name, total_calls, total_duration, average, percentage from top_kernels").fetchall()
+    print(f"# rocprofv3 kernel-trace summary\n# command: {command}\n# durations in microseconds")
+    print(f"{'calls':>6} {'total_us':>12} {'avg_us':>12} {'pct':>6}  kernel")
+    for name, calls, tot, avg, pct in rows:
+        print(f"{calls:>6} {tot:>12.1f} {avg:>12.1f} {pct:>6.2f}  {name[:150]}")
+    res = db.execute("select name, vgpr_count, accum_vgpr_count, scratch_size, lds_size, grid_x, grid_y, workgroup_x "
+                     "from kernels group by name").fetchall()
+    print("\n# resources (vgpr, agpr, scratch B/lane, lds B, grid x, grid y, wg)")
+    for r in res:
+        if "dg::k_" in r[0]:
+            print(f"{r[1]:>4} {r[2]:>4} {r[3]:>6} {r[4]:>7} {r[5]:>8} {r[6]:>5} {r[7]:>4}  {r[0][:110]}")
+
+
+# ---------------------------------------------------------------------------
+def read_counter(run_dir, counter):
+    acc = defaultdict(float)
+    dbs = glob.glob(os.path.join(run_dir, "**", "*results.db"), recursive=True)
+    for f in dbs:
+        db = sqlite3.connect(f)
+        for name, did, v in db.execute("select kernel_name, dispatch_id, value from counters_collection "
+                                       "where counter_name = ?", (counter,)):
+            acc[(name, f, did)] += float(v)
+    files = [] if dbs else glob.glob(os.path.join(run_dir, "**", "*counter_collection.csv"), recursive=True)
+    if not dbs and not files:
+        sys.exit(f"no rocprofv3 results under {run_dir}")
+    for f in files:
+        with open(f, newline="") as fh:
+            for row in csv.DictReader(fh):
+                if row.get("Counter_Name") != counter:
+                    continue
+                # rows of one dispatch (one per instance) are summed
+                acc[(row["Kernel_Name"], f, row.get("Dispatch_Id", ""))] += float(row["Counter_Value"])
+    out = defaultdict(list)
+    for (name, _, _), v in acc.items():
+        out[name].append(v)
+    return out
+
+
+def lattice_mode(name):
+    m = re.search(r"k_lattice_(schur|plain)\w*<([^>]*)>", name)
+    if not m:
+        return None
+    args = [x.strip() for x in m.group(2).split(",")]
+    idx = 2 if m.group(1) == "plain" else 1
+    return {"0": "FWD", "1": "DENS", "2": "VJP"}.get(args[idx] if len(args) > idx else "")
+
+
+def pmc(argv):
+    ap = argparse.ArgumentParser(prog="rocprof_summary pmc")
+    ap.add_argument("--fetch", required=True)
+    ap.add_argument("--write", required=True)
+    ap.add_argument("--copy-mib", type=float, required=True)
+    ap.add_argument("--method", required=True)
+    ap.add_argument("--json", default=None)
+    ap.add_argument("--command", default="")
+    a = ap.parse_args(argv)
+    fetch = read_counter(a.fetch, "FETCH_SIZE")
+    write = read_counter(a.write, "WRITE_SIZE")
+    copy_bytes = a.copy_mib * (1 << 20)
+
+    def copy_kib(d):
+        vals = [max(v) for k, v in d.items() if "copyBuffer" in k]
+        return max(vals) if vals else None
+
+    cf, cw = copy_kib(fetch), copy_kib(write)
+    if not cf or not cw:
+        sys.exit("calibration copy not found in the runs")
+    ff, fw = copy_bytes / (cf * 1024.0), copy_bytes / (cw * 1024.0)
+    print("# HBM traffic per launch from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (separate runs)")
+    if a.command:
+        print(f"# command: {a.command}")
+    print(f"# calibration: a {a.copy_mib:.0f} MiB copy reads FETCH_SIZE {cf:.0f} KiB (factor {ff:.4f}) and "
+          f"WRITE_SIZE {cw:.0f} KiB (factor {fw:.4f})")
+    print(f"{'dispatches':>10} {'fetch_MB':>10} {'write_MB':>10} {'total_MB':>10}  kernel")
+    result = {}
+    for name in sorted(set(fetch) | set(write)):
+        fv, wv = fetch.get(name, []), write.get(name, [])
+        if not fv or not wv:
+            continue
+        mf = sum(fv) / len(fv) * 1024.0 * ff
+        mw = sum(wv) / len(wv) * 1024.0 * fw
+        print(f"{len(fv):>10} {mf / 1e6:>10.2f} {mw / 1e6:>10.2f} {(mf + mw) / 1e6:>10.2f}  {name[:120]}")
+        mode = lattice_mode(name)
+        if mode:
+            result[mode] = dict(bytes_per_launch=mf + mw, fetch_bytes=mf, write_bytes=mw, kernel=name)
+    if a.json:
+        db = json.load(open(a.json)) if os.path.exists(a.json) else {}
+        db[a.method] = {k: v["bytes_per_launch"] for k, v in result.items()}
+        db.setdefault("_detail", {})[a.method] = result
+        db["_calibration"] = dict(copy_mib=a.copy_mib, fetch_factor=ff, write_factor=fw)
+        with open(a.json, "w") as fh:
+            json.dump(db, fh, indent=1)
+
+
+def main():
+    if len(sys.argv) < 2 or sys.argv[1] not in ("trace", "pmc"):
+        sys.exit(__doc__)
+    if sys.argv[1] == "trace":
+        trace(sys.argv[2], sys.argv[3] if len(sys.argv) > 3 else "")
+    else:
+        pmc(sys.argv[2:])
+
+
+if __name__ == "__main__":
+    main()

@@ -1,5 +1,9 @@
 """Schur-complement paths against the FP64 oracle.
 
+n_c <= 6 runs the one-matrix-per-lane kernels, 7 <= n_c <= 16 the
+column-per-lane kernels (buckets 8/10/12/16, with padding), n_c = n the
+degenerate Schur complement with no d block.
+
 SCHUR_F64: same bar as the plain path, 1e-12 normwise relative.
 SCHUR_F32R (FP32 solve + FP64-residual refinement): max |dG_loc| <= 1e-9 and
 relative 1e-9 on NumMatrix, N, dN/dmu; gradients relative 1e-6.
@@ -26,7 +30,8 @@ def case(name, seed=2):
     return Hk, sig, 30.0, 0.2
 
 
-CASES = ["dp14", "c_6_3", "c_9_4", "c_20_5", "c_8_6", "c_4_1", "c_3_3", "c_5_2"]
+CASES = ["dp14", "c_6_3", "c_9_4", "c_20_5", "c_8_6", "c_4_1", "c_3_3", "c_5_2",
+         "c_12_7", "c_16_8", "c_16_10", "c_11_9", "c_10_10", "c_20_12", "c_24_16", "c_16_16"]
 METHODS = [("f64", 1, 1e-12), ("f32r", 2, 1e-9)]
 
 
@@ -53,7 +58,7 @@ def test_greenftn_w_schur(gpu, name, mname, method, tol):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["dp14", "c_9_4", "c_3_3", "c_20_5"])
+@pytest.mark.parametrize("name", ["dp14", "c_9_4", "c_3_3", "c_20_5", "c_12_7", "c_16_10", "c_10_10", "c_24_16"])
 @pytest.mark.parametrize("mname,method,tol", METHODS)
 def test_density_schur(gpu, name, mname, method, tol):
     Hk, sig, beta, mu = case(name)
@@ -67,7 +72,7 @@ def test_density_schur(gpu, name, mname, method, tol):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["dp14", "c_9_4", "c_20_5"])
+@pytest.mark.parametrize("name", ["dp14", "c_9_4", "c_20_5", "c_12_7", "c_16_10", "c_11_9", "c_20_12", "c_24_16"])
 @pytest.mark.parametrize("mname,method,tol", [("f64", 1, 1e-12), ("f32r", 2, 1e-8)])
 def test_vjp_schur(gpu, name, mname, method, tol):
     Hk, sig, beta, mu = case(name)
@@ -95,4 +100,24 @@ def test_schur_matches_plain_at_headline_shape(gpu):
     assert rel(s["gloc"], p["gloc"]) <= 1e-12
     assert np.max(np.abs(r["gloc"] - p["gloc"])) <= 1e-9
     assert rel(r["num_matrix"], p["num_matrix"]) <= 1e-9
+    lat.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mname,method,tol", METHODS)
+def test_large32_two_clusters(gpu, mname, method, tol):
+    """The 32-orbital shape: two 5x5 correlated blocks (n_c = 10, NumCluster = 2,
+    G_loc returned block-diagonal), small mesh."""
+    w = L.workload("large32", seed=4, nk_side=2, n_freq=40)
+    Hk, sig, beta = w["Hk"], w["sigma"], w["beta"]
+    mom = O.asymto_moments(sig, beta)
+    ref = O.greenftn_w(Hk, sig, mom, -0.2, beta, n_cluster=2)
+    lat = gpu.LatticeGF(Hk, 10, sig.shape[0], beta, mom, n_cluster=2)
+    out = lat.greenftn_w(sig, -0.2, method=method)
+    assert np.max(np.abs(out["gloc"] - ref["gloc"])) <= tol
+    assert rel(out["num_matrix"], ref["num_matrix"]) <= tol
+    N_ref, dN_ref = O.density(Hk, sig, mom, -0.2, beta)
+    N, dN = lat.density(sig, -0.2, method=method)
+    assert abs(N - N_ref) <= tol * max(abs(N_ref), 1.0)
+    assert abs(dN - dN_ref) <= tol * max(abs(dN_ref), 1.0)
     lat.close()
