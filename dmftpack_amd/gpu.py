@@ -121,6 +121,33 @@ class Handle:
                                              C.byref(viol)))
         return from_dev_cm(out), viol.value
 
+    def hyb_delta(self, gloc, sigma, h_imp, orbit, beta, mu):
+        """Construct_hyb_delta (Construct_hyb_delta.cpp:47-119): Delta(iw_n) of the
+        impurity whose orbitals are `orbit` (indices into the correlated block)."""
+        g, s = to_dev_cm(gloc), to_dev_cm(sigma)
+        hi = to_dev_cm(np.asarray(h_imp)[None])[0].contiguous()
+        nf, nc = g.shape[0], g.shape[1]
+        orbit = np.ascontiguousarray(np.asarray(orbit, dtype=np.int32))
+        dim = orbit.size
+        out = torch.zeros((nf, max(dim, 1), max(dim, 1)), dtype=torch.complex128, device=g.device)
+        self.check(self.lib.dmftgpu_hyb_delta(self.h, nc, nf, beta, mu, _ptr(g), _ptr(s), _ptr(hi), dim,
+                                              orbit.ctypes.data_as(C.POINTER(C.c_int)), _ptr(out)))
+        return from_dev_cm(out)[:, :dim, :dim]
+
+    def hk_from_hr(self, n, hops, kphase):
+        """H(k) = sum_R H(R) exp(-i k.a.R) (ConstructModelHamiltonian.cpp:58-77) from a hopping
+        list [(R1, R2, R3, i, j, t), ...] (0-based orbitals); kphase [nk][3] = k.a per axis.
+        Returns the column-major device tensor [nk][n][n] (the layout LatticeGF takes)."""
+        idx = np.ascontiguousarray(np.array([h[:5] for h in hops], dtype=np.int32).reshape(-1, 5))
+        t = np.ascontiguousarray(np.array([complex(h[5]) for h in hops], dtype=np.complex128))
+        kp = torch.from_numpy(np.ascontiguousarray(np.asarray(kphase, dtype=np.float64))).to("cuda")
+        nk = kp.shape[0]
+        out = torch.empty((nk, n, n), dtype=torch.complex128, device="cuda")
+        self.check(self.lib.dmftgpu_hk_from_hr(self.h, n, len(hops), idx.ctypes.data_as(C.POINTER(C.c_int)),
+                                               t.ctypes.data_as(C.POINTER(C.c_double)), nk, _ptr(kp), _ptr(out)))
+        out._dmft_colmajor = True
+        return out
+
 
 class LatticeGF(Handle):
     """One lattice (H(k) + Sigma tail moments) bound to a library handle."""

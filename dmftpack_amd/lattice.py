@@ -121,6 +121,60 @@ def random_tb_hk(n, nx, seed, ny=None, nz=None):
     return hk_from_hr(Rs, HR, nx, ny, nz)
 
 
+def hopping_list(Rs, HR, drop_zero=True):
+    """Full hopping list [(R1, R2, R3, i, j, t), ...] (0-based orbitals) of a
+    half-shell (Rs, HR) model: each R != 0 also contributes H(-R) = H(R)^H, the
+    form an Hk.HWR file lists (read_TB_parm.cpp:36-62)."""
+    hops = []
+    for R, H in zip(Rs, HR):
+        n = H.shape[0]
+        parts = [(R, H)] if tuple(R) == (0, 0, 0) else [(R, H), (tuple(-np.asarray(R)), H.conj().T)]
+        for RR, HH in parts:
+            for j in range(n):
+                for i in range(n):
+                    if drop_zero and HH[i, j] == 0:
+                        continue
+                    hops.append((int(RR[0]), int(RR[1]), int(RR[2]), i, j, complex(HH[i, j])))
+    return hops
+
+
+def read_hwr(path):
+    """Hk.HWR text reader (read_HR, read_TB_parm.cpp:6-76): the header lines
+    "Norbital of Atom = N" (one per atom), then one line per hopping
+    "R1 R2 R3 atom1 atom2 i j Re Im" with 1-based atoms and orbitals.
+    Returns (orbitals per atom, hops [(R1, R2, R3, i0, j0, t)]) with
+    i0 = (orbitals of the atoms before atom1) + i - 1, likewise j0."""
+    norbs, hops = [], []
+    with open(path) as fh:
+        for line in fh:
+            s = line.strip()
+            if not s:
+                continue
+            if s.startswith("Norbital of Atom"):
+                norbs.append(int(s.split("=")[1]))
+                continue
+            f = s.split()
+            R = [int(v) for v in f[:3]]
+            a1, a2, i, j = (int(v) for v in f[3:7])
+            acc = np.concatenate([[0], np.cumsum(norbs)])
+            hops.append((R[0], R[1], R[2], int(acc[a1 - 1]) + i - 1, int(acc[a2 - 1]) + j - 1,
+                         float(f[7]) + 1j * float(f[8])))
+    return norbs, hops
+
+
+def t2g_hopping_list(t=0.25, tz=0.03):
+    """Hopping list of the t2g model of t2g_hk (orbital xy: -t along x and y,
+    -tz/2 along z; yz and xz likewise)."""
+    axes = {0: ((1, 0, 0), (0, 1, 0), (0, 0, 1)), 1: ((0, 1, 0), (0, 0, 1), (1, 0, 0)),
+            2: ((1, 0, 0), (0, 0, 1), (0, 1, 0))}
+    hops = []
+    for a, (u, v, w) in axes.items():
+        for R, amp in ((u, -t), (v, -t), (w, -tz / 2.0)):
+            for sgn in (1, -1):
+                hops.append((sgn * R[0], sgn * R[1], sgn * R[2], a, a, complex(amp)))
+    return hops
+
+
 # --------------------------------------------------------------------------
 # Self-energies
 # --------------------------------------------------------------------------

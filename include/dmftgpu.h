@@ -190,6 +190,33 @@ int dmftgpu_ft_w2tau_points(dmftgpu_handle h, int d, int n_freq, double beta, co
 int dmftgpu_ft_tau2w(dmftgpu_handle h, int d, int n_freq, int n_tau, double beta, const double* gtau, double* fw,
                      int* violations);
 
+/* ---- callers either side of the path ----------------------------------- */
+
+/* Construct_hyb_delta (Construct_hyb_delta.cpp:47-119, off-diagonal form
+ * :96-117): hybridisation function of one impurity whose dim orbitals are
+ * orbit[0..dim) of the correlated block,
+ *   delta(iw_n) = (iw_n + mu) I - H_imp - Sigma_imp(iw_n) - [G_imp(iw_n)]^{-1},  n < n_freq,
+ * X_imp(a, b) = X(orbit[a], orbit[b]).
+ *   gloc   [n_freq][nc][nc]   G_loc (dmftgpu_greenftn_w output)
+ *   sigma  [n_freq][nc][nc]   SelfE_w
+ *   h_imp  [nc][nc]           impurity_site_Hamiltonian
+ *   orbit  [dim]              HOST, 0 <= orbit[a] < nc (impurityOrbit)
+ *   delta  [n_freq][dim][dim] weiss_fieldTB block of this atom
+ * dim = 0 returns without work, as the reference skips the inverse (:98).
+ * Delta -> tau is dmftgpu_moments + dmftgpu_ft_w2tau with green_asym = 0
+ * (write_hyb_t, Construct_hyb_delta.cpp:126-138). */
+int dmftgpu_hyb_delta(dmftgpu_handle h, int nc, int n_freq, double beta, double mu, const double* gloc,
+                      const double* sigma, const double* h_imp, int dim, const int* orbit, double* delta);
+
+/* H(k) = sum_R H(R) exp(-i k.a.R) (ConstructModelHamiltonian.cpp:58-77) from
+ * the hopping list of read_HR (read_TB_parm.cpp:6-76):
+ *   hop_idx [n_hop][5]  HOST int: R1 R2 R3 i j (0-based orbitals i, j < n)
+ *   hop     [n_hop]     HOST complex t
+ *   kphase  [n_k][3]    k_x a_x, k_y a_y, k_z a_z
+ *   Hk      [n_k][n][n] output; hops of one element are summed in list order. */
+int dmftgpu_hk_from_hr(dmftgpu_handle h, int n, int n_hop, const int* hop_idx, const double* hop, int n_k,
+                       const double* kphase, double* Hk);
+
 #ifdef __cplusplus
 }
 #endif

@@ -30,6 +30,10 @@
  *   orc_ft_w2tau         two-pole closed form, G(0)+G(beta) = -m1
  *   orc_ft_tau2w         exact transform of piecewise-linear G(tau)
  *   orc_gloc_vjp         central finite differences of orc_density/orc_loss
+ *   orc_hyb_delta        round trip: G_imp built from a known causal Delta
+ *                        returns that Delta; scalar closed form
+ *   orc_hk_from_hr       closed-form dispersions of the t2g model and of the
+ *                        Hubbard-model Hk.HWR fixture, Hermiticity
  */
 #include <complex.h>
 #include <math.h>
@@ -998,4 +1002,50 @@ int orc_num_threads(void)
 #else
     return 1;
 #endif
+}
+
+/* ------------------------------------------------------------------------- */
+/* Construct_hyb_delta (Construct_hyb_delta.cpp:47-119), off-diagonal form
+ * (:96-117): for n < n_freq
+ *   delta_n = (iw_n + mu) I - H_imp - Sigma_imp,n - (G_imp,n)^{-1},
+ * X_imp(a, b) = X(orbit[a], orbit[b]); the inverse is Eigen's inverse() (LU). */
+int orc_hyb_delta(int nc, int n_freq, double beta, double mu, const cplx *gloc, const cplx *sigma, const cplx *h_imp,
+                  int dim, const int *orbit, cplx *delta)
+{
+    if (dim == 0) return 0;
+    const size_t m2 = (size_t)nc * nc, d2 = (size_t)dim * dim;
+    cplx *Gi = malloc(sizeof(cplx) * d2), *Ginv = malloc(sizeof(cplx) * d2);
+    int rc = 0;
+    for (int n = 0; n < n_freq && rc == 0; ++n) {
+        const cplx iw = I * PI_REF * (2.0 * n + 1.0) / beta;
+        for (int b = 0; b < dim; ++b)
+            for (int a = 0; a < dim; ++a) Gi[IDX(a, b, dim)] = gloc[n * m2 + IDX(orbit[a], orbit[b], nc)];
+        rc = orc_lu_inverse(dim, Gi, Ginv);
+        for (int b = 0; b < dim; ++b)
+            for (int a = 0; a < dim; ++a) {
+                cplx v = -h_imp[IDX(orbit[a], orbit[b], nc)] - sigma[n * m2 + IDX(orbit[a], orbit[b], nc)] -
+                         Ginv[IDX(a, b, dim)];
+                if (a == b) v += iw + mu;
+                delta[n * d2 + IDX(a, b, dim)] = v;
+            }
+    }
+    free(Gi); free(Ginv);
+    return rc;
+}
+
+/* H(k) = sum_R H(R) exp(-i (k_x a R1 + k_y a R2 + k_z a R3))
+ * (ConstructModelHamiltonian.cpp:58-77), hops in the list order of read_HR
+ * (read_TB_parm.cpp:36-62): hop_idx [n_hop][5] = R1 R2 R3 i j (0-based). */
+void orc_hk_from_hr(int n, int n_hop, const int *hop_idx, const cplx *hop, int n_k, const double *kphase, cplx *Hk)
+{
+    const size_t nn = (size_t)n * n;
+    for (int k = 0; k < n_k; ++k) {
+        cplx *H = Hk + k * nn;
+        for (size_t e = 0; e < nn; ++e) H[e] = 0.0;
+        const double *p = kphase + 3 * (size_t)k;
+        for (int q = 0; q < n_hop; ++q) {
+            const int *r = hop_idx + 5 * (size_t)q;
+            H[IDX(r[3], r[4], n)] += hop[q] * cexp(-I * (p[0] * r[0] + p[1] * r[1] + p[2] * r[2]));
+        }
+    }
 }

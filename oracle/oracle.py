@@ -52,6 +52,8 @@ def lib():
         _lib.orc_lu_inverse.argtypes = [C.c_int, P, P]
         _lib.orc_herm_eig.argtypes = [C.c_int, P, P, P]
         _lib.orc_num_threads.restype = C.c_int
+        _lib.orc_hyb_delta.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, P, P, P, C.c_int, P, P]
+        _lib.orc_hk_from_hr.argtypes = [C.c_int, C.c_int, P, P, C.c_int, P, P]
     return _lib
 
 
@@ -197,3 +199,28 @@ def ft_tau2w(gt, beta, n_freq):
     viol = C.c_int(0)
     lib().orc_ft_tau2w(d, n_freq, nt1 - 1, beta, _p(g), _p(out), C.byref(viol))
     return _from_cm(out), viol.value
+
+
+def hyb_delta(gloc, sigma, h_imp, orbit, beta, mu):
+    """Construct_hyb_delta restated: Delta(iw_n) of the impurity `orbit` (indices into the correlated block)."""
+    gloc = np.asarray(gloc, dtype=np.complex128)
+    nf, nc, _ = gloc.shape
+    orbit = np.ascontiguousarray(np.asarray(orbit, dtype=np.int32))
+    dim = orbit.size
+    out = np.zeros((nf, dim, dim), np.complex128)
+    g, sg, hi = _cm(gloc), _cm(sigma), _cm(np.asarray(h_imp)[None])
+    rc = lib().orc_hyb_delta(nc, nf, beta, mu, _p(g), _p(sg), _p(hi), dim, _p(orbit), _p(out))
+    if rc:
+        raise ValueError("singular G_imp")
+    return _from_cm(out)
+
+
+def hk_from_hr(n, hops, kphase):
+    """H(k) = sum_R H(R) exp(-i k.a.R) from a hopping list [(R1, R2, R3, i, j, t), ...]; kphase [nk][3]."""
+    idx = np.ascontiguousarray(np.array([h[:5] for h in hops], dtype=np.int32).reshape(-1, 5))
+    t = np.ascontiguousarray(np.array([complex(h[5]) for h in hops], dtype=np.complex128))
+    kp = np.ascontiguousarray(np.asarray(kphase, dtype=np.float64))
+    nk = kp.shape[0]
+    out = np.zeros((nk, n, n), np.complex128)
+    lib().orc_hk_from_hr(n, len(hops), _p(idx), _p(t), nk, _p(kp), _p(out))
+    return _from_cm(out)

@@ -1,0 +1,97 @@
+"""GPU hybridisation function and H(R) -> H(k) builder against the oracle.
+
+Bar: 1e-12 normwise relative (hybridisation: one small inverse per frequency,
+no pivoting on the GPU against the oracle's partial-pivot LU), 1e-13 absolute
+for H(k) (a sum of a few exp terms per element).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from dmftpack_amd import lattice as L
+from oracle import oracle as O
+from test_cpu_model import hyb_case
+
+HWR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "hubbard_Hk.HWR")
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    from dmftpack_amd import gpu as G
+    h = G.Handle()
+    yield G, h
+    h.close()
+
+
+def rel(x, ref):
+    return float(np.max(np.abs(np.asarray(x) - ref)) / max(np.max(np.abs(ref)), 1e-300))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nc,dim", [(5, 1), (5, 3), (5, 5), (10, 7), (10, 10), (16, 13), (20, 17), (32, 32)])
+def test_hyb_delta(gpu, nc, dim):
+    G, h = gpu
+    gloc, sig, h_imp, orbit, beta, mu, delta0 = hyb_case(nc, dim, nf=37)
+    print(f"hyb_delta nc={nc} dim={dim}", flush=True)
+    ref = O.hyb_delta(gloc, sig, h_imp, orbit, beta, mu)
+    got = h.hyb_delta(gloc, sig, h_imp, orbit, beta, mu)
+    assert rel(got, ref) <= 1e-12
+    assert rel(got, delta0) <= 1e-11
+
+
+@pytest.mark.gpu
+def test_hyb_delta_from_lattice_gloc(gpu):
+    """The chain the DMFT loop runs: G_loc from the lattice path -> Delta of the
+    5 d orbitals; causal: -Im Delta(iw_n) is positive semidefinite."""
+    G, h = gpu
+    w = L.workload("dp14", seed=1, nk_side=4, n_freq=64)
+    mom = O.asymto_moments(w["sigma"], w["beta"])
+    lat = G.LatticeGF(w["Hk"], 5, 64, w["beta"], mom)
+    out = lat.greenftn_w(w["sigma"], 0.3, method=G.SCHUR_F64)
+    lat.close()
+    h_imp = np.mean(w["Hk"], axis=0)[:5, :5]
+    got = h.hyb_delta(out["gloc"], w["sigma"], h_imp, range(5), w["beta"], 0.3)
+    ref = O.hyb_delta(out["gloc"], w["sigma"], h_imp, list(range(5)), w["beta"], 0.3)
+    assert rel(got, ref) <= 1e-12
+    for n in range(64):
+        anti = (got[n] - got[n].conj().T) / 2j
+        assert np.max(np.linalg.eigvalsh(anti)) <= 1e-9
+
+
+@pytest.mark.gpu
+def test_hyb_delta_errors(gpu):
+    G, h = gpu
+    gloc, sig, h_imp, orbit, beta, mu, _ = hyb_case(5, 3)
+    with pytest.raises(G.capi.DmftGpuError):
+        h.hyb_delta(gloc, sig, h_imp, [0, 1, 7], beta, mu)
+
+
+@pytest.mark.gpu
+def test_hk_from_hr_random(gpu):
+    G, h = gpu
+    Rs, HR = L.random_tb_hr(14, 5)
+    hops = L.hopping_list(Rs, HR)
+    th = L.kmesh_phases(6)
+    ref = O.hk_from_hr(14, hops, th)
+    dev = h.hk_from_hr(14, hops, th)
+    got = G.from_dev_cm(dev)
+    assert np.max(np.abs(got - ref)) <= 1e-13
+
+
+@pytest.mark.gpu
+def test_hk_from_hr_feeds_lattice(gpu):
+    """H(k) built on the GPU from the Hk.HWR fixture goes straight into the
+    lattice path; G_loc agrees with the oracle on the same H(k)."""
+    G, h = gpu
+    _, hops = L.read_hwr(HWR)
+    th = L.kmesh_phases(8, 8, 1)
+    Hk_dev = h.hk_from_hr(2, hops, th)
+    Hk = O.hk_from_hr(2, hops, th)
+    sig, _ = L.pole_sigma(2, 32, 40.0, 3)
+    mom = O.asymto_moments(sig, 40.0)
+    ref = O.greenftn_w(Hk, sig, mom, 0.1, 40.0)
+    lat = G.LatticeGF(Hk_dev, 2, 32, 40.0, mom)
+    out = lat.greenftn_w(sig, 0.1, method=G.PLAIN_F64)
+    assert rel(out["gloc"], ref["gloc"]) <= 1e-12
+    lat.close()
