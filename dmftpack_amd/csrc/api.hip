@@ -161,6 +161,10 @@ extern "C" int dmftgpu_set_option(dmftgpu_handle h, int option, int value)
             if (value != 64 && value != 128) return fail(h, DMFTGPU_ERR_ARG, "Schur workgroup width %d not 64/128", value);
             h->schur_wg = value;
             return DMFTGPU_OK;
+        case DMFTGPU_OPT_SCHUR_LAYOUT:
+            if (value != 0 && value != 1) return fail(h, DMFTGPU_ERR_ARG, "Schur layout %d not 0/1", value);
+            h->schur_layout = value;
+            return DMFTGPU_OK;
         default: return fail(h, DMFTGPU_ERR_ARG, "unknown option %d", option);
     }
 }
@@ -220,6 +224,22 @@ extern "C" int dmftgpu_prepare(dmftgpu_handle h, const dmftgpu_lattice* L, const
         h->schur_ready = true;
     }
     h->prepared = true;
+    ++h->prep_gen;
+    return DMFTGPU_OK;
+}
+
+// New Sigma tail moments for the prepared H(k): only the static part
+// eig(H(k) + P m0 P^H) (GreenFtn.cpp:197-215) depends on them; the Schur data
+// eig(H_dd(k)) and W = H_cd V_d depend on H(k) alone and are kept.
+extern "C" int dmftgpu_update_moments(dmftgpu_handle h, const double* moments)
+{
+    (void)hipGetLastError();  // drop an error left by an earlier failed call
+    if (!h) return DMFTGPU_ERR_ARG;
+    if (!h->prepared) return fail(h, DMFTGPU_ERR_STATE, "dmftgpu_prepare has not been called");
+    if (!moments) return fail(h, DMFTGPU_ERR_ARG, "moments is null");
+    const int n = h->lat.n, nc = h->lat.n_corr, nk = h->lat.n_k;
+    HIPCHK(hipMemcpyAsync(h->mom, moments, sizeof(cd) * 4 * nc * nc, hipMemcpyDefault, h->stream));
+    HIPCHK(dg::launch_herm_eig(h->Hk, n, 0, n, h->mom, nc, h->ev, h->U, nk, h->stream));
     ++h->prep_gen;
     return DMFTGPU_OK;
 }
@@ -313,7 +333,7 @@ int run_lattice(dmftgpu_handle h, int method, int mode, const double* sigma, dou
     const int nw_tot = L.n_freq * L.high_freq;
     int slots;
     if (method == DMFTGPU_PLAIN_F64) slots = dg::plain_slots(n, mode);
-    else slots = dg::schur_slots(nc, mode, method == DMFTGPU_SCHUR_F32R);
+    else slots = dg::schur_slots(nc, mode, method == DMFTGPU_SCHUR_F32R, h->schur_layout == 1);
     Plan P = make_plan(nk, nw_tot, slots, h->target_wg);
     dg::LatArgs a{};
     a.Hk = h->Hk;
@@ -352,7 +372,8 @@ int run_lattice(dmftgpu_handle h, int method, int mode, const double* sigma, dou
     if (method == DMFTGPU_PLAIN_F64) {
         HIPCHK(dg::launch_lattice_plain(a, mode, h->stream, nullptr));
     } else {
-        HIPCHK(dg::launch_lattice_schur(a, mode, method == DMFTGPU_SCHUR_F32R, h->stream, h->schur_wg));
+        HIPCHK(dg::launch_lattice_schur(a, mode, method == DMFTGPU_SCHUR_F32R, h->stream, h->schur_wg,
+                                        h->schur_layout == 1));
     }
     if (h->timing) {
         HIPCHK(hipEventRecord(h->ev1[mode], h->stream));
@@ -382,7 +403,8 @@ size_t lattice_ws_bytes(dmftgpu_handle h, int method, int mode)
     const int nc = L.n_corr, m2 = nc * nc;
     const size_t nw_tot = (size_t)L.n_freq * L.high_freq;
     int slots = method == DMFTGPU_PLAIN_F64 ? dg::plain_slots(L.n, mode)
-                                            : dg::schur_slots(nc, mode, method == DMFTGPU_SCHUR_F32R);
+                                            : dg::schur_slots(nc, mode, method == DMFTGPU_SCHUR_F32R,
+                                                              h->schur_layout == 1);
     Plan P = make_plan(L.n_k, (int)nw_tot, slots, h->target_wg);
     size_t C = P.n_chunks + 1;  // +1 for the reduced copy
     size_t b = (C + 1) * nw_tot * 16 * (2 + (size_t)m2) + C * L.n_freq * 16 * (m2 + 1) + 64 * 256;

@@ -40,8 +40,10 @@ int plain_slots(int n, int mode);
 // densityMatDFT: dens [nk][n][n] from U, static weights wt [nk][n]
 hipError_t launch_density_matrix(const LatArgs& a, const cd* U, const double* wt, cd* dens, hipStream_t s);
 
-// Schur complement (FP64 or FP32 + refinement); wg = workgroup width (64 or 128)
-hipError_t launch_lattice_schur(const LatArgs& a, int mode, bool fp32, hipStream_t s, int wg = 64);
+// Schur complement (FP64 or FP32 + refinement); wg = workgroup width of the
+// one-matrix-per-lane kernels (64 or 128); cols selects the column-per-lane
+// kernels for n_c <= 6 (always used above 6)
+hipError_t launch_lattice_schur(const LatArgs& a, int mode, bool fp32, hipStream_t s, int wg = 64, bool cols = false);
 // Sigma expanded over all nw_tot frequencies into a.pSigAll
 hipError_t launch_sigma_all(const LatArgs& a, hipStream_t s);
 // Persistent launch over a work list of several lattices sharing n_c (and mode,
@@ -58,11 +60,14 @@ hipError_t launch_lattice_schur_multi(const LatArgs& proto, int mode, bool fp32,
                                       const SchurMulti& m);
 size_t schur_lds_bytes(int nc, int nd, int mode, bool fp32, int wg);
 // frequencies per workgroup of the Schur kernel for n_c (grid planning)
-int schur_slots(int nc, int mode, bool fp32);
+int schur_slots(int nc, int mode, bool fp32, bool cols);
 bool schur_supported(int n, int nc);
 // the persistent multi-lattice launch covers the one-matrix-per-lane kernels
 bool schur_multi_supported(int nc);
-// column-per-lane Schur kernels for 7 <= n_c <= 16 (lattice_cols{8,10,12,16}.hip)
+// column-per-lane Schur kernels (lattice_cols{4,5,6,8,10,12,16}.hip)
+hipError_t launch_cols_4(const LatArgs& a, int mode, bool fp32, const cd* sig_all, hipStream_t s);
+hipError_t launch_cols_5(const LatArgs& a, int mode, bool fp32, const cd* sig_all, hipStream_t s);
+hipError_t launch_cols_6(const LatArgs& a, int mode, bool fp32, const cd* sig_all, hipStream_t s);
 hipError_t launch_cols_8(const LatArgs& a, int mode, bool fp32, const cd* sig_all, hipStream_t s);
 hipError_t launch_cols_10(const LatArgs& a, int mode, bool fp32, const cd* sig_all, hipStream_t s);
 hipError_t launch_cols_12(const LatArgs& a, int mode, bool fp32, const cd* sig_all, hipStream_t s);

@@ -415,14 +415,17 @@ __global__ __launch_bounds__(WG) void k_lattice_schur_multi(const LatArgs* __res
 }
 
 // -------------------------------------------------------------------------
-// n_c <= 6: one matrix per lane (this file); 7..16: column per lane
-// (lattice_schur_cols.hpp, buckets 8/10/12/16)
+// n_c <= 6: one matrix per lane (this file) or, on request, column per lane;
+// 7..16: column per lane (lattice_schur_cols.hpp, buckets 4/5/6/8/10/12/16)
 bool schur_supported(int n, int nc) { return nc >= 1 && nc <= 16 && n >= nc && n <= 32; }
 bool schur_multi_supported(int nc) { return nc >= 1 && nc <= 6; }
-static int cols_bucket(int nc) { return nc <= 8 ? 8 : nc <= 10 ? 10 : nc <= 12 ? 12 : 16; }
-int schur_slots(int nc, int mode, bool fp32)
+static int cols_bucket(int nc)
 {
-    if (nc <= 6) return 64;
+    return nc <= 4 ? 4 : nc <= 6 ? nc : nc <= 8 ? 8 : nc <= 10 ? 10 : nc <= 12 ? 12 : 16;
+}
+int schur_slots(int nc, int mode, bool fp32, bool cols)
+{
+    if (nc <= 6 && !cols) return 64;
     const int N = cols_bucket(nc), prec = fp32 ? 1 : 0;
     const int wg = (mode == MODE_FWD && prec == 0) ? 256 : 64;
     return wg / 64 * (64 / N);
@@ -525,12 +528,15 @@ hipError_t launch_sigma_all(const LatArgs& a, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_lattice_schur(const LatArgs& a, int mode, bool fp32, hipStream_t s, int wg)
+hipError_t launch_lattice_schur(const LatArgs& a, int mode, bool fp32, hipStream_t s, int wg, bool cols)
 {
     hipError_t e = launch_sigma_all(a, s);
     if (e != hipSuccess) return e;
-    if (a.nc > 6) {
+    if (a.nc > 6 || cols) {
         switch (cols_bucket(a.nc)) {
+            case 4: return launch_cols_4(a, mode, fp32, a.pSigAll, s);
+            case 5: return launch_cols_5(a, mode, fp32, a.pSigAll, s);
+            case 6: return launch_cols_6(a, mode, fp32, a.pSigAll, s);
             case 8: return launch_cols_8(a, mode, fp32, a.pSigAll, s);
             case 10: return launch_cols_10(a, mode, fp32, a.pSigAll, s);
             case 12: return launch_cols_12(a, mode, fp32, a.pSigAll, s);

@@ -160,11 +160,13 @@ class LatticeGF(Handle):
         self.n, self.nc, self.nk = n, n_corr, nk
         self.n_freq, self.beta = n_freq, beta
         self.lat = capi.Lattice(n, n_corr, n_cluster, nk, n_k_global or nk, n_freq, high_freq, beta)
-        self.set_moments(moments)
-
-    def set_moments(self, moments):
         self.mom = to_dev_cm(moments)
         self.check(self.lib.dmftgpu_prepare(self.h, C.byref(self.lat), _ptr(self.Hk), _ptr(self.mom)))
+
+    def set_moments(self, moments):
+        """New Sigma tail moments, same H(k): dmftgpu_update_moments."""
+        self.mom = to_dev_cm(moments)
+        self.check(self.lib.dmftgpu_update_moments(self.h, _ptr(self.mom)))
 
     def sigma_dev(self, sigma):
         if isinstance(sigma, torch.Tensor) and getattr(sigma, "_dmft_colmajor", False):

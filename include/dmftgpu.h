@@ -73,6 +73,8 @@ const char* dmftgpu_version(void);
 #define DMFTGPU_OPT_REFINE_STEPS 1      /* FP64-residual refinement steps of SCHUR_F32R (1 or 2, default 2) */
 #define DMFTGPU_OPT_TARGET_WORKGROUPS 2 /* workgroups per lattice launch, sets the k chunking (default 2048) */
 #define DMFTGPU_OPT_SCHUR_WG 3          /* frequencies per workgroup of the Schur kernels (64 or 128) */
+#define DMFTGPU_OPT_SCHUR_LAYOUT 4      /* n_c <= 6: 0 = one (k,iw) matrix per lane (default), 1 = one column
+                                           per lane (always used for n_c > 6) */
 int dmftgpu_set_option(dmftgpu_handle h, int option, int value);
 
 /* Instrumentation (no reference counterpart): with timing enabled, HIP events
@@ -89,6 +91,11 @@ int dmftgpu_kernel_time(dmftgpu_handle h, int mode, double* ms);
  * (GreenFtn.cpp:197-215).  Hk must stay valid until the next prepare.
  * For the Schur methods it also diagonalises H_dd(k) once per k. */
 int dmftgpu_prepare(dmftgpu_handle h, const dmftgpu_lattice* lat, const double* Hk, const double* moments);
+
+/* New tail moments for the H(k) bound by dmftgpu_prepare (a DMFT iteration
+ * changes Sigma, H(k) stays): redoes only the static eigen-decomposition
+ * (GreenFtn.cpp:197-215); the Schur data of H(k) is kept. */
+int dmftgpu_update_moments(dmftgpu_handle h, const double* moments);
 
 /* Copies the static eigenpairs out: eps [n_k][n] ascending, U [n_k][n][n]. */
 int dmftgpu_static_eig(dmftgpu_handle h, double* eps, double* U);

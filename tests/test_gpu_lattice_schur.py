@@ -121,3 +121,53 @@ def test_large32_two_clusters(gpu, mname, method, tol):
     assert abs(N - N_ref) <= tol * max(abs(N_ref), 1.0)
     assert abs(dN - dN_ref) <= tol * max(abs(dN_ref), 1.0)
     lat.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["dp14", "c_6_3", "c_4_1", "c_8_6", "c_5_2", "c_3_3"])
+@pytest.mark.parametrize("mname,method,tol", METHODS)
+def test_column_layout_small_nc(gpu, name, mname, method, tol):
+    """n_c <= 6 through the column-per-lane kernels (buckets 4/5/6): forward,
+    density and adjoint against the oracle."""
+    Hk, sig, beta, mu = case(name)
+    mom = O.asymto_moments(sig, beta)
+    print(f"columns {name} {mname}", flush=True)
+    lat = gpu.LatticeGF(Hk, sig.shape[1], sig.shape[0], beta, mom)
+    lat.set_option(gpu.capi.OPT_SCHUR_LAYOUT, 1)
+    ref = O.greenftn_w(Hk, sig, mom, mu, beta)
+    out = lat.greenftn_w(sig, mu, method=method)
+    assert np.max(np.abs(out["gloc"] - ref["gloc"])) <= tol
+    assert rel(out["num_matrix"], ref["num_matrix"]) <= tol
+    N_ref, dN_ref = O.density(Hk, sig, mom, mu, beta)
+    N, dN = lat.density(sig, mu, method=method)
+    assert abs(N - N_ref) <= tol * max(abs(N_ref), 1.0)
+    assert abs(dN - dN_ref) <= tol * max(abs(dN_ref), 1.0)
+    rng = np.random.default_rng(5)
+    gbar = rng.normal(size=sig.shape) + 1j * rng.normal(size=sig.shape)
+    sb_ref, mb_ref = O.gloc_vjp(Hk, sig, mom, mu, beta, gbar, -0.4)
+    sb, mb = lat.vjp(sig, mu, gbar, -0.4, method=method)
+    gtol = 1e-12 if method == 1 else 1e-8
+    assert rel(sb, sb_ref) <= gtol
+    assert abs(mb - mb_ref) <= gtol * max(abs(mb_ref), 1.0)
+    lat.close()
+
+
+@pytest.mark.gpu
+def test_update_moments_matches_prepare(gpu):
+    """New tail moments through dmftgpu_update_moments give what a fresh
+    prepare gives (the Schur data of H(k) is reused)."""
+    Hk, sig, beta, mu = case("dp14")
+    mom = O.asymto_moments(sig, beta)
+    sig2 = L.perturbed(sig, seed=9)
+    mom2 = O.asymto_moments(sig2, beta)
+    lat = gpu.LatticeGF(Hk, 5, sig.shape[0], beta, mom)
+    lat.greenftn_w(sig, mu, method=gpu.SCHUR_F64)
+    lat.set_moments(mom2)
+    a = lat.greenftn_w(sig2, mu, method=gpu.SCHUR_F64)
+    fresh = gpu.LatticeGF(Hk, 5, sig.shape[0], beta, mom2)
+    b = fresh.greenftn_w(sig2, mu, method=gpu.SCHUR_F64)
+    ref = O.greenftn_w(Hk, sig2, mom2, mu, beta)
+    assert np.array_equal(a["gloc"], b["gloc"]) and np.array_equal(a["num_matrix"], b["num_matrix"])
+    assert rel(a["num_matrix"], ref["num_matrix"]) <= 1e-12
+    lat.close()
+    fresh.close()
