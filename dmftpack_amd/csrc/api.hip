@@ -202,6 +202,8 @@ extern "C" int dmftgpu_prepare(dmftgpu_handle h, const dmftgpu_lattice* L, const
     size_t o_ed = cv.take<double>((size_t)nk * (nd > 0 ? nd : 1));
     size_t o_Vd = cv.take<cd>((size_t)nk * (nd > 0 ? nd * nd : 1));
     size_t o_W = cv.take<cd>((size_t)nk * nc * (nd > 0 ? nd : 1));
+    const bool lane_kernels = dg::schur_supported(n, nc) && dg::schur_multi_supported(nc);
+    size_t o_kd = cv.take<cd>(lane_kernels ? (size_t)nk * dg::schur_kdata_size(n, nc) : 1);
     hipError_t e;
     char* base = (char*)h->prep.get(cv.off, &e);
     if (!base) return fail(h, DMFTGPU_ERR_HIP, "prepare alloc %zu bytes: %s", cv.off, hipGetErrorString(e));
@@ -211,6 +213,7 @@ extern "C" int dmftgpu_prepare(dmftgpu_handle h, const dmftgpu_lattice* L, const
     h->ed = (double*)(base + o_ed);
     h->Vd = (cd*)(base + o_Vd);
     h->W = (cd*)(base + o_W);
+    h->kdata = lane_kernels ? (cd*)(base + o_kd) : nullptr;
     HIPCHK(hipMemcpyAsync(h->mom, moments, sizeof(cd) * 4 * nc * nc, hipMemcpyDefault, h->stream));
     // static part: eig(H(k) + P m0 P^H)   (GreenFtn.cpp:197-215)
     HIPCHK(dg::launch_herm_eig(h->Hk, n, 0, n, h->mom, nc, h->ev, h->U, nk, h->stream));
@@ -221,6 +224,7 @@ extern "C" int dmftgpu_prepare(dmftgpu_handle h, const dmftgpu_lattice* L, const
             HIPCHK(dg::launch_herm_eig(h->Hk, n, nc, nd, nullptr, 0, h->ed, h->Vd, nk, h->stream));
             HIPCHK(dg::launch_schur_prep(h->Hk, h->Vd, n, nc, nk, h->W, h->stream));
         }
+        if (lane_kernels) HIPCHK(dg::launch_schur_kdata(h->Hk, h->W, h->ed, n, nc, nk, h->kdata, h->stream));
         h->schur_ready = true;
     }
     h->prepared = true;
@@ -325,15 +329,22 @@ struct Sums {
     cd *Gsum = nullptr, *tr = nullptr, *tr2 = nullptr, *Sb = nullptr, *mu = nullptr;
 };
 
+// Real-axis frequency grid z_n = e0 + n de + i eta, n < n_e (retarded_GreenFtn2).
+struct RealAxis {
+    int n_e;
+    double e0, de, eta;
+};
+
 int run_lattice(dmftgpu_handle h, int method, int mode, const double* sigma, double mu, const double* gbar,
-                double nbar, char* ws, size_t& off, Sums& S, bool want_trace = false)
+                double nbar, char* ws, size_t& off, Sums& S, bool want_trace = false, const RealAxis* ra = nullptr)
 {
     const dmftgpu_lattice& L = h->lat;
     const int n = L.n, nc = L.n_corr, nk = L.n_k, m2 = nc * nc;
-    const int nw_tot = L.n_freq * L.high_freq;
+    const int nw_tot = ra ? ra->n_e : L.n_freq * L.high_freq;
+    const int n_freq = ra ? ra->n_e : L.n_freq;
     int slots;
     if (method == DMFTGPU_PLAIN_F64) slots = dg::plain_slots(n, mode);
-    else slots = dg::schur_slots(nc, mode, method == DMFTGPU_SCHUR_F32R, h->schur_layout == 1);
+    else slots = dg::schur_slots(nc, mode, method == DMFTGPU_SCHUR_F32R, h->schur_layout == 1 || ra);
     Plan P = make_plan(nk, nw_tot, slots, h->target_wg);
     dg::LatArgs a{};
     a.Hk = h->Hk;
@@ -342,8 +353,14 @@ int run_lattice(dmftgpu_handle h, int method, int mode, const double* sigma, dou
     a.n = n;
     a.nc = nc;
     a.nk = nk;
-    a.n_freq = L.n_freq;
+    a.n_freq = n_freq;
     a.nw_tot = nw_tot;
+    if (ra) {
+        a.real_axis = 1;
+        a.e0 = ra->e0;
+        a.de = ra->de;
+        a.eta = ra->eta;
+    }
     a.beta = L.beta;
     a.mu = mu;
     a.k_per_chunk = P.k_per_chunk;
@@ -352,6 +369,7 @@ int run_lattice(dmftgpu_handle h, int method, int mode, const double* sigma, dou
     a.nbar = nbar;
     a.ed = h->ed;
     a.W = h->W;
+    a.kdata = h->kdata;
     a.refine = h->refine_steps;
     a.want_trace = want_trace ? 1 : 0;
     auto take = [&](size_t bytes) {
@@ -364,8 +382,8 @@ int run_lattice(dmftgpu_handle h, int method, int mode, const double* sigma, dou
     if (mode == dg::MODE_FWD) a.pG = (cd*)take(C * nw_tot * m2 * sizeof(cd));
     if (mode != dg::MODE_FWD) a.pTr2 = (cd*)take(C * nw_tot * sizeof(cd));
     if (mode == dg::MODE_VJP) {
-        a.pSb = (cd*)take(C * L.n_freq * m2 * sizeof(cd));
-        a.pMu = (cd*)take(C * L.n_freq * sizeof(cd));
+        a.pSb = (cd*)take(C * n_freq * m2 * sizeof(cd));
+        a.pMu = (cd*)take(C * n_freq * sizeof(cd));
     }
     if (method != DMFTGPU_PLAIN_F64) a.pSigAll = (cd*)take((size_t)nw_tot * m2 * sizeof(cd));
     if (h->timing) HIPCHK(hipEventRecord(h->ev0[mode], h->stream));
@@ -389,25 +407,26 @@ int run_lattice(dmftgpu_handle h, int method, int mode, const double* sigma, dou
         HIPCHK(dg::launch_reduce_chunks(a.pTr2, P.n_chunks, nw_tot, S.tr2, h->stream));
     }
     if (mode == dg::MODE_VJP) {
-        S.Sb = (cd*)take((size_t)L.n_freq * m2 * sizeof(cd));
-        S.mu = (cd*)take(L.n_freq * sizeof(cd));
-        HIPCHK(dg::launch_reduce_chunks(a.pSb, P.n_chunks, (size_t)L.n_freq * m2, S.Sb, h->stream));
-        HIPCHK(dg::launch_reduce_chunks(a.pMu, P.n_chunks, L.n_freq, S.mu, h->stream));
+        S.Sb = (cd*)take((size_t)n_freq * m2 * sizeof(cd));
+        S.mu = (cd*)take(n_freq * sizeof(cd));
+        HIPCHK(dg::launch_reduce_chunks(a.pSb, P.n_chunks, (size_t)n_freq * m2, S.Sb, h->stream));
+        HIPCHK(dg::launch_reduce_chunks(a.pMu, P.n_chunks, n_freq, S.mu, h->stream));
     }
     return DMFTGPU_OK;
 }
 
-size_t lattice_ws_bytes(dmftgpu_handle h, int method, int mode)
+size_t lattice_ws_bytes(dmftgpu_handle h, int method, int mode, const RealAxis* ra = nullptr)
 {
     const dmftgpu_lattice& L = h->lat;
     const int nc = L.n_corr, m2 = nc * nc;
-    const size_t nw_tot = (size_t)L.n_freq * L.high_freq;
+    const size_t nw_tot = ra ? (size_t)ra->n_e : (size_t)L.n_freq * L.high_freq;
+    const size_t n_freq = ra ? (size_t)ra->n_e : (size_t)L.n_freq;
     int slots = method == DMFTGPU_PLAIN_F64 ? dg::plain_slots(L.n, mode)
                                             : dg::schur_slots(nc, mode, method == DMFTGPU_SCHUR_F32R,
-                                                              h->schur_layout == 1);
+                                                              h->schur_layout == 1 || ra);
     Plan P = make_plan(L.n_k, (int)nw_tot, slots, h->target_wg);
     size_t C = P.n_chunks + 1;  // +1 for the reduced copy
-    size_t b = (C + 1) * nw_tot * 16 * (2 + (size_t)m2) + C * L.n_freq * 16 * (m2 + 1) + 64 * 256;
+    size_t b = (C + 1) * nw_tot * 16 * (2 + (size_t)m2) + C * n_freq * 16 * (m2 + 1) + 64 * 256;
     return b;
 }
 
@@ -422,7 +441,7 @@ extern "C" int dmftgpu_greenftn_w(dmftgpu_handle h, int method, const double* si
     if (!h->prepared) return fail(h, DMFTGPU_ERR_STATE, "dmftgpu_prepare has not been called");
     if (!valid_method(method)) return fail(h, DMFTGPU_ERR_ARG, "unknown method %d", method);
     if (method != DMFTGPU_PLAIN_F64 && !h->schur_ready)
-        return fail(h, DMFTGPU_ERR_UNSUPPORTED, "Schur method needs n_corr <= 6 (n=%d, n_corr=%d)", h->lat.n, h->lat.n_corr);
+        return fail(h, DMFTGPU_ERR_UNSUPPORTED, "Schur method needs n_corr <= 16 (n=%d, n_corr=%d)", h->lat.n, h->lat.n_corr);
     if (!sigma || !gloc || !num_matrix) return fail(h, DMFTGPU_ERR_ARG, "sigma, gloc and num_matrix are required");
     if (!std::isfinite(mu)) return fail(h, DMFTGPU_ERR_ARG, "mu is not finite");
     const dmftgpu_lattice& L = h->lat;
@@ -470,6 +489,38 @@ extern "C" int dmftgpu_greenftn_w(dmftgpu_handle h, int method, const double* si
     }
     HIPCHK(hipMemcpyAsync(num_matrix, num.data(), sizeof(cd) * m2, hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
+    return DMFTGPU_OK;
+}
+
+// retarded_GreenFtn2 (GreenFtn.cpp:425-466) summed over k, as qs_spectra
+// accumulates it (qs_spectra.cpp:118-131)
+extern "C" int dmftgpu_greenftn_real(dmftgpu_handle h, int method, int n_e, double e0, double de, double eta,
+                                     const double* sigma_r, double mu, double* gloc_r, double* trace)
+{
+    (void)hipGetLastError();  // drop an error left by an earlier failed call
+    if (!h) return DMFTGPU_ERR_ARG;
+    if (!h->prepared) return fail(h, DMFTGPU_ERR_STATE, "dmftgpu_prepare has not been called");
+    if (!valid_method(method)) return fail(h, DMFTGPU_ERR_ARG, "unknown method %d", method);
+    if (method != DMFTGPU_PLAIN_F64 && !h->schur_ready)
+        return fail(h, DMFTGPU_ERR_UNSUPPORTED, "Schur method needs n_corr <= 16 (n=%d, n_corr=%d)", h->lat.n, h->lat.n_corr);
+    if (n_e < 1 || !std::isfinite(e0) || !std::isfinite(de) || !(eta > 0.0) || !std::isfinite(eta) ||
+        !std::isfinite(mu))
+        return fail(h, DMFTGPU_ERR_ARG, "n_e=%d e0=%g de=%g eta=%g mu=%g (eta must be > 0)", n_e, e0, de, eta, mu);
+    if (!sigma_r || !gloc_r) return fail(h, DMFTGPU_ERR_ARG, "sigma_r and gloc_r are required");
+    const dmftgpu_lattice& L = h->lat;
+    const int m2 = L.n_corr * L.n_corr;
+    const RealAxis ra{n_e, e0, de, eta};
+    hipError_t e;
+    size_t need = lattice_ws_bytes(h, method, dg::MODE_FWD, &ra) + 8192;
+    char* ws = (char*)h->ws.get(need, &e);
+    if (!ws) return fail(h, DMFTGPU_ERR_HIP, "workspace %zu bytes: %s", need, hipGetErrorString(e));
+    size_t off = 0;
+    Sums S;
+    int rc = run_lattice(h, method, dg::MODE_FWD, sigma_r, mu, nullptr, 0.0, ws, off, S, trace != nullptr, &ra);
+    if (rc) return rc;
+    const double inv_nk = 1.0 / (double)L.n_k_global;
+    HIPCHK(dg::launch_scale(S.Gsum, (size_t)n_e * m2, inv_nk, (cd*)gloc_r, h->stream));
+    if (trace) HIPCHK(dg::launch_scale(S.tr, (size_t)n_e, inv_nk, (cd*)trace, h->stream));
     return DMFTGPU_OK;
 }
 

@@ -32,7 +32,25 @@ struct LatArgs {
     cd* pSigAll;       // Schur: Sigma expanded to all nw_tot frequencies [nw_tot][nc*nc]
     int want_trace;    // FWD: also accumulate tr G (total N)
     const double* nbar_dev;  // VJP: device copy of nbar (overrides nbar when set)
+    // real axis (retarded_GreenFtn2): frequency point nw is z = e0 + nw*de + i eta
+    // instead of i w_n (plain and column-per-lane kernels, FWD mode)
+    int real_axis;
+    double e0, de, eta;
+    // per-k blocks of the one-matrix-per-lane Schur kernels [nk][schur_kdata_size]
+    const cd* kdata;
 };
+
+// (real shift added to mu, imaginary part) of frequency point nw
+__device__ __forceinline__ void freq_point(const LatArgs& a, int nw, double& mu, double& w)
+{
+    if (a.real_axis) {
+        mu = a.mu + a.e0 + nw * a.de;
+        w = a.eta;
+    } else {
+        mu = a.mu;
+        w = matsubara(nw, a.beta);
+    }
+}
 
 // plain FP64 full inverse; returns 0 or a HIP error
 hipError_t launch_lattice_plain(const LatArgs& a, int mode, hipStream_t s, int* wtile_out);
@@ -75,6 +93,10 @@ hipError_t launch_cols_16(const LatArgs& a, int mode, bool fp32, const cd* sig_a
 
 // W = H_cd V_d per k (Schur preparation)
 hipError_t launch_schur_prep(const cd* Hk, const cd* Vd, int n, int nc, int nk, cd* W, hipStream_t s);
+// per-k blocks [H_cc][P_d pairs][e_d] of the one-matrix-per-lane kernels (n_c <= 6)
+int schur_kdata_size(int n, int nc);
+hipError_t launch_schur_kdata(const cd* Hk, const cd* W, const double* ed, int n, int nc, int nk, cd* out,
+                              hipStream_t s);
 
 // Batched Hermitian eigen-decomposition of X_b[off:off+m, off:off+m] (+ add on
 // the leading nadd x nadd block).  w [batch][m] ascending, V [batch][m][m] (may be null).

@@ -51,8 +51,9 @@ __global__ __launch_bounds__(WG) void k_lattice_plain(LatArgs a)
     const bool wvalid = nw < a.nw_tot;
     const int k0 = blockIdx.y * a.k_per_chunk;
     const int k1 = min(a.nk, k0 + a.k_per_chunk);
-    const double wn = matsubara(nw, a.beta);
-    const cd zmu = cd{a.mu, wn};
+    double mun, wn;
+    freq_point(a, nw, mun, wn);
+    const cd zmu = cd{mun, wn};
     const bool vjp_here = (MODE == MODE_VJP) && wvalid && nw < a.n_freq;
 
     for (int e = tid; e < 2 * NN; e += WG) sH[e] = cd{0.0, 0.0};

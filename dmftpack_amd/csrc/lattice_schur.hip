@@ -111,7 +111,12 @@ __device__ __forceinline__ void stage_k(const LatArgs& a, int k, cd* buf, int nd
     for (int t = threadIdx.x; t < nd; t += WG) e[t] = ed[t];
 }
 
-template <int NC, int MODE, bool F32R, int WG, bool TRACE, int REFINE>
+// GK: the per-k block (same layout as the LDS staging buffer) is read straight
+// from a.kdata in global memory; its addresses are wave-uniform, so the loads
+// become scalar loads and the products against it take SGPR operands, with no
+// LDS traffic and no barriers in the k loop.  Without GK the block is staged
+// in LDS (double buffered) by the workgroup.
+template <int NC, int MODE, bool F32R, int WG, bool TRACE, int REFINE, bool GK>
 __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restrict__ sig_all, int wtile, int chunk,
                                            cd* lds)
 {
@@ -146,12 +151,12 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
 #pragma unroll
             for (int j = 0; j < (ACC_LDS ? 1 : NC); ++j) acc[i][j] = cd{0.0, 0.0};
     }
-    if (k0 < k1) stage_k<NC, WG>(a, k0, sbuf, nd);
+    if (!GK && k0 < k1) stage_k<NC, WG>(a, k0, sbuf, nd);
     cd trA = cd{0.0, 0.0}, tr2A = cd{0.0, 0.0}, muA = cd{0.0, 0.0};
-    __syncthreads();
+    if (!GK) __syncthreads();
 
     for (int k = k0; k < k1; ++k) {
-        const cd* B = sbuf + ((k - k0) & 1) * bs;
+        const cd* B = GK ? a.kdata + (size_t)k * bs : sbuf + ((k - k0) & 1) * bs;
         const cd* Hcc = B;
         const cd* P = B + M2;
         const double* ed = (const double*)(P + nd * NP);
@@ -364,9 +369,11 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
                 }
             }
         }
-        __syncthreads();
-        if (k + 1 < k1) stage_k<NC, WG>(a, k + 1, sbuf + (((k + 1 - k0) & 1) * bs), nd);
-        __syncthreads();
+        if (!GK) {
+            __syncthreads();
+            if (k + 1 < k1) stage_k<NC, WG>(a, k + 1, sbuf + (((k + 1 - k0) & 1) * bs), nd);
+            __syncthreads();
+        }
     }
 
     if (!wvalid) return;
@@ -395,7 +402,7 @@ template <int NC, int MODE, bool F32R, int WG, bool TRACE, int REFINE>
 __global__ __launch_bounds__(WG) void k_lattice_schur(LatArgs a, const cd* __restrict__ sig_all)
 {
     extern __shared__ cd lds[];
-    schur_item<NC, MODE, F32R, WG, TRACE, REFINE>(a, sig_all, blockIdx.x, blockIdx.y, lds);
+    schur_item<NC, MODE, F32R, WG, TRACE, REFINE, true>(a, sig_all, blockIdx.x, blockIdx.y, lds);
 }
 
 // Persistent form over a work list spanning several lattices of the same n_c:
@@ -409,7 +416,7 @@ __global__ __launch_bounds__(WG) void k_lattice_schur_multi(const LatArgs* __res
     for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
         const int4 item = items[it];  // {system, wtile, chunk, -}
         const LatArgs a = args[item.x];
-        schur_item<NC, MODE, F32R, WG, TRACE, REFINE>(a, a.pSigAll, item.y, item.z, lds);
+        schur_item<NC, MODE, F32R, WG, TRACE, REFINE, false>(a, a.pSigAll, item.y, item.z, lds);
         __syncthreads();
     }
 }
@@ -432,17 +439,17 @@ int schur_slots(int nc, int mode, bool fp32, bool cols)
 }
 
 template <int NC, int MODE, bool F32R, int WG>
-size_t schur_shm(int nd)
+size_t schur_shm(int nd, bool staged)
 {
     constexpr bool ACC_LDS = (MODE == MODE_VJP) || (MODE == MODE_FWD && F32R);
-    return sizeof(cd) * (2 * (size_t)stage_size<NC>(nd) + (ACC_LDS ? (size_t)NC * NC * WG : 0));
+    return sizeof(cd) * ((staged ? 2 * (size_t)stage_size<NC>(nd) : 0) + (ACC_LDS ? (size_t)NC * NC * WG : 0));
 }
 
 template <int NC, int MODE, bool F32R, bool TRACE, int REFINE, int WG>
 static hipError_t launch_nc_w(const LatArgs& a, const cd* sig_all, hipStream_t s, const SchurMulti* multi)
 {
     if (multi) {
-        size_t shm = schur_shm<NC, MODE, F32R, WG>(multi->max_nd);
+        size_t shm = schur_shm<NC, MODE, F32R, WG>(multi->max_nd, true);
         hipError_t e = hipFuncSetAttribute((const void*)k_lattice_schur_multi<NC, MODE, F32R, WG, TRACE, REFINE>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         if (e != hipSuccess) return e;
@@ -451,7 +458,7 @@ static hipError_t launch_nc_w(const LatArgs& a, const cd* sig_all, hipStream_t s
         return hipGetLastError();
     }
     const int nd = a.n - NC;
-    size_t shm = schur_shm<NC, MODE, F32R, WG>(nd);
+    size_t shm = schur_shm<NC, MODE, F32R, WG>(nd, false);
     dim3 grid((a.nw_tot + WG - 1) / WG, a.n_chunks);
     hipError_t e = hipFuncSetAttribute((const void*)k_lattice_schur<NC, MODE, F32R, WG, TRACE, REFINE>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
@@ -532,7 +539,7 @@ hipError_t launch_lattice_schur(const LatArgs& a, int mode, bool fp32, hipStream
 {
     hipError_t e = launch_sigma_all(a, s);
     if (e != hipSuccess) return e;
-    if (a.nc > 6 || cols) {
+    if (a.nc > 6 || cols || a.real_axis) {  // the per-lane kernels take Matsubara points only
         switch (cols_bucket(a.nc)) {
             case 4: return launch_cols_4(a, mode, fp32, a.pSigAll, s);
             case 5: return launch_cols_5(a, mode, fp32, a.pSigAll, s);
@@ -574,6 +581,47 @@ __global__ void k_schur_prep(const cd* __restrict__ Hk, const cd* __restrict__ V
     cd s = cd{0.0, 0.0};
     for (int e = 0; e < nd; ++e) s = cfma(H[aa + (size_t)(nc + e) * n], V[e + (size_t)d * nd], s);
     W[t] = s;
+}
+
+// Per-k block of the one-matrix-per-lane kernels, laid out as the LDS
+// staging buffer of stage_k: [H_cc nc*nc][P_d(a<=b) nd*np][e_d nd doubles],
+// P_d(a, b) = W_ad conj(W_bd), pairs ordered b-major.
+__global__ void k_schur_kdata(const cd* __restrict__ Hk, const cd* __restrict__ W, const double* __restrict__ ed, int n,
+                              int nc, int nk, int bs, cd* __restrict__ out)
+{
+    const int nd = n - nc, np = nc * (nc + 1) / 2;
+    const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= (size_t)nk * bs) return;
+    const int k = (int)(gid / bs), t = (int)(gid % bs);
+    cd v = cd{0.0, 0.0};
+    if (t < nc * nc) {
+        v = Hk[(size_t)k * n * n + (t % nc) + (size_t)(t / nc) * n];
+    } else if (t < nc * nc + nd * np) {
+        const int q = t - nc * nc, d = q / np, p = q % np;
+        int ib = 0;
+        while ((ib + 1) * (ib + 2) / 2 <= p) ++ib;
+        const int ia = p - ib * (ib + 1) / 2;
+        const cd* Wk = W + (size_t)k * nc * nd;
+        v = cmulc(Wk[ia + d * nc], Wk[ib + d * nc]);
+    } else {
+        const int q = 2 * (t - nc * nc - nd * np);  // two e_d per cd slot
+        const double* e = ed + (size_t)k * nd;
+        v.x = q < nd ? e[q] : 0.0;
+        v.y = q + 1 < nd ? e[q + 1] : 0.0;
+    }
+    out[gid] = v;
+}
+
+int schur_kdata_size(int n, int nc) { return nc * nc + (n - nc) * (nc * (nc + 1) / 2) + (n - nc + 1) / 2; }
+
+hipError_t launch_schur_kdata(const cd* Hk, const cd* W, const double* ed, int n, int nc, int nk, cd* out,
+                              hipStream_t s)
+{
+    const int bs = schur_kdata_size(n, nc);
+    const size_t total = (size_t)nk * bs;
+    hipLaunchKernelGGL(k_schur_kdata, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, Hk, W, ed, n, nc, nk, bs,
+                       out);
+    return hipGetLastError();
 }
 
 hipError_t launch_schur_prep(const cd* Hk, const cd* Vd, int n, int nc, int nk, cd* W, hipStream_t s)

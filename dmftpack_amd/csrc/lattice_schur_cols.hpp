@@ -95,8 +95,8 @@ __global__ __launch_bounds__(cols_wg(MODE, PREC)) void k_lattice_schur_cols(LatA
     const int nwc = nw < a.nw_tot ? nw : a.nw_tot - 1;
     const int k0 = blockIdx.y * a.k_per_chunk;
     const int k1 = min(a.nk, k0 + a.k_per_chunk);
-    const double w = matsubara(nwc, a.beta);
-    const double mu = a.mu;
+    double mu, w;
+    freq_point(a, nwc, mu, w);
     const bool vjp_here = (MODE == MODE_VJP) && wvalid && nw < a.n_freq;
     const double cn = (MODE == MODE_VJP) ? 2.0 * (a.nbar_dev ? *a.nbar_dev : a.nbar) / a.beta : 0.0;
 

@@ -211,6 +211,16 @@ class LatticeGF(Handle):
         self.check(self.lib.dmftgpu_density_matrix(self.h, _ptr(s), mu, _ptr(out)))
         return from_dev_cm(out)
 
+    def greenftn_real(self, sigma_r, mu, e0, de, eta, method=PLAIN_F64):
+        """retarded_GreenFtn2 (GreenFtn.cpp:425-466) summed over k on E_j = e0 + j de:
+        returns (G^R_loc [n_e][nc][nc], (1/Nk) sum_k tr G^R [n_e])."""
+        s = to_dev_cm(sigma_r)
+        ne = s.shape[0]
+        g = torch.empty((ne, self.nc, self.nc), dtype=torch.complex128, device="cuda")
+        tr = torch.empty((ne,), dtype=torch.complex128, device="cuda")
+        self.check(self.lib.dmftgpu_greenftn_real(self.h, method, ne, e0, de, eta, _ptr(s), mu, _ptr(g), _ptr(tr)))
+        return from_dev_cm(g), tr.cpu().numpy()
+
     def density(self, sigma, mu, method=PLAIN_F64):
         """(N(mu), dN/dmu) -- GiwToNele (GreenFtn.cpp:513-635) plus its derivative."""
         s = self.sigma_dev(sigma)

@@ -110,6 +110,19 @@ int dmftgpu_static_eig(dmftgpu_handle h, double* eps, double* U);
 int dmftgpu_greenftn_w(dmftgpu_handle h, int method, const double* sigma, double mu, double* gloc, double* gloc_hf,
                        double* num_matrix, double* n_total);
 
+/* Real-axis Green's function of retarded_GreenFtn2 (GreenFtn.cpp:425-466),
+ * summed over k as qs_spectra accumulates it (qs_spectra.cpp:118-131):
+ *   G^R(k, E_j) = [(E_j + mu + i eta) I - H(k) - P Sigma^R(E_j) P^H]^{-1},
+ *   E_j = e0 + j de, j < n_e, eta > 0 (the reference's `infinitesimal`)
+ *   sigma_r [n_e][nc][nc]  Sigma^R(E_j)
+ *   gloc_r  [n_e][nc][nc]  (1/n_k_global) sum_k G^R_cc (full correlated block)
+ *   trace   [n_e] complex, optional (NULL): (1/n_k_global) sum_k tr G^R over
+ *           the model space (-Im trace / pi is the total spectral weight)
+ * Uses the H(k) bound by dmftgpu_prepare.  The Schur methods run the
+ * column-per-lane kernels here. */
+int dmftgpu_greenftn_real(dmftgpu_handle h, int method, int n_e, double e0, double de, double eta,
+                          const double* sigma_r, double mu, double* gloc_r, double* trace);
+
 /* Per-k density matrix densityMatDFT (GreenFtn.cpp:291-319): dens [n_k][n][n].
  * Full n x n inverse, DMFTGPU_PLAIN_F64 only. */
 int dmftgpu_density_matrix(dmftgpu_handle h, const double* sigma, double mu, double* dens);

@@ -322,4 +322,115 @@ void FT_t_to_w(dmftgpu_handle h, const Globals& g, Matrix* delta_w, Matrix* delt
     }
 }
 
+// void Construct_hyb_delta(int impurityDim, std::vector<int> impurityOrbit, ImgFreqFtn& SelfE_w,
+//                          std::vector<MatrixXcd> Gw, double mu, ImgFreqFtn& weiss_fieldTB, int atom,
+//                          MatrixXcd& SolverBasis)  (Construct_hyb_delta.cpp:47)
+// The global impurity_site_Hamiltonian the reference reads becomes an argument;
+// weiss_fieldTB needs setValueSubMat(n, atom, h1, h2, value) (ImgTimeFtn.h).
+template <class Matrix, class SelfEnergy, class Weiss>
+void Construct_hyb_delta(dmftgpu_handle h, const Globals& g, int impurityDim, const std::vector<int>& impurityOrbit,
+                         SelfEnergy& SelfE_w, const std::vector<Matrix>& Gw, double mu, Weiss& weiss_fieldTB, int atom,
+                         const Matrix& impurity_site_Hamiltonian)
+{
+    if (impurityDim == 0) return;
+    const int nc = (int)Gw[0].rows();
+    DevBuf gl((size_t)g.N_freq * nc * nc), sig((size_t)g.N_freq * nc * nc), hi((size_t)nc * nc),
+        dl((size_t)g.N_freq * impurityDim * impurityDim);
+    gl.upload(pack(Gw, g.N_freq, nc).data());
+    sig.upload(pack_sigma(SelfE_w, g.N_freq, nc).data());
+    std::vector<cplx> hh((size_t)nc * nc);
+    for (int j = 0; j < nc; ++j)
+        for (int i = 0; i < nc; ++i) hh[i + (size_t)j * nc] = impurity_site_Hamiltonian(i, j);
+    hi.upload(hh.data());
+    die(h, dmftgpu_hyb_delta(h, nc, g.N_freq, g.beta, mu, (const double*)gl.p, (const double*)sig.p,
+                             (const double*)hi.p, impurityDim, impurityOrbit.data(), (double*)dl.p),
+        "Construct_hyb_delta");
+    std::vector<cplx> hd(dl.n);
+    dl.download(hd.data());
+    const size_t d2 = (size_t)impurityDim * impurityDim;
+    for (int n = 0; n < g.N_freq; ++n)
+        for (int h2 = 0; h2 < impurityDim; ++h2)
+            for (int h1 = 0; h1 < impurityDim; ++h1)
+                weiss_fieldTB.setValueSubMat(n, atom, h1, h2, hd[n * d2 + h1 + (size_t)h2 * impurityDim]);
+}
+
+// void write_hyb_t(ImgFreqFtn& weiss_field, MatrixXcd* delta_t, int atom)  (Construct_hyb_delta.cpp:126)
+// Delta(iw) of the atom's NSpinOrbit_per_atom orbitals (getValue(n, h1, h2)) -> Delta(tau)
+// through the moment fit and transform (FourierTransform(delta_w, delta_t, false), :136).
+template <class Matrix, class Weiss>
+void write_hyb_t(dmftgpu_handle h, const Globals& g, Weiss& weiss_field, Matrix* delta_t, int NSpinOrbit_per_atom)
+{
+    std::vector<Matrix> delta_w(g.N_freq);
+    for (int n = 0; n < g.N_freq; ++n) {
+        delta_w[n].setZero(NSpinOrbit_per_atom, NSpinOrbit_per_atom);
+        for (int h2 = 0; h2 < NSpinOrbit_per_atom; ++h2)
+            for (int h1 = 0; h1 < NSpinOrbit_per_atom; ++h1) delta_w[n](h1, h2) = weiss_field.getValue(n, h1, h2);
+    }
+    FourierTransform(h, g, delta_w.data(), delta_t, false);
+}
+
+// Hopping list of an Hk.HWR file (read_HR, read_TB_parm.cpp:6-76): the header
+// lines "Norbital of Atom = N", then "R1 R2 R3 atom1 atom2 i j Re Im" lines
+// with 1-based atoms and orbitals.  Orbital indices are returned 0-based and
+// global: (orbitals of the atoms before atom1) + i - 1.
+struct HoppingList {
+    std::vector<int> accumulated;  // accumulated_Num_SpinOrbital, size atoms + 1
+    std::vector<int> idx;          // [n_hop][5]: R1 R2 R3 i j
+    std::vector<cplx> t;           // [n_hop]
+    int n_orbital() const { return accumulated.empty() ? 0 : accumulated.back(); }
+};
+
+inline HoppingList read_HR(const char* path)
+{
+    HoppingList out;
+    std::FILE* f = std::fopen(path, "r");
+    if (!f) {
+        std::fprintf(stderr, "Cannot open Hopping data file...\n");  // the reference exits here (:9-11)
+        std::exit(1);
+    }
+    out.accumulated.push_back(0);
+    char line[512];
+    while (std::fgets(line, sizeof(line), f)) {
+        int no = 0;
+        if (std::sscanf(line, " Norbital of Atom = %d", &no) == 1) {
+            out.accumulated.push_back(out.accumulated.back() + no);
+            continue;
+        }
+        int r[3], a1, a2, i, j;
+        double re, im;
+        if (std::sscanf(line, "%d %d %d %d %d %d %d %lf %lf", &r[0], &r[1], &r[2], &a1, &a2, &i, &j, &re, &im) != 9)
+            continue;
+        const int i0 = out.accumulated[a1 - 1] + i - 1, j0 = out.accumulated[a2 - 1] + j - 1;
+        out.idx.insert(out.idx.end(), {r[0], r[1], r[2], i0, j0});
+        out.t.emplace_back(re, im);
+    }
+    std::fclose(f);
+    return out;
+}
+
+// H_k_inModelSpace[k] = sum_R H(R) exp(-i k.a.R)  (ConstructModelHamiltonian.cpp:58-77)
+// kphase [knum][3] holds kmesh[k][axis] * a_axis.
+template <class Matrix>
+void Construct_Hk(dmftgpu_handle h, const HoppingList& hr, const std::vector<double>& kphase,
+                  std::vector<Matrix>& H_k)
+{
+    const int n = hr.n_orbital();
+    const int nk = (int)(kphase.size() / 3);
+    DevBuf hk((size_t)nk * n * n);
+    double* dk = nullptr;
+    hip_ok(hipMalloc((void**)&dk, sizeof(double) * kphase.size()), "hipMalloc");
+    hip_ok(hipMemcpy(dk, kphase.data(), sizeof(double) * kphase.size(), hipMemcpyHostToDevice), "upload");
+    const int rc = dmftgpu_hk_from_hr(h, n, (int)hr.t.size(), hr.idx.data(), (const double*)hr.t.data(), nk, dk,
+                                      (double*)hk.p);
+    (void)hipFree(dk);
+    die(h, rc, "Construct_Hk");
+    std::vector<cplx> hh(hk.n);
+    hk.download(hh.data());
+    H_k.resize(nk);
+    for (int k = 0; k < nk; ++k) {
+        H_k[k].setZero(n, n);
+        std::memcpy(H_k[k].data(), &hh[(size_t)k * n * n], sizeof(cplx) * n * n);
+    }
+}
+
 }  // namespace dmftgpu_shim

@@ -34,6 +34,8 @@
  *                        returns that Delta; scalar closed form
  *   orc_hk_from_hr       closed-form dispersions of the t2g model and of the
  *                        Hubbard-model Hk.HWR fixture, Hermiticity
+ *   orc_greenftn_real    scalar closed form of the orbital-diagonal t2g model,
+ *                        spectral sum rule, numpy.linalg.inv brute force
  */
 #include <complex.h>
 #include <math.h>
@@ -1048,4 +1050,39 @@ void orc_hk_from_hr(int n, int n_hop, const int *hop_idx, const cplx *hop, int n
             H[IDX(r[3], r[4], n)] += hop[q] * cexp(-I * (p[0] * r[0] + p[1] * r[1] + p[2] * r[2]));
         }
     }
+}
+
+/* retarded_GreenFtn2 (GreenFtn.cpp:425-466) in the model space, summed over k
+ * as qs_spectra accumulates it (qs_spectra.cpp:118-131):
+ *   A = (E_j + mu + i eta) I - H(k) - P Sigma_j P^H,  G = A^{-1} (Eigen inverse),
+ *   gloc_r[j] = sum_k G_cc / n_k_global,  trace[j] = sum_k tr G / n_k_global. */
+int orc_greenftn_real(const orc_lattice *L, const cplx *Hk, int n_e, double e0, double de, double eta,
+                      const cplx *sigma_r, double mu, cplx *gloc_r, cplx *trace, int nthreads)
+{
+    const int n = L->n, nc = L->n_corr;
+    const size_t nn = (size_t)n * n, m2 = (size_t)nc * nc;
+    int bad = 0;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1) reduction(| : bad)
+#endif
+    for (int j = 0; j < n_e; ++j) {
+        cplx *A = malloc(sizeof(cplx) * nn), *G = malloc(sizeof(cplx) * nn);
+        const cplx z = e0 + j * de + mu + I * eta;
+        cplx tr = 0.0;
+        for (size_t e = 0; e < m2; ++e) gloc_r[j * m2 + e] = 0.0;
+        for (int k = 0; k < L->n_k; ++k) {
+            for (size_t e = 0; e < nn; ++e) A[e] = -Hk[k * nn + e];
+            for (int b = 0; b < nc; ++b)
+                for (int a = 0; a < nc; ++a) A[IDX(a, b, n)] -= sigma_r[j * m2 + IDX(a, b, nc)];
+            for (int a = 0; a < n; ++a) A[IDX(a, a, n)] += z;
+            bad |= orc_lu_inverse(n, A, G);
+            for (int b = 0; b < nc; ++b)
+                for (int a = 0; a < nc; ++a) gloc_r[j * m2 + IDX(a, b, nc)] += G[IDX(a, b, n)];
+            for (int a = 0; a < n; ++a) tr += G[IDX(a, a, n)];
+        }
+        for (size_t e = 0; e < m2; ++e) gloc_r[j * m2 + e] /= (double)L->n_k_global;
+        if (trace) trace[j] = tr / (double)L->n_k_global;
+        free(A); free(G);
+    }
+    return bad;
 }

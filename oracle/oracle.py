@@ -54,6 +54,8 @@ def lib():
         _lib.orc_num_threads.restype = C.c_int
         _lib.orc_hyb_delta.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, P, P, P, C.c_int, P, P]
         _lib.orc_hk_from_hr.argtypes = [C.c_int, C.c_int, P, P, C.c_int, P, P]
+        _lib.orc_greenftn_real.argtypes = [C.POINTER(OrcLattice), P, C.c_int, C.c_double, C.c_double, C.c_double,
+                                           P, C.c_double, P, P, C.c_int]
     return _lib
 
 
@@ -224,3 +226,19 @@ def hk_from_hr(n, hops, kphase):
     out = np.zeros((nk, n, n), np.complex128)
     lib().orc_hk_from_hr(n, len(hops), _p(idx), _p(t), nk, _p(kp), _p(out))
     return _from_cm(out)
+
+
+def greenftn_real(Hk, sigma_r, mu, e0, de, eta, nthreads=1):
+    """retarded_GreenFtn2 summed over k: (G^R_loc [n_e][nc][nc], (1/Nk) sum_k tr G^R [n_e])."""
+    Hk = np.asarray(Hk, dtype=np.complex128)
+    sigma_r = np.asarray(sigma_r, dtype=np.complex128)
+    nk, n, _ = Hk.shape
+    ne, nc, _ = sigma_r.shape
+    L = _lat(n, nc, 1, nk, nk, ne, 1.0, 1)
+    h, s = _cm(Hk), _cm(sigma_r)
+    g = np.zeros((ne, nc, nc), np.complex128)
+    tr = np.zeros(ne, np.complex128)
+    rc = lib().orc_greenftn_real(C.byref(L), _p(h), ne, e0, de, eta, _p(s), mu, _p(g), _p(tr), nthreads)
+    if rc:
+        raise ValueError("singular matrix")
+    return _from_cm(g), tr

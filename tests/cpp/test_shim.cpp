@@ -26,6 +26,8 @@ struct Mat {  // the subset of Eigen::MatrixXcd the adapter uses
 struct Sigma {  // the subset of ImgFreqFtn the adapter uses
     std::vector<Mat> w;
     Mat getMatrix(int n) { return w[n]; }
+    cplx getValue(int n, int i, int j) { return w[n](i, j); }
+    void setValueSubMat(int n, int /*atom*/, int i, int j, cplx v) { w[n](i, j) = v; }
 };
 
 extern "C" {
@@ -34,6 +36,8 @@ int orc_greenftn_w(const orc_lattice*, const cplx*, const cplx*, const cplx*, do
 int orc_asymto_moments(int, int, double, const cplx*, int, cplx*, int*);
 int orc_ft_w2tau(int, int, int, double, const cplx*, const cplx*, cplx*, int*, int);
 int orc_ft_tau2w(int, int, int, double, const cplx*, cplx*, int*);
+int orc_hyb_delta(int, int, double, double, const cplx*, const cplx*, const cplx*, int, const int*, cplx*);
+void orc_hk_from_hr(int, int, const int*, const cplx*, int, const double*, cplx*);
 }
 
 static unsigned long long rng_state = 12345;
@@ -144,6 +148,60 @@ int main()
     orc_ft_tau2w(nc, nf, ntau, beta, ggt.data(), ow.data(), &viol);
     for (int w = 0; w < nf; ++w) std::copy(gw2[w].v.begin(), gw2[w].v.end(), gw.begin() + (size_t)w * nc * nc);
     check("FT_t_to_w", relerr(gw.data(), ow.data(), ow.size()), 1e-12);
+
+    // Construct_hyb_delta for the impurity of cluster 1 (orbitals 2, 3), then write_hyb_t
+    {
+        std::vector<int> orbit = {2, 3};
+        Mat himp;
+        himp.setZero(nc, nc);
+        for (int a = 0; a < nc; ++a) himp(a, a) = 0.1 * a;
+        himp(2, 3) = himp(3, 2) = 0.05;
+        Sigma weiss;
+        weiss.w.resize(nf);
+        for (auto& m : weiss.w) m.setZero(2, 2);
+        dmftgpu_shim::Construct_hyb_delta(h, g, 2, orbit, S, Gw, mu, weiss, 1, himp);
+        std::vector<cplx> od((size_t)nf * 4), gd((size_t)nf * 4);
+        orc_hyb_delta(nc, nf, beta, mu, got.data(), sg.data(), himp.v.data(), 2, orbit.data(), od.data());
+        for (int w = 0; w < nf; ++w) std::copy(weiss.w[w].v.begin(), weiss.w[w].v.end(), gd.begin() + (size_t)w * 4);
+        check("Construct_hyb_delta", relerr(gd.data(), od.data(), od.size()), 1e-12);
+        std::vector<Mat> dt(ntau + 1);
+        dmftgpu_shim::write_hyb_t(h, g, weiss, dt.data(), 2);
+        std::vector<cplx> dm4(16), odt((size_t)(ntau + 1) * 4), gdt(odt.size());
+        orc_asymto_moments(2, nf, beta, od.data(), 0, dm4.data(), nullptr);
+        int pt = 0;
+        orc_ft_w2tau(2, nf, ntau, beta, od.data(), dm4.data(), odt.data(), &pt, 1);
+        for (int t = 0; t <= ntau; ++t) std::copy(dt[t].v.begin(), dt[t].v.end(), gdt.begin() + (size_t)t * 4);
+        check("write_hyb_t", relerr(gdt.data(), odt.data(), odt.size()), 1e-8);
+    }
+
+    // read_HR of a two-atom Hk.HWR file, then Construct_Hk on a small mesh
+    {
+        const char* path = "/tmp/dmftgpu_shim_test.HWR";
+        std::FILE* f = std::fopen(path, "w");
+        std::fprintf(f, "Norbital of Atom = 2\nNorbital of Atom = 1\n");
+        std::fprintf(f, "+0 +0 +0 1 1 1 1  0.50 0.0\n+0 +0 +0 2 2 1 1 -0.70 0.0\n");
+        std::fprintf(f, "+1 +0 +0 1 2 2 1 -0.25 0.10\n-1 +0 +0 2 1 1 2 -0.25 -0.10\n");
+        std::fprintf(f, "+0 +1 +0 1 1 1 2  0.05 0.02\n+0 -1 +0 1 1 2 1  0.05 -0.02\n");
+        std::fclose(f);
+        dmftgpu_shim::HoppingList hr = dmftgpu_shim::read_HR(path);
+        const int nkk = 12;
+        std::vector<double> kp(3 * nkk);
+        for (int k = 0; k < nkk; ++k)
+            for (int a = 0; a < 3; ++a) kp[3 * k + a] = -PI + PI / 4 + 2 * PI * ((k >> a) % 4) / 4.0;
+        std::vector<Mat> Hb;
+        dmftgpu_shim::Construct_Hk(h, hr, kp, Hb);
+        const int no = hr.n_orbital();
+        std::vector<cplx> oh((size_t)nkk * no * no), gh(oh.size());
+        orc_hk_from_hr(no, (int)hr.t.size(), hr.idx.data(), hr.t.data(), nkk, kp.data(), oh.data());
+        for (int k = 0; k < nkk; ++k) std::copy(Hb[k].v.begin(), Hb[k].v.end(), gh.begin() + (size_t)k * no * no);
+        check("read_HR + Construct_Hk", relerr(gh.data(), oh.data(), oh.size()) + (no == 3 && hr.t.size() == 6 ? 0 : 1),
+              1e-14);
+        double herm = 0;
+        for (int k = 0; k < nkk; ++k)
+            for (int i = 0; i < no; ++i)
+                for (int j = 0; j < no; ++j) herm = std::fmax(herm, std::abs(Hb[k](i, j) - std::conj(Hb[k](j, i))));
+        check("Construct_Hk Hermitian", herm, 1e-15);
+    }
 
     dmftgpu_destroy(h);
     std::printf(failures ? "SHIM FAILED\n" : "SHIM OK\n");

@@ -98,3 +98,50 @@ def test_hyb_delta_scalar_closed_form():
     d = O.hyb_delta(g[:, None, None], s[:, None, None], np.array([[0.25]]), [0], beta, mu)
     ref = iw + mu - 0.25 - s - (iw + 0.7)
     assert np.max(np.abs(d[:, 0, 0] - ref)) < 1e-14
+
+
+def t2g_real_sigma(ne, e0, de, seed=2):
+    """Causal diagonal real-axis self-energy S_inf + A/(E - b + i Gamma)."""
+    rng = np.random.default_rng(seed)
+    E = e0 + de * np.arange(ne)
+    sig = np.zeros((ne, 3, 3), complex)
+    for a in range(3):
+        s0, A, b, gam = rng.uniform(0, 1), rng.uniform(0.2, 1), rng.uniform(-1, 1), rng.uniform(0.2, 0.6)
+        sig[:, a, a] = s0 + A / (E - b + 1j * gam)
+    return sig
+
+
+def test_greenftn_real_t2g_closed_form():
+    """Orbital-diagonal H and Sigma: G^R_a(E) = mean_k 1/(E + mu + i eta - eps_a(k) - Sigma_a(E))."""
+    Hk, eps = L.t2g_hk(6)
+    ne, e0, de, eta, mu = 40, -2.0, 0.1, 0.05, 0.2
+    sig = t2g_real_sigma(ne, e0, de)
+    g, tr = O.greenftn_real(Hk, sig, mu, e0, de, eta)
+    E = e0 + de * np.arange(ne)
+    for a in range(3):
+        ref = np.mean(1.0 / (E[:, None] + mu + 1j * eta - eps[None, :, a] - sig[:, a, a][:, None]), axis=1)
+        assert np.max(np.abs(g[:, a, a] - ref)) < 1e-13
+    assert np.max(np.abs(tr - np.trace(g, axis1=1, axis2=2))) < 1e-13
+
+
+def test_greenftn_real_brute_force_and_sum_rule():
+    """d+p model: numpy inverse on a few points; with Sigma = 0 the spectral
+    weight -Im tr G / pi integrates to n (Lorentzian tails outside the window
+    counted analytically)."""
+    Hk = L.random_tb_hk(5, 2, 7)
+    ne, e0, de, eta, mu = 3000, -15.0, 0.01, 0.05, 0.0
+    sig = np.zeros((ne, 2, 2), complex)
+    g, tr = O.greenftn_real(Hk, sig, mu, e0, de, eta, nthreads=4)
+    for j in (0, 777, 2999):
+        z = e0 + j * de + mu + 1j * eta
+        G = np.mean([np.linalg.inv(z * np.eye(5) - H) for H in Hk], axis=0)
+        assert np.max(np.abs(g[j] - G[:2, :2])) < 1e-13
+        assert abs(tr[j] - np.trace(G)) < 1e-13
+    ev = np.concatenate([np.linalg.eigvalsh(H) for H in Hk])
+    lo, hi = e0 - de / 2, e0 + (ne - 0.5) * de
+    # weight of each Lorentzian inside [lo, hi]
+    inside = np.mean([np.sum((np.arctan((hi - x) / eta) - np.arctan((lo - x) / eta)) / np.pi)
+                      for x in ev.reshape(len(Hk), -1)])
+    weight = np.sum(-tr.imag / np.pi) * de
+    assert abs(weight - inside) < 2e-3 * 5
+    assert abs(inside - 5) < 0.02

@@ -95,3 +95,34 @@ def test_hk_from_hr_feeds_lattice(gpu):
     out = lat.greenftn_w(sig, 0.1, method=G.PLAIN_F64)
     assert rel(out["gloc"], ref["gloc"]) <= 1e-12
     lat.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,method,tol", [("t2g", 0, 1e-12), ("dp14", 0, 1e-12), ("dp14", 1, 1e-12),
+                                             ("dp14", 2, 1e-9), ("large32", 1, 1e-12), ("large32", 2, 1e-9)])
+def test_greenftn_real(gpu, name, method, tol):
+    """retarded_GreenFtn2 summed over k on a real-energy grid against the
+    oracle; eta = 0.05 (the reference's infinitesimal), so |G| <= 20."""
+    G, _ = gpu
+    if name == "t2g":
+        Hk, _ = L.t2g_hk(6)
+        nc = 3
+    elif name == "dp14":
+        Hk, nc = L.random_tb_hk(14, 3, 4), 5
+    else:
+        Hk, nc = L.random_tb_hk(32, 2, 4), 10
+    ne, e0, de, eta, mu = 150, -4.0, 0.05, 0.05, 0.1
+    rng = np.random.default_rng(1)
+    E = e0 + de * np.arange(ne)
+    sig = np.zeros((ne, nc, nc), complex)
+    for p in range(2):
+        B = rng.normal(size=(nc, nc)) + 1j * rng.normal(size=(nc, nc))
+        sig += (B @ B.conj().T / nc)[None] / (E - rng.uniform(-1, 1) + 0.3j)[:, None, None]
+    print(f"greenftn_real {name} method {method}", flush=True)
+    ref, tr_ref = O.greenftn_real(Hk, sig, mu, e0, de, eta, nthreads=8)
+    mom = np.zeros((4, nc, nc), complex)
+    lat = G.LatticeGF(Hk, nc, 16, 40.0, mom)
+    got, tr = lat.greenftn_real(sig, mu, e0, de, eta, method=method)
+    assert np.max(np.abs(got - ref)) <= tol * max(1.0, np.max(np.abs(ref)))
+    assert np.max(np.abs(tr - tr_ref)) <= tol * max(1.0, np.max(np.abs(tr_ref)))
+    lat.close()
