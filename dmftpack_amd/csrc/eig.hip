@@ -2,9 +2,10 @@
 //
 // Replaces the per-k SelfAdjointEigenSolver calls of the static part of
 // GreenFtn_w (GreenFtn.cpp:197-215) and the small Hermitian problems of the
-// Fourier transform.  Cyclic Jacobi with complex rotations on a matrix held
-// in LDS; every rotation is applied by up to 64 lanes in parallel (lane i owns
-// row/column i).  Eigenvalues are returned in ascending order like Eigen.
+// Fourier transform.  Jacobi with complex rotations on a matrix held in LDS,
+// in parallel (round-robin) order: the m/2 disjoint rotations of a round are
+// applied together by the 64 lanes.  Eigenvalues are returned in ascending
+// order like Eigen.
 #include "lattice_kernels.hpp"
 
 namespace dg {
@@ -37,6 +38,14 @@ __global__ __launch_bounds__(64) void k_herm_eig(const cd* __restrict__ X, int l
     double nrm = 0.0;
     for (int e = lane; e < m * m; e += 64) nrm += A[e].x * A[e].x + A[e].y * A[e].y;
     for (int o = 32; o > 0; o >>= 1) nrm += __shfl_xor(nrm, o);
+    // Parallel (round-robin) ordering: each round rotates M/2 disjoint pairs at
+    // once (A <- J^H A J with J a product of commuting plane rotations), so a
+    // sweep is M-1 rounds of wave-wide updates instead of m(m-1)/2 rotations
+    // applied one after another.  M = m rounded up to even; pairs touching the
+    // padding index are skipped.
+    const int M = m + (m & 1);
+    const int np = M / 2;
+    double* rc = (double*)(Vs + m * m);  // per pair: c, s, phase (x, y) and p, q
     for (int sweep = 0; sweep < 60; ++sweep) {
         double off2 = 0.0;
         for (int e = lane; e < m * m; e += 64) {
@@ -45,42 +54,80 @@ __global__ __launch_bounds__(64) void k_herm_eig(const cd* __restrict__ X, int l
         }
         for (int o = 32; o > 0; o >>= 1) off2 += __shfl_xor(off2, o);
         if (off2 <= 1e-34 * nrm || off2 == 0.0) break;
-        for (int p = 0; p < m - 1; ++p) {
-            for (int q = p + 1; q < m; ++q) {
-                const cd bpq = A[p + q * m];
-                const double ab = sqrt(bpq.x * bpq.x + bpq.y * bpq.y);
-                if (ab == 0.0) continue;  // uniform across the wave
-                const double app = A[p + p * m].x, aqq = A[q + q * m].x;
-                const cd ph = cd{bpq.x / ab, bpq.y / ab};
-                const cd phc = cconj(ph);
-                const double zeta = (aqq - app) / (2.0 * ab);
-                const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-                const double c = 1.0 / sqrt(1.0 + t * t);
-                const double s = t * c;
-                __syncthreads();
-                if (lane < m) {  // columns p, q of A and V
-                    cd xp = A[lane + p * m], xq = A[lane + q * m];
-                    A[lane + p * m] = csub(cscale(xp, c), cscale(cmul(phc, xq), s));
-                    A[lane + q * m] = cadd(cscale(xp, s), cscale(cmul(phc, xq), c));
-                    cd vp = Vs[lane + p * m], vq = Vs[lane + q * m];
-                    Vs[lane + p * m] = csub(cscale(vp, c), cscale(cmul(phc, vq), s));
-                    Vs[lane + q * m] = cadd(cscale(vp, s), cscale(cmul(phc, vq), c));
-                }
-                __syncthreads();
-                if (lane < m) {  // rows p, q of A
-                    cd xp = A[p + lane * m], xq = A[q + lane * m];
-                    A[p + lane * m] = csub(cscale(xp, c), cscale(cmul(ph, xq), s));
-                    A[q + lane * m] = cadd(cscale(xp, s), cscale(cmul(ph, xq), c));
-                }
-                __syncthreads();
+        for (int r = 0; r < M - 1; ++r) {
+            if (lane < np) {
+                // circle method: (r, M-1) and (r+k, r-k) mod (M-1)
+                int a_, b_;
                 if (lane == 0) {
+                    a_ = r;
+                    b_ = M - 1;
+                } else {
+                    a_ = (r + lane) % (M - 1);
+                    b_ = (r - lane + M - 1) % (M - 1);
+                }
+                const int p = min(a_, b_), q = max(a_, b_);
+                double c = 1.0, sn = 0.0, phx = 1.0, phy = 0.0;
+                if (q < m) {
+                    const cd bpq = A[p + q * m];
+                    const double ab = sqrt(bpq.x * bpq.x + bpq.y * bpq.y);
+                    if (ab != 0.0) {
+                        const double app = A[p + p * m].x, aqq = A[q + q * m].x;
+                        phx = bpq.x / ab;
+                        phy = bpq.y / ab;
+                        const double zeta = (aqq - app) / (2.0 * ab);
+                        const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                        c = 1.0 / sqrt(1.0 + t * t);
+                        sn = t * c;
+                    }
+                }
+                rc[6 * lane] = c;
+                rc[6 * lane + 1] = sn;
+                rc[6 * lane + 2] = phx;
+                rc[6 * lane + 3] = phy;
+                rc[6 * lane + 4] = (double)p;
+                rc[6 * lane + 5] = (double)(q < m ? q : -1);
+            }
+            __syncthreads();
+            // columns p, q of A and V: A <- A J
+            for (int t = lane; t < np * m; t += 64) {
+                const int k = t / m, i = t % m;
+                const int q = (int)rc[6 * k + 5];
+                if (q < 0) continue;
+                const int p = (int)rc[6 * k + 4];
+                const double c = rc[6 * k], sn = rc[6 * k + 1];
+                const cd phc = cd{rc[6 * k + 2], -rc[6 * k + 3]};
+                cd xp = A[i + p * m], xq = A[i + q * m];
+                A[i + p * m] = csub(cscale(xp, c), cscale(cmul(phc, xq), sn));
+                A[i + q * m] = cadd(cscale(xp, sn), cscale(cmul(phc, xq), c));
+                cd vp = Vs[i + p * m], vq = Vs[i + q * m];
+                Vs[i + p * m] = csub(cscale(vp, c), cscale(cmul(phc, vq), sn));
+                Vs[i + q * m] = cadd(cscale(vp, sn), cscale(cmul(phc, vq), c));
+            }
+            __syncthreads();
+            // rows p, q of A: A <- J^H A
+            for (int t = lane; t < np * m; t += 64) {
+                const int k = t / m, j = t % m;
+                const int q = (int)rc[6 * k + 5];
+                if (q < 0) continue;
+                const int p = (int)rc[6 * k + 4];
+                const double c = rc[6 * k], sn = rc[6 * k + 1];
+                const cd ph = cd{rc[6 * k + 2], rc[6 * k + 3]};
+                cd xp = A[p + j * m], xq = A[q + j * m];
+                A[p + j * m] = csub(cscale(xp, c), cscale(cmul(ph, xq), sn));
+                A[q + j * m] = cadd(cscale(xp, sn), cscale(cmul(ph, xq), c));
+            }
+            __syncthreads();
+            if (lane < np) {
+                const int q = (int)rc[6 * lane + 5];
+                if (q >= 0 && rc[6 * lane + 1] != 0.0) {
+                    const int p = (int)rc[6 * lane + 4];
                     A[p + q * m] = cd{0.0, 0.0};
                     A[q + p * m] = cd{0.0, 0.0};
                     A[p + p * m].y = 0.0;
                     A[q + q * m].y = 0.0;
                 }
-                __syncthreads();
             }
+            __syncthreads();
         }
     }
     __syncthreads();
@@ -102,7 +149,8 @@ hipError_t launch_herm_eig(const cd* X, int ldx, int off, int m, const cd* add, 
                            hipStream_t s)
 {
     if (batch <= 0 || m <= 0) return hipSuccess;
-    size_t shm = sizeof(cd) * 2 * (size_t)m * m;
+    if (m > 32) return hipErrorInvalidValue;  // the pair table holds 16 rotations
+    size_t shm = sizeof(cd) * 2 * (size_t)m * m + sizeof(double) * 6 * 17;
     hipLaunchKernelGGL(k_herm_eig, dim3(batch), dim3(64), shm, s, X, ldx, off, m, add, nadd, w, V, batch);
     return hipGetLastError();
 }

@@ -24,12 +24,27 @@
 // >= w_n I for causal Sigma, so elimination without pivoting is stable).
 //
 // FP32R: the Schur complement S is formed in FP64, inverted in FP32, and the
-// FP32 inverse X0 preconditions FP64 iterative refinement
-//   X <- X + X0 (I - S X),
-// one column at a time, `refine` times.
+// FP32 inverse X0 preconditions iterative refinement with FP64 residuals
+//   X = X0 + D,  D <- D + X0 (I - S X),
+// one column at a time, `refine` times; the correction D is kept in FP32.
+#include <type_traits>
+
 #include "lattice_kernels.hpp"
 
 namespace dg {
+
+// Data read through the constant address space: the compiler treats it as
+// invariant for the kernel's duration, so wave-uniform loads become scalar
+// loads (s_load) whose results feed the VALU as SGPR operands.
+typedef __attribute__((address_space(4))) cd kconst_cd;
+typedef __attribute__((address_space(4))) double kconst_double;
+
+__device__ __forceinline__ cd ldc(const cd* p) { return *p; }
+__device__ __forceinline__ cd ldc(const kconst_cd* p)
+{
+    const kconst_double* q = (const kconst_double*)p;
+    return cd{q[0], q[1]};
+}
 
 template <int NC>
 struct Mat {
@@ -120,6 +135,9 @@ template <int NC, int MODE, bool F32R, int WG, bool TRACE, int REFINE, bool GK>
 __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restrict__ sig_all, int wtile, int chunk,
                                            cd* lds)
 {
+    // element types of the k block: constant address space (scalar loads) or LDS
+    using BT = std::conditional_t<GK, kconst_cd, cd>;
+    using BD = std::conditional_t<GK, kconst_double, double>;
     constexpr int NP = NC * (NC + 1) / 2;
     constexpr int M2 = NC * NC;
     // accumulators that do not fit next to the working matrices live in LDS
@@ -156,34 +174,64 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
     if (!GK) __syncthreads();
 
     for (int k = k0; k < k1; ++k) {
-        const cd* B = GK ? a.kdata + (size_t)k * bs : sbuf + ((k - k0) & 1) * bs;
-        const cd* Hcc = B;
-        const cd* P = B + M2;
-        const double* ed = (const double*)(P + nd * NP);
-        // ---- Schur complement S = (z+mu) I - H_cc - Sigma - sum_d g_d P_d
+        // Sigma(iw) is re-read from the cache every k rather than hoisted out
+        // of the loop, where it would hold 4*NC*NC registers for its duration.
+        // The k block is read through the constant address space, which the
+        // compiler treats as invariant: those loads stay scalar despite the
+        // clobber.
+        if (GK) __asm__ __volatile__("" ::: "memory");
+        const cd* sigk = sig;
+        const BT* B;
+        if constexpr (GK) B = (const BT*)(a.kdata + (size_t)k * bs);
+        else B = sbuf + ((k - k0) & 1) * bs;
+        const BT* Hcc = B;
+        const BT* P = B + M2;
+        const BD* ed = (const BD*)(P + nd * NP);
+        // ---- Schur complement S = (z+mu) I - H_cc - Sigma - sum_d g_d P_d.
+        // sum_d g_d P_d = R + i J with R = sum_d Re(g_d) P_d and
+        // J = sum_d Im(g_d) P_d both Hermitian: their upper triangles (real
+        // diagonals) are accumulated in place of S, G[a][b] = R_ab,
+        // G[b][a] = J_ab (a < b), G[a][a] = (R_aa, J_aa), a real-by-complex
+        // update per stored entry (4 FMA per pair instead of 8).
         cd G[NC][NC];
 #pragma unroll
         for (int j = 0; j < NC; ++j)
 #pragma unroll
-            for (int i = 0; i < NC; ++i) {
-                cd v = csub(cneg(Hcc[i + j * NC]), sig[i + j * NC]);
-                if (i == j) v = cadd(v, cd{mu, w});
-                G[i][j] = v;
-            }
+            for (int i = 0; i < NC; ++i) G[i][j] = cd{0.0, 0.0};
         for (int d = 0; d < nd; ++d) {
             const double x = mu - ed[d];
             const double r = 1.0 / fma(x, x, w * w);
-            const cd g = cd{x * r, -w * r};
-            const cd* Pd = P + d * NP;
+            const double gr = x * r, gi = -w * r;
+            const BT* Pd = P + d * NP;
 #pragma unroll
             for (int ib = 0; ib < NC; ++ib)
 #pragma unroll
                 for (int ia = 0; ia <= ib; ++ia) {
-                    const cd p = Pd[ib * (ib + 1) / 2 + ia];
-                    G[ia][ib] = cfms(p, g, G[ia][ib]);
-                    if (ia != ib) G[ib][ia] = cfms(cconj(p), g, G[ib][ia]);
+                    const cd p = ldc(Pd + ib * (ib + 1) / 2 + ia);
+                    if (ia == ib) {  // P_aa is real
+                        G[ia][ia].x = fma(gr, p.x, G[ia][ia].x);
+                        G[ia][ia].y = fma(gi, p.x, G[ia][ia].y);
+                    } else {
+                        G[ia][ib] = cd{fma(gr, p.x, G[ia][ib].x), fma(gr, p.y, G[ia][ib].y)};
+                        G[ib][ia] = cd{fma(gi, p.x, G[ib][ia].x), fma(gi, p.y, G[ib][ia].y)};
+                    }
                 }
         }
+#pragma unroll
+        for (int ib = 0; ib < NC; ++ib)
+#pragma unroll
+            for (int ia = 0; ia <= ib; ++ia) {
+                cd bab = csub(cneg(ldc(Hcc + ia + ib * NC)), sigk[ia + ib * NC]);
+                if (ia == ib) {
+                    bab = cadd(bab, cd{mu, w});
+                    G[ia][ia] = csub(bab, G[ia][ia]);  // - (R_aa + i J_aa)
+                } else {
+                    const cd R = G[ia][ib], J = G[ib][ia];
+                    const cd bba = csub(cneg(ldc(Hcc + ib + ia * NC)), sigk[ib + ia * NC]);
+                    G[ia][ib] = csub(bab, cd{R.x - J.y, R.y + J.x});   // - (R + i J)_ab
+                    G[ib][ia] = csub(bba, cd{R.x + J.y, J.x - R.y});   // - (R + i J)_ba
+                }
+            }
         // ---- G_cc = S^{-1}
         if (!F32R) {
             gj_inverse<NC>(G);
@@ -195,36 +243,44 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
 #pragma unroll
                 for (int j = 0; j < NC; ++j) X0[i][j] = cf{(float)G[i][j].x, (float)G[i][j].y};
             gj_inverse_f32<NC>(X0);
-            cd X[NC][NC];
+            // X = X0 + D with the correction D held in FP32: D is ~kappa*eps32 of
+            // |X|, so its own FP32 rounding (~kappa*eps32^2 of |X|) is the
+            // accuracy floor of the refinement, the same as adding an FP32
+            // correction into an FP64 X, at half the registers.
+            cf D[NC][NC];
 #pragma unroll
             for (int i = 0; i < NC; ++i)
 #pragma unroll
-                for (int j = 0; j < NC; ++j) X[i][j] = cd{(double)X0[i][j].x, (double)X0[i][j].y};
+                for (int j = 0; j < NC; ++j) D[i][j] = cf{0.f, 0.f};
             for (int it = 0; it < REFINE; ++it) {
 #pragma unroll
                 for (int c = 0; c < NC; ++c) {
+                    cd xc[NC];  // column c of X in FP64
+#pragma unroll
+                    for (int l = 0; l < NC; ++l)
+                        xc[l] = cd{(double)X0[l][c].x + (double)D[l][c].x, (double)X0[l][c].y + (double)D[l][c].y};
                     cf rc[NC];  // r = e_c - S x_c, formed in FP64, stored in FP32
 #pragma unroll
                     for (int i = 0; i < NC; ++i) {
                         cd s = cd{i == c ? 1.0 : 0.0, 0.0};
 #pragma unroll
-                        for (int l = 0; l < NC; ++l) s = cfms(G[i][l], X[l][c], s);
+                        for (int l = 0; l < NC; ++l) s = cfms(G[i][l], xc[l], s);
                         rc[i] = cf{(float)s.x, (float)s.y};
                     }
-                    // x_c += X0 r: the correction is ~kappa*eps32 of |x|, so FP32 suffices
 #pragma unroll
                     for (int i = 0; i < NC; ++i) {
-                        cf s = cf{0.f, 0.f};
+                        cf t = D[i][c];
 #pragma unroll
-                        for (int l = 0; l < NC; ++l) s = ffms(cf{-X0[i][l].x, -X0[i][l].y}, rc[l], s);
-                        X[i][c] = cadd(X[i][c], cd{(double)s.x, (double)s.y});
+                        for (int l = 0; l < NC; ++l) t = ffma(X0[i][l], rc[l], t);
+                        D[i][c] = t;
                     }
                 }
             }
 #pragma unroll
             for (int i = 0; i < NC; ++i)
 #pragma unroll
-                for (int j = 0; j < NC; ++j) G[i][j] = X[i][j];
+                for (int j = 0; j < NC; ++j)
+                    G[i][j] = cd{(double)X0[i][j].x + (double)D[i][j].x, (double)X0[i][j].y + (double)D[i][j].y};
         }
 
         if (MODE == MODE_FWD) {
@@ -246,19 +302,38 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
                 cd t = cd{0.0, 0.0};
 #pragma unroll
                 for (int i = 0; i < NC; ++i) t = cadd(t, G[i][i]);
+                // M_d = sum_ab G_ab P_d(b, a): for a < b the pair contributes
+                // Re = (G_ab + G_ba).x p.x + (G_ab - G_ba).y p.y and
+                // Im = (G_ab + G_ba).y p.x + (G_ba - G_ab).x p.y (p = P_d(a, b)),
+                // so G is folded once into those sums (in place; G is not needed
+                // after this) and each pair costs 4 FMA per d.
+#pragma unroll
+                for (int ib = 0; ib < NC; ++ib)
+#pragma unroll
+                    for (int ia = 0; ia < ib; ++ia) {
+                        const cd x1 = G[ia][ib], x2 = G[ib][ia];
+                        G[ia][ib] = cd{x1.x + x2.x, x1.y - x2.y};
+                        G[ib][ia] = cd{x1.y + x2.y, x2.x - x1.x};
+                    }
                 for (int d = 0; d < nd; ++d) {
                     const double x = mu - ed[d];
                     const double r = 1.0 / fma(x, x, w * w);
                     const cd g = cd{x * r, -w * r};
-                    const cd* Pd = P + d * NP;
+                    const BT* Pd = P + d * NP;
                     cd Md = cd{0.0, 0.0};
 #pragma unroll
                     for (int ib = 0; ib < NC; ++ib)
 #pragma unroll
                         for (int ia = 0; ia <= ib; ++ia) {
-                            const cd p = Pd[ib * (ib + 1) / 2 + ia];
-                            Md = cfma(G[ib][ia], p, Md);
-                            if (ia != ib) Md = cfma(G[ia][ib], cconj(p), Md);
+                            const cd p = ldc(Pd + ib * (ib + 1) / 2 + ia);
+                            if (ia == ib) {
+                                Md.x = fma(G[ia][ia].x, p.x, Md.x);
+                                Md.y = fma(G[ia][ia].y, p.x, Md.y);
+                            } else {
+                                const cd sd = G[ia][ib], sm = G[ib][ia];
+                                Md.x = fma(sd.x, p.x, fma(sd.y, p.y, Md.x));
+                                Md.y = fma(sm.x, p.x, fma(sm.y, p.y, Md.y));
+                            }
                         }
                     t = cadd(t, cadd(g, cmul(cmul(g, g), Md)));
                 }
@@ -272,20 +347,27 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
 #pragma unroll
                 for (int j = 0; j < NC; ++j) Y[i][j] = cd{0.0, 0.0};
             cd sg = cd{0.0, 0.0}, sg2 = cd{0.0, 0.0}, sgM2 = cd{0.0, 0.0}, sgM3 = cd{0.0, 0.0};
+            // Y is accumulated as R + i J (R = sum Re(g^2) P_d, J = sum Im(g^2) P_d,
+            // upper triangles in place: Y[a][b] = R_ab, Y[b][a] = J_ab), as S is
             for (int d = 0; d < nd; ++d) {
                 const double x = mu - ed[d];
                 const double r = 1.0 / fma(x, x, w * w);
                 const cd g = cd{x * r, -w * r};
                 const cd g2 = cmul(g, g);
-                const cd* Pd = P + d * NP;
+                const BT* Pd = P + d * NP;
                 cd Md = cd{0.0, 0.0};
 #pragma unroll
                 for (int ib = 0; ib < NC; ++ib)
 #pragma unroll
                     for (int ia = 0; ia <= ib; ++ia) {
-                        const cd p = Pd[ib * (ib + 1) / 2 + ia];  // P_{ia ib} = w_a conj(w_b)
-                        Y[ia][ib] = cfma(p, g2, Y[ia][ib]);
-                        if (ia != ib) Y[ib][ia] = cfma(cconj(p), g2, Y[ib][ia]);
+                        const cd p = ldc(Pd + ib * (ib + 1) / 2 + ia);  // P_{ia ib} = w_a conj(w_b)
+                        if (ia == ib) {
+                            Y[ia][ia].x = fma(g2.x, p.x, Y[ia][ia].x);
+                            Y[ia][ia].y = fma(g2.y, p.x, Y[ia][ia].y);
+                        } else {
+                            Y[ia][ib] = cd{fma(g2.x, p.x, Y[ia][ib].x), fma(g2.x, p.y, Y[ia][ib].y)};
+                            Y[ib][ia] = cd{fma(g2.y, p.x, Y[ib][ia].x), fma(g2.y, p.y, Y[ib][ia].y)};
+                        }
                         // M_d = sum_ab conj(w_a) G_ab w_b = sum_ab G_ab P_{ba}
                         Md = cfma(G[ib][ia], p, Md);
                         if (ia != ib) Md = cfma(G[ia][ib], cconj(p), Md);
@@ -295,6 +377,14 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
                 sgM2 = cfma(g2, Md, sgM2);
                 sgM3 = cfma(cmul(g2, g), Md, sgM3);
             }
+#pragma unroll
+            for (int ib = 0; ib < NC; ++ib)
+#pragma unroll
+                for (int ia = 0; ia < ib; ++ia) {
+                    const cd R = Y[ia][ib], J = Y[ib][ia];
+                    Y[ia][ib] = cd{R.x - J.y, R.y + J.x};
+                    Y[ib][ia] = cd{R.x + J.y, J.x - R.y};
+                }
             cd trG = cadd(sg, sgM2);
 #pragma unroll
             for (int i = 0; i < NC; ++i) trG = cadd(trG, G[i][i]);
