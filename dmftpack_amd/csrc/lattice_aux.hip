@@ -23,8 +23,13 @@ __global__ __launch_bounds__(256) void k_static_terms(const double* __restrict__
     double s = 0.0, s2 = 0.0;
     for (int nw = 0; nw < nw_tot; ++nw) {
         const double w = matsubara(nw, beta);
-        const double den = 1.0 / (x2 + w * w);
-        s = fma(2.0 * x, den, s);              // 2 Re 1/(x + iw)
+        const double q = fma(w, w, x2);
+        // 1/q: hardware reciprocal estimate refined by two Newton-Raphson steps
+        // r <- r + r (1 - q r) (each doubles the correct bits; ~1 ulp result)
+        double den = __builtin_amdgcn_rcp(q);
+        den = fma(den, fma(-q, den, 1.0), den);
+        den = fma(den, fma(-q, den, 1.0), den);
+        s = fma(2.0 * x, den, s);                     // 2 Re 1/(x + iw)
         s2 = fma(2.0 * (x2 - w * w), den * den, s2);  // 2 Re 1/(x + iw)^2
     }
     const double xb = beta * (e - mu);
@@ -56,15 +61,40 @@ __global__ __launch_bounds__(256) void k_static_reduce(const double* __restrict_
     const int ch = blockIdx.x;
     const int k0 = ch * k_per_chunk, k1 = min(nk, k0 + k_per_chunk);
     const int m2 = nc * nc;
-    for (int e = threadIdx.x; e < m2; e += blockDim.x) {
-        const int i = e % nc, j = e / nc;
-        cd acc = cd{0.0, 0.0};
-        for (int k = k0; k < k1; ++k) {
-            const cd* Uk = U + (size_t)k * n * n;
-            const double* w = wt + (size_t)k * n;
-            for (int l = 0; l < n; ++l) acc = cadd(acc, cscale(cmulc(Uk[i + l * n], Uk[j + l * n]), w[l]));
+    // groups of m2 threads take every ng-th k of the chunk; the group partials
+    // are added in group order (deterministic)
+    // (m2 > 256: one element per thread, strided)
+    __shared__ cd part[256];
+    const int ng = m2 <= 256 ? 256 / m2 : 1;
+    if (m2 <= 256) {
+        const int grp = threadIdx.x / m2, e = threadIdx.x % m2;
+        if (grp < ng) {
+            const int i = e % nc, j = e / nc;
+            cd acc = cd{0.0, 0.0};
+            for (int k = k0 + grp; k < k1; k += ng) {
+                const cd* Uk = U + (size_t)k * n * n;
+                const double* w = wt + (size_t)k * n;
+                for (int l = 0; l < n; ++l) acc = cadd(acc, cscale(cmulc(Uk[i + l * n], Uk[j + l * n]), w[l]));
+            }
+            part[threadIdx.x] = acc;
         }
-        pNum[(size_t)ch * m2 + e] = acc;
+        __syncthreads();
+        for (int t = threadIdx.x; t < m2; t += blockDim.x) {
+            cd acc = cd{0.0, 0.0};
+            for (int g = 0; g < ng; ++g) acc = cadd(acc, part[g * m2 + t]);
+            pNum[(size_t)ch * m2 + t] = acc;
+        }
+    } else {
+        for (int e = threadIdx.x; e < m2; e += blockDim.x) {
+            const int i = e % nc, j = e / nc;
+            cd acc = cd{0.0, 0.0};
+            for (int k = k0; k < k1; ++k) {
+                const cd* Uk = U + (size_t)k * n * n;
+                const double* w = wt + (size_t)k * n;
+                for (int l = 0; l < n; ++l) acc = cadd(acc, cscale(cmulc(Uk[i + l * n], Uk[j + l * n]), w[l]));
+            }
+            pNum[(size_t)ch * m2 + e] = acc;
+        }
     }
     double sN = 0.0, sD = 0.0;
     for (int t = k0 * n + threadIdx.x; t < k1 * n; t += blockDim.x) {
