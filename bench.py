@@ -181,12 +181,26 @@ def main():
             traffic = json.load(open(tpath)).get(args.method, {}).get(kname.split("<")[1].rstrip(">"))
         except Exception:
             traffic = None
+    # executed VALU FLOPs of the same kernel (rocprofv3 SQ_INSTS_VALU_FLOPS_* pass summarised in profiles/)
+    executed = None
+    fpath = os.path.join(ROOT, "profiles", "pmc_flops.json")
+    if os.path.exists(fpath):
+        try:
+            ex = json.load(open(fpath)).get(args.method, {}).get(kname.split("<")[1].rstrip(">"))
+            if ex:
+                fl = ex["fp64_flops"] + ex["fp32_flops"]
+                executed = dict(flop_per_launch=fl, tflops=fl / t_k / 1e12, frac=fl / t_k / 1e12 / PEAK_FP64_TFLOPS,
+                                source="profiles/valu_counters_dp14_step_" + args.method + ".txt")
+        except Exception:
+            executed = None
     roofline = dict(bound="mfma", achieved=achieved, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
                     frac=achieved / PEAK_FP64_TFLOPS, traffic=traffic, kernel=kname,
                     kernel_ms=t_k * 1e3, fwd_kernel_ms=t_f * 1e3, vjp_kernel_ms=t_v * 1e3,
-                    flop_per_unit=flop, units_per_launch=units,
+                    flop_per_unit=flop, units_per_launch=units, executed=executed,
                     note="achieved = algorithmic 8n^3-based FLOPs (adjoint: 8n^3+16nc^3+8n nc^2+8n^2) per (k,iw) "
-                         "x units / measured kernel time; the Schur methods execute fewer FLOPs than this")
+                         "x units / measured kernel time; the Schur methods execute fewer FLOPs than this "
+                         "(executed: counted VALU FLOPs of the same kernel over the same live time). "
+                         "Bound: FP64 VALU, whose MI355X peak equals the FP64 matrix peak (78.6 TFLOP/s)")
 
     # ---- CPU baseline: the oracle's GreenFtn_w on a k subset ---------------------
     cpu = None

@@ -12,6 +12,10 @@
       each way, so factor = M MiB / (counter KiB * 1024) for each counter, and a
       kernel's traffic per launch is the mean over its dispatches of
       FETCH * factor_fetch + WRITE * factor_write.
+
+  python -m dmftpack_amd.rocprof_summary sq --run <dir> --method NAME [--json profiles/pmc_flops.json]
+      executed FP64/FP32 VALU FLOPs per launch (SQ_INSTS_VALU_FLOPS_* x 64 lanes)
+      and the instruction mix, from one --pmc pass of SQ counters.
 """
 import argparse
 import csv
@@ -121,11 +125,60 @@ def pmc(argv):
             json.dump(db, fh, indent=1)
 
 
+def sq(argv):
+    """Executed FP64/FP32 VALU FLOPs and instruction mix per launch from one
+    --pmc pass of SQ counters.  SQ_INSTS_VALU_FLOPS_* count FLOPs per wave
+    instruction, so they are scaled by the 64 lanes of a wavefront."""
+    ap = argparse.ArgumentParser(prog="rocprof_summary sq")
+    ap.add_argument("--run", required=True)
+    ap.add_argument("--command", default="")
+    ap.add_argument("--json", default=None)
+    ap.add_argument("--method", default="")
+    a = ap.parse_args(argv)
+    rows = defaultdict(lambda: defaultdict(float))
+    dur = defaultdict(dict)
+    for f in glob.glob(os.path.join(a.run, "**", "*results.db"), recursive=True):
+        db = sqlite3.connect(f)
+        for name, did, cn, v, d in db.execute("select kernel_name, dispatch_id, counter_name, value, duration "
+                                              "from counters_collection"):
+            rows[name][cn] += v
+            dur[name][(f, did)] = d
+    print("# executed VALU work per launch from rocprofv3 --pmc SQ_INSTS_VALU_FLOPS_FP64 SQ_INSTS_VALU_FLOPS_FP32 "
+          "SQ_INSTS_VALU SQ_ACTIVE_INST_VALU SQ_BUSY_CYCLES SQ_WAVE_CYCLES SQ_INSTS_LDS SQ_INSTS_SMEM")
+    if a.command:
+        print(f"# command: {a.command}")
+    print("# FLOPs = SQ_INSTS_VALU_FLOPS_* x 64 lanes; TFLOP/s over the launch duration under the profiler")
+    print(f"{'launch_ms':>9} {'fp64_GF':>9} {'fp32_GF':>9} {'TFLOP/s':>8} {'valu_Minst':>10} {'lds_Minst':>9} "
+          f"{'smem_Minst':>10}  kernel")
+    out = {}
+    for name in sorted(rows):
+        if "dg::k_" not in name:
+            continue
+        nd = len(dur[name])
+        t = sum(dur[name].values()) / nd * 1e-9
+        c = {k: v / nd for k, v in rows[name].items()}
+        f64 = c.get("SQ_INSTS_VALU_FLOPS_FP64", 0.0) * 64
+        f32 = c.get("SQ_INSTS_VALU_FLOPS_FP32", 0.0) * 64
+        rate = (f64 + f32) / t / 1e12 if t > 0 else 0.0
+        print(f"{t * 1e3:>9.2f} {f64 / 1e9:>9.1f} {f32 / 1e9:>9.1f} {rate:>8.1f} {c.get('SQ_INSTS_VALU', 0) / 1e6:>10.1f} "
+              f"{c.get('SQ_INSTS_LDS', 0) / 1e6:>9.1f} {c.get('SQ_INSTS_SMEM', 0) / 1e6:>10.1f}  {name[:100]}")
+        mode = lattice_mode(name)
+        if mode:
+            out[mode] = dict(fp64_flops=f64, fp32_flops=f32, launch_s=t, kernel=name)
+    if a.json:
+        db = json.load(open(a.json)) if os.path.exists(a.json) else {}
+        db[a.method] = out
+        with open(a.json, "w") as fh:
+            json.dump(db, fh, indent=1)
+
+
 def main():
-    if len(sys.argv) < 2 or sys.argv[1] not in ("trace", "pmc"):
+    if len(sys.argv) < 2 or sys.argv[1] not in ("trace", "pmc", "sq"):
         sys.exit(__doc__)
     if sys.argv[1] == "trace":
         trace(sys.argv[2], sys.argv[3] if len(sys.argv) > 3 else "")
+    elif sys.argv[1] == "sq":
+        sq(sys.argv[2:])
     else:
         pmc(sys.argv[2:])
 
