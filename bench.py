@@ -32,6 +32,13 @@ PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 
 
+WORKLOADS = {
+    "dp14": "dp14: 14-orbital d+p, n_corr=5, Nk=32^3, N_freq=2048, beta=50, 3-pole Sigma",
+    "large32": "large32: 32 orbitals, two 5x5 correlated blocks (n_corr=10), Nk=40^3, N_freq=4096, beta=100",
+    "batch64": "batch64: 64 lattices, n in {3,5,7,10,14,20}, Nk in {8^3,12^3,16^3,24^3}, N_freq=1024, beta=40",
+}
+
+
 def f_fwd(n):
     return 8.0 * n ** 3
 
@@ -46,8 +53,12 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--method", default="schur_f32r", choices=["schur_f32r", "schur_f64", "plain_f64"])
-    ap.add_argument("--nk-side", type=int, default=32)
-    ap.add_argument("--n-freq", type=int, default=2048)
+    ap.add_argument("--workload", default="dp14", choices=["dp14", "large32", "batch64"],
+                    help="dp14: the 14-orbital precision configuration (default, the headline); large32: 32 orbitals, "
+                         "two 5x5 correlated blocks, Nk=40^3, N_freq=4096, beta=100; batch64: 64 uneven lattices "
+                         "in one replayable step")
+    ap.add_argument("--nk-side", type=int, default=None)
+    ap.add_argument("--n-freq", type=int, default=None)
     ap.add_argument("--refine", type=int, default=1)
     ap.add_argument("--layout", type=int, default=0, choices=[0, 1],
                     help="Schur kernels for n_corr <= 6: 0 one (k,iw) matrix per lane, 1 one column per lane")
@@ -77,15 +88,20 @@ def main():
     torch.cuda.init()
     dev = torch.device("cuda:0")
 
+    if args.workload == "batch64":
+        return bench_batch(args, torch, capi, G, L)
+
     # ---- synthetic workload -------------------------------------------------
-    w = L.workload("dp14", seed=1, nk_side=args.nk_side, n_freq=args.n_freq)
+    w = L.workload(args.workload, seed=1, nk_side=args.nk_side, n_freq=args.n_freq)
     Hk, sig, beta, mom = w["Hk"], w["sigma"], w["beta"], w["exact_moments"]
     nk, n = Hk.shape[0], Hk.shape[1]
     nc, nf = sig.shape[1], sig.shape[0]
+    ncl = w.get("n_cluster", 1)
     N0 = w["n_target"]
-    mu = 0.25
+    mu = 0.25 if args.workload == "dp14" else w.get("mu", 0.0)
     units = nk * 2 * nf
-    lat = G.LatticeGF(Hk, nc, nf, beta, mom)
+    lat = G.LatticeGF(Hk, nc, nf, beta, mom, n_cluster=ncl)
+    del w
     lat.set_option(capi.OPT_REFINE_STEPS, args.refine)
     lat.set_option(capi.OPT_SCHUR_LAYOUT, args.layout)
     s_dev = lat.sigma_dev(sig)
@@ -152,7 +168,7 @@ def main():
         extra["errors_vs_plain_fp64"] = errs
         others = {}
         for name in ("plain_f64", "schur_f64", "schur_f32r"):
-            for layout in ((0,) if name == "plain_f64" else (0, 1)):
+            for layout in ((0,) if name == "plain_f64" or nc > 6 else (0, 1)):
                 if name == args.method and layout == args.layout:
                     continue
                 lat.set_option(capi.OPT_SCHUR_LAYOUT, layout)
@@ -236,7 +252,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32+f64" if args.method == "schur_f32r" else "f64",
         "data": "synthetic",
-        "config": {"workload": "dp14: 14-orbital d+p, n_corr=5, Nk=32^3, N_freq=2048, beta=50, 3-pole Sigma",
+        "config": {"workload": WORKLOADS[args.workload],
                    "n": n, "n_corr": nc, "n_k": nk, "n_freq": nf, "beta": beta, "method": args.method,
                    "refine_steps": args.refine, "schur_layout": args.layout, "units_per_step": units,
                    "step": "static eig for the Sigma tail + GreenFtn_w forward (G_loc, NumMatrix, N) + VJP (dL/dSigma, dL/dmu)"},
@@ -253,6 +269,67 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(f"calibration copy: {args.calib_copy} MiB read + {args.calib_copy} MiB written per launch", flush=True)
+
+
+def bench_batch(args, torch, capi, G, L):
+    """64 independent lattices of uneven size in one replayable step (persistent
+    launches per n_corr, hipGraph, autotuned tiles): prepare + forward + loss
+    cotangents + adjoint for every system."""
+    methods = {"plain_f64": G.PLAIN_F64, "schur_f64": G.SCHUR_F64, "schur_f32r": G.SCHUR_F32R}
+    method = methods[args.method] if args.method != "plain_f64" else G.SCHUR_F64
+    nf = args.n_freq or 1024
+    systems = L.batch_systems(64, seed=1, n_freq=nf)
+    h = G.Handle()
+    h.set_option(capi.OPT_REFINE_STEPS, args.refine)
+    units = 0
+    for i, sd in enumerate(systems):
+        sd["moments"] = sd["exact_moments"]
+        sd["mu"] = 0.1 * (i % 3)
+        sd["n_target"] = 0.5 * sd["n_corr"]
+        # target: G_loc of a perturbed Sigma (single-lattice FP64, untimed)
+        lat = G.LatticeGF(sd["Hk"], sd["n_corr"], nf, sd["beta"], sd["moments"])
+        sd["gloc_target"] = lat.greenftn_w(L.perturbed(sd["sigma"], seed=100 + i), sd["mu"], method=G.SCHUR_F64)["gloc"]
+        lat.close()
+        units += sd["Hk"].shape[0] * 2 * nf
+    b = G.Batch(h, systems, method=method, tune=True, graph=True, prepare=True)
+    for _ in range(args.warmup):
+        b.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.run()
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / args.steps
+    got = b.results()
+    # per-system errors against the single-lattice FP64 path (untimed)
+    worst = dict(max_abs_dGloc=0.0, rel_dNumMatrix=0.0, abs_dN=0.0, rel_dSigmaBar=0.0, rel_dMuBar=0.0)
+    if not args.no_compare:
+        for sd, r in zip(systems, got):
+            lat = G.LatticeGF(sd["Hk"], sd["n_corr"], nf, sd["beta"], sd["moments"])
+            ref = lat.greenftn_w(sd["sigma"], sd["mu"], method=G.SCHUR_F64)
+            gbar = 2.0 * (ref["gloc"] - sd["gloc_target"])
+            sb, mb = lat.vjp(sd["sigma"], sd["mu"], gbar, 2.0 * (ref["n_total"] - sd["n_target"]), method=G.SCHUR_F64)
+            lat.close()
+            worst["max_abs_dGloc"] = max(worst["max_abs_dGloc"], float(np.max(np.abs(r["gloc"] - ref["gloc"]))))
+            worst["rel_dNumMatrix"] = max(worst["rel_dNumMatrix"], float(np.max(np.abs(r["num_matrix"] - ref["num_matrix"]))
+                                                                       / np.max(np.abs(ref["num_matrix"]))))
+            worst["abs_dN"] = max(worst["abs_dN"], abs(r["n_total"] - ref["n_total"]))
+            worst["rel_dSigmaBar"] = max(worst["rel_dSigmaBar"], float(np.max(np.abs(r["sigma_bar"] - sb)) / np.max(np.abs(sb))))
+            worst["rel_dMuBar"] = max(worst["rel_dMuBar"], abs(r["mu_bar"] - mb) / max(abs(mb), 1e-300))
+    out = {
+        "metric": METRIC, "value": units / sec, "unit": "(k,w) inversions/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": sec * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64" if method == G.SCHUR_F32R else "f64",
+        "data": "synthetic",
+        "config": {"workload": WORKLOADS["batch64"], "systems": len(systems), "units_per_step": units,
+                   "method": args.method, "refine_steps": args.refine,
+                   "step": "per system: prepare + GreenFtn_w forward + loss cotangents + VJP, one graph replay"},
+        "worst_error_vs_single_fp64": worst if not args.no_compare else None,
+        "tuning": b.tuning(),
+    }
+    print(json.dumps(out), flush=True)
+    b.close()
+    h.close()
 
 
 if __name__ == "__main__":
