@@ -39,6 +39,11 @@ WORKLOADS = {
 }
 
 
+def log(msg):
+    """Progress of the long phases (setup, references, timing) on stderr."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def f_fwd(n):
     return 8.0 * n ** 3
 
@@ -92,6 +97,7 @@ def main():
         return bench_batch(args, torch, capi, G, L)
 
     # ---- synthetic workload -------------------------------------------------
+    log(f"building workload {args.workload}")
     w = L.workload(args.workload, seed=1, nk_side=args.nk_side, n_freq=args.n_freq)
     Hk, sig, beta, mom = w["Hk"], w["sigma"], w["beta"], w["exact_moments"]
     nk, n = Hk.shape[0], Hk.shape[1]
@@ -107,6 +113,7 @@ def main():
     s_dev = lat.sigma_dev(sig)
     # target G_loc from a seeded perturbation of Sigma (plain FP64, untimed)
     s_tgt = lat.sigma_dev(L.perturbed(sig, seed=7))
+    log("target G_loc (plain FP64, untimed)")
     gtgt, _ = lat.greenftn_w_dev(s_tgt, mu, G.PLAIN_F64)
     gtgt = gtgt.clone()
     mom_dev = G.to_dev_cm(mom)
@@ -141,6 +148,7 @@ def main():
         return (t1 - t0) / steps, kf, kv
 
     head = methods[args.method]
+    log(f"timing {args.method}: {args.warmup} warmup + {args.steps} steps")
     ms_head, kf, kv = timed(head, args.steps, args.warmup, kernel_times=True)
     value = units / ms_head
 
@@ -149,6 +157,7 @@ def main():
     if not args.no_compare:
         res = {}
         for name in ("plain_f64", "schur_f64", "schur_f32r"):
+            log(f"reference outputs {name}")
             m = methods[name]
             gl, num, N = lat.greenftn_w_dev(s_dev, mu, m, want_n=True)
             gl = gl.clone()
@@ -173,6 +182,7 @@ def main():
                     continue
                 lat.set_option(capi.OPT_SCHUR_LAYOUT, layout)
                 steps = args.plain_steps if name == "plain_f64" else max(3, args.steps // 2)
+                log(f"timing {name} layout {layout}: {steps} steps")
                 ms, _, _ = timed(methods[name], steps, 1)
                 key = name if name == "plain_f64" else f"{name}_layout{layout}"
                 others[key] = dict(value=units / ms, ms_per_step=ms * 1e3, steps=steps)
@@ -223,6 +233,7 @@ def main():
     if not args.no_cpu:
         try:
             from oracle import oracle as O
+            log("CPU baseline (oracle)")
             cores = len(os.sched_getaffinity(0))
             probe_k = max(1, min(nk, 4 * cores))
             t0 = time.perf_counter()
