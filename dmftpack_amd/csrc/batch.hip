@@ -65,7 +65,7 @@ struct Sys {
     dmftgpu_system d;
     int n, nc, nd, nk, nf, nw_tot, m2;
     // device data
-    cd *mom, *U, *Vd, *W, *sig_all, *gbar;
+    cd *mom, *U, *Vd, *W, *kdata, *sig_all, *gbar;
     double *ev, *ed, *nbar;
     // static part
     double *wt, *dt, *pN;
@@ -155,6 +155,7 @@ LatArgs make_args(const dmftgpu_batch_s* b, const Sys& s, int mode, int chunks, 
     a.n_chunks = chunks;
     a.ed = s.ed;
     a.W = s.W;
+    a.kdata = s.kdata;
     a.refine = b->h->refine_steps;
     a.pSigAll = s.sig_all;
     if (mode == MODE_FWD) {
@@ -243,6 +244,7 @@ int issue_prepare(dmftgpu_batch_s* b)
             HIPCHK(dg::launch_herm_eig((const cd*)s.d.Hk, s.n, s.nc, s.nd, nullptr, 0, s.ed, s.Vd, s.nk, h->stream));
             HIPCHK(dg::launch_schur_prep((const cd*)s.d.Hk, s.Vd, s.n, s.nc, s.nk, s.W, h->stream));
         }
+        HIPCHK(dg::launch_schur_kdata((const cd*)s.d.Hk, s.W, s.ed, s.n, s.nc, s.nk, s.kdata, h->stream));
     }
     return DMFTGPU_OK;
 }
@@ -442,6 +444,7 @@ static int batch_create_impl(dmftgpu_batch_s* b, int count, const dmftgpu_system
         size_t t = 0;
         t += rnd(16 * 4 * s.m2) + rnd(8 * (size_t)s.nk * s.n) + rnd(16 * (size_t)s.nk * s.n * s.n);
         t += rnd(8 * (size_t)s.nk * ndd) + rnd(16 * (size_t)s.nk * ndd * ndd) + rnd(16 * (size_t)s.nk * s.nc * ndd);
+        t += rnd(16 * (size_t)s.nk * dg::schur_kdata_size(s.n, s.nc));
         t += rnd(16 * (size_t)s.nw_tot * s.m2) + rnd(16 * (size_t)s.nf * s.m2) + rnd(8);
         t += 2 * rnd(8 * (size_t)s.nk * s.n) + rnd(8 * 2 * (size_t)s.st_chunks) + rnd(16 * (size_t)s.st_chunks * s.m2);
         t += rnd(16 * s.m2) + 2 * rnd(16 * (s.m2 + 3)) + rnd(16);
@@ -470,6 +473,7 @@ static int batch_create_impl(dmftgpu_batch_s* b, int count, const dmftgpu_system
         s.ed = (double*)take(8 * (size_t)s.nk * ndd);
         s.Vd = (cd*)take(16 * (size_t)s.nk * ndd * ndd);
         s.W = (cd*)take(16 * (size_t)s.nk * s.nc * ndd);
+        s.kdata = (cd*)take(16 * (size_t)s.nk * dg::schur_kdata_size(s.n, s.nc));
         s.sig_all = (cd*)take(16 * (size_t)s.nw_tot * s.m2);
         s.gbar = (cd*)take(16 * (size_t)s.nf * s.m2);
         s.nbar = (double*)take(8);

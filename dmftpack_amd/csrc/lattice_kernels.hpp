@@ -76,7 +76,6 @@ struct SchurMulti {
 };
 hipError_t launch_lattice_schur_multi(const LatArgs& proto, int mode, bool fp32, hipStream_t s, int wg,
                                       const SchurMulti& m);
-size_t schur_lds_bytes(int nc, int nd, int mode, bool fp32, int wg);
 // frequencies per workgroup of the Schur kernel for n_c (grid planning)
 int schur_slots(int nc, int mode, bool fp32, bool cols);
 bool schur_supported(int n, int nc);

@@ -497,7 +497,8 @@ __global__ __launch_bounds__(WG) void k_lattice_schur(LatArgs a, const cd* __res
 
 // Persistent form over a work list spanning several lattices of the same n_c:
 // workgroup b takes items b, b + gridDim.x, ... (static striding, no atomics,
-// so every replay does the same work in the same order).
+// so every replay does the same work in the same order).  Each system's per-k
+// blocks come from its LatArgs (kdata), read through the constant address space.
 template <int NC, int MODE, bool F32R, int WG, bool TRACE, int REFINE>
 __global__ __launch_bounds__(WG) void k_lattice_schur_multi(const LatArgs* __restrict__ args,
                                                             const int4* __restrict__ items, int n_items)
@@ -506,7 +507,7 @@ __global__ __launch_bounds__(WG) void k_lattice_schur_multi(const LatArgs* __res
     for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
         const int4 item = items[it];  // {system, wtile, chunk, -}
         const LatArgs a = args[item.x];
-        schur_item<NC, MODE, F32R, WG, TRACE, REFINE, false>(a, a.pSigAll, item.y, item.z, lds);
+        schur_item<NC, MODE, F32R, WG, TRACE, REFINE, true>(a, a.pSigAll, item.y, item.z, lds);
         __syncthreads();
     }
 }
@@ -539,7 +540,7 @@ template <int NC, int MODE, bool F32R, bool TRACE, int REFINE, int WG>
 static hipError_t launch_nc_w(const LatArgs& a, const cd* sig_all, hipStream_t s, const SchurMulti* multi)
 {
     if (multi) {
-        size_t shm = schur_shm<NC, MODE, F32R, WG>(multi->max_nd, true);
+        size_t shm = schur_shm<NC, MODE, F32R, WG>(multi->max_nd, false);
         hipError_t e = hipFuncSetAttribute((const void*)k_lattice_schur_multi<NC, MODE, F32R, WG, TRACE, REFINE>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         if (e != hipSuccess) return e;
@@ -647,13 +648,6 @@ hipError_t launch_lattice_schur_multi(const LatArgs& proto, int mode, bool fp32,
                                       const SchurMulti& m)
 {
     return dispatch_all(proto, mode, fp32, nullptr, s, wg, &m);
-}
-
-size_t schur_lds_bytes(int nc, int nd, int mode, bool fp32, int wg)
-{
-    const bool acc_lds = (mode == MODE_VJP) || (mode == MODE_FWD && fp32);
-    const int np = nc * (nc + 1) / 2;
-    return sizeof(cd) * (2 * (size_t)(nc * nc + nd * np + (nd + 1) / 2) + (acc_lds ? (size_t)nc * nc * wg : 0));
 }
 
 // W(k) = H_cd(k) V_d(k): W_ad = sum_e H(a, nc+e) V_d(e, d)
