@@ -9,6 +9,20 @@ namespace dg {
 
 enum LatMode { MODE_FWD = 0, MODE_DENS = 1, MODE_VJP = 2 };
 
+// Data read through the constant address space: the compiler treats it as
+// invariant for the kernel's duration, so wave-uniform loads become scalar
+// loads (s_load) whose results feed the VALU as SGPR operands.  Used for the
+// per-k data of the Schur kernels, which no kernel writes while they run.
+typedef __attribute__((address_space(4))) cd kconst_cd;
+typedef __attribute__((address_space(4))) double kconst_double;
+
+__device__ __forceinline__ cd ldc(const cd* p) { return *p; }
+__device__ __forceinline__ cd ldc(const kconst_cd* p)
+{
+    const kconst_double* q = (const kconst_double*)p;
+    return cd{q[0], q[1]};
+}
+
 struct LatArgs {
     const cd* Hk;     // [nk][n][n]
     const cd* sigma;  // [n_freq][nc][nc]

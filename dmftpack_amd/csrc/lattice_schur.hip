@@ -33,18 +33,6 @@
 
 namespace dg {
 
-// Data read through the constant address space: the compiler treats it as
-// invariant for the kernel's duration, so wave-uniform loads become scalar
-// loads (s_load) whose results feed the VALU as SGPR operands.
-typedef __attribute__((address_space(4))) cd kconst_cd;
-typedef __attribute__((address_space(4))) double kconst_double;
-
-__device__ __forceinline__ cd ldc(const cd* p) { return *p; }
-__device__ __forceinline__ cd ldc(const kconst_cd* p)
-{
-    const kconst_double* q = (const kconst_double*)p;
-    return cd{q[0], q[1]};
-}
 
 template <int NC>
 struct Mat {
@@ -524,9 +512,9 @@ static int cols_bucket(int nc)
 int schur_slots(int nc, int mode, bool fp32, bool cols)
 {
     if (nc <= 6 && !cols) return 64;
-    const int N = cols_bucket(nc), prec = fp32 ? 1 : 0;
-    const int wg = (mode == MODE_FWD && prec == 0) ? 256 : 64;
-    return wg / 64 * (64 / N);
+    (void)mode;
+    (void)fp32;
+    return 64 / cols_bucket(nc);  // one wavefront of 64/N slots per workgroup
 }
 
 template <int NC, int MODE, bool F32R, int WG>

@@ -26,19 +26,17 @@
 //   * DENS/VJP exchange G_cc and Z through per-slot LDS matrices for the
 //     products Z = G + G Y, tr(Z^2), (G^2)_cc = Z G and G^H gbar G^H; lane c
 //     accumulates row c of sigma_bar.
-// The per-k data (H_cc, W = H_cd V_d, e_d) is staged in LDS, double buffered,
-// shared by all slots of the workgroup.
+// The per-k data (H_cc, W = H_cd V_d, e_d) is read from global memory through
+// the constant address space (wave-uniform reads become scalar loads).
 #pragma once
 #include "gj.hpp"
 #include "lattice_kernels.hpp"
 
 namespace dg {
 
-// FWD FP64 needs no per-slot matrices: 4 wavefronts share each staged k.
-__host__ __device__ constexpr int cols_wg(int mode, int prec) { return (mode == MODE_FWD && prec == 0) ? 256 : 64; }
-
-// per-k staging: [H_cc nc*nc][W nc*nd][e_d nd doubles]
-__host__ __device__ inline int cols_stage(int nc, int nd) { return nc * nc + nc * nd + (nd + 1) / 2; }
+// one wavefront per workgroup: the per-k data comes from global memory, so
+// wavefronts share nothing but the cache
+__host__ __device__ constexpr int cols_wg(int, int) { return 64; }
 
 // per-slot LDS (cd units): bc 2N (GJ broadcast), gsh 32 (g_d), MA N*N, MB N*N, GB N*N
 __host__ __device__ constexpr int cols_slot_size(int N, int mode, int prec)
@@ -48,23 +46,11 @@ __host__ __device__ constexpr int cols_slot_size(int N, int mode, int prec)
 
 __host__ __device__ constexpr int cols_slots(int N, int mode, int prec) { return cols_wg(mode, prec) / 64 * (64 / N); }
 
-inline size_t cols_lds_bytes(int N, int mode, int prec, int nc, int nd)
+inline size_t cols_lds_bytes(int N, int mode, int prec, int, int)
 {
     const int nwave = cols_wg(mode, prec) / 64;
     const int spw = 64 / N;
-    return sizeof(cd) * (2 * (size_t)cols_stage(nc, nd) + (size_t)nwave * (spw + 1) * cols_slot_size(N, mode, prec));
-}
-
-template <int WG>
-__device__ __forceinline__ void cols_stage_k(const LatArgs& a, int k, cd* buf, int nc, int nd)
-{
-    const int n = a.n;
-    const cd* Hk = a.Hk + (size_t)k * n * n;
-    for (int t = threadIdx.x; t < nc * nc; t += WG) buf[t] = Hk[(t % nc) + (size_t)(t / nc) * n];
-    const cd* W = a.W + (size_t)k * nc * nd;
-    for (int t = threadIdx.x; t < nc * nd; t += WG) buf[nc * nc + t] = W[t];
-    double* e = (double*)(buf + nc * nc + nc * nd);
-    for (int t = threadIdx.x; t < nd; t += WG) e[t] = a.ed[(size_t)k * nd + t];
+    return sizeof(cd) * ((size_t)nwave * (spw + 1) * cols_slot_size(N, mode, prec));
 }
 
 template <int N, int MODE, int PREC, bool TRACE>
@@ -76,15 +62,14 @@ __global__ __launch_bounds__(cols_wg(MODE, PREC)) void k_lattice_schur_cols(LatA
     constexpr int SS = cols_slot_size(N, MODE, PREC);
     constexpr bool MATS = (PREC > 0 || MODE != MODE_FWD);
     extern __shared__ cd lds[];
-    const int nc = a.nc, nd = a.n - nc, m2 = nc * nc;
-    const int bs = cols_stage(nc, nd);
+    const int n = a.n, nc = a.nc, nd = a.n - nc, m2 = nc * nc;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int sl = lane / N;  // == SPW for the spare lanes of a wavefront
     const int c = lane - sl * N;
     const bool active = sl < SPW;
     const bool live = active && c < nc;  // lanes c >= nc hold identity padding
     const int cc = live ? c : 0;         // clamped column index for loads
-    cd* my = lds + 2 * bs + (size_t)(wave * (SPW + 1) + sl) * SS;  // spare lanes get a scratch slot
+    cd* my = lds + (size_t)(wave * (SPW + 1) + sl) * SS;  // spare lanes get a scratch slot
     cd* bc = my;
     cd* gsh = my + 2 * N;
     cd* MA = gsh + 32;
@@ -118,14 +103,14 @@ __global__ __launch_bounds__(cols_wg(MODE, PREC)) void k_lattice_schur_cols(LatA
 #pragma unroll
     for (int i = 0; i < N; ++i) acc[i] = cd{0.0, 0.0};
     cd trA = cd{0.0, 0.0}, tr2A = cd{0.0, 0.0}, muA = cd{0.0, 0.0};
-    if (k0 < k1) cols_stage_k<WG>(a, k0, lds, nc, nd);
     __syncthreads();
 
     for (int k = k0; k < k1; ++k) {
-        const cd* B = lds + ((k - k0) & 1) * bs;
-        const cd* Hcc = B;
-        const cd* Wk = B + m2;
-        const double* ed = (const double*)(Wk + nc * nd);
+        // per-k data straight from global memory through the constant address
+        // space: the wave-uniform reads (W_id, e_d) become scalar loads
+        const kconst_cd* Hkk = (const kconst_cd*)(a.Hk + (size_t)k * n * n);  // H_cc(i, c) at i + c*n
+        const kconst_cd* Wk = (const kconst_cd*)(a.W + (size_t)k * nc * nd);
+        const kconst_double* ed = (const kconst_double*)(a.ed + (size_t)k * nd);
         // g_d = 1/(iw + mu - e_d) of this slot's frequency, shared by its lanes
         for (int d = c; d < nd; d += N) {
             const double x = mu - ed[d];
@@ -140,7 +125,7 @@ __global__ __launch_bounds__(cols_wg(MODE, PREC)) void k_lattice_schur_cols(LatA
 #pragma unroll
         for (int i = 0; i < N; ++i) {
             cd v = (i == c) ? cd{1.0, 0.0} : cd{0.0, 0.0};
-            if (live) v = (i < nc) ? csub(E[i], Hcc[i + cc * nc]) : cd{0.0, 0.0};
+            if (live) v = (i < nc) ? csub(E[i], ldc(Hkk + i + (size_t)cc * n)) : cd{0.0, 0.0};
             col[i] = v;
         }
         if constexpr (MODE != MODE_FWD) {
@@ -149,16 +134,16 @@ __global__ __launch_bounds__(cols_wg(MODE, PREC)) void k_lattice_schur_cols(LatA
         }
         for (int d = 0; d < nd; ++d) {
             const cd g = gsh[d];
-            const cd* Wd = Wk + d * nc;
-            const cd u = live ? cmulc(g, Wd[cc]) : cd{0.0, 0.0};  // g_d conj(W_cd)
+            const kconst_cd* Wd = Wk + d * nc;
+            const cd u = live ? cmulc(g, ldc(Wd + cc)) : cd{0.0, 0.0};  // g_d conj(W_cd)
 #pragma unroll
             for (int i = 0; i < N; ++i)
-                if (i < nc) col[i] = cfms(Wd[i], u, col[i]);
+                if (i < nc) col[i] = cfms(ldc(Wd + i), u, col[i]);
             if constexpr (MODE != MODE_FWD) {
                 const cd u2 = cmul(g, u);
 #pragma unroll
                 for (int i = 0; i < N; ++i)
-                    if (i < nc) y[i] = cfma(Wd[i], u2, y[i]);
+                    if (i < nc) y[i] = cfma(ldc(Wd + i), u2, y[i]);
             }
         }
 
@@ -213,12 +198,12 @@ __global__ __launch_bounds__(cols_wg(MODE, PREC)) void k_lattice_schur_cols(LatA
                 cd t = live ? gdiag : cd{0.0, 0.0};
                 for (int d = 0; d < nd; ++d) {
                     const cd g = gsh[d];
-                    const cd* Wd = Wk + d * nc;
+                    const kconst_cd* Wd = Wk + d * nc;
                     cd v = cd{0.0, 0.0};
 #pragma unroll
                     for (int i = 0; i < N; ++i)
-                        if (i < nc) v = cfmac(Wd[i], col[i], v);
-                    const cd m = live ? cmul(Wd[cc], v) : cd{0.0, 0.0};
+                        if (i < nc) v = cfmac(ldc(Wd + i), col[i], v);
+                    const cd m = live ? cmul(ldc(Wd + cc), v) : cd{0.0, 0.0};
                     t = cfma(cmul(g, g), m, t);
                     if (c == 0) t = cadd(t, g);
                 }
@@ -246,12 +231,12 @@ __global__ __launch_bounds__(cols_wg(MODE, PREC)) void k_lattice_schur_cols(LatA
             for (int d = 0; d < nd; ++d) {
                 const cd g = gsh[d];
                 const cd g2 = cmul(g, g);
-                const cd* Wd = Wk + d * nc;
+                const kconst_cd* Wd = Wk + d * nc;
                 cd v = cd{0.0, 0.0};
 #pragma unroll
                 for (int i = 0; i < N; ++i)
-                    if (i < nc) v = cfmac(Wd[i], col[i], v);
-                const cd m = live ? cmul(Wd[cc], v) : cd{0.0, 0.0};
+                    if (i < nc) v = cfmac(ldc(Wd + i), col[i], v);
+                const cd m = live ? cmul(ldc(Wd + cc), v) : cd{0.0, 0.0};
                 tg = cfma(g2, m, tg);
                 t2 = cfma(cscale(cmul(g2, g), 2.0), m, t2);
                 if (c == 0) {
@@ -311,9 +296,7 @@ __global__ __launch_bounds__(cols_wg(MODE, PREC)) void k_lattice_schur_cols(LatA
             }
             wave_sync();
         }
-        __syncthreads();
-        if (k + 1 < k1) cols_stage_k<WG>(a, k + 1, lds + ((k + 1 - k0) & 1) * bs, nc, nd);
-        __syncthreads();
+        wave_sync();  // gsh of this k is read by every lane of the slot before it is rewritten
     }
 
     // ---- sum the trace shares over the lanes of each slot
