@@ -337,6 +337,18 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
             cd sg = cd{0.0, 0.0}, sg2 = cd{0.0, 0.0}, sgM2 = cd{0.0, 0.0}, sgM3 = cd{0.0, 0.0};
             // Y is accumulated as R + i J (R = sum Re(g^2) P_d, J = sum Im(g^2) P_d,
             // upper triangles in place: Y[a][b] = R_ab, Y[b][a] = J_ab), as S is
+            // M_d = sum_ab conj(w_a) G_ab w_b = sum_ab G_ab P_{ba} with G folded
+            // into pair sums for the loop (as in the forward trace) and
+            // unfolded after it: for a < b, G[a][b] <- (G_ab + G_ba).x, (G_ab - G_ba).y
+            // and G[b][a] <- (G_ab + G_ba).y, (G_ba - G_ab).x
+#pragma unroll
+            for (int ib = 0; ib < NC; ++ib)
+#pragma unroll
+                for (int ia = 0; ia < ib; ++ia) {
+                    const cd x1 = G[ia][ib], x2 = G[ib][ia];
+                    G[ia][ib] = cd{x1.x + x2.x, x1.y - x2.y};
+                    G[ib][ia] = cd{x1.y + x2.y, x2.x - x1.x};
+                }
             for (int d = 0; d < nd; ++d) {
                 const double x = mu - ed[d];
                 const double r = 1.0 / fma(x, x, w * w);
@@ -352,19 +364,29 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
                         if (ia == ib) {
                             Y[ia][ia].x = fma(g2.x, p.x, Y[ia][ia].x);
                             Y[ia][ia].y = fma(g2.y, p.x, Y[ia][ia].y);
+                            Md.x = fma(G[ia][ia].x, p.x, Md.x);
+                            Md.y = fma(G[ia][ia].y, p.x, Md.y);
                         } else {
                             Y[ia][ib] = cd{fma(g2.x, p.x, Y[ia][ib].x), fma(g2.x, p.y, Y[ia][ib].y)};
                             Y[ib][ia] = cd{fma(g2.y, p.x, Y[ib][ia].x), fma(g2.y, p.y, Y[ib][ia].y)};
+                            const cd sd = G[ia][ib], sm = G[ib][ia];
+                            Md.x = fma(sd.x, p.x, fma(sd.y, p.y, Md.x));
+                            Md.y = fma(sm.x, p.x, fma(sm.y, p.y, Md.y));
                         }
-                        // M_d = sum_ab conj(w_a) G_ab w_b = sum_ab G_ab P_{ba}
-                        Md = cfma(G[ib][ia], p, Md);
-                        if (ia != ib) Md = cfma(G[ia][ib], cconj(p), Md);
                     }
                 sg = cadd(sg, g);
                 sg2 = cadd(sg2, g2);
                 sgM2 = cfma(g2, Md, sgM2);
                 sgM3 = cfma(cmul(g2, g), Md, sgM3);
             }
+#pragma unroll
+            for (int ib = 0; ib < NC; ++ib)
+#pragma unroll
+                for (int ia = 0; ia < ib; ++ia) {
+                    const cd a1 = G[ia][ib], a2 = G[ib][ia];
+                    G[ia][ib] = cd{0.5 * (a1.x - a2.y), 0.5 * (a2.x + a1.y)};
+                    G[ib][ia] = cd{0.5 * (a1.x + a2.y), 0.5 * (a2.x - a1.y)};
+                }
 #pragma unroll
             for (int ib = 0; ib < NC; ++ib)
 #pragma unroll
@@ -376,13 +398,13 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
             cd trG = cadd(sg, sgM2);
 #pragma unroll
             for (int i = 0; i < NC; ++i) trG = cadd(trG, G[i][i]);
-            // Y <- G Y (column by column, in place)
+            // Y <- Z = G S' = G + G Y (column by column, in place)
 #pragma unroll
             for (int j = 0; j < NC; ++j) {
                 cd t[NC];
 #pragma unroll
                 for (int i = 0; i < NC; ++i) {
-                    cd s = cd{0.0, 0.0};
+                    cd s = G[i][j];
 #pragma unroll
                     for (int l = 0; l < NC; ++l) s = cfma(G[i][l], Y[l][j], s);
                     t[i] = s;
@@ -390,22 +412,24 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
 #pragma unroll
                 for (int i = 0; i < NC; ++i) Y[i][j] = t[i];
             }
-            // tr((G S')^2) = tr(G^2) + 2 tr(G Y) + tr(Y^2),  G S' = G + Y
+            // tr(Z^2) = sum_i Z_ii^2 + 2 sum_{i<j} Z_ij Z_ji
             cd tz = cadd(sg2, cscale(sgM3, 2.0));
+            {
+                cd off = cd{0.0, 0.0};
 #pragma unroll
-            for (int i = 0; i < NC; ++i)
+                for (int i = 0; i < NC; ++i) {
+                    tz = cfma(Y[i][i], Y[i][i], tz);
 #pragma unroll
-                for (int j = 0; j < NC; ++j) {
-                    tz = cfma(G[i][j], G[j][i], tz);
-                    tz = cfma(cscale(G[i][j], 2.0), Y[j][i], tz);
-                    tz = cfma(Y[i][j], Y[j][i], tz);
+                    for (int j = i + 1; j < NC; ++j) off = cfma(Y[i][j], Y[j][i], off);
                 }
+                tz = cadd(tz, cscale(off, 2.0));
+            }
             if (wvalid) {
                 trA = cadd(trA, trG);
                 tr2A = cadd(tr2A, tz);
             }
             if (MODE == MODE_VJP && vjp_here) {
-                // (G^2)_cc = (G + Y) G, column j: q.  mu term += tr(gbar^H q-column),
+                // (G^2)_cc = Z G, column j: q.  mu term += tr(gbar^H q-column),
                 // sigma_bar += cn * ((G^2)_cc)^H  -> row j of sigma_bar gets cn conj(q)
                 cd ms = cd{0.0, 0.0};
 #pragma unroll
@@ -414,7 +438,7 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
                     for (int i = 0; i < NC; ++i) {
                         cd q = cd{0.0, 0.0};
 #pragma unroll
-                        for (int l = 0; l < NC; ++l) q = cfma(cadd(G[i][l], Y[i][l]), G[l][j], q);
+                        for (int l = 0; l < NC; ++l) q = cfma(Y[i][l], G[l][j], q);  // (Z G)_ij
                         ms = cfmac(gb[i + j * NC], q, ms);
                         cd* p = &sAcc[(j + i * NC) * WG + lane];
                         *p = cadd(*p, cscale(cconj(q), cn));
