@@ -323,13 +323,21 @@ int tune_group(dmftgpu_batch_s* b, Group& g, int mode)
 {
     dmftgpu_handle h = b->h;
     const int wgs[2] = {64, 128};
-    const double targets[3] = {2.5e5, 1.0e6, 4.0e6};
-    hipEvent_t e0, e1;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
+    const double targets[4] = {6.25e4, 2.5e5, 1.0e6, 4.0e6};
+    struct Events {  // destroyed on every return path
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        ~Events()
+        {
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+        }
+    } ev;
+    HIPCHK(hipEventCreate(&ev.e0));
+    HIPCHK(hipEventCreate(&ev.e1));
+    hipEvent_t e0 = ev.e0, e1 = ev.e1;
     double best = 1e300;
     int bw = 64;
-    double bt = targets[1];
+    double bt = targets[2];
     for (int wg : wgs)
         for (double t : targets) {
             int rc = build_group(b, g, mode, wg, t);
@@ -349,8 +357,6 @@ int tune_group(dmftgpu_batch_s* b, Group& g, int mode)
                 bt = t;
             }
         }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     g.ms[slot(mode)] = best;
     return build_group(b, g, mode, bw, bt);
 }
@@ -420,7 +426,7 @@ static int batch_create_impl(dmftgpu_batch_s* b, int count, const dmftgpu_system
         s.nf = L.n_freq;
         s.nw_tot = L.n_freq * L.high_freq;
         s.m2 = s.nc * s.nc;
-        s.max_chunks = std::max(1, std::min(s.nk, 64));
+        s.max_chunks = std::max(1, std::min(s.nk, 128));
         s.st_kpc = (s.nk + 255) / 256;
         s.st_chunks = (s.nk + s.st_kpc - 1) / s.st_kpc;
         auto it = gidx.find(s.nc);
