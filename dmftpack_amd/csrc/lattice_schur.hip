@@ -17,16 +17,16 @@
 // Per (k, iw) this costs ~ n_c^2 n_d + n_c^3 complex operations instead of n^3.
 //
 // Mapping: one (k, iw) matrix per lane, everything in registers (n_c <= 6):
-// a workgroup of one wavefront owns 64 consecutive frequencies and walks a chunk
-// of k points; the per-k data (H_cc, e_d, P_d for pairs a <= b) is staged in
-// LDS, double buffered, and read as wave-wide broadcasts.  The n_c x n_c
-// inversion is Gauss-Jordan without pivoting (the Hermitian part of -iS is
-// >= w_n I for causal Sigma, so elimination without pivoting is stable).
+// a wavefront owns 64 consecutive frequencies and walks a chunk of k points;
+// the per-k data (H_cc, e_d, P_d for pairs a <= b; built once by k_schur_kdata)
+// is read with wave-uniform scalar loads.  The n_c x n_c inversion is
+// Gauss-Jordan without pivoting (the Hermitian part of -iS is >= w_n I for
+// causal Sigma, so elimination without pivoting is stable).
 //
 // FP32R: the Schur complement S is formed in FP64, inverted in FP32, and the
 // FP32 inverse X0 preconditions iterative refinement with FP64 residuals
 //   X = X0 + D,  D <- D + X0 (I - S X),
-// one column at a time, `refine` times; the correction D is kept in FP32.
+// `refine` times; the correction D is kept in FP32.
 #include <type_traits>
 
 #include "lattice_kernels.hpp"
@@ -168,7 +168,6 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
         // compiler treats as invariant: those loads stay scalar despite the
         // clobber.
         if (GK) __asm__ __volatile__("" ::: "memory");
-        const cd* sigk = sig;
         const BT* B;
         if constexpr (GK) B = (const BT*)(a.kdata + (size_t)k * bs);
         else B = sbuf + ((k - k0) & 1) * bs;
@@ -209,13 +208,13 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
         for (int ib = 0; ib < NC; ++ib)
 #pragma unroll
             for (int ia = 0; ia <= ib; ++ia) {
-                cd bab = csub(cneg(ldc(Hcc + ia + ib * NC)), sigk[ia + ib * NC]);
+                cd bab = csub(cneg(ldc(Hcc + ia + ib * NC)), sig[ia + ib * NC]);
                 if (ia == ib) {
                     bab = cadd(bab, cd{mu, w});
                     G[ia][ia] = csub(bab, G[ia][ia]);  // - (R_aa + i J_aa)
                 } else {
                     const cd R = G[ia][ib], J = G[ib][ia];
-                    const cd bba = csub(cneg(ldc(Hcc + ib + ia * NC)), sigk[ib + ia * NC]);
+                    const cd bba = csub(cneg(ldc(Hcc + ib + ia * NC)), sig[ib + ia * NC]);
                     G[ia][ib] = csub(bab, cd{R.x - J.y, R.y + J.x});   // - (R + i J)_ab
                     G[ib][ia] = csub(bba, cd{R.x + J.y, J.x - R.y});   // - (R + i J)_ba
                 }
@@ -231,6 +230,32 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
 #pragma unroll
                 for (int j = 0; j < NC; ++j) X0[i][j] = cf{(float)G[i][j].x, (float)G[i][j].y};
             gj_inverse_f32<NC>(X0);
+            if constexpr (REFINE == 1) {
+                // One step: the whole residual R = I - S X0 (FP64 products,
+                // FP32 storage) is formed first, after which S is dead and
+                // G = X0 + X0 R is written over it.  Peak: S + X0 + R.
+                // R is formed row by row so that each row of S dies as soon as
+                // its row of R is done.
+                cf Rm[NC][NC];
+#pragma unroll
+                for (int i = 0; i < NC; ++i)
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) {
+                        cd s = cd{i == c ? 1.0 : 0.0, 0.0};
+#pragma unroll
+                        for (int l = 0; l < NC; ++l) s = cfms(G[i][l], cd{(double)X0[l][c].x, (double)X0[l][c].y}, s);
+                        Rm[i][c] = cf{(float)s.x, (float)s.y};
+                    }
+#pragma unroll
+                for (int c = 0; c < NC; ++c)
+#pragma unroll
+                    for (int i = 0; i < NC; ++i) {
+                        cf t = cf{0.f, 0.f};
+#pragma unroll
+                        for (int l = 0; l < NC; ++l) t = ffma(X0[i][l], Rm[l][c], t);
+                        G[i][c] = cd{(double)X0[i][c].x + (double)t.x, (double)X0[i][c].y + (double)t.y};
+                    }
+            } else {
             // X = X0 + D with the correction D held in FP32: D is ~kappa*eps32 of
             // |X|, so its own FP32 rounding (~kappa*eps32^2 of |X|) is the
             // accuracy floor of the refinement, the same as adding an FP32
@@ -269,6 +294,7 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
 #pragma unroll
                 for (int j = 0; j < NC; ++j)
                     G[i][j] = cd{(double)X0[i][j].x + (double)D[i][j].x, (double)X0[i][j].y + (double)D[i][j].y};
+            }
         }
 
         if (MODE == MODE_FWD) {
@@ -398,7 +424,10 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
             cd trG = cadd(sg, sgM2);
 #pragma unroll
             for (int i = 0; i < NC; ++i) trG = cadd(trG, G[i][i]);
-            // Y <- Z = G S' = G + G Y (column by column, in place)
+            // Z = G S' = G + G Y.  DENS: in place of Y.  VJP: apart, as S' = I + Y
+            // is used again below.
+            cd Z[MODE == MODE_VJP ? NC : 1][MODE == MODE_VJP ? NC : 1];
+            auto Zr = [&](int i, int j) -> cd& { return MODE == MODE_VJP ? Z[i][j] : Y[i][j]; };
 #pragma unroll
             for (int j = 0; j < NC; ++j) {
                 cd t[NC];
@@ -410,65 +439,64 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
                     t[i] = s;
                 }
 #pragma unroll
-                for (int i = 0; i < NC; ++i) Y[i][j] = t[i];
+                for (int i = 0; i < NC; ++i) Zr(i, j) = t[i];
             }
             // tr(Z^2) = sum_i Z_ii^2 + 2 sum_{i<j} Z_ij Z_ji
-            cd tz = cadd(sg2, cscale(sgM3, 2.0));
+            cd tzz = cd{0.0, 0.0};
             {
                 cd off = cd{0.0, 0.0};
 #pragma unroll
                 for (int i = 0; i < NC; ++i) {
-                    tz = cfma(Y[i][i], Y[i][i], tz);
+                    tzz = cfma(Zr(i, i), Zr(i, i), tzz);
 #pragma unroll
-                    for (int j = i + 1; j < NC; ++j) off = cfma(Y[i][j], Y[j][i], off);
+                    for (int j = i + 1; j < NC; ++j) off = cfma(Zr(i, j), Zr(j, i), off);
                 }
-                tz = cadd(tz, cscale(off, 2.0));
+                tzz = cadd(tzz, cscale(off, 2.0));
             }
+            const cd tz = cadd(cadd(sg2, cscale(sgM3, 2.0)), tzz);
             if (wvalid) {
                 trA = cadd(trA, trG);
                 tr2A = cadd(tr2A, tz);
             }
             if (MODE == MODE_VJP && vjp_here) {
-                // (G^2)_cc = Z G, column j: q.  mu term += tr(gbar^H q-column),
-                // sigma_bar += cn * ((G^2)_cc)^H  -> row j of sigma_bar gets cn conj(q)
-                cd ms = cd{0.0, 0.0};
-#pragma unroll
-                for (int j = 0; j < NC; ++j) {
-#pragma unroll
-                    for (int i = 0; i < NC; ++i) {
-                        cd q = cd{0.0, 0.0};
-#pragma unroll
-                        for (int l = 0; l < NC; ++l) q = cfma(Y[i][l], G[l][j], q);  // (Z G)_ij
-                        ms = cfmac(gb[i + j * NC], q, ms);
-                        cd* p = &sAcc[(j + i * NC) * WG + lane];
-                        *p = cadd(*p, cscale(cconj(q), cn));
-                    }
-                }
-                muA = cadd(muA, ms);
-                // sigma_bar += G^H gbar G^H ; gbar staged in Y
+                // sigma_bar += G^H gbar G^H + cn ((G^2)_cc)^H with (G^2)_cc = G S' G:
+                // both terms are C = G^H V G^H, V = gbar + cn S'^H.
+                // mu term tr(gbar^H (G^2)_cc) = tr(S' G gbar^H G)
+                //   = sum_ab S'_ab conj(C_ab) - cn tr(Z^2)
+                // (G gbar^H G is (G^H gbar G^H)^H = (C - cn G^H S'^H G^H)^H, and
+                // tr(S' G S' G) = tr(Z^2)).
+                cd V[NC][NC];
 #pragma unroll
                 for (int j = 0; j < NC; ++j)
 #pragma unroll
-                    for (int i = 0; i < NC; ++i) Y[i][j] = gb[i + j * NC];
+                    for (int i = 0; i < NC; ++i) {
+                        const cd gv = gb[i + j * NC];
+                        const double dg = i == j ? cn : 0.0;
+                        V[i][j] = cd{fma(cn, Y[j][i].x, gv.x + dg), fma(-cn, Y[j][i].y, gv.y)};
+                    }
+                cd ms = cscale(tzz, -cn);
 #pragma unroll
                 for (int b = 0; b < NC; ++b) {
-                    cd t[NC];  // t = gbar conj(row b of G)
+                    cd t[NC];  // column b of L = V G^H: t_i = sum_l V_il conj(G_bl)
 #pragma unroll
                     for (int i = 0; i < NC; ++i) {
                         cd s = cd{0.0, 0.0};
 #pragma unroll
-                        for (int l = 0; l < NC; ++l) s = cfma(Y[i][l], cconj(G[b][l]), s);
+                        for (int l = 0; l < NC; ++l) s = cfma(V[i][l], cconj(G[b][l]), s);
                         t[i] = s;
                     }
 #pragma unroll
-                    for (int aa = 0; aa < NC; ++aa) {
+                    for (int aa = 0; aa < NC; ++aa) {  // C_ab = sum_i conj(G_ia) t_i
                         cd s = cd{0.0, 0.0};
 #pragma unroll
                         for (int i = 0; i < NC; ++i) s = cfmac(G[i][aa], t[i], s);
                         cd* p = &sAcc[(aa + b * NC) * WG + lane];
                         *p = cadd(*p, s);
+                        const cd sp = aa == b ? cadd(Y[aa][b], cd{1.0, 0.0}) : Y[aa][b];
+                        ms = cfmac(s, sp, ms);  // S'_ab conj(C_ab)
                     }
                 }
+                muA = cadd(muA, ms);
             }
         }
         if (!GK) {
@@ -500,8 +528,15 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
     }
 }
 
+// Waves per SIMD the register allocator is held to.  At n_c = 5 the forward and
+// density kernels (FP64, or FP32 with one refinement step) need a few registers
+// more than the 256 of two waves; two waves with a few bytes of scratch beat one.
+template <int NC, int MODE, int REFINE>
+constexpr int schur_waves() { return (NC == 5 && MODE != MODE_VJP && REFINE <= 1) ? 2 : 1; }
+
 template <int NC, int MODE, bool F32R, int WG, bool TRACE, int REFINE>
-__global__ __launch_bounds__(WG) void k_lattice_schur(LatArgs a, const cd* __restrict__ sig_all)
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(schur_waves<NC, MODE, REFINE>())))
+void k_lattice_schur(LatArgs a, const cd* __restrict__ sig_all)
 {
     extern __shared__ cd lds[];
     schur_item<NC, MODE, F32R, WG, TRACE, REFINE, true>(a, sig_all, blockIdx.x, blockIdx.y, lds);
@@ -512,7 +547,8 @@ __global__ __launch_bounds__(WG) void k_lattice_schur(LatArgs a, const cd* __res
 // so every replay does the same work in the same order).  Each system's per-k
 // blocks come from its LatArgs (kdata), read through the constant address space.
 template <int NC, int MODE, bool F32R, int WG, bool TRACE, int REFINE>
-__global__ __launch_bounds__(WG) void k_lattice_schur_multi(const LatArgs* __restrict__ args,
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(schur_waves<NC, MODE, REFINE>())))
+void k_lattice_schur_multi(const LatArgs* __restrict__ args,
                                                             const int4* __restrict__ items, int n_items)
 {
     extern __shared__ cd lds[];
