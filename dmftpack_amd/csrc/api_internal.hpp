@@ -43,7 +43,7 @@ struct dmftgpu_handle_s {
     double static_mu = 0.0;
     dgi::Workspace static_cache;  // [m2 cd][2 doubles]
     int refine_steps = 2;    // FP64-residual refinement steps of the FP32 Schur solve
-    int schur_wg = 64;       // workgroup width of the Schur kernels (64 or 128)
+    int schur_wg = 128;      // workgroup width of the Schur kernels (64 or 128)
     int schur_layout = 0;    // n_c <= 6: 0 one matrix per lane, 1 column per lane
     bool static_cache_on = true;  // reuse static weights for a repeated mu
     dgi::Workspace prep;     // prepare-time data

@@ -41,6 +41,16 @@ __device__ __forceinline__ cd cfmac(cd a, cd b, cd acc)
 {
     return cd{fma(a.x, b.x, fma(a.y, b.y, acc.x)), fma(a.x, b.y, fma(-a.y, b.x, acc.y))};
 }
+// 1/q for finite q > 0 away from the denormal range: hardware reciprocal
+// estimate refined by two Newton-Raphson steps r <- r + r (1 - q r), each of
+// which doubles the correct bits (~1 ulp result, 5 instructions instead of the
+// IEEE division sequence).
+__device__ __forceinline__ double rcp_pos(double q)
+{
+    double r = __builtin_amdgcn_rcp(q);
+    r = fma(r, fma(-q, r, 1.0), r);
+    return fma(r, fma(-q, r, 1.0), r);
+}
 __device__ __forceinline__ cd crcp(cd a)
 {
     double d = 1.0 / fma(a.x, a.x, a.y * a.y);

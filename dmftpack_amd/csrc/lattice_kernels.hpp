@@ -52,6 +52,9 @@ struct LatArgs {
     double e0, de, eta;
     // per-k blocks of the one-matrix-per-lane Schur kernels [nk][schur_kdata_size]
     const cd* kdata;
+    // one-matrix-per-lane Schur kernels: frequency range [nw_begin, nw_end) of
+    // this launch (nw_end = 0: up to nw_tot); partial sums keep the nw_tot stride
+    int nw_begin, nw_end;
 };
 
 // (real shift added to mu, imaginary part) of frequency point nw

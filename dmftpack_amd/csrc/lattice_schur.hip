@@ -135,9 +135,10 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
     cd* sAcc = lds;            // [M2][WG], lane-contiguous
     cd* sbuf = lds + (ACC_LDS ? M2 * WG : 0);  // [2][bs]
     const int lane = threadIdx.x;
-    const int nw = wtile * WG + lane;
-    const bool wvalid = nw < a.nw_tot;
-    const int nwc = wvalid ? nw : a.nw_tot - 1;  // clamp for loads
+    const int nw_hi = a.nw_end ? a.nw_end : a.nw_tot;
+    const int nw = a.nw_begin + wtile * WG + lane;
+    const bool wvalid = nw < nw_hi;
+    const int nwc = wvalid ? nw : nw_hi - 1;  // clamp for loads
     const int k0 = chunk * a.k_per_chunk;
     const int k1 = min(a.nk, k0 + a.k_per_chunk);
     const double w = matsubara(nwc, a.beta);
@@ -187,7 +188,7 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
             for (int i = 0; i < NC; ++i) G[i][j] = cd{0.0, 0.0};
         for (int d = 0; d < nd; ++d) {
             const double x = mu - ed[d];
-            const double r = 1.0 / fma(x, x, w * w);
+            const double r = rcp_pos(fma(x, x, w * w));
             const double gr = x * r, gi = -w * r;
             const BT* Pd = P + d * NP;
 #pragma unroll
@@ -331,7 +332,7 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
                     }
                 for (int d = 0; d < nd; ++d) {
                     const double x = mu - ed[d];
-                    const double r = 1.0 / fma(x, x, w * w);
+                    const double r = rcp_pos(fma(x, x, w * w));
                     const cd g = cd{x * r, -w * r};
                     const BT* Pd = P + d * NP;
                     cd Md = cd{0.0, 0.0};
@@ -377,7 +378,7 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
                 }
             for (int d = 0; d < nd; ++d) {
                 const double x = mu - ed[d];
-                const double r = 1.0 / fma(x, x, w * w);
+                const double r = rcp_pos(fma(x, x, w * w));
                 const cd g = cd{x * r, -w * r};
                 const cd g2 = cmul(g, g);
                 const BT* Pd = P + d * NP;
@@ -598,7 +599,8 @@ static hipError_t launch_nc_w(const LatArgs& a, const cd* sig_all, hipStream_t s
     }
     const int nd = a.n - NC;
     size_t shm = schur_shm<NC, MODE, F32R, WG>(nd, false);
-    dim3 grid((a.nw_tot + WG - 1) / WG, a.n_chunks);
+    const int span = (a.nw_end ? a.nw_end : a.nw_tot) - a.nw_begin;
+    dim3 grid((span + WG - 1) / WG, a.n_chunks);
     hipError_t e = hipFuncSetAttribute((const void*)k_lattice_schur<NC, MODE, F32R, WG, TRACE, REFINE>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
     if (e != hipSuccess) return e;
@@ -688,6 +690,16 @@ hipError_t launch_lattice_schur(const LatArgs& a, int mode, bool fp32, hipStream
             case 12: return launch_cols_12(a, mode, fp32, a.pSigAll, s);
             default: return launch_cols_16(a, mode, fp32, a.pSigAll, s);
         }
+    }
+    if (mode == MODE_VJP && a.nw_tot > a.n_freq) {
+        // Frequencies beyond n_freq contribute only tr G and tr G^2 (dN/dmu):
+        // the density kernel does that at twice the occupancy of the adjoint.
+        LatArgs lo = a, hi = a;
+        lo.nw_end = a.n_freq;
+        hi.nw_begin = a.n_freq;
+        e = dispatch_all(lo, MODE_VJP, fp32, a.pSigAll, s, wg, nullptr);
+        if (e != hipSuccess) return e;
+        return dispatch_all(hi, MODE_DENS, fp32, a.pSigAll, s, wg, nullptr);
     }
     return dispatch_all(a, mode, fp32, a.pSigAll, s, wg, nullptr);
 }

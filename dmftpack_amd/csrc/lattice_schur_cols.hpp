@@ -23,9 +23,9 @@
 //     x_c <- x_c + X0 (e_c - S x_c);
 //   * FWD accumulates column c of G_cc over the k chunk in registers; the
 //     trace terms are per-lane partials summed over the slot at the end;
-//   * DENS/VJP exchange G_cc and Z through per-slot LDS matrices for the
-//     products Z = G + G Y, tr(Z^2), (G^2)_cc = Z G and G^H gbar G^H; lane c
-//     accumulates row c of sigma_bar.
+//   * DENS/VJP exchange G_cc, Z and V = gbar + cn (I + Y)^H through per-slot
+//     LDS matrices for the products Z = G + G Y, tr(Z^2) and
+//     sigma_bar += G^H V G^H; lane c accumulates column c of sigma_bar.
 // The per-k data (H_cc, W = H_cd V_d, e_d) is read from global memory through
 // the constant address space (wave-uniform reads become scalar loads).
 #pragma once
@@ -114,7 +114,7 @@ __global__ __launch_bounds__(cols_wg(MODE, PREC)) void k_lattice_schur_cols(LatA
         // g_d = 1/(iw + mu - e_d) of this slot's frequency, shared by its lanes
         for (int d = c; d < nd; d += N) {
             const double x = mu - ed[d];
-            const double r = 1.0 / fma(x, x, w * w);
+            const double r = rcp_pos(fma(x, x, w * w));
             gsh[d] = cd{x * r, -w * r};
         }
         wave_sync();
@@ -247,51 +247,57 @@ __global__ __launch_bounds__(cols_wg(MODE, PREC)) void k_lattice_schur_cols(LatA
 #pragma unroll
             for (int i = 0; i < N; ++i) MB[i + c * N] = z[i];
             wave_sync();
+            cd zz = cd{0.0, 0.0};  // lane share of tr(Z^2): sum_i Z_ic Z_ci
             if (live) {
 #pragma unroll
                 for (int i = 0; i < N; ++i)
-                    if (i < nc) t2 = cfma(z[i], MB[c + i * N], t2);
+                    if (i < nc) zz = cfma(z[i], MB[c + i * N], zz);
             }
+            t2 = cadd(t2, zz);
             if (wvalid) {
                 trA = cadd(trA, tg);
                 tr2A = cadd(tr2A, t2);
             }
             if constexpr (MODE == MODE_VJP) {
+                // sigma_bar += G^H gbar G^H + cn ((G^2)_cc)^H = C = G^H V G^H with
+                // V = gbar + cn S'^H, S' = I + Y ((G^2)_cc = G S' G); lane c forms
+                // column c of C.  mu term tr(gbar^H (G^2)_cc) = sum_ab S'_ab conj(C_ab)
+                // - cn tr(Z^2) (as in lattice_schur.hip).
+                wave_sync();  // every lane has read Z from MB
+                if (live) {  // row c of V into MB
+#pragma unroll
+                    for (int j = 0; j < N; ++j)
+                        if (j < nc)
+                            MB[c + j * N] = cadd(GB[c + j * N], cscale(cadd(cconj(y[j]), cd{j == c ? 1.0 : 0.0, 0.0}), cn));
+                }
+                wave_sync();
                 if (vjp_here && live) {
-                    // q = column c of (G^2)_cc = Z G
-                    cd q[N];
+                    cd l[N];  // column c of L = V G^H: l_i = sum_j V_ij conj(G_cj)
 #pragma unroll
-                    for (int i = 0; i < N; ++i) q[i] = cd{0.0, 0.0};
-#pragma unroll
-                    for (int l = 0; l < N; ++l) {
-                        if (l < nc) {
-                            const cd gl = col[l];
-#pragma unroll
-                            for (int i = 0; i < N; ++i) q[i] = cfma(MB[i + l * N], gl, q[i]);
-                        }
-                    }
-                    cd ms = cd{0.0, 0.0};
-#pragma unroll
-                    for (int i = 0; i < N; ++i) {
-                        if (i < nc) {
-                            ms = cfmac(GB[i + c * N], q[i], ms);
-                            acc[i] = cadd(acc[i], cscale(cconj(q[i]), cn));  // sigma_bar(c, i) += cn conj((G^2)_ic)
-                        }
-                    }
-                    muA = cadd(muA, ms);
-                    // sigma_bar(c, b) += sum_j t_j conj(G_bj),  t_j = sum_i conj(G_ic) gbar_ij
+                    for (int i = 0; i < N; ++i) l[i] = cd{0.0, 0.0};
 #pragma unroll
                     for (int j = 0; j < N; ++j) {
                         if (j < nc) {
-                            cd t = cd{0.0, 0.0};
+                            const cd gc = cconj(MA[c + j * N]);
 #pragma unroll
                             for (int i = 0; i < N; ++i)
-                                if (i < nc) t = cfmac(col[i], GB[i + j * N], t);
-#pragma unroll
-                            for (int b = 0; b < N; ++b)
-                                if (b < nc) acc[b] = cfma(t, cconj(MA[b + j * N]), acc[b]);
+                                if (i < nc) l[i] = cfma(MB[i + j * N], gc, l[i]);
                         }
                     }
+                    cd ms = cscale(zz, -cn);
+#pragma unroll
+                    for (int aa = 0; aa < N; ++aa) {
+                        if (aa < nc) {  // C_ac = sum_i conj(G_ia) l_i
+                            cd s = cd{0.0, 0.0};
+#pragma unroll
+                            for (int i = 0; i < N; ++i)
+                                if (i < nc) s = cfmac(MA[i + aa * N], l[i], s);
+                            acc[aa] = cadd(acc[aa], s);
+                            const cd sp = aa == c ? cadd(y[aa], cd{1.0, 0.0}) : y[aa];  // S'_ac
+                            ms = cfmac(s, sp, ms);
+                        }
+                    }
+                    muA = cadd(muA, ms);
                 }
             }
             wave_sync();
@@ -326,7 +332,7 @@ __global__ __launch_bounds__(cols_wg(MODE, PREC)) void k_lattice_schur_cols(LatA
         if (nw < a.n_freq) {
 #pragma unroll
             for (int b = 0; b < N; ++b)
-                if (b < nc) a.pSb[(ch * a.n_freq + nw) * m2 + c + b * nc] = acc[b];
+                if (b < nc) a.pSb[(ch * a.n_freq + nw) * m2 + b + c * nc] = acc[b];  // column c
         }
     }
     (void)MATS;
