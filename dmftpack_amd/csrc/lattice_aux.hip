@@ -7,9 +7,35 @@
 //     w_kl = f(eps_kl - mu) - (1/beta) sum_{n < high*N_freq} 2 Re 1/(iw_n + mu - eps_kl)
 // so that NumMatrix_static = sum_k (U diag(w) U^H)_cc and N_static = sum w;
 // dN/dmu gets beta f (1-f) + (1/beta) sum_n 2 Re 1/(iw_n+mu-eps)^2 per level.
+//
+// Closed form (M = high*N_freq frequencies, x = mu - eps, a = pi/beta): the
+// full Matsubara sums are sum_{n>=0} 2x/(x^2+w_n^2) = (beta/2) tanh(beta x/2)
+// and, its x-derivative, sum_{n>=0} 2 Re (x+iw_n)^-2 = -(beta^2/4) sech^2, while
+// f = (1 + tanh(beta x/2))/2 and beta f (1-f) = (beta/4) sech^2, so
+//     w_kl = 1/2 + T/beta,   dN/dmu term = -T2/beta
+// with the tails T = sum_{n>=M} 2 Re 1/(x+iw_n), T2 = sum_{n>=M} 2 Re 1/(x+iw_n)^2.
+// The tails follow from the Euler-Maclaurin formula with v = a(2M+1),
+// z = x + iv, w_j = z^-j (terms through the fifth derivative; the remainder is
+// ~ (2M+1)^-7 of the tail):
+//     T  = atan(x/v)/a + Re w1 - (a/3) Im w2 - (2/15) a^3 Im w4 - (16/63) a^5 Im w6
+//     T2 = Im w1 / a   + Re w2 - (2a/3) Im w3 - (8/15) a^3 Im w5 - (32/21) a^5 Im w7
+// The reference's clamp of f at |beta(eps - mu)| > 100 changes the result by
+// < e^-100.  Below M = 256 the finite sums are taken term by term.
 #include "lattice_kernels.hpp"
 
 namespace dg {
+
+__device__ __forceinline__ void matsubara_tails(double x, double beta, int M, double& T, double& T2)
+{
+    const double a = kPi / beta;
+    const double v = a * (2.0 * M + 1.0);
+    const cd w1 = crcp(cd{x, v});
+    const cd w2 = cmul(w1, w1), w3 = cmul(w2, w1), w4 = cmul(w2, w2);
+    const cd w5 = cmul(w4, w1), w6 = cmul(w4, w2), w7 = cmul(w6, w1);
+    const double a3 = a * a * a, a5 = a3 * a * a;
+    T = atan(x / v) / a + w1.x - (a / 3.0) * w2.y - (2.0 / 15.0) * a3 * w4.y - (16.0 / 63.0) * a5 * w6.y;
+    T2 = w1.y / a + w2.x - (2.0 * a / 3.0) * w3.y - (8.0 / 15.0) * a3 * w5.y - (32.0 / 21.0) * a5 * w7.y;
+}
 
 __global__ __launch_bounds__(256) void k_static_terms(const double* __restrict__ ev, int total, int nw_tot,
                                                       double beta, double mu, double* __restrict__ wt,
@@ -19,16 +45,19 @@ __global__ __launch_bounds__(256) void k_static_terms(const double* __restrict__
     if (t >= total) return;
     const double e = ev[t];
     const double x = mu - e;
+    if (nw_tot >= 256) {
+        double T, T2;
+        matsubara_tails(x, beta, nw_tot, T, T2);
+        wt[t] = 0.5 + T / beta;
+        dt[t] = -T2 / beta;
+        return;
+    }
     const double x2 = x * x;
     double s = 0.0, s2 = 0.0;
     for (int nw = 0; nw < nw_tot; ++nw) {
         const double w = matsubara(nw, beta);
         const double q = fma(w, w, x2);
-        // 1/q: hardware reciprocal estimate refined by two Newton-Raphson steps
-        // r <- r + r (1 - q r) (each doubles the correct bits; ~1 ulp result)
-        double den = __builtin_amdgcn_rcp(q);
-        den = fma(den, fma(-q, den, 1.0), den);
-        den = fma(den, fma(-q, den, 1.0), den);
+        const double den = rcp_pos(q);
         s = fma(2.0 * x, den, s);                     // 2 Re 1/(x + iw)
         s2 = fma(2.0 * (x2 - w * w), den * den, s2);  // 2 Re 1/(x + iw)^2
     }
