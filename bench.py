@@ -206,10 +206,14 @@ def main():
 
     # ---- roofline of the dominant kernel --------------------------------------
     t_f, t_v = float(np.mean(kf)) / 1e3, float(np.mean(kv)) / 1e3
+    # the adjoint call of the Schur path is two launches (VJP kernel below N_freq,
+    # density kernel above), timed together by the library's events
+    split = head != G.PLAIN_F64 and nc <= 6 and args.layout == 0
     if t_v >= t_f:
-        kname, t_k, flop = "k_lattice_schur<VJP>" if head else "k_lattice_plain<VJP>", t_v, f_adj(n, nc)
+        kname, t_k, flop = ("k_lattice_schur<VJP>+<DENS>" if split else "k_lattice_schur<VJP>"), t_v, f_adj(n, nc)
+        pkey = "VJP_CALL" if split else "VJP"
     else:
-        kname, t_k, flop = "k_lattice_schur<FWD>" if head else "k_lattice_plain<FWD>", t_f, f_fwd(n)
+        kname, t_k, flop, pkey = "k_lattice_schur<FWD>", t_f, f_fwd(n), "FWD"
     if head == G.PLAIN_F64:
         kname = kname.replace("schur", "plain")
     achieved = flop * units / t_k / 1e12
@@ -217,7 +221,7 @@ def main():
     tpath = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     if os.path.exists(tpath):
         try:
-            traffic = json.load(open(tpath)).get(args.method, {}).get(kname.split("<")[1].rstrip(">"))
+            traffic = json.load(open(tpath)).get(args.method, {}).get(pkey)
         except Exception:
             traffic = None
     # executed VALU FLOPs of the same kernel (rocprofv3 SQ_INSTS_VALU_FLOPS_* pass summarised in profiles/)
@@ -225,7 +229,7 @@ def main():
     fpath = os.path.join(ROOT, "profiles", "pmc_flops.json")
     if os.path.exists(fpath):
         try:
-            ex = json.load(open(fpath)).get(args.method, {}).get(kname.split("<")[1].rstrip(">"))
+            ex = json.load(open(fpath)).get(args.method, {}).get(pkey)
             if ex:
                 fl = ex["fp64_flops"] + ex["fp32_flops"]
                 executed = dict(flop_per_launch=fl, tflops=fl / t_k / 1e12, frac=fl / t_k / 1e12 / PEAK_FP64_TFLOPS,

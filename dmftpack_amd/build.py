@@ -35,7 +35,8 @@ def _compile(src):
 
 def build(verbose=True, jobs=None):
     os.makedirs(OBJ, exist_ok=True)
-    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
+    # largest translation units first, so that they do not start last
+    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")), key=lambda f: -os.path.getsize(f))
     jobs = jobs or min(8, os.cpu_count() or 4)
     with cf.ThreadPoolExecutor(jobs) as ex:
         objs = list(ex.map(_compile, srcs))

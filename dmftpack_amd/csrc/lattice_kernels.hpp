@@ -98,6 +98,19 @@ int schur_slots(int nc, int mode, bool fp32, bool cols);
 bool schur_supported(int n, int nc);
 // the persistent multi-lattice launch covers the one-matrix-per-lane kernels
 bool schur_multi_supported(int nc);
+// one-matrix-per-lane Schur kernels of one n_c (lattice_lane{1..6}.hip)
+hipError_t launch_lane_1(const LatArgs& a, int mode, bool fp32, const cd* sig_all, hipStream_t s, int wg,
+                         const SchurMulti* m);
+hipError_t launch_lane_2(const LatArgs& a, int mode, bool fp32, const cd* sig_all, hipStream_t s, int wg,
+                         const SchurMulti* m);
+hipError_t launch_lane_3(const LatArgs& a, int mode, bool fp32, const cd* sig_all, hipStream_t s, int wg,
+                         const SchurMulti* m);
+hipError_t launch_lane_4(const LatArgs& a, int mode, bool fp32, const cd* sig_all, hipStream_t s, int wg,
+                         const SchurMulti* m);
+hipError_t launch_lane_5(const LatArgs& a, int mode, bool fp32, const cd* sig_all, hipStream_t s, int wg,
+                         const SchurMulti* m);
+hipError_t launch_lane_6(const LatArgs& a, int mode, bool fp32, const cd* sig_all, hipStream_t s, int wg,
+                         const SchurMulti* m);
 // column-per-lane Schur kernels (lattice_cols{4,5,6,8,10,12,16}.hip)
 hipError_t launch_cols_4(const LatArgs& a, int mode, bool fp32, const cd* sig_all, hipStream_t s);
 hipError_t launch_cols_5(const LatArgs& a, int mode, bool fp32, const cd* sig_all, hipStream_t s);

@@ -78,6 +78,15 @@ def lattice_mode(name):
     return {"0": "FWD", "1": "DENS", "2": "VJP"}.get(args[idx] if len(args) > idx else "")
 
 
+def add_adjoint_call(d, keys):
+    """The adjoint call of the one-matrix-per-lane Schur path is two launches:
+    the VJP kernel over the first N_freq frequencies and the density kernel over
+    the rest (the bench step makes no other density launch).  VJP_CALL sums them."""
+    if "VJP" in d and "DENS" in d:
+        d["VJP_CALL"] = {k: d["VJP"][k] + d["DENS"][k] for k in keys}
+        d["VJP_CALL"]["kernel"] = d["VJP"]["kernel"] + " + " + d["DENS"]["kernel"]
+
+
 def pmc(argv):
     ap = argparse.ArgumentParser(prog="rocprof_summary pmc")
     ap.add_argument("--fetch", required=True)
@@ -116,6 +125,7 @@ def pmc(argv):
         mode = lattice_mode(name)
         if mode:
             result[mode] = dict(bytes_per_launch=mf + mw, fetch_bytes=mf, write_bytes=mw, kernel=name)
+    add_adjoint_call(result, ("bytes_per_launch", "fetch_bytes", "write_bytes"))
     if a.json:
         db = json.load(open(a.json)) if os.path.exists(a.json) else {}
         db[a.method] = {k: v["bytes_per_launch"] for k, v in result.items()}
@@ -165,6 +175,7 @@ def sq(argv):
         mode = lattice_mode(name)
         if mode:
             out[mode] = dict(fp64_flops=f64, fp32_flops=f32, launch_s=t, kernel=name)
+    add_adjoint_call(out, ("fp64_flops", "fp32_flops", "launch_s"))
     if a.json:
         db = json.load(open(a.json)) if os.path.exists(a.json) else {}
         db[a.method] = out
