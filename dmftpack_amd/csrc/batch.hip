@@ -92,6 +92,10 @@ struct Group {
     LatArgs* dargs[2] = {nullptr, nullptr};
     int4* ditems[2] = {nullptr, nullptr};
     int n_items[2] = {0, 0};
+    // adjoint: frequencies n >= N_freq (tr G, tr G^2 only) run in the density kernel
+    LatArgs* dargs_hi = nullptr;
+    int4* ditems_hi = nullptr;
+    int n_items_hi = 0;
     int max_nd = 0;
 };
 
@@ -177,8 +181,8 @@ LatArgs make_args(const dmftgpu_batch_s* b, const Sys& s, int mode, int chunks, 
 int build_group(dmftgpu_batch_s* b, Group& g, int mode, int wg, double target)
 {
     dmftgpu_handle h = b->h;
-    std::vector<LatArgs> args;
-    std::vector<int4> items;
+    std::vector<LatArgs> args, args_hi;
+    std::vector<int4> items, items_hi;
     for (size_t q = 0; q < g.systems.size(); ++q) {
         Sys& s = b->sys[g.systems[q]];
         int kpc;
@@ -190,13 +194,29 @@ int build_group(dmftgpu_batch_s* b, Group& g, int mode, int wg, double target)
             s.chunks_v = chunks;
             s.kpc_v = kpc;
         }
-        args.push_back(make_args(b, s, mode, chunks, kpc));
-        const int nwt = (s.nw_tot + wg - 1) / wg;
+        LatArgs a = make_args(b, s, mode, chunks, kpc);
+        const bool split = mode == MODE_VJP && s.nw_tot > s.nf;
+        if (split) {  // [0, nf) adjoint, [nf, nw_tot) density
+            LatArgs hi = a;
+            a.nw_end = s.nf;
+            hi.nw_begin = s.nf;
+            args_hi.push_back(hi);
+        } else {
+            args_hi.push_back(a);  // unused placeholder keeps the system index
+        }
+        args.push_back(a);
+        const int nwt = ((split ? s.nf : s.nw_tot) + wg - 1) / wg;
         for (int c = 0; c < chunks; ++c)
             for (int t = 0; t < nwt; ++t) items.push_back(int4{(int)q, t, c, 0});
+        if (split) {
+            const int nwh = (s.nw_tot - s.nf + wg - 1) / wg;
+            for (int c = 0; c < chunks; ++c)
+                for (int t = 0; t < nwh; ++t) items_hi.push_back(int4{(int)q, t, c, 0});
+        }
     }
     // interleave systems so neighbouring workgroups mix large and small items
     std::stable_sort(items.begin(), items.end(), [](const int4& x, const int4& y) { return x.z < y.z; });
+    std::stable_sort(items_hi.begin(), items_hi.end(), [](const int4& x, const int4& y) { return x.z < y.z; });
     if (!g.dargs[slot(mode)]) {
         void* p = nullptr;
         HIPCHK(hipMalloc(&p, sizeof(LatArgs) * args.size()));
@@ -212,6 +232,23 @@ int build_group(dmftgpu_batch_s* b, Group& g, int mode, int wg, double target)
     g.target[slot(mode)] = target;
     HIPCHK(hipMemcpy(g.dargs[slot(mode)], args.data(), sizeof(LatArgs) * args.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(g.ditems[slot(mode)], items.data(), sizeof(int4) * items.size(), hipMemcpyHostToDevice));
+    if (mode == MODE_VJP) {
+        g.n_items_hi = (int)items_hi.size();
+        if (!g.dargs_hi) {
+            void* p = nullptr;
+            HIPCHK(hipMalloc(&p, sizeof(LatArgs) * args_hi.size()));
+            b->extra.push_back(p);
+            g.dargs_hi = (LatArgs*)p;
+        }
+        HIPCHK(hipMemcpy(g.dargs_hi, args_hi.data(), sizeof(LatArgs) * args_hi.size(), hipMemcpyHostToDevice));
+        if (!items_hi.empty()) {
+            void* p = nullptr;
+            HIPCHK(hipMalloc(&p, sizeof(int4) * items_hi.size()));
+            b->extra.push_back(p);
+            g.ditems_hi = (int4*)p;
+            HIPCHK(hipMemcpy(g.ditems_hi, items_hi.data(), sizeof(int4) * items_hi.size(), hipMemcpyHostToDevice));
+        }
+    }
     return DMFTGPU_OK;
 }
 
@@ -230,6 +267,14 @@ int launch_group(dmftgpu_batch_s* b, Group& g, int mode)
     m.grid = std::min(g.n_items[slot(mode)], 256 * 16);
     m.max_nd = g.max_nd;
     HIPCHK(dg::launch_lattice_schur_multi(proto, mode, fp32, h->stream, g.wg[slot(mode)], m));
+    if (mode == MODE_VJP && g.n_items_hi > 0) {
+        SchurMulti mh = m;
+        mh.args = g.dargs_hi;
+        mh.items = g.ditems_hi;
+        mh.n_items = g.n_items_hi;
+        mh.grid = std::min(g.n_items_hi, 256 * 16);
+        HIPCHK(dg::launch_lattice_schur_multi(proto, MODE_DENS, fp32, h->stream, g.wg[slot(mode)], mh));
+    }
     return DMFTGPU_OK;
 }
 
