@@ -8,6 +8,10 @@
 namespace dg {
 
 enum LatMode { MODE_FWD = 0, MODE_DENS = 1, MODE_VJP = 2 };
+// One-matrix-per-lane kernels only: the forward over frequencies n >= N_freq,
+// whose G_cc enters NumMatrix alone (through G + G^H), accumulates the
+// Hermitian part (G + G^H)/2 (n_c^2 reals instead of n_c^2 complex).
+constexpr int MODE_FWD_HERM = 3;
 
 // Data read through the constant address space: the compiler treats it as
 // invariant for the kernel's duration, so wave-uniform loads become scalar

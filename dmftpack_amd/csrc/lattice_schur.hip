@@ -71,6 +71,16 @@ hipError_t launch_lattice_schur(const LatArgs& a, int mode, bool fp32, hipStream
             default: return launch_cols_16(a, mode, fp32, a.pSigAll, s);
         }
     }
+    if (mode == MODE_FWD && a.nw_tot > a.n_freq) {
+        // Frequencies beyond n_freq enter NumMatrix (and N) only: the Hermitian
+        // part of G_cc is accumulated there, half the accumulators.
+        LatArgs lo = a, hi = a;
+        lo.nw_end = a.n_freq;
+        hi.nw_begin = a.n_freq;
+        e = dispatch_nc(lo, MODE_FWD, fp32, a.pSigAll, s, wg, nullptr);
+        if (e != hipSuccess) return e;
+        return dispatch_nc(hi, MODE_FWD_HERM, fp32, a.pSigAll, s, wg, nullptr);
+    }
     if (mode == MODE_VJP && a.nw_tot > a.n_freq) {
         // Frequencies beyond n_freq contribute only tr G and tr G^2 (dN/dmu):
         // the density kernel does that at twice the occupancy of the adjoint.

@@ -100,11 +100,14 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
     using BD = kconst_double;
     constexpr int NP = NC * (NC + 1) / 2;
     constexpr int M2 = NC * NC;
+    constexpr bool HERM = MODE == MODE_FWD_HERM;
+    constexpr bool FWDLIKE = MODE == MODE_FWD || HERM;
     // accumulators that do not fit next to the working matrices live in LDS
-    constexpr bool ACC_LDS = (MODE == MODE_VJP) || (MODE == MODE_FWD && F32R);
+    constexpr bool ACC_LDS = (MODE == MODE_VJP) || (FWDLIKE && F32R);
     const int nd = a.n - NC;
     const int bs = kblock_size<NC>(nd);
-    cd* sAcc = lds;  // [M2][WG], lane-contiguous
+    cd* sAcc = lds;                   // [M2][WG] complex, lane-contiguous
+    double* sAccH = (double*)lds;     // HERM: [M2][WG] reals
     const int lane = threadIdx.x;
     const int nw_hi = a.nw_end ? a.nw_end : a.nw_tot;
     const int nw = a.nw_begin + wtile * WG + lane;
@@ -120,8 +123,18 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
     // nbar: host value, or (batched adjoint) a device scalar written by an earlier kernel
     const double cn = 2.0 * (a.nbar_dev ? *a.nbar_dev : a.nbar) / a.beta;
 
-    cd acc[ACC_LDS ? 1 : NC][ACC_LDS ? 1 : NC];
-    if (ACC_LDS) {
+    cd acc[ACC_LDS || HERM ? 1 : NC][ACC_LDS || HERM ? 1 : NC];
+    // HERM: sums of Re G_ii and, for i < j, of (G_ij + G_ji).x, (G_ij - G_ji).y,
+    // in the order (j ascending, i <= j ascending)
+    double accH[HERM && !ACC_LDS ? M2 : 1];
+    if (HERM) {
+        if (ACC_LDS) {
+            for (int e = 0; e < M2; ++e) sAccH[e * WG + lane] = 0.0;
+        } else {
+#pragma unroll
+            for (int e = 0; e < (HERM && !ACC_LDS ? M2 : 1); ++e) accH[e] = 0.0;
+        }
+    } else if (ACC_LDS) {
         for (int e = 0; e < M2; ++e) sAcc[e * WG + lane] = cd{0.0, 0.0};
     } else {
 #pragma unroll
@@ -265,8 +278,25 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
             }
         }
 
-        if (MODE == MODE_FWD) {
-            if (ACC_LDS) {
+        if (FWDLIKE) {
+            if constexpr (HERM) {
+                int q = 0;
+#pragma unroll
+                for (int j = 0; j < NC; ++j)
+#pragma unroll
+                    for (int i = 0; i <= j; ++i) {
+                        const double h0 = i == j ? G[i][i].x : G[i][j].x + G[j][i].x;
+                        const double h1 = G[i][j].y - G[j][i].y;
+                        if (ACC_LDS) {
+                            sAccH[q * WG + lane] += h0;
+                            if (i != j) sAccH[(q + 1) * WG + lane] += h1;
+                        } else {
+                            accH[q] += h0;
+                            if (i != j) accH[q + 1] += h1;
+                        }
+                        q += i == j ? 1 : 2;
+                    }
+            } else if (ACC_LDS) {
 #pragma unroll
                 for (int j = 0; j < NC; ++j)
 #pragma unroll
@@ -276,9 +306,9 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
                     }
             } else {
 #pragma unroll
-                for (int i = 0; i < (ACC_LDS ? 1 : NC); ++i)
+                for (int i = 0; i < (ACC_LDS || HERM ? 1 : NC); ++i)
 #pragma unroll
-                    for (int j = 0; j < (ACC_LDS ? 1 : NC); ++j) acc[i][j] = cadd(acc[i][j], G[i][j]);
+                    for (int j = 0; j < (ACC_LDS || HERM ? 1 : NC); ++j) acc[i][j] = cadd(acc[i][j], G[i][j]);
             }
             if (TRACE) {  // tr G = sum_d g_d + tr G_cc + sum_d g_d^2 M_d
                 cd t = cd{0.0, 0.0};
@@ -471,15 +501,36 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
 
     if (!wvalid) return;
     const size_t ch = chunk;
-    if (MODE == MODE_FWD) {
+    if (HERM) {
+        // (G + G^H)/2 in full layout: NumMatrix sums G_ij + conj(G_ji) over frequencies
+        a.pTr[ch * a.nw_tot + nw] = trA;
+        cd* out = a.pG + (ch * a.nw_tot + nw) * M2;
+        int q = 0;
+#pragma unroll
+        for (int j = 0; j < NC; ++j)
+#pragma unroll
+            for (int i = 0; i <= j; ++i) {
+                const double h0 = ACC_LDS ? sAccH[q * WG + lane] : accH[q];
+                if (i == j) {
+                    out[i + i * NC] = cd{h0, 0.0};
+                    q += 1;
+                } else {
+                    const double h1 = ACC_LDS ? sAccH[(q + 1) * WG + lane] : accH[q + 1];
+                    out[i + j * NC] = cd{0.5 * h0, 0.5 * h1};
+                    out[j + i * NC] = cd{0.5 * h0, -0.5 * h1};
+                    q += 2;
+                }
+            }
+    } else if (MODE == MODE_FWD) {
         a.pTr[ch * a.nw_tot + nw] = trA;
         if (ACC_LDS) {
             for (int e = 0; e < M2; ++e) a.pG[(ch * a.nw_tot + nw) * M2 + e] = sAcc[e * WG + lane];
         } else {
 #pragma unroll
-            for (int i = 0; i < (ACC_LDS ? 1 : NC); ++i)
+            for (int i = 0; i < (ACC_LDS || HERM ? 1 : NC); ++i)
 #pragma unroll
-                for (int j = 0; j < (ACC_LDS ? 1 : NC); ++j) a.pG[(ch * a.nw_tot + nw) * M2 + i + j * NC] = acc[i][j];
+                for (int j = 0; j < (ACC_LDS || HERM ? 1 : NC); ++j)
+                    a.pG[(ch * a.nw_tot + nw) * M2 + i + j * NC] = acc[i][j];
         }
     } else {
         a.pTr[ch * a.nw_tot + nw] = trA;
@@ -526,6 +577,7 @@ void k_lattice_schur_multi(const LatArgs* __restrict__ args,
 template <int NC, int MODE, bool F32R, int WG>
 constexpr size_t schur_shm()
 {
+    if constexpr (MODE == MODE_FWD_HERM) return F32R ? sizeof(double) * (size_t)NC * NC * WG : 0;
     constexpr bool ACC_LDS = (MODE == MODE_VJP) || (MODE == MODE_FWD && F32R);
     return ACC_LDS ? sizeof(cd) * (size_t)NC * NC * WG : 0;
 }
@@ -580,6 +632,13 @@ hipError_t launch_lane_nc(const LatArgs& a, int mode, bool fp32, const cd* sig_a
                         : launch_nc<NC, MODE_FWD, false, true>(a, sig_all, s, wg, m);
         return fp32 ? launch_nc<NC, MODE_FWD, true, false>(a, sig_all, s, wg, m)
                     : launch_nc<NC, MODE_FWD, false, false>(a, sig_all, s, wg, m);
+    }
+    if (mode == MODE_FWD_HERM) {
+        if (a.want_trace)
+            return fp32 ? launch_nc<NC, MODE_FWD_HERM, true, true>(a, sig_all, s, wg, m)
+                        : launch_nc<NC, MODE_FWD_HERM, false, true>(a, sig_all, s, wg, m);
+        return fp32 ? launch_nc<NC, MODE_FWD_HERM, true, false>(a, sig_all, s, wg, m)
+                    : launch_nc<NC, MODE_FWD_HERM, false, false>(a, sig_all, s, wg, m);
     }
     if (mode == MODE_DENS)
         return fp32 ? launch_nc<NC, MODE_DENS, true, false>(a, sig_all, s, wg, m)
