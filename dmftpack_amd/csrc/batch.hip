@@ -92,10 +92,11 @@ struct Group {
     LatArgs* dargs[2] = {nullptr, nullptr};
     int4* ditems[2] = {nullptr, nullptr};
     int n_items[2] = {0, 0};
-    // adjoint: frequencies n >= N_freq (tr G, tr G^2 only) run in the density kernel
-    LatArgs* dargs_hi = nullptr;
-    int4* ditems_hi = nullptr;
-    int n_items_hi = 0;
+    // frequencies n >= N_freq: the forward accumulates the Hermitian part of G_cc
+    // (MODE_FWD_HERM), the adjoint only needs tr G, tr G^2 (density kernel)
+    LatArgs* dargs_hi[2] = {nullptr, nullptr};
+    int4* ditems_hi[2] = {nullptr, nullptr};
+    int n_items_hi[2] = {0, 0};
     int max_nd = 0;
 };
 
@@ -195,8 +196,8 @@ int build_group(dmftgpu_batch_s* b, Group& g, int mode, int wg, double target)
             s.kpc_v = kpc;
         }
         LatArgs a = make_args(b, s, mode, chunks, kpc);
-        const bool split = mode == MODE_VJP && s.nw_tot > s.nf;
-        if (split) {  // [0, nf) adjoint, [nf, nw_tot) density
+        const bool split = s.nw_tot > s.nf;
+        if (split) {  // [0, nf) forward / adjoint, [nf, nw_tot) Hermitian forward / density
             LatArgs hi = a;
             a.nw_end = s.nf;
             hi.nw_begin = s.nf;
@@ -232,22 +233,21 @@ int build_group(dmftgpu_batch_s* b, Group& g, int mode, int wg, double target)
     g.target[slot(mode)] = target;
     HIPCHK(hipMemcpy(g.dargs[slot(mode)], args.data(), sizeof(LatArgs) * args.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(g.ditems[slot(mode)], items.data(), sizeof(int4) * items.size(), hipMemcpyHostToDevice));
-    if (mode == MODE_VJP) {
-        g.n_items_hi = (int)items_hi.size();
-        if (!g.dargs_hi) {
-            void* p = nullptr;
-            HIPCHK(hipMalloc(&p, sizeof(LatArgs) * args_hi.size()));
-            b->extra.push_back(p);
-            g.dargs_hi = (LatArgs*)p;
-        }
-        HIPCHK(hipMemcpy(g.dargs_hi, args_hi.data(), sizeof(LatArgs) * args_hi.size(), hipMemcpyHostToDevice));
-        if (!items_hi.empty()) {
-            void* p = nullptr;
-            HIPCHK(hipMalloc(&p, sizeof(int4) * items_hi.size()));
-            b->extra.push_back(p);
-            g.ditems_hi = (int4*)p;
-            HIPCHK(hipMemcpy(g.ditems_hi, items_hi.data(), sizeof(int4) * items_hi.size(), hipMemcpyHostToDevice));
-        }
+    const int sl = slot(mode);
+    g.n_items_hi[sl] = (int)items_hi.size();
+    if (!g.dargs_hi[sl]) {
+        void* p = nullptr;
+        HIPCHK(hipMalloc(&p, sizeof(LatArgs) * args_hi.size()));
+        b->extra.push_back(p);
+        g.dargs_hi[sl] = (LatArgs*)p;
+    }
+    HIPCHK(hipMemcpy(g.dargs_hi[sl], args_hi.data(), sizeof(LatArgs) * args_hi.size(), hipMemcpyHostToDevice));
+    if (!items_hi.empty()) {
+        void* p = nullptr;
+        HIPCHK(hipMalloc(&p, sizeof(int4) * items_hi.size()));
+        b->extra.push_back(p);
+        g.ditems_hi[sl] = (int4*)p;
+        HIPCHK(hipMemcpy(g.ditems_hi[sl], items_hi.data(), sizeof(int4) * items_hi.size(), hipMemcpyHostToDevice));
     }
     return DMFTGPU_OK;
 }
@@ -267,13 +267,14 @@ int launch_group(dmftgpu_batch_s* b, Group& g, int mode)
     m.grid = std::min(g.n_items[slot(mode)], 256 * 16);
     m.max_nd = g.max_nd;
     HIPCHK(dg::launch_lattice_schur_multi(proto, mode, fp32, h->stream, g.wg[slot(mode)], m));
-    if (mode == MODE_VJP && g.n_items_hi > 0) {
+    if (g.n_items_hi[slot(mode)] > 0) {
         SchurMulti mh = m;
-        mh.args = g.dargs_hi;
-        mh.items = g.ditems_hi;
-        mh.n_items = g.n_items_hi;
-        mh.grid = std::min(g.n_items_hi, 256 * 16);
-        HIPCHK(dg::launch_lattice_schur_multi(proto, MODE_DENS, fp32, h->stream, g.wg[slot(mode)], mh));
+        mh.args = g.dargs_hi[slot(mode)];
+        mh.items = g.ditems_hi[slot(mode)];
+        mh.n_items = g.n_items_hi[slot(mode)];
+        mh.grid = std::min(mh.n_items, 256 * 16);
+        HIPCHK(dg::launch_lattice_schur_multi(proto, mode == MODE_VJP ? MODE_DENS : MODE_FWD_HERM, fp32, h->stream,
+                                              g.wg[slot(mode)], mh));
     }
     return DMFTGPU_OK;
 }
