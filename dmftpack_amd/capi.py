@@ -72,6 +72,7 @@ _SIGS = {
     "dmftgpu_ft_w2tau_points": ([P, I, I, D, P, P, I, PD, P], I),
     "dmftgpu_update_moments": ([P, P], I),
     "dmftgpu_greenftn_real": ([P, I, I, D, D, D, P, D, P, P], I),
+    "dmftgpu_greenftn_real_k": ([P, I, I, D, D, D, P, D, P, P], I),
     "dmftgpu_hyb_delta": ([P, I, I, D, D, P, P, P, I, PI, P], I),
     "dmftgpu_hk_from_hr": ([P, I, I, PI, PD, I, P, P], I),
 }

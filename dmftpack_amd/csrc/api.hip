@@ -335,8 +335,16 @@ struct RealAxis {
     double e0, de, eta;
 };
 
+// Per-k outputs: one k per chunk, the per-chunk partials are the results
+// (g [n_k][nw][nc*nc], tr [n_k][nw]); no reduction over k.
+struct PerK {
+    cd* g;
+    cd* tr;
+};
+
 int run_lattice(dmftgpu_handle h, int method, int mode, const double* sigma, double mu, const double* gbar,
-                double nbar, char* ws, size_t& off, Sums& S, bool want_trace = false, const RealAxis* ra = nullptr)
+                double nbar, char* ws, size_t& off, Sums& S, bool want_trace = false, const RealAxis* ra = nullptr,
+                const PerK* pk = nullptr)
 {
     const dmftgpu_lattice& L = h->lat;
     const int n = L.n, nc = L.n_corr, nk = L.n_k, m2 = nc * nc;
@@ -345,7 +353,7 @@ int run_lattice(dmftgpu_handle h, int method, int mode, const double* sigma, dou
     int slots;
     if (method == DMFTGPU_PLAIN_F64) slots = dg::plain_slots(n, mode);
     else slots = dg::schur_slots(nc, mode, method == DMFTGPU_SCHUR_F32R, h->schur_layout == 1 || ra);
-    Plan P = make_plan(nk, nw_tot, slots, h->target_wg);
+    Plan P = pk ? Plan{nk, 1} : make_plan(nk, nw_tot, slots, h->target_wg);
     dg::LatArgs a{};
     a.Hk = h->Hk;
     a.sigma = (const cd*)sigma;
@@ -378,8 +386,13 @@ int run_lattice(dmftgpu_handle h, int method, int mode, const double* sigma, dou
         return p;
     };
     const size_t C = P.n_chunks;
-    a.pTr = (cd*)take(C * nw_tot * sizeof(cd));
-    if (mode == dg::MODE_FWD) a.pG = (cd*)take(C * nw_tot * m2 * sizeof(cd));
+    if (pk) {
+        a.pTr = pk->tr;
+        a.pG = pk->g;
+    } else {
+        a.pTr = (cd*)take(C * nw_tot * sizeof(cd));
+        if (mode == dg::MODE_FWD) a.pG = (cd*)take(C * nw_tot * m2 * sizeof(cd));
+    }
     if (mode != dg::MODE_FWD) a.pTr2 = (cd*)take(C * nw_tot * sizeof(cd));
     if (mode == dg::MODE_VJP) {
         a.pSb = (cd*)take(C * n_freq * m2 * sizeof(cd));
@@ -397,6 +410,7 @@ int run_lattice(dmftgpu_handle h, int method, int mode, const double* sigma, dou
         HIPCHK(hipEventRecord(h->ev1[mode], h->stream));
         h->ev_used[mode] = true;
     }
+    if (pk) return DMFTGPU_OK;
     S.tr = (cd*)take(nw_tot * sizeof(cd));
     HIPCHK(dg::launch_reduce_chunks(a.pTr, P.n_chunks, nw_tot, S.tr, h->stream));
     if (mode == dg::MODE_FWD) {
@@ -522,6 +536,39 @@ extern "C" int dmftgpu_greenftn_real(dmftgpu_handle h, int method, int n_e, doub
     HIPCHK(dg::launch_scale(S.Gsum, (size_t)n_e * m2, inv_nk, (cd*)gloc_r, h->stream));
     if (trace) HIPCHK(dg::launch_scale(S.tr, (size_t)n_e, inv_nk, (cd*)trace, h->stream));
     return DMFTGPU_OK;
+}
+
+// Per-k real-axis Green's function (qs_spectra.cpp:197-221, the band spectra)
+extern "C" int dmftgpu_greenftn_real_k(dmftgpu_handle h, int method, int n_e, double e0, double de, double eta,
+                                       const double* sigma_r, double mu, double* gcc_k, double* trace_k)
+{
+    (void)hipGetLastError();  // drop an error left by an earlier failed call
+    if (!h) return DMFTGPU_ERR_ARG;
+    if (!h->prepared) return fail(h, DMFTGPU_ERR_STATE, "dmftgpu_prepare has not been called");
+    if (!valid_method(method)) return fail(h, DMFTGPU_ERR_ARG, "unknown method %d", method);
+    if (method != DMFTGPU_PLAIN_F64 && !h->schur_ready)
+        return fail(h, DMFTGPU_ERR_UNSUPPORTED, "Schur method needs n_corr <= 16 (n=%d, n_corr=%d)", h->lat.n, h->lat.n_corr);
+    if (n_e < 1 || !std::isfinite(e0) || !std::isfinite(de) || !(eta > 0.0) || !std::isfinite(eta) ||
+        !std::isfinite(mu))
+        return fail(h, DMFTGPU_ERR_ARG, "n_e=%d e0=%g de=%g eta=%g mu=%g (eta must be > 0)", n_e, e0, de, eta, mu);
+    if (!sigma_r || (!gcc_k && !trace_k)) return fail(h, DMFTGPU_ERR_ARG, "sigma_r and gcc_k or trace_k are required");
+    const dmftgpu_lattice& L = h->lat;
+    const size_t m2 = (size_t)L.n_corr * L.n_corr;
+    const RealAxis ra{n_e, e0, de, eta};
+    // the kernels always write both partials: scratch for the one not asked for
+    const size_t g_scratch = gcc_k ? 0 : (size_t)L.n_k * n_e * m2 * sizeof(cd);
+    const size_t t_scratch = trace_k ? 0 : (size_t)L.n_k * n_e * sizeof(cd);
+    hipError_t e;
+    size_t need = lattice_ws_bytes(h, method, dg::MODE_FWD, &ra) + g_scratch + t_scratch + 8192;
+    char* ws = (char*)h->ws.get(need, &e);
+    if (!ws) return fail(h, DMFTGPU_ERR_HIP, "workspace %zu bytes: %s", need, hipGetErrorString(e));
+    size_t off = 0;
+    PerK pk{gcc_k ? (cd*)gcc_k : (cd*)(ws + off), nullptr};
+    off += (g_scratch + 255) / 256 * 256;
+    pk.tr = trace_k ? (cd*)trace_k : (cd*)(ws + off);
+    off += (t_scratch + 255) / 256 * 256;
+    Sums S;
+    return run_lattice(h, method, dg::MODE_FWD, sigma_r, mu, nullptr, 0.0, ws, off, S, trace_k != nullptr, &ra, &pk);
 }
 
 // N(mu), dN/dmu

@@ -221,6 +221,16 @@ class LatticeGF(Handle):
         self.check(self.lib.dmftgpu_greenftn_real(self.h, method, ne, e0, de, eta, _ptr(s), mu, _ptr(g), _ptr(tr)))
         return from_dev_cm(g), tr.cpu().numpy()
 
+    def greenftn_real_k(self, sigma_r, mu, e0, de, eta, method=PLAIN_F64):
+        """Per-k retarded G (the band spectra of qs_spectra.cpp:197-221): returns
+        (G^R_cc [n_k][n_e][nc][nc], tr G^R [n_k][n_e]) with no k average."""
+        s = to_dev_cm(sigma_r)
+        ne = s.shape[0]
+        g = torch.empty((self.nk, ne, self.nc, self.nc), dtype=torch.complex128, device="cuda")
+        tr = torch.empty((self.nk, ne), dtype=torch.complex128, device="cuda")
+        self.check(self.lib.dmftgpu_greenftn_real_k(self.h, method, ne, e0, de, eta, _ptr(s), mu, _ptr(g), _ptr(tr)))
+        return from_dev_cm(g), tr.cpu().numpy()
+
     def density(self, sigma, mu, method=PLAIN_F64):
         """(N(mu), dN/dmu) -- GiwToNele (GreenFtn.cpp:513-635) plus its derivative."""
         s = self.sigma_dev(sigma)

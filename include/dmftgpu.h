@@ -123,6 +123,16 @@ int dmftgpu_greenftn_w(dmftgpu_handle h, int method, const double* sigma, double
 int dmftgpu_greenftn_real(dmftgpu_handle h, int method, int n_e, double e0, double de, double eta,
                           const double* sigma_r, double mu, double* gloc_r, double* trace);
 
+/* Per-k form of dmftgpu_greenftn_real, for the band spectra of qs_spectra
+ * (qs_spectra.cpp:197-221: A(k, E) = -Im tr G^R(k, E) / pi and the partial
+ * weights -Im G^R_ii(k, E) / pi of the correlated orbitals), H(k) bound by
+ * dmftgpu_prepare being the k path:
+ *   gcc_k   [n_k][n_e][nc][nc]  G^R_cc(k, E_j), or NULL
+ *   trace_k [n_k][n_e] complex  tr G^R(k, E_j) over the model space, or NULL
+ * (at least one of them); nothing is averaged over k. */
+int dmftgpu_greenftn_real_k(dmftgpu_handle h, int method, int n_e, double e0, double de, double eta,
+                            const double* sigma_r, double mu, double* gcc_k, double* trace_k);
+
 /* Per-k density matrix densityMatDFT (GreenFtn.cpp:291-319): dens [n_k][n][n].
  * Full n x n inverse, DMFTGPU_PLAIN_F64 only. */
 int dmftgpu_density_matrix(dmftgpu_handle h, const double* sigma, double mu, double* dens);

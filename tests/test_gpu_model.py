@@ -126,3 +126,40 @@ def test_greenftn_real(gpu, name, method, tol):
     assert np.max(np.abs(got - ref)) <= tol * max(1.0, np.max(np.abs(ref)))
     assert np.max(np.abs(tr - tr_ref)) <= tol * max(1.0, np.max(np.abs(tr_ref)))
     lat.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,method,tol", [("t2g", 0, 1e-12), ("dp14", 1, 1e-12), ("dp14", 2, 1e-9),
+                                             ("large32", 1, 1e-12)])
+def test_greenftn_real_per_k(gpu, name, method, tol):
+    """Band spectra on a k path (qs_spectra.cpp:197-221): the per-k retarded G
+    against the oracle run on one k at a time, and its k average against the
+    summed entry point."""
+    G, _ = gpu
+    if name == "t2g":
+        Hk, _ = L.t2g_hk(4)
+        nc = 3
+    elif name == "dp14":
+        Hk, nc = L.random_tb_hk(14, 2, 4), 5
+    else:
+        Hk, nc = L.random_tb_hk(32, 2, 4), 10
+    Hk = Hk[:7]  # a short k path
+    ne, e0, de, eta, mu = 90, -3.0, 0.07, 0.05, 0.1
+    rng = np.random.default_rng(3)
+    E = e0 + de * np.arange(ne)
+    B = rng.normal(size=(nc, nc)) + 1j * rng.normal(size=(nc, nc))
+    sig = (B @ B.conj().T / nc)[None] / (E - 0.2 + 0.4j)[:, None, None]
+    print(f"greenftn_real_k {name} method {method}", flush=True)
+    lat = G.LatticeGF(Hk, nc, 16, 40.0, np.zeros((4, nc, nc), complex))
+    gk, trk = lat.greenftn_real_k(sig, mu, e0, de, eta, method=method)
+    assert gk.shape == (7, ne, nc, nc) and trk.shape == (7, ne)
+    for k in range(7):
+        ref, tr_ref = O.greenftn_real(Hk[k:k + 1], sig, mu, e0, de, eta)
+        assert np.max(np.abs(gk[k] - ref)) <= tol * max(1.0, np.max(np.abs(ref)))
+        assert np.max(np.abs(trk[k] - tr_ref)) <= tol * max(1.0, np.max(np.abs(tr_ref)))
+    g_avg, tr_avg = lat.greenftn_real(sig, mu, e0, de, eta, method=method)
+    assert np.max(np.abs(gk.mean(axis=0) - g_avg)) <= 1e-13 * max(1.0, np.max(np.abs(g_avg)))
+    assert np.max(np.abs(trk.mean(axis=0) - tr_avg)) <= 1e-13 * max(1.0, np.max(np.abs(tr_avg)))
+    # spectral function of the band plot: A(k, E) = -Im tr G^R / pi >= 0
+    assert np.min(-trk.imag) > -1e-12
+    lat.close()
