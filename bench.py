@@ -137,10 +137,8 @@ def main():
     def step(method):
         # a new Sigma brings new tail moments: static eigen-decomposition of H(k) + P m0 P^H
         lat.check(lat.lib.dmftgpu_update_moments(lat.h, G._ptr(mom_dev)))
-        gl, num, N = lat.greenftn_w_dev(s_dev, mu, method, gloc=gbuf, num=nbuf, want_n=True)
-        gbar = 2.0 * (gl - gtgt)
-        nbar = 2.0 * (N - N0)
-        sb, mb = lat.vjp_dev(s_dev, mu, gbar, nbar, method, out=sbar)
+        # forward (G_loc, NumMatrix, N), loss cotangents and adjoint (dmftgpu_gloc_loss_grad)
+        _, _, N, _, _, mb = lat.loss_grad_dev(s_dev, mu, gtgt, N0, method, gloc=gbuf, num=nbuf, out=sbar)
         return N, mb
 
     def timed(method, steps, warmup, kernel_times=False):
@@ -172,11 +170,8 @@ def main():
         for name in ("plain_f64", "schur_f64", "schur_f32r"):
             log(f"reference outputs {name}")
             m = methods[name]
-            gl, num, N = lat.greenftn_w_dev(s_dev, mu, m, want_n=True)
-            gl = gl.clone()
-            gbar = 2.0 * (gl - gtgt)
-            sb, mb = lat.vjp_dev(s_dev, mu, gbar, 2.0 * (N - N0), m, out=torch.empty_like(s_dev))
-            res[name] = dict(gloc=gl, num=num.clone(), N=N, sb=sb, mb=mb)
+            gl, num, N, _, sb, mb = lat.loss_grad_dev(s_dev, mu, gtgt, N0, m)
+            res[name] = dict(gloc=gl, num=num, N=N, sb=sb, mb=mb)
         ref = res["plain_f64"]
         errs = {}
         for name in ("schur_f64", "schur_f32r"):
@@ -206,17 +201,21 @@ def main():
 
     # ---- roofline of the dominant kernel --------------------------------------
     t_f, t_v = float(np.mean(kf)) / 1e3, float(np.mean(kv)) / 1e3
-    # the adjoint call of the Schur path is two launches (VJP kernel below N_freq,
-    # density kernel above), timed together by the library's events
-    split = head != G.PLAIN_F64 and nc <= 6 and args.layout == 0
+    # On the one-matrix-per-lane Schur path dmftgpu_gloc_loss_grad is fused: the
+    # forward call is two launches (forward below N_freq; Hermitian-part forward +
+    # tr G^2 above), the adjoint call one launch over the first N_freq frequencies.
+    # The library's events time each call as a whole.
+    fused = head != G.PLAIN_F64 and nc <= 6 and args.layout == 0
     if t_v >= t_f:
-        kname, t_k, flop = ("k_lattice_schur<VJP>+<DENS>" if split else "k_lattice_schur<VJP>"), t_v, f_adj(n, nc)
-        pkey = "VJP_CALL" if split else "VJP"
+        kname = "k_lattice_schur<VJP> (first N_freq)" if fused else "k_lattice_schur<VJP>"
+        t_k, flop, pkey = t_v, f_adj(n, nc), "VJP_CALL"
+        units_k = nk * nf if fused else units
     else:
-        kname, t_k, flop, pkey = "k_lattice_schur<FWD>", t_f, f_fwd(n), "FWD"
+        kname = "k_lattice_schur<FWD>+<DENS_HERM>" if fused else "k_lattice_schur<FWD>"
+        t_k, flop, pkey, units_k = t_f, f_fwd(n), "FWD_CALL", units
     if head == G.PLAIN_F64:
         kname = kname.replace("schur", "plain")
-    achieved = flop * units / t_k / 1e12
+    achieved = flop * units_k / t_k / 1e12
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     if os.path.exists(tpath):
@@ -239,7 +238,7 @@ def main():
     roofline = dict(bound="mfma", achieved=achieved, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
                     frac=achieved / PEAK_FP64_TFLOPS, traffic=traffic, kernel=kname,
                     kernel_ms=t_k * 1e3, fwd_kernel_ms=t_f * 1e3, vjp_kernel_ms=t_v * 1e3,
-                    flop_per_unit=flop, units_per_launch=units, executed=executed,
+                    flop_per_unit=flop, units_per_launch=units_k, executed=executed,
                     note="achieved = algorithmic 8n^3-based FLOPs (adjoint: 8n^3+16nc^3+8n nc^2+8n^2) per (k,iw) "
                          "x units / measured kernel time; the Schur methods execute fewer FLOPs than this "
                          "(executed: counted VALU FLOPs of the same kernel over the same live time). "
@@ -291,7 +290,8 @@ def main():
         "config": {"workload": WORKLOADS[args.workload],
                    "n": n, "n_corr": nc, "n_k": nk, "n_freq": nf, "beta": beta, "method": args.method,
                    "refine_steps": args.refine, "schur_layout": args.layout, "units_per_step": units,
-                   "step": "static eig for the Sigma tail + GreenFtn_w forward (G_loc, NumMatrix, N) + VJP (dL/dSigma, dL/dmu)"},
+                   "step": "static eig for the Sigma tail + GreenFtn_w forward (G_loc, NumMatrix, N) + loss cotangents "
+                           "+ VJP (dL/dSigma, dL/dmu), one dmftgpu_gloc_loss_grad call after dmftgpu_update_moments"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "host_to_device": {"hk_bytes": int(Hk.nbytes), "hk_upload_ms": upload_hk_ms,

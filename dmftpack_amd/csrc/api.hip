@@ -107,6 +107,8 @@ extern "C" int dmftgpu_destroy(dmftgpu_handle h)
     h->prep.release();
     h->stat.release();
     h->static_cache.release();
+    h->fused.release();
+    h->cot.release();
     delete h;
     return DMFTGPU_OK;
 }
@@ -342,9 +344,12 @@ struct PerK {
     cd* tr;
 };
 
+// fused_tr2 (one-matrix-per-lane Schur only): the [chunks][nw_tot] tr G^2 partials
+// shared by a forward that fills them beyond n_freq and the adjoint that follows
+// (dmftgpu_gloc_loss_grad), which fills the rest and reduces them.
 int run_lattice(dmftgpu_handle h, int method, int mode, const double* sigma, double mu, const double* gbar,
                 double nbar, char* ws, size_t& off, Sums& S, bool want_trace = false, const RealAxis* ra = nullptr,
-                const PerK* pk = nullptr)
+                const PerK* pk = nullptr, cd* fused_tr2 = nullptr)
 {
     const dmftgpu_lattice& L = h->lat;
     const int n = L.n, nc = L.n_corr, nk = L.n_k, m2 = nc * nc;
@@ -393,7 +398,8 @@ int run_lattice(dmftgpu_handle h, int method, int mode, const double* sigma, dou
         a.pTr = (cd*)take(C * nw_tot * sizeof(cd));
         if (mode == dg::MODE_FWD) a.pG = (cd*)take(C * nw_tot * m2 * sizeof(cd));
     }
-    if (mode != dg::MODE_FWD) a.pTr2 = (cd*)take(C * nw_tot * sizeof(cd));
+    if (fused_tr2) a.pTr2 = fused_tr2;
+    else if (mode != dg::MODE_FWD) a.pTr2 = (cd*)take(C * nw_tot * sizeof(cd));
     if (mode == dg::MODE_VJP) {
         a.pSb = (cd*)take(C * n_freq * m2 * sizeof(cd));
         a.pMu = (cd*)take(C * n_freq * sizeof(cd));
@@ -404,7 +410,7 @@ int run_lattice(dmftgpu_handle h, int method, int mode, const double* sigma, dou
         HIPCHK(dg::launch_lattice_plain(a, mode, h->stream, nullptr));
     } else {
         HIPCHK(dg::launch_lattice_schur(a, mode, method == DMFTGPU_SCHUR_F32R, h->stream, h->schur_wg,
-                                        h->schur_layout == 1));
+                                        h->schur_layout == 1, fused_tr2 != nullptr));
     }
     if (h->timing) {
         HIPCHK(hipEventRecord(h->ev1[mode], h->stream));
@@ -447,17 +453,11 @@ size_t lattice_ws_bytes(dmftgpu_handle h, int method, int mode, const RealAxis* 
 }  // namespace
 
 // GreenFtn_w (GreenFtn.cpp:142-422)
-extern "C" int dmftgpu_greenftn_w(dmftgpu_handle h, int method, const double* sigma, double mu, double* gloc,
-                                  double* gloc_hf, double* num_matrix, double* n_total)
+namespace {
+
+int forward_impl(dmftgpu_handle h, int method, const double* sigma, double mu, double* gloc, double* gloc_hf,
+                 double* num_matrix, double* n_total, cd* fused_tr2)
 {
-    (void)hipGetLastError();  // drop an error left by an earlier failed call
-    if (!h) return DMFTGPU_ERR_ARG;
-    if (!h->prepared) return fail(h, DMFTGPU_ERR_STATE, "dmftgpu_prepare has not been called");
-    if (!valid_method(method)) return fail(h, DMFTGPU_ERR_ARG, "unknown method %d", method);
-    if (method != DMFTGPU_PLAIN_F64 && !h->schur_ready)
-        return fail(h, DMFTGPU_ERR_UNSUPPORTED, "Schur method needs n_corr <= 16 (n=%d, n_corr=%d)", h->lat.n, h->lat.n_corr);
-    if (!sigma || !gloc || !num_matrix) return fail(h, DMFTGPU_ERR_ARG, "sigma, gloc and num_matrix are required");
-    if (!std::isfinite(mu)) return fail(h, DMFTGPU_ERR_ARG, "mu is not finite");
     const dmftgpu_lattice& L = h->lat;
     const int nc = L.n_corr, m2 = nc * nc;
     const int nw_tot = L.n_freq * L.high_freq;
@@ -467,7 +467,8 @@ extern "C" int dmftgpu_greenftn_w(dmftgpu_handle h, int method, const double* si
     if (!ws) return fail(h, DMFTGPU_ERR_HIP, "workspace %zu bytes: %s", need, hipGetErrorString(e));
     size_t off = 0;
     Sums S;
-    int rc = run_lattice(h, method, dg::MODE_FWD, sigma, mu, nullptr, 0.0, ws, off, S, n_total != nullptr);
+    int rc = run_lattice(h, method, dg::MODE_FWD, sigma, mu, nullptr, 0.0, ws, off, S, n_total != nullptr, nullptr,
+                         nullptr, fused_tr2);
     if (rc) return rc;
     cd* tot = (cd*)(ws + off);
     off += ((size_t)(m2 + 3) * sizeof(cd) + 255) / 256 * 256;
@@ -504,6 +505,22 @@ extern "C" int dmftgpu_greenftn_w(dmftgpu_handle h, int method, const double* si
     HIPCHK(hipMemcpyAsync(num_matrix, num.data(), sizeof(cd) * m2, hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     return DMFTGPU_OK;
+}
+
+}  // namespace
+
+extern "C" int dmftgpu_greenftn_w(dmftgpu_handle h, int method, const double* sigma, double mu, double* gloc,
+                                  double* gloc_hf, double* num_matrix, double* n_total)
+{
+    (void)hipGetLastError();  // drop an error left by an earlier failed call
+    if (!h) return DMFTGPU_ERR_ARG;
+    if (!h->prepared) return fail(h, DMFTGPU_ERR_STATE, "dmftgpu_prepare has not been called");
+    if (!valid_method(method)) return fail(h, DMFTGPU_ERR_ARG, "unknown method %d", method);
+    if (method != DMFTGPU_PLAIN_F64 && !h->schur_ready)
+        return fail(h, DMFTGPU_ERR_UNSUPPORTED, "Schur method needs n_corr <= 16 (n=%d, n_corr=%d)", h->lat.n, h->lat.n_corr);
+    if (!sigma || !gloc || !num_matrix) return fail(h, DMFTGPU_ERR_ARG, "sigma, gloc and num_matrix are required");
+    if (!std::isfinite(mu)) return fail(h, DMFTGPU_ERR_ARG, "mu is not finite");
+    return forward_impl(h, method, sigma, mu, gloc, gloc_hf, num_matrix, n_total, nullptr);
 }
 
 // retarded_GreenFtn2 (GreenFtn.cpp:425-466) summed over k, as qs_spectra
@@ -616,7 +633,7 @@ extern "C" int dmftgpu_density(dmftgpu_handle h, int method, const double* sigma
     if (!h->prepared) return fail(h, DMFTGPU_ERR_STATE, "dmftgpu_prepare has not been called");
     if (!valid_method(method)) return fail(h, DMFTGPU_ERR_ARG, "unknown method %d", method);
     if (method != DMFTGPU_PLAIN_F64 && !h->schur_ready)
-        return fail(h, DMFTGPU_ERR_UNSUPPORTED, "Schur method needs n_corr <= 6");
+        return fail(h, DMFTGPU_ERR_UNSUPPORTED, "Schur method needs n_corr <= 16 (n=%d, n_corr=%d)", h->lat.n, h->lat.n_corr);
     if (!sigma) return fail(h, DMFTGPU_ERR_ARG, "sigma is null");
     if (!std::isfinite(mu)) return fail(h, DMFTGPU_ERR_ARG, "mu is not finite");
     return density_impl(h, method, sigma, mu, n_out, dn_dmu_out);
@@ -665,16 +682,11 @@ extern "C" int dmftgpu_adjust_mu(dmftgpu_handle h, int method, const double* sig
     return DMFTGPU_OK;
 }
 
-extern "C" int dmftgpu_gloc_vjp(dmftgpu_handle h, int method, const double* sigma, double mu, const double* gloc_bar,
-                                double n_bar, double* sigma_bar, double* mu_bar)
+namespace {
+
+int vjp_impl(dmftgpu_handle h, int method, const double* sigma, double mu, const double* gloc_bar, double n_bar,
+             double* sigma_bar, double* mu_bar, cd* fused_tr2)
 {
-    (void)hipGetLastError();  // drop an error left by an earlier failed call
-    if (!h) return DMFTGPU_ERR_ARG;
-    if (!h->prepared) return fail(h, DMFTGPU_ERR_STATE, "dmftgpu_prepare has not been called");
-    if (!valid_method(method)) return fail(h, DMFTGPU_ERR_ARG, "unknown method %d", method);
-    if (method != DMFTGPU_PLAIN_F64 && !h->schur_ready)
-        return fail(h, DMFTGPU_ERR_UNSUPPORTED, "Schur method needs n_corr <= 6");
-    if (!sigma || !gloc_bar || !sigma_bar) return fail(h, DMFTGPU_ERR_ARG, "sigma, gloc_bar and sigma_bar are required");
     const dmftgpu_lattice& L = h->lat;
     const int nc = L.n_corr, m2 = nc * nc;
     const int nw_tot = L.n_freq * L.high_freq;
@@ -684,7 +696,8 @@ extern "C" int dmftgpu_gloc_vjp(dmftgpu_handle h, int method, const double* sigm
     if (!ws) return fail(h, DMFTGPU_ERR_HIP, "workspace %zu bytes: %s", need, hipGetErrorString(e));
     size_t off = 0;
     Sums S;
-    int rc = run_lattice(h, method, dg::MODE_VJP, sigma, mu, gloc_bar, n_bar, ws, off, S);
+    int rc = run_lattice(h, method, dg::MODE_VJP, sigma, mu, gloc_bar, n_bar, ws, off, S, false, nullptr, nullptr,
+                         fused_tr2);
     if (rc) return rc;
     cd* tot = (cd*)(ws + off);
     off += ((size_t)(m2 + 3) * sizeof(cd) + 255) / 256 * 256;
@@ -706,6 +719,80 @@ extern "C" int dmftgpu_gloc_vjp(dmftgpu_handle h, int method, const double* sigm
     const double mub = -htot[m2 + 2].x * inv_nk + n_bar * dN;
     if (!std::isfinite(mub)) return fail(h, DMFTGPU_ERR_NUMERIC, "mu_bar is not finite");
     if (mu_bar) *mu_bar = mub;
+    return DMFTGPU_OK;
+}
+
+bool method_ok(dmftgpu_handle h, int method, int* rc)
+{
+    *rc = DMFTGPU_OK;
+    if (!h->prepared) *rc = fail(h, DMFTGPU_ERR_STATE, "dmftgpu_prepare has not been called");
+    else if (!valid_method(method)) *rc = fail(h, DMFTGPU_ERR_ARG, "unknown method %d", method);
+    else if (method != DMFTGPU_PLAIN_F64 && !h->schur_ready)
+        *rc = fail(h, DMFTGPU_ERR_UNSUPPORTED, "Schur method needs n_corr <= 16 (n=%d, n_corr=%d)", h->lat.n,
+                   h->lat.n_corr);
+    return *rc == DMFTGPU_OK;
+}
+
+}  // namespace
+
+extern "C" int dmftgpu_gloc_vjp(dmftgpu_handle h, int method, const double* sigma, double mu, const double* gloc_bar,
+                                double n_bar, double* sigma_bar, double* mu_bar)
+{
+    (void)hipGetLastError();  // drop an error left by an earlier failed call
+    if (!h) return DMFTGPU_ERR_ARG;
+    int rc;
+    if (!method_ok(h, method, &rc)) return rc;
+    if (!sigma || !gloc_bar || !sigma_bar) return fail(h, DMFTGPU_ERR_ARG, "sigma, gloc_bar and sigma_bar are required");
+    if (!std::isfinite(mu) || !std::isfinite(n_bar)) return fail(h, DMFTGPU_ERR_ARG, "mu or n_bar is not finite");
+    return vjp_impl(h, method, sigma, mu, gloc_bar, n_bar, sigma_bar, mu_bar, nullptr);
+}
+
+// Loss L = sum_{n<N_freq} ||G_loc(iw_n) - G_tgt(iw_n)||_F^2 + (N - N_tgt)^2 and its
+// gradient: forward, cotangents on the device, adjoint.  On the one-matrix-per-lane
+// Schur path the forward also forms tr G^2 beyond N_freq (fused), so the adjoint
+// only covers the first N_freq frequencies.
+extern "C" int dmftgpu_gloc_loss_grad(dmftgpu_handle h, int method, const double* sigma, double mu,
+                                      const double* gloc_target, double n_target, double* gloc, double* num_matrix,
+                                      double* n_total, double* loss, double* sigma_bar, double* mu_bar)
+{
+    (void)hipGetLastError();  // drop an error left by an earlier failed call
+    if (!h) return DMFTGPU_ERR_ARG;
+    int rc;
+    if (!method_ok(h, method, &rc)) return rc;
+    if (!sigma || !gloc_target || !gloc || !num_matrix || !sigma_bar)
+        return fail(h, DMFTGPU_ERR_ARG, "sigma, gloc_target, gloc, num_matrix and sigma_bar are required");
+    if (!std::isfinite(mu) || !std::isfinite(n_target)) return fail(h, DMFTGPU_ERR_ARG, "mu or n_target is not finite");
+    const dmftgpu_lattice& L = h->lat;
+    const int nc = L.n_corr, m2 = nc * nc;
+    const int nw_tot = L.n_freq * L.high_freq;
+    hipError_t e;
+    cd* ftr2 = nullptr;
+    if (method != DMFTGPU_PLAIN_F64 && dg::schur_can_fuse(nc, h->schur_layout == 1, false) && nw_tot > L.n_freq) {
+        const int slots = dg::schur_slots(nc, dg::MODE_FWD, method == DMFTGPU_SCHUR_F32R, false);
+        const Plan P = make_plan(L.n_k, nw_tot, slots, h->target_wg);
+        ftr2 = (cd*)h->fused.get((size_t)P.n_chunks * nw_tot * sizeof(cd), &e);
+        if (!ftr2) return fail(h, DMFTGPU_ERR_HIP, "fused partials: %s", hipGetErrorString(e));
+    }
+    double N = 0.0;
+    rc = forward_impl(h, method, sigma, mu, gloc, nullptr, num_matrix, &N, ftr2);
+    if (rc) return rc;
+    // cotangents: gbar = 2 (G_loc - G_tgt) on the returned (cluster-block-diagonal)
+    // entries, 0 on the inter-cluster ones, which the output fixes at zero
+    cd* gbar = (cd*)h->cot.get((size_t)L.n_freq * m2 * sizeof(cd) + 256, &e);
+    if (!gbar) return fail(h, DMFTGPU_ERR_HIP, "cotangent buffer: %s", hipGetErrorString(e));
+    double* dloss = (double*)(gbar + (size_t)L.n_freq * m2);
+    HIPCHK(dg::launch_loss_cotangent((const cd*)gloc, (const cd*)gloc_target, L.n_freq, nc, nc / L.n_cluster, gbar, dloss,
+                                     h->stream));
+    const double nbar = 2.0 * (N - n_target);
+    rc = vjp_impl(h, method, sigma, mu, (const double*)gbar, nbar, sigma_bar, mu_bar, ftr2);
+    if (rc) return rc;
+    if (n_total) *n_total = N;
+    if (loss) {
+        double hl = 0.0;
+        HIPCHK(hipMemcpyAsync(&hl, dloss, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        *loss = hl + (N - n_target) * (N - n_target);
+    }
     return DMFTGPU_OK;
 }
 

@@ -50,6 +50,8 @@ struct dmftgpu_handle_s {
     dgi::Workspace ws;       // per-call scratch
     dgi::Workspace ft;       // Fourier-transform scratch
     dgi::Workspace stat;     // static weights wt/dt [nk][n] of the cached mu
+    dgi::Workspace fused;    // tr G^2 partials shared by a fused forward and its adjoint
+    dgi::Workspace cot;      // loss cotangents gbar [n_freq][nc][nc] + the loss sum
     // optional kernel timing (dmftgpu_enable_timing): events around the main
     // lattice kernel of each mode (FWD, DENS, VJP)
     bool timing = false;

@@ -41,13 +41,18 @@ __global__ void k_fin_fwd(const cd* __restrict__ tot, const cd* __restrict__ st,
     if (q == 0 && n_total) *n_total = (tot[m2].x / beta + sts[0]) * inv_nk;
 }
 
-// loss cotangents: gbar = 2 (G_loc - G_tgt), nbar = 2 (N - N_tgt)
-__global__ void k_loss_cot(const cd* __restrict__ gloc, const cd* __restrict__ gtgt, size_t len,
+// loss cotangents: gbar = 2 (G_loc - G_tgt) inside cluster blocks (npc orbitals
+// each), 0 across them (G_loc is returned block-diagonal, so those entries do not
+// depend on Sigma); nbar = 2 (N - N_tgt)
+__global__ void k_loss_cot(const cd* __restrict__ gloc, const cd* __restrict__ gtgt, size_t len, int nc, int npc,
                            const double* __restrict__ n_total, double n_tgt, cd* __restrict__ gbar,
                            double* __restrict__ nbar)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < len) gbar[i] = cscale(csub(gloc[i], gtgt[i]), 2.0);
+    if (i < len) {
+        const int e = (int)(i % ((size_t)nc * nc)), a = e % nc, b = e / nc;
+        gbar[i] = (a / npc == b / npc) ? cscale(csub(gloc[i], gtgt[i]), 2.0) : cd{0.0, 0.0};
+    }
     if (i == 0) *nbar = 2.0 * (*n_total - n_tgt);
 }
 
@@ -342,7 +347,8 @@ int issue_step(dmftgpu_batch_s* b)
     for (Sys& s : b->sys) {
         const size_t len = (size_t)s.nf * s.m2;
         hipLaunchKernelGGL(k_loss_cot, dim3(nblk(len)), dim3(256), 0, h->stream, (const cd*)s.d.gloc,
-                           (const cd*)s.d.gloc_target, len, s.d.n_total, s.d.n_target, s.gbar, s.nbar);
+                           (const cd*)s.d.gloc_target, len, s.nc, s.nc / s.d.lat.n_cluster, s.d.n_total, s.d.n_target, s.gbar,
+                           s.nbar);
         HIPCHK(hipGetLastError());
     }
     for (Group& g : b->groups) {  // adjoint: one persistent launch per n_c

@@ -12,6 +12,10 @@ enum LatMode { MODE_FWD = 0, MODE_DENS = 1, MODE_VJP = 2 };
 // whose G_cc enters NumMatrix alone (through G + G^H), accumulates the
 // Hermitian part (G + G^H)/2 (n_c^2 reals instead of n_c^2 complex).
 constexpr int MODE_FWD_HERM = 3;
+// The same Hermitian-part forward fused with the density terms (tr G^2): when an
+// adjoint follows at the same (Sigma, mu), the frequencies beyond N_freq are done
+// once, in the forward, instead of once there and once in the adjoint.
+constexpr int MODE_DENS_HERM = 4;
 
 // Data read through the constant address space: the compiler treats it as
 // invariant for the kernel's duration, so wave-uniform loads become scalar
@@ -82,7 +86,12 @@ hipError_t launch_density_matrix(const LatArgs& a, const cd* U, const double* wt
 // Schur complement (FP64 or FP32 + refinement); wg = workgroup width of the
 // one-matrix-per-lane kernels (64 or 128); cols selects the column-per-lane
 // kernels for n_c <= 6 (always used above 6)
-hipError_t launch_lattice_schur(const LatArgs& a, int mode, bool fp32, hipStream_t s, int wg = 64, bool cols = false);
+// fused (one-matrix-per-lane kernels, schur_can_fuse): a forward that also
+// writes tr G^2 (a.pTr2) beyond n_freq, and the adjoint at the same (Sigma, mu)
+// that then covers the first n_freq frequencies only (same k chunking).
+hipError_t launch_lattice_schur(const LatArgs& a, int mode, bool fp32, hipStream_t s, int wg = 64, bool cols = false,
+                                bool fused = false);
+bool schur_can_fuse(int nc, bool cols, bool real_axis);
 // Sigma expanded over all nw_tot frequencies into a.pSigAll
 hipError_t launch_sigma_all(const LatArgs& a, hipStream_t s);
 // Persistent launch over a work list of several lattices sharing n_c (and mode,
@@ -143,6 +152,10 @@ hipError_t launch_static_terms(const double* ev, int n, int nk, int nw_tot, doub
 hipError_t launch_static_reduce(const double* wt, const double* dt, const cd* U, int n, int nc, int nk,
                                 int k_per_chunk, int n_chunks, cd* pNum, double* pN, double* pdN, hipStream_t s);
 
+// loss cotangents gbar = 2 (G_loc - G_tgt) inside cluster blocks (npc orbitals per
+// cluster), 0 across them; *loss = sum |G_loc - G_tgt|^2 (device scalar)
+hipError_t launch_loss_cotangent(const cd* gloc, const cd* gtgt, int n_freq, int nc, int npc, cd* gbar, double* loss,
+                                 hipStream_t s);
 // GwHF (optional output): gloc_hf[nw][a][b] = (1/nkg) sum_k sum_l U_al conj(U_bl)/(iw+mu-e_kl)
 hipError_t launch_gloc_hf(const double* ev, const cd* U, int n, int nc, int nk, int n_freq, double beta, double mu,
                           double inv_nkg, int n_cluster, cd* out, hipStream_t s);

@@ -56,8 +56,11 @@ hipError_t launch_sigma_all(const LatArgs& a, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_lattice_schur(const LatArgs& a, int mode, bool fp32, hipStream_t s, int wg, bool cols)
+bool schur_can_fuse(int nc, bool cols, bool real_axis) { return nc >= 1 && nc <= 6 && !cols && !real_axis; }
+
+hipError_t launch_lattice_schur(const LatArgs& a, int mode, bool fp32, hipStream_t s, int wg, bool cols, bool fused)
 {
+    if (fused && !schur_can_fuse(a.nc, cols, a.real_axis)) return hipErrorInvalidValue;
     hipError_t e = launch_sigma_all(a, s);
     if (e != hipSuccess) return e;
     if (a.nc > 6 || cols || a.real_axis) {  // the per-lane kernels take Matsubara points only
@@ -79,7 +82,8 @@ hipError_t launch_lattice_schur(const LatArgs& a, int mode, bool fp32, hipStream
         hi.nw_begin = a.n_freq;
         e = dispatch_nc(lo, MODE_FWD, fp32, a.pSigAll, s, wg, nullptr);
         if (e != hipSuccess) return e;
-        return dispatch_nc(hi, MODE_FWD_HERM, fp32, a.pSigAll, s, wg, nullptr);
+        // fused: the adjoint that follows needs tr G^2 there too (a.pTr2)
+        return dispatch_nc(hi, fused ? MODE_DENS_HERM : MODE_FWD_HERM, fp32, a.pSigAll, s, wg, nullptr);
     }
     if (mode == MODE_VJP && a.nw_tot > a.n_freq) {
         // Frequencies beyond n_freq contribute only tr G and tr G^2 (dN/dmu):
@@ -88,7 +92,7 @@ hipError_t launch_lattice_schur(const LatArgs& a, int mode, bool fp32, hipStream
         lo.nw_end = a.n_freq;
         hi.nw_begin = a.n_freq;
         e = dispatch_nc(lo, MODE_VJP, fp32, a.pSigAll, s, wg, nullptr);
-        if (e != hipSuccess) return e;
+        if (e != hipSuccess || fused) return e;  // fused: the forward wrote tr G^2 beyond n_freq
         return dispatch_nc(hi, MODE_DENS, fp32, a.pSigAll, s, wg, nullptr);
     }
     return dispatch_nc(a, mode, fp32, a.pSigAll, s, wg, nullptr);

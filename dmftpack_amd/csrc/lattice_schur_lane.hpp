@@ -100,10 +100,10 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
     using BD = kconst_double;
     constexpr int NP = NC * (NC + 1) / 2;
     constexpr int M2 = NC * NC;
-    constexpr bool HERM = MODE == MODE_FWD_HERM;
-    constexpr bool FWDLIKE = MODE == MODE_FWD || HERM;
+    constexpr bool HERM = MODE == MODE_FWD_HERM || MODE == MODE_DENS_HERM;
+    constexpr bool FWDLIKE = MODE == MODE_FWD || MODE == MODE_FWD_HERM;
     // accumulators that do not fit next to the working matrices live in LDS
-    constexpr bool ACC_LDS = (MODE == MODE_VJP) || (FWDLIKE && F32R);
+    constexpr bool ACC_LDS = (MODE == MODE_VJP) || (MODE == MODE_DENS_HERM) || (FWDLIKE && F32R);
     const int nd = a.n - NC;
     const int bs = kblock_size<NC>(nd);
     cd* sAcc = lds;                   // [M2][WG] complex, lane-contiguous
@@ -278,24 +278,26 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
             }
         }
 
+        if constexpr (HERM) {  // Hermitian-part accumulation (before G is folded below)
+            int q = 0;
+#pragma unroll
+            for (int j = 0; j < NC; ++j)
+#pragma unroll
+                for (int i = 0; i <= j; ++i) {
+                    const double h0 = i == j ? G[i][i].x : G[i][j].x + G[j][i].x;
+                    const double h1 = G[i][j].y - G[j][i].y;
+                    if (ACC_LDS) {
+                        sAccH[q * WG + lane] += h0;
+                        if (i != j) sAccH[(q + 1) * WG + lane] += h1;
+                    } else {
+                        accH[q] += h0;
+                        if (i != j) accH[q + 1] += h1;
+                    }
+                    q += i == j ? 1 : 2;
+                }
+        }
         if (FWDLIKE) {
             if constexpr (HERM) {
-                int q = 0;
-#pragma unroll
-                for (int j = 0; j < NC; ++j)
-#pragma unroll
-                    for (int i = 0; i <= j; ++i) {
-                        const double h0 = i == j ? G[i][i].x : G[i][j].x + G[j][i].x;
-                        const double h1 = G[i][j].y - G[j][i].y;
-                        if (ACC_LDS) {
-                            sAccH[q * WG + lane] += h0;
-                            if (i != j) sAccH[(q + 1) * WG + lane] += h1;
-                        } else {
-                            accH[q] += h0;
-                            if (i != j) accH[q + 1] += h1;
-                        }
-                        q += i == j ? 1 : 2;
-                    }
             } else if (ACC_LDS) {
 #pragma unroll
                 for (int j = 0; j < NC; ++j)
@@ -504,6 +506,7 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
     if (HERM) {
         // (G + G^H)/2 in full layout: NumMatrix sums G_ij + conj(G_ji) over frequencies
         a.pTr[ch * a.nw_tot + nw] = trA;
+        if (MODE == MODE_DENS_HERM) a.pTr2[ch * a.nw_tot + nw] = tr2A;
         cd* out = a.pG + (ch * a.nw_tot + nw) * M2;
         int q = 0;
 #pragma unroll
@@ -578,6 +581,7 @@ template <int NC, int MODE, bool F32R, int WG>
 constexpr size_t schur_shm()
 {
     if constexpr (MODE == MODE_FWD_HERM) return F32R ? sizeof(double) * (size_t)NC * NC * WG : 0;
+    if constexpr (MODE == MODE_DENS_HERM) return sizeof(double) * (size_t)NC * NC * WG;
     constexpr bool ACC_LDS = (MODE == MODE_VJP) || (MODE == MODE_FWD && F32R);
     return ACC_LDS ? sizeof(cd) * (size_t)NC * NC * WG : 0;
 }
@@ -643,6 +647,9 @@ hipError_t launch_lane_nc(const LatArgs& a, int mode, bool fp32, const cd* sig_a
     if (mode == MODE_DENS)
         return fp32 ? launch_nc<NC, MODE_DENS, true, false>(a, sig_all, s, wg, m)
                     : launch_nc<NC, MODE_DENS, false, false>(a, sig_all, s, wg, m);
+    if (mode == MODE_DENS_HERM)
+        return fp32 ? launch_nc<NC, MODE_DENS_HERM, true, false>(a, sig_all, s, wg, m)
+                    : launch_nc<NC, MODE_DENS_HERM, false, false>(a, sig_all, s, wg, m);
     return fp32 ? launch_nc<NC, MODE_VJP, true, false>(a, sig_all, s, wg, m)
                 : launch_nc<NC, MODE_VJP, false, false>(a, sig_all, s, wg, m);
 }

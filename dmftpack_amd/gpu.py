@@ -253,6 +253,23 @@ class LatticeGF(Handle):
                                              C.byref(mb)))
         return out, mb.value
 
+    def loss_grad_dev(self, sigma_dev, mu, gtgt_dev, n_target, method=PLAIN_F64, gloc=None, num=None, out=None):
+        """dmftgpu_gloc_loss_grad: L = sum ||G_loc - G_tgt||^2 + (N - n_target)^2 and (dL/dSigma, dL/dmu).
+        Returns (gloc, num_matrix, N, L, sigma_bar, mu_bar); tensors stay on the device."""
+        gloc = torch.empty_like(gtgt_dev) if gloc is None else gloc
+        num = torch.empty((self.nc, self.nc), dtype=torch.complex128, device="cuda") if num is None else num
+        out = torch.empty_like(sigma_dev) if out is None else out
+        N, loss, mb = C.c_double(0), C.c_double(0), C.c_double(0)
+        self.check(self.lib.dmftgpu_gloc_loss_grad(self.h, method, _ptr(sigma_dev), mu, _ptr(gtgt_dev), n_target,
+                                                   _ptr(gloc), _ptr(num), C.byref(N), C.byref(loss), _ptr(out),
+                                                   C.byref(mb)))
+        return gloc, num, N.value, loss.value, out, mb.value
+
+    def loss_grad(self, sigma, mu, gtgt, n_target, method=PLAIN_F64):
+        gl, num, N, loss, sb, mb = self.loss_grad_dev(self.sigma_dev(sigma), mu, to_dev_cm(gtgt), n_target, method)
+        return dict(gloc=from_dev_cm(gl), num_matrix=from_dev_cm(num), n_total=N, loss=loss,
+                    sigma_bar=from_dev_cm(sb), mu_bar=mb)
+
     def vjp(self, sigma, mu, gbar, nbar, method=PLAIN_F64):
         """(sigma_bar, mu_bar) for cotangents (gbar of G_loc, nbar of N); dL/dRe + i dL/dIm."""
         s = self.sigma_dev(sigma)

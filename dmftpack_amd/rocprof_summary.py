@@ -75,16 +75,19 @@ def lattice_mode(name):
         return None
     args = [x.strip() for x in m.group(2).split(",")]
     idx = 2 if m.group(1) == "plain" else 1
-    return {"0": "FWD", "1": "DENS", "2": "VJP"}.get(args[idx] if len(args) > idx else "")
+    return {"0": "FWD", "1": "DENS", "2": "VJP", "3": "FWD_HERM", "4": "DENS_HERM"}.get(args[idx] if len(args) > idx else "")
 
 
-def add_adjoint_call(d, keys):
-    """The adjoint call of the one-matrix-per-lane Schur path is two launches:
-    the VJP kernel over the first N_freq frequencies and the density kernel over
-    the rest (the bench step makes no other density launch).  VJP_CALL sums them."""
-    if "VJP" in d and "DENS" in d:
-        d["VJP_CALL"] = {k: d["VJP"][k] + d["DENS"][k] for k in keys}
-        d["VJP_CALL"]["kernel"] = d["VJP"]["kernel"] + " + " + d["DENS"]["kernel"]
+def add_calls(d, keys):
+    """A forward or adjoint call of the one-matrix-per-lane Schur path can be more
+    than one launch: FWD below N_freq plus FWD_HERM or DENS_HERM (fused with the
+    adjoint's tr G^2) above; VJP below N_freq plus DENS above when not fused (the
+    bench step makes no other launch of those modes).  FWD_CALL and VJP_CALL sum them."""
+    for call, parts in (("FWD_CALL", ("FWD", "FWD_HERM", "DENS_HERM")), ("VJP_CALL", ("VJP", "DENS"))):
+        have = [m for m in parts if m in d]
+        if have:
+            d[call] = {k: sum(d[m][k] for m in have) for k in keys}
+            d[call]["kernel"] = " + ".join(d[m]["kernel"] for m in have)
 
 
 def pmc(argv):
@@ -125,7 +128,7 @@ def pmc(argv):
         mode = lattice_mode(name)
         if mode:
             result[mode] = dict(bytes_per_launch=mf + mw, fetch_bytes=mf, write_bytes=mw, kernel=name)
-    add_adjoint_call(result, ("bytes_per_launch", "fetch_bytes", "write_bytes"))
+    add_calls(result, ("bytes_per_launch", "fetch_bytes", "write_bytes"))
     if a.json:
         db = json.load(open(a.json)) if os.path.exists(a.json) else {}
         db[a.method] = {k: v["bytes_per_launch"] for k, v in result.items()}
@@ -175,7 +178,7 @@ def sq(argv):
         mode = lattice_mode(name)
         if mode:
             out[mode] = dict(fp64_flops=f64, fp32_flops=f32, launch_s=t, kernel=name)
-    add_adjoint_call(out, ("fp64_flops", "fp32_flops", "launch_s"))
+    add_calls(out, ("fp64_flops", "fp32_flops", "launch_s"))
     if a.json:
         db = json.load(open(a.json)) if os.path.exists(a.json) else {}
         db[a.method] = out

@@ -158,11 +158,25 @@ int dmftgpu_adjust_mu(dmftgpu_handle h, int method, const double* sigma, double 
 int dmftgpu_gloc_vjp(dmftgpu_handle h, int method, const double* sigma, double mu, const double* gloc_bar,
                      double n_bar, double* sigma_bar, double* mu_bar);
 
+/* Loss and gradient in one call, for fitting / Newton-Krylov updates:
+ *   L = sum_{n<N_freq} ||G_loc(iw_n) - G_tgt(iw_n)||_F^2 + (N - n_target)^2
+ * with G_loc as dmftgpu_greenftn_w returns it (cluster-block-diagonal).
+ *   gloc_target [n_freq][nc][nc]   in;  gloc [n_freq][nc][nc], num_matrix [nc][nc] out
+ *   n_total, loss, mu_bar          HOST outputs (each may be NULL)
+ *   sigma_bar [n_freq][nc][nc]     out, dL/dRe + i dL/dIm (as dmftgpu_gloc_vjp)
+ * Equal to dmftgpu_greenftn_w, the cotangents 2 (G_loc - G_tgt) and 2 (N - n_target),
+ * and dmftgpu_gloc_vjp; on the one-matrix-per-lane Schur path the forward also forms
+ * tr G^2 beyond N_freq, which the adjoint then skips.  No reference counterpart. */
+int dmftgpu_gloc_loss_grad(dmftgpu_handle h, int method, const double* sigma, double mu, const double* gloc_target,
+                           double n_target, double* gloc, double* num_matrix, double* n_total, double* loss,
+                           double* sigma_bar, double* mu_bar);
+
 /* ---- batched lattices ----------------------------------------------------------
  * No reference counterpart: DMFTpack handles one lattice per call.  Many
  * independent lattices (uneven n, n_corr <= 6, n_k) run as one replayable
  * step: per system GreenFtn_w (G_loc, NumMatrix, N) and, when a target is
- * given, the VJP of L = sum_w ||G_loc - gloc_target||^2 + (N - n_target)^2.
+ * given, the VJP of L = sum_w ||G_loc - gloc_target||^2 + (N - n_target)^2
+ * (G_loc cluster-block-diagonal as returned, as in dmftgpu_gloc_loss_grad).
  * All pointers in dmftgpu_system are DEVICE pointers; n_total and mu_bar
  * point to one double each.  The batch owns a stream (default priority) and
  * orders itself after / before the caller's handle stream. */

@@ -131,6 +131,18 @@ def test_errors_are_reported(gpu):
         lat.greenftn_w(sig, mu, method=17)
     with pytest.raises(gpu.capi.DmftGpuError):
         lat.density(sig, float("nan"))
+    gbar = np.ones_like(sig)
+    with pytest.raises(gpu.capi.DmftGpuError):
+        lat.vjp(sig, float("inf"), gbar, 0.0)
+    with pytest.raises(gpu.capi.DmftGpuError):
+        lat.vjp(sig, mu, gbar, float("nan"))
+    with pytest.raises(gpu.capi.DmftGpuError):  # eta must be > 0
+        lat.greenftn_real(sig, mu, -1.0, 0.1, 0.0)
+    with pytest.raises(gpu.capi.DmftGpuError):
+        lat.greenftn_real_k(sig, mu, -1.0, 0.1, -0.05)
+    # a failed call leaves the handle usable
+    out = lat.greenftn_w(sig, mu)
+    assert np.all(np.isfinite(out["gloc"]))
     lat.close()
 
 

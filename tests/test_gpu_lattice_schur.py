@@ -171,3 +171,44 @@ def test_update_moments_matches_prepare(gpu):
     assert rel(a["num_matrix"], ref["num_matrix"]) <= 1e-12
     lat.close()
     fresh.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,ncl", [("dp14", 1), ("c_9_4", 1), ("c_4_1", 1), ("c_8_6", 1), ("c_16_10", 1),
+                                      ("large32", 2)])
+@pytest.mark.parametrize("mname,method,tol", [("plain", 0, 1e-12), ("f64", 1, 1e-12), ("f32r", 2, 1e-8)])
+def test_loss_grad(gpu, name, ncl, mname, method, tol):
+    """dmftgpu_gloc_loss_grad (fused on the one-matrix-per-lane path) equals the
+    forward, the cotangents and the adjoint called one by one, and the oracle."""
+    if name == "large32":
+        w = L.workload("large32", seed=4, nk_side=2, n_freq=24)
+        Hk, sig, beta, mu = w["Hk"], w["sigma"], w["beta"], -0.2
+    else:
+        Hk, sig, beta, mu = case(name)
+    nc = sig.shape[1]
+    mom = O.asymto_moments(sig, beta)
+    print(f"loss_grad {name} {mname}", flush=True)
+    lat = gpu.LatticeGF(Hk, nc, sig.shape[0], beta, mom, n_cluster=ncl)
+    tgt = lat.greenftn_w(L.perturbed(sig, seed=11), mu, method=gpu.PLAIN_F64)["gloc"]
+    n0 = 0.4 * nc
+    r = lat.loss_grad(sig, mu, tgt, n0, method=method)
+    fwd = lat.greenftn_w(sig, mu, method=method)
+    assert np.array_equal(r["gloc"], fwd["gloc"]) and np.array_equal(r["num_matrix"], fwd["num_matrix"])
+    N, _ = lat.density(sig, mu, method=method)
+    assert abs(r["n_total"] - N) <= 1e-12 * max(1.0, abs(N))
+    loss_ref = np.sum(np.abs(fwd["gloc"] - tgt) ** 2) + (r["n_total"] - n0) ** 2
+    assert abs(r["loss"] - loss_ref) <= 1e-12 * loss_ref
+    # the cotangent of the inter-cluster entries is zero (G_loc is returned block-diagonal)
+    npc = nc // ncl
+    gbar = 2.0 * (fwd["gloc"] - tgt)
+    for a in range(nc):
+        for b in range(nc):
+            if a // npc != b // npc:
+                gbar[:, a, b] = 0.0
+    sb, mb = lat.vjp(sig, mu, gbar, 2.0 * (r["n_total"] - n0), method=method)
+    assert rel(r["sigma_bar"], sb) <= 1e-12
+    assert abs(r["mu_bar"] - mb) <= 1e-12 * max(1.0, abs(mb))
+    sb_ref, mb_ref = O.gloc_vjp(Hk, sig, mom, mu, beta, gbar, 2.0 * (r["n_total"] - n0))
+    assert rel(r["sigma_bar"], sb_ref) <= tol
+    assert abs(r["mu_bar"] - mb_ref) <= tol * max(1.0, abs(mb_ref))
+    lat.close()
