@@ -98,7 +98,11 @@ struct Group {
     int4* ditems[2] = {nullptr, nullptr};
     int n_items[2] = {0, 0};
     // frequencies n >= N_freq: the forward accumulates the Hermitian part of G_cc
-    // (MODE_FWD_HERM), the adjoint only needs tr G, tr G^2 (density kernel)
+    // (MODE_FWD_HERM), the adjoint only needs tr G, tr G^2 (density kernel); with
+    // an adjoint in the step both happen in the forward (MODE_DENS_HERM, writing
+    // the adjoint's tr G^2 partials, same k chunking) and the adjoint has no
+    // upper work list
+    int mode_hi[2] = {MODE_FWD_HERM, MODE_DENS};
     LatArgs* dargs_hi[2] = {nullptr, nullptr};
     int4* ditems_hi[2] = {nullptr, nullptr};
     int n_items_hi[2] = {0, 0};
@@ -202,10 +206,11 @@ int build_group(dmftgpu_batch_s* b, Group& g, int mode, int wg, double target)
         }
         LatArgs a = make_args(b, s, mode, chunks, kpc);
         const bool split = s.nw_tot > s.nf;
-        if (split) {  // [0, nf) forward / adjoint, [nf, nw_tot) Hermitian forward / density
+        if (split) {  // [0, nf) forward / adjoint, [nf, nw_tot) see Group::mode_hi
             LatArgs hi = a;
             a.nw_end = s.nf;
             hi.nw_begin = s.nf;
+            if (mode == MODE_FWD && b->vjp) hi.pTr2 = s.pTr2;  // fused: the adjoint's partials
             args_hi.push_back(hi);
         } else {
             args_hi.push_back(a);  // unused placeholder keeps the system index
@@ -214,12 +219,13 @@ int build_group(dmftgpu_batch_s* b, Group& g, int mode, int wg, double target)
         const int nwt = ((split ? s.nf : s.nw_tot) + wg - 1) / wg;
         for (int c = 0; c < chunks; ++c)
             for (int t = 0; t < nwt; ++t) items.push_back(int4{(int)q, t, c, 0});
-        if (split) {
+        if (split && !(mode == MODE_VJP && b->vjp)) {  // fused adjoint: the forward did these
             const int nwh = (s.nw_tot - s.nf + wg - 1) / wg;
             for (int c = 0; c < chunks; ++c)
                 for (int t = 0; t < nwh; ++t) items_hi.push_back(int4{(int)q, t, c, 0});
         }
     }
+    g.mode_hi[0] = b->vjp ? MODE_DENS_HERM : MODE_FWD_HERM;
     // interleave systems so neighbouring workgroups mix large and small items
     std::stable_sort(items.begin(), items.end(), [](const int4& x, const int4& y) { return x.z < y.z; });
     std::stable_sort(items_hi.begin(), items_hi.end(), [](const int4& x, const int4& y) { return x.z < y.z; });
@@ -278,8 +284,7 @@ int launch_group(dmftgpu_batch_s* b, Group& g, int mode)
         mh.items = g.ditems_hi[slot(mode)];
         mh.n_items = g.n_items_hi[slot(mode)];
         mh.grid = std::min(mh.n_items, 256 * 16);
-        HIPCHK(dg::launch_lattice_schur_multi(proto, mode == MODE_VJP ? MODE_DENS : MODE_FWD_HERM, fp32, h->stream,
-                                              g.wg[slot(mode)], mh));
+        HIPCHK(dg::launch_lattice_schur_multi(proto, g.mode_hi[slot(mode)], fp32, h->stream, g.wg[slot(mode)], mh));
     }
     return DMFTGPU_OK;
 }
@@ -411,6 +416,69 @@ int tune_group(dmftgpu_batch_s* b, Group& g, int mode)
         }
     g.ms[slot(mode)] = best;
     return build_group(b, g, mode, bw, bt);
+}
+
+// Forward and adjoint of a fused step share the k chunking (the forward writes the
+// adjoint's tr G^2 partials beyond N_freq), so one target is chosen for both: per
+// target the best workgroup width of each, the target with the least sum.
+int tune_group_pair(dmftgpu_batch_s* b, Group& g)
+{
+    dmftgpu_handle h = b->h;
+    const int wgs[2] = {64, 128};
+    const double targets[4] = {6.25e4, 2.5e5, 1.0e6, 4.0e6};
+    struct Events {  // destroyed on every return path
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        ~Events()
+        {
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+        }
+    } ev;
+    HIPCHK(hipEventCreate(&ev.e0));
+    HIPCHK(hipEventCreate(&ev.e1));
+    double best_sum = 1e300, best_ms[2] = {0, 0};
+    int best_wg[2] = {64, 64};
+    double best_t = targets[2];
+    for (double t : targets) {
+        double ms_mode[2];
+        int wg_mode[2];
+        for (int mode : {MODE_FWD, MODE_VJP}) {
+            double bm = 1e300;
+            int bw = 64;
+            for (int wg : wgs) {
+                int rc = build_group(b, g, mode, wg, t);
+                if (rc) return rc;
+                rc = launch_group(b, g, mode);  // warm-up
+                if (rc) return rc;
+                HIPCHK(hipEventRecord(ev.e0, h->stream));
+                rc = launch_group(b, g, mode);
+                if (rc) return rc;
+                HIPCHK(hipEventRecord(ev.e1, h->stream));
+                HIPCHK(hipEventSynchronize(ev.e1));
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, ev.e0, ev.e1));
+                if (ms < bm) {
+                    bm = ms;
+                    bw = wg;
+                }
+            }
+            ms_mode[slot(mode)] = bm;
+            wg_mode[slot(mode)] = bw;
+        }
+        if (ms_mode[0] + ms_mode[1] < best_sum) {
+            best_sum = ms_mode[0] + ms_mode[1];
+            best_t = t;
+            for (int q = 0; q < 2; ++q) {
+                best_ms[q] = ms_mode[q];
+                best_wg[q] = wg_mode[q];
+            }
+        }
+    }
+    g.ms[0] = best_ms[0];
+    g.ms[1] = best_ms[1];
+    int rc = build_group(b, g, MODE_FWD, best_wg[0], best_t);
+    if (rc) return rc;
+    return build_group(b, g, MODE_VJP, best_wg[1], best_t);
 }
 
 }  // namespace
@@ -567,17 +635,19 @@ static int batch_create_impl(dmftgpu_batch_s* b, int count, const dmftgpu_system
         }
     }
     if (flags & DMFTGPU_BATCH_TUNE) {
-        // the adjoint needs the forward outputs (loss cotangents) in place while it is timed
-        for (Group& g : b->groups) {
-            if ((rc = tune_group(b, g, MODE_FWD))) break;
-        }
-        if (!rc && b->vjp) {
+        if (!b->vjp) {
             for (Group& g : b->groups)
-                if ((rc = build_group(b, g, MODE_VJP, 64, 1.0e6))) break;
+                if ((rc = tune_group(b, g, MODE_FWD))) break;
+        } else {
+            // the adjoint needs the forward outputs (loss cotangents) in place while it is timed
+            for (Group& g : b->groups)
+                if (!rc) rc = build_group(b, g, MODE_FWD, 64, 1.0e6);
+            for (Group& g : b->groups)
+                if (!rc) rc = build_group(b, g, MODE_VJP, 64, 1.0e6);
             if (!rc) rc = issue_step(b);
             if (!rc && hipStreamSynchronize(h->stream) != hipSuccess) rc = fail(h, DMFTGPU_ERR_HIP, "tuning step");
             for (Group& g : b->groups)
-                if (!rc) rc = tune_group(b, g, MODE_VJP);
+                if (!rc) rc = tune_group_pair(b, g);
         }
         if (rc) return rc;
     }
