@@ -349,7 +349,7 @@ struct PerK {
 // (dmftgpu_gloc_loss_grad), which fills the rest and reduces them.
 int run_lattice(dmftgpu_handle h, int method, int mode, const double* sigma, double mu, const double* gbar,
                 double nbar, char* ws, size_t& off, Sums& S, bool want_trace = false, const RealAxis* ra = nullptr,
-                const PerK* pk = nullptr, cd* fused_tr2 = nullptr)
+                const PerK* pk = nullptr, cd* fused_tr2 = nullptr, int fused_chunks = 0)
 {
     const dmftgpu_lattice& L = h->lat;
     const int n = L.n, nc = L.n_corr, nk = L.n_k, m2 = nc * nc;
@@ -359,6 +359,9 @@ int run_lattice(dmftgpu_handle h, int method, int mode, const double* sigma, dou
     if (method == DMFTGPU_PLAIN_F64) slots = dg::plain_slots(n, mode);
     else slots = dg::schur_slots(nc, mode, method == DMFTGPU_SCHUR_F32R, h->schur_layout == 1 || ra);
     Plan P = pk ? Plan{nk, 1} : make_plan(nk, nw_tot, slots, h->target_wg);
+    if (fused_tr2 && P.n_chunks != fused_chunks)  // both halves of a fused pair must chunk k alike
+        return fail(h, DMFTGPU_ERR_STATE, "fused partials laid out for %d k chunks, this call uses %d", fused_chunks,
+                    P.n_chunks);
     dg::LatArgs a{};
     a.Hk = h->Hk;
     a.sigma = (const cd*)sigma;
@@ -456,7 +459,7 @@ size_t lattice_ws_bytes(dmftgpu_handle h, int method, int mode, const RealAxis* 
 namespace {
 
 int forward_impl(dmftgpu_handle h, int method, const double* sigma, double mu, double* gloc, double* gloc_hf,
-                 double* num_matrix, double* n_total, cd* fused_tr2)
+                 double* num_matrix, double* n_total, cd* fused_tr2, int fused_chunks)
 {
     const dmftgpu_lattice& L = h->lat;
     const int nc = L.n_corr, m2 = nc * nc;
@@ -468,7 +471,7 @@ int forward_impl(dmftgpu_handle h, int method, const double* sigma, double mu, d
     size_t off = 0;
     Sums S;
     int rc = run_lattice(h, method, dg::MODE_FWD, sigma, mu, nullptr, 0.0, ws, off, S, n_total != nullptr, nullptr,
-                         nullptr, fused_tr2);
+                         nullptr, fused_tr2, fused_chunks);
     if (rc) return rc;
     cd* tot = (cd*)(ws + off);
     off += ((size_t)(m2 + 3) * sizeof(cd) + 255) / 256 * 256;
@@ -520,7 +523,7 @@ extern "C" int dmftgpu_greenftn_w(dmftgpu_handle h, int method, const double* si
         return fail(h, DMFTGPU_ERR_UNSUPPORTED, "Schur method needs n_corr <= 16 (n=%d, n_corr=%d)", h->lat.n, h->lat.n_corr);
     if (!sigma || !gloc || !num_matrix) return fail(h, DMFTGPU_ERR_ARG, "sigma, gloc and num_matrix are required");
     if (!std::isfinite(mu)) return fail(h, DMFTGPU_ERR_ARG, "mu is not finite");
-    return forward_impl(h, method, sigma, mu, gloc, gloc_hf, num_matrix, n_total, nullptr);
+    return forward_impl(h, method, sigma, mu, gloc, gloc_hf, num_matrix, n_total, nullptr, 0);
 }
 
 // retarded_GreenFtn2 (GreenFtn.cpp:425-466) summed over k, as qs_spectra
@@ -685,7 +688,7 @@ extern "C" int dmftgpu_adjust_mu(dmftgpu_handle h, int method, const double* sig
 namespace {
 
 int vjp_impl(dmftgpu_handle h, int method, const double* sigma, double mu, const double* gloc_bar, double n_bar,
-             double* sigma_bar, double* mu_bar, cd* fused_tr2)
+             double* sigma_bar, double* mu_bar, cd* fused_tr2, int fused_chunks)
 {
     const dmftgpu_lattice& L = h->lat;
     const int nc = L.n_corr, m2 = nc * nc;
@@ -697,7 +700,7 @@ int vjp_impl(dmftgpu_handle h, int method, const double* sigma, double mu, const
     size_t off = 0;
     Sums S;
     int rc = run_lattice(h, method, dg::MODE_VJP, sigma, mu, gloc_bar, n_bar, ws, off, S, false, nullptr, nullptr,
-                         fused_tr2);
+                         fused_tr2, fused_chunks);
     if (rc) return rc;
     cd* tot = (cd*)(ws + off);
     off += ((size_t)(m2 + 3) * sizeof(cd) + 255) / 256 * 256;
@@ -709,7 +712,8 @@ int vjp_impl(dmftgpu_handle h, int method, const double* sigma, double mu, const
     if (rc) return rc;
     const double inv_nk = 1.0 / (double)L.n_k_global;
     HIPCHK(dg::launch_scale(S.Sb, (size_t)L.n_freq * m2, inv_nk, (cd*)sigma_bar, h->stream));
-    HIPCHK(dg::launch_freq_totals(nullptr, m2, nc, S.tr, S.tr2, nw_tot, S.mu, L.n_freq, tot, h->stream));
+    // (tr G itself is not needed here; after a fused forward only its first N_freq entries are formed)
+    HIPCHK(dg::launch_freq_totals(nullptr, m2, nc, nullptr, S.tr2, nw_tot, S.mu, L.n_freq, tot, h->stream));
     std::vector<cd> htot(m2 + 3);
     double hsts[2];
     HIPCHK(hipMemcpyAsync(htot.data(), tot, sizeof(cd) * (m2 + 3), hipMemcpyDeviceToHost, h->stream));
@@ -744,7 +748,7 @@ extern "C" int dmftgpu_gloc_vjp(dmftgpu_handle h, int method, const double* sigm
     if (!method_ok(h, method, &rc)) return rc;
     if (!sigma || !gloc_bar || !sigma_bar) return fail(h, DMFTGPU_ERR_ARG, "sigma, gloc_bar and sigma_bar are required");
     if (!std::isfinite(mu) || !std::isfinite(n_bar)) return fail(h, DMFTGPU_ERR_ARG, "mu or n_bar is not finite");
-    return vjp_impl(h, method, sigma, mu, gloc_bar, n_bar, sigma_bar, mu_bar, nullptr);
+    return vjp_impl(h, method, sigma, mu, gloc_bar, n_bar, sigma_bar, mu_bar, nullptr, 0);
 }
 
 // Loss L = sum_{n<N_freq} ||G_loc(iw_n) - G_tgt(iw_n)||_F^2 + (N - N_tgt)^2 and its
@@ -767,14 +771,16 @@ extern "C" int dmftgpu_gloc_loss_grad(dmftgpu_handle h, int method, const double
     const int nw_tot = L.n_freq * L.high_freq;
     hipError_t e;
     cd* ftr2 = nullptr;
+    int fchunks = 0;
     if (method != DMFTGPU_PLAIN_F64 && dg::schur_can_fuse(nc, h->schur_layout == 1, false) && nw_tot > L.n_freq) {
         const int slots = dg::schur_slots(nc, dg::MODE_FWD, method == DMFTGPU_SCHUR_F32R, false);
         const Plan P = make_plan(L.n_k, nw_tot, slots, h->target_wg);
+        fchunks = P.n_chunks;
         ftr2 = (cd*)h->fused.get((size_t)P.n_chunks * nw_tot * sizeof(cd), &e);
         if (!ftr2) return fail(h, DMFTGPU_ERR_HIP, "fused partials: %s", hipGetErrorString(e));
     }
     double N = 0.0;
-    rc = forward_impl(h, method, sigma, mu, gloc, nullptr, num_matrix, &N, ftr2);
+    rc = forward_impl(h, method, sigma, mu, gloc, nullptr, num_matrix, &N, ftr2, fchunks);
     if (rc) return rc;
     // cotangents: gbar = 2 (G_loc - G_tgt) on the returned (cluster-block-diagonal)
     // entries, 0 on the inter-cluster ones, which the output fixes at zero
@@ -784,7 +790,7 @@ extern "C" int dmftgpu_gloc_loss_grad(dmftgpu_handle h, int method, const double
     HIPCHK(dg::launch_loss_cotangent((const cd*)gloc, (const cd*)gloc_target, L.n_freq, nc, nc / L.n_cluster, gbar, dloss,
                                      h->stream));
     const double nbar = 2.0 * (N - n_target);
-    rc = vjp_impl(h, method, sigma, mu, (const double*)gbar, nbar, sigma_bar, mu_bar, ftr2);
+    rc = vjp_impl(h, method, sigma, mu, (const double*)gbar, nbar, sigma_bar, mu_bar, ftr2, fchunks);
     if (rc) return rc;
     if (n_total) *n_total = N;
     if (loss) {

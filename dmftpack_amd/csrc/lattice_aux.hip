@@ -246,7 +246,8 @@ __global__ __launch_bounds__(256) void k_freq_totals(const cd* __restrict__ G, i
             }
         }
     } else if (q == m2) {
-        for (int nw = threadIdx.x; nw < nw_tot; nw += 256) ax += 2.0 * tr[nw].x;
+        if (tr)
+            for (int nw = threadIdx.x; nw < nw_tot; nw += 256) ax += 2.0 * tr[nw].x;
     } else if (q == m2 + 1) {
         if (tr2)
             for (int nw = threadIdx.x; nw < nw_tot; nw += 256) ax += 2.0 * tr2[nw].x;
