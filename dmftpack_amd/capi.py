@@ -14,7 +14,7 @@ HEADER = os.path.join(HERE, "..", "include", "dmftgpu.h")
 
 OK, ERR_ARG, ERR_STATE, ERR_HIP, ERR_NUMERIC, ERR_UNSUPPORTED = 0, 1, 2, 3, 4, 5
 PLAIN_F64, SCHUR_F64, SCHUR_F32R = 0, 1, 2
-OPT_REFINE_STEPS, OPT_TARGET_WORKGROUPS, OPT_SCHUR_WG, OPT_SCHUR_LAYOUT = 1, 2, 3, 4
+OPT_REFINE_STEPS, OPT_TARGET_WORKGROUPS, OPT_SCHUR_WG, OPT_SCHUR_LAYOUT, OPT_UPFOLD_CHUNK = 1, 2, 3, 4, 5
 MODE_FWD, MODE_DENS, MODE_VJP = 0, 1, 2
 
 
@@ -76,6 +76,7 @@ _SIGS = {
     "dmftgpu_greenftn_real_k": ([P, I, I, D, D, D, P, D, P, P], I),
     "dmftgpu_hyb_delta": ([P, I, I, D, D, P, P, P, I, PI, P], I),
     "dmftgpu_hk_from_hr": ([P, I, I, PI, PD, I, P, P], I),
+    "dmftgpu_upfold_density": ([P, I, I, I, I, D, D, P, P, P, PI, P, P, P, I, PI, P, P, P], I),
 }
 
 

@@ -167,6 +167,10 @@ extern "C" int dmftgpu_set_option(dmftgpu_handle h, int option, int value)
             if (value != 0 && value != 1) return fail(h, DMFTGPU_ERR_ARG, "Schur layout %d not 0/1", value);
             h->schur_layout = value;
             return DMFTGPU_OK;
+        case DMFTGPU_OPT_UPFOLD_CHUNK:
+            if (value < 0) return fail(h, DMFTGPU_ERR_ARG, "upfolding chunk %d < 0", value);
+            h->upfold_chunk = value;
+            return DMFTGPU_OK;
         default: return fail(h, DMFTGPU_ERR_ARG, "unknown option %d", option);
     }
 }

@@ -148,6 +148,29 @@ class Handle:
         out._dmft_colmajor = True
         return out
 
+    def upfold_density(self, dm_model, corr_base, ks_energy, val_to_ks, transform, ks_ortho, overlap, beta, mu,
+                       r_idx, kphase, n_k_global=None, want_dm_k=True):
+        """upfolding_density (upfolding.cpp:38-139): returns (D [nk][n_orb][n_orb] or None,
+        dmr [3][n_r]) with dmr = dual-dual, dual-direct, direct-direct DM(R); overlap None means S = I."""
+        X = to_dev_cm(dm_model)
+        nk, n = X.shape[0], X.shape[1]
+        eps = torch.from_numpy(np.ascontiguousarray(np.asarray(ks_energy, dtype=np.float64))).to("cuda")
+        n_orb = eps.shape[1]
+        v = np.ascontiguousarray(np.asarray(val_to_ks, dtype=np.int32))
+        ri = np.ascontiguousarray(np.asarray(r_idx, dtype=np.int32).reshape(-1, 5))
+        kp = torch.from_numpy(np.ascontiguousarray(np.asarray(kphase, dtype=np.float64))).to("cuda")
+        c, t, ko = to_dev_cm(corr_base), to_dev_cm(transform), to_dev_cm(ks_ortho)
+        s = None if overlap is None else to_dev_cm(overlap)
+        dm = torch.empty((nk, n_orb, n_orb), dtype=torch.complex128, device="cuda") if want_dm_k else None
+        n_r = ri.shape[0]
+        dmr = torch.empty((3, max(n_r, 1)), dtype=torch.complex128, device="cuda")
+        self.check(self.lib.dmftgpu_upfold_density(
+            self.h, n, n_orb, nk, n_k_global or nk, beta, mu, _ptr(X), _ptr(c), _ptr(eps),
+            v.ctypes.data_as(C.POINTER(C.c_int)), _ptr(t), _ptr(ko), None if s is None else _ptr(s), n_r,
+            ri.ctypes.data_as(C.POINTER(C.c_int)), _ptr(kp), None if dm is None else _ptr(dm),
+            _ptr(dmr) if n_r else None))
+        return (None if dm is None else from_dev_cm(dm)), dmr[:, :n_r].cpu().numpy()
+
 
 class LatticeGF(Handle):
     """One lattice (H(k) + Sigma tail moments) bound to a library handle."""

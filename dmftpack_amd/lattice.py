@@ -274,3 +274,61 @@ def batch_systems(count=64, seed=1, n_freq=1024, beta=40.0, norbs=(3, 5, 7, 10, 
         systems.append(dict(Hk=Hk, sigma=sig, exact_moments=mom, beta=beta, n_cluster=1, nk_side=nx, n=n, n_corr=nc,
                             mu=0.0))
     return systems
+
+
+def _cnormal(seed, stream, shape):
+    g = normal(seed, stream, 2 * int(np.prod(shape))).reshape((2,) + tuple(shape))
+    return (g[0] + 1j * g[1]) / np.sqrt(2.0)
+
+
+def _unitary(seed, stream, count, d):
+    q, r = np.linalg.qr(_cnormal(seed, stream, (count, d, d)))
+    ph = np.diagonal(r, axis1=1, axis2=2)
+    return q * (ph / np.abs(ph))[:, None, :]
+
+
+def upfold_case(n, n_orb, nk_side, seed, overlap=True, identity=False):
+    """Synthetic inputs of upfolding_density (upfolding.cpp:38-86):
+    dm_model [nk][n][n] Hermitian, corr_base [nk][n][n] unitary (DF_CorrBase),
+    ks_energy [nk][n_orb], val_to_ks [nk][n] distinct band indices (FromValToKS),
+    transform [nk][n_orb][n_orb] (transformMatrix_k, well conditioned),
+    ks_ortho [nk][n_orb][n_orb] unitary (KS_eigenVectors_orthoBasis),
+    overlap [nk][n_orb][n_orb] Hermitian positive definite or None,
+    r_idx [n_r][5] = every (R, i0, m0) with R in {-1,0,1}^3, kphase [nk][3].
+    identity=True sets every basis change to the identity (n = n_orb)."""
+    kp = kmesh_phases(nk_side)
+    nk = kp.shape[0]
+    A = _cnormal(seed, 501, (nk, n, n))
+    X = 0.5 * (A + A.conj().transpose(0, 2, 1)) / np.sqrt(n)
+    eps = np.sort(-4.0 + 8.0 * uniform(seed, 502, nk * n_orb).reshape(nk, n_orb), axis=1)
+    eps[:, 0] = -30.0  # beta (eps - mu) beyond -100 and +100: the clamped Fermi factors
+    eps[:, -1] = 30.0
+    if identity:
+        assert n == n_orb
+        eye_n = np.broadcast_to(np.eye(n, dtype=complex), (nk, n, n)).copy()
+        eye_o = eye_n.copy()
+        return dict(dm_model=X, corr_base=eye_n, ks_energy=eps, val_to_ks=np.tile(np.arange(n), (nk, 1)),
+                    transform=eye_o, ks_ortho=eye_o.copy(), overlap=None if not overlap else eye_o.copy(),
+                    r_idx=_all_r_idx(n_orb), kphase=kp)
+    C = _unitary(seed, 503, nk, n)
+    order = np.argsort(uniform(seed, 504, nk * n_orb).reshape(nk, n_orb), axis=1)
+    v = np.ascontiguousarray(order[:, :n]).astype(np.int32)
+    T = np.eye(n_orb)[None] + 0.3 * _cnormal(seed, 505, (nk, n_orb, n_orb)) / np.sqrt(n_orb)
+    Ko = _unitary(seed, 506, nk, n_orb)
+    S = None
+    if overlap:
+        B = _cnormal(seed, 507, (nk, n_orb, n_orb))
+        S = np.eye(n_orb)[None] + 0.3 * (B @ B.conj().transpose(0, 2, 1)) / n_orb
+    return dict(dm_model=X, corr_base=C, ks_energy=eps, val_to_ks=v, transform=T, ks_ortho=Ko, overlap=S,
+                r_idx=_all_r_idx(n_orb), kphase=kp)
+
+
+def _all_r_idx(n_orb):
+    rows = []
+    for r1 in (-1, 0, 1):
+        for r2 in (-1, 0, 1):
+            for r3 in (-1, 0, 1):
+                for i in range(n_orb):
+                    for j in range(n_orb):
+                        rows.append((r1, r2, r3, i, j))
+    return np.array(rows, dtype=np.int32)

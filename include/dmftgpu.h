@@ -75,6 +75,8 @@ const char* dmftgpu_version(void);
 #define DMFTGPU_OPT_SCHUR_WG 3          /* frequencies per workgroup of the Schur kernels (64 or 128, default 128) */
 #define DMFTGPU_OPT_SCHUR_LAYOUT 4      /* n_c <= 6: 0 = one (k,iw) matrix per lane (default), 1 = one column
                                            per lane (always used for n_c > 6) */
+#define DMFTGPU_OPT_UPFOLD_CHUNK 5      /* k points per chunk of dmftgpu_upfold_density (0 = as many as 512 MiB of
+                                           scratch holds, the default) */
 int dmftgpu_set_option(dmftgpu_handle h, int option, int value);
 
 /* Instrumentation (no reference counterpart): with timing enabled, HIP events
@@ -260,6 +262,37 @@ int dmftgpu_hyb_delta(dmftgpu_handle h, int nc, int n_freq, double beta, double 
  *   Hk      [n_k][n][n] output; hops of one element are summed in list order. */
 int dmftgpu_hk_from_hr(dmftgpu_handle h, int n, int n_hop, const int* hop_idx, const double* hop, int n_k,
                        const double* kphase, double* Hk);
+
+/* upfolding_density (upfolding.cpp:38-139) with upfolding_k (:10-35), the
+ * downfolding == 1 branch (:45-51): the per-k density matrix of
+ * dmftgpu_density_matrix taken to the full DFT basis and summed to R,
+ *   E_k  = (T_k^H)^{-1} K_k                       (:46)
+ *   K'_k = diag f(eps_kb - mu), f = 0 / 1 where beta (eps - mu) > 100 / < -100,
+ *          with K'_k(v_ki, v_kj) = (C_k^H X_k C_k)(i, j), i, j < n   (:13-32)
+ *   D_k  = E_k K'_k E_k^H                         (:34)
+ *   dmr[0][r] = sum_k (S_k^{-1} D_k S_k^{-1})(i0, m0) exp(+i kphase_k . R) / n_k_global
+ *   dmr[1][r] = same with S_k^{-1} D_k,  dmr[2][r] = same with D_k   (:49-50, :67-86)
+ * (the DM_DFT_DMFT.HWR, dual_DM_direct.HWR and direct_DM_direct.HWR values).
+ *   n, n_orb   NBAND (uniform over k) and NumOrbit, 1 <= n <= n_orb <= 512
+ *   dm_model   [n_k][n][n]          densityMatDFT in the model basis
+ *   corr_base  [n_k][n][n]          DF_CorrBase
+ *   ks_energy  [n_k][n_orb]         KS_eigenEnergy (double)
+ *   val_to_ks  [n_k][n]   HOST int  FromValToKS, distinct bands per k
+ *   transform  [n_k][n_orb][n_orb]  transformMatrix_k
+ *   ks_ortho   [n_k][n_orb][n_orb]  KS_eigenVectors_orthoBasis
+ *   overlap    [n_k][n_orb][n_orb]  S_overlap, or NULL for S = I
+ *   r_idx      [n_r][5]   HOST int  H_Rindex rows: R1 R2 R3 i0 m0 (0-based)
+ *   kphase     [n_k][3]             k_x a_x, k_y a_y, k_z a_z
+ *   dm_k       [n_k][n_orb][n_orb]  output D_k (densityMatDFT[k] after upfolding_k), or NULL
+ *   dmr        [3][n_r] complex     output (n_r = 0: dm_k only)
+ * Inverses are Gauss-Jordan with partial pivoting; a singular T_k or S_k
+ * returns DMFTGPU_ERR_NUMERIC.  With k split over ranks the R sums are the
+ * rank's share (the operands of the reference's MPI_Allreduce, :80-82). */
+int dmftgpu_upfold_density(dmftgpu_handle h, int n, int n_orb, int n_k, int n_k_global, double beta, double mu,
+                           const double* dm_model, const double* corr_base, const double* ks_energy,
+                           const int* val_to_ks, const double* transform, const double* ks_ortho,
+                           const double* overlap, int n_r, const int* r_idx, const double* kphase, double* dm_k,
+                           double* dmr);
 
 #ifdef __cplusplus
 }

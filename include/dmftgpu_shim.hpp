@@ -433,4 +433,65 @@ void Construct_Hk(dmftgpu_handle h, const HoppingList& hr, const std::vector<dou
     }
 }
 
+// upfolding_density (upfolding.cpp:38-139).  densityMatDFT[k] is taken to the
+// DFT basis in place (upfolding_k, :10-35), as the reference leaves it; the
+// three R sums come back in H_Rindex row order (DM_DFT_DMFT.HWR,
+// dual_DM_direct.HWR, direct_DM_direct.HWR values, :89-137), after
+// Globals::allreduce (:80-82).  Printing them needs the caller's
+// orbital-to-atom tables and stays with the caller.  The reference's globals
+// arrive as arguments: DF_CorrBase, KS_eigenEnergy (anything indexable as
+// [k][band]: Eigen::VectorXd*), FromValToKS, and kphase [knum][3] =
+// kmesh[k][axis] * a_axis.  An empty S_overlap means S = I.
+template <class Matrix, class IndexMatrix, class Energies>
+void upfolding_density(dmftgpu_handle h, const Globals& g, std::vector<Matrix>& densityMatDFT,
+                       std::vector<Matrix>& KS_eigenVectors_orthoBasis, const IndexMatrix& H_Rindex, double mu,
+                       std::vector<Matrix>& S_overlap, std::vector<Matrix>& transformMatrix_k,
+                       const std::vector<Matrix>& DF_CorrBase, const Energies& KS_eigenEnergy,
+                       const std::vector<std::vector<int>>& FromValToKS, const std::vector<double>& kphase,
+                       std::vector<cplx> (&dmr)[3])
+{
+    const int nk = g.knum;
+    const int n = (int)densityMatDFT[0].rows();
+    const int no = (int)KS_eigenVectors_orthoBasis[0].rows();
+    const int nr = (int)H_Rindex.rows();
+    DevBuf X((size_t)nk * n * n), Cb((size_t)nk * n * n), T((size_t)nk * no * no), Ko((size_t)nk * no * no),
+        S(S_overlap.empty() ? 1 : (size_t)nk * no * no), D((size_t)nk * no * no), R(3 * (size_t)(nr > 0 ? nr : 1));
+    X.upload(pack(densityMatDFT, nk, n).data());
+    Cb.upload(pack(DF_CorrBase, nk, n).data());
+    T.upload(pack(transformMatrix_k, nk, no).data());
+    Ko.upload(pack(KS_eigenVectors_orthoBasis, nk, no).data());
+    if (!S_overlap.empty()) S.upload(pack(S_overlap, nk, no).data());
+    std::vector<double> eps((size_t)nk * no), kp(kphase.begin(), kphase.end());
+    std::vector<int> v((size_t)nk * n), ridx(5 * (size_t)(nr > 0 ? nr : 1));
+    for (int k = 0; k < nk; ++k) {
+        for (int b = 0; b < no; ++b) eps[(size_t)k * no + b] = KS_eigenEnergy[k][b];
+        for (int i = 0; i < n; ++i) v[(size_t)k * n + i] = FromValToKS[k][i];
+    }
+    for (int r = 0; r < nr; ++r)
+        for (int c = 0; c < 5; ++c) ridx[5 * (size_t)r + c] = H_Rindex(r, c);
+    double *de = nullptr, *dk = nullptr;
+    hip_ok(hipMalloc((void**)&de, sizeof(double) * eps.size()), "hipMalloc");
+    hip_ok(hipMalloc((void**)&dk, sizeof(double) * kp.size()), "hipMalloc");
+    hip_ok(hipMemcpy(de, eps.data(), sizeof(double) * eps.size(), hipMemcpyHostToDevice), "upload");
+    hip_ok(hipMemcpy(dk, kp.data(), sizeof(double) * kp.size(), hipMemcpyHostToDevice), "upload");
+    const int rc = dmftgpu_upfold_density(h, n, no, nk, g.knum_mpiGlobal, g.beta, mu, (const double*)X.p,
+                                          (const double*)Cb.p, de, v.data(), (const double*)T.p, (const double*)Ko.p,
+                                          S_overlap.empty() ? nullptr : (const double*)S.p, nr, ridx.data(), dk,
+                                          (double*)D.p, nr > 0 ? (double*)R.p : nullptr);
+    (void)hipFree(de);
+    (void)hipFree(dk);
+    die(h, rc, "upfolding_density");
+    std::vector<cplx> hd(D.n), hr(R.n);
+    D.download(hd.data());
+    R.download(hr.data());
+    for (int k = 0; k < nk; ++k) {
+        densityMatDFT[k].setZero(no, no);
+        std::memcpy(densityMatDFT[k].data(), &hd[(size_t)k * no * no], sizeof(cplx) * no * no);
+    }
+    for (int x = 0; x < 3; ++x) {
+        dmr[x].assign(hr.begin() + (size_t)x * nr, hr.begin() + (size_t)(x + 1) * nr);
+        if (g.allreduce && nr > 0) g.allreduce(dmr[x].data(), dmr[x].size());
+    }
+}
+
 }  // namespace dmftgpu_shim

@@ -36,6 +36,10 @@
  *                        Hubbard-model Hk.HWR fixture, Hermiticity
  *   orc_greenftn_real    scalar closed form of the orbital-diagonal t2g model,
  *                        spectral sum rule, numpy.linalg.inv brute force
+ *   orc_upfold_density   k -> R round trip against orc_hk_from_hr on the
+ *                        complete R set, recovery of K = E^{-1} D E^{-H}
+ *                        (window block C^H X C, Fermi diagonal elsewhere),
+ *                        S DD S = D and S DDir = D, trace of D
  */
 #include <complex.h>
 #include <math.h>
@@ -1084,5 +1088,75 @@ int orc_greenftn_real(const orc_lattice *L, const cplx *Hk, int n_e, double e0, 
         if (trace) trace[j] = tr / (double)L->n_k_global;
         free(A); free(G);
     }
+    return bad;
+}
+
+/* upfolding_density (upfolding.cpp:38-86) with upfolding_k (:10-35), the
+ * downfolding == 1 branch (:45-51):
+ *   E   = (T^H)^{-1} K_o                           (:46, Eigen inverse)
+ *   M   = C^H X C                                   (:13)
+ *   K   = diag f(eps_b - mu), f clamped at |beta(eps-mu)| > 100 (:15-26)
+ *   K(v_i, v_j) = M(i, j), i, j < n                (:28-32, assignment)
+ *   D   = E K E^H                                   (:34)
+ *   DD  = S^{-1} D S^{-1},  DDir = S^{-1} D          (:49-50, Eigen inverse)
+ *   dmr[x][r] = sum_k Y_x,k(i0, m0) exp(+i p_k . R) / n_k_global, k in order
+ *               (:67-86), Y_0 = DD, Y_1 = DDir, Y_2 = D.
+ * X, C [n_k][n][n]; eps [n_k][n_orb]; v [n_k][n] (FromValToKS);
+ * T, K_o, S [n_k][n_orb][n_orb] (S NULL = identity); r_idx [n_r][5] = R1 R2 R3 i0 m0;
+ * kphase [n_k][3]; out: dm_k [n_k][n_orb][n_orb] (D, may be NULL), dmr [3][n_r]. */
+int orc_upfold_density(int n, int n_orb, int n_k, int n_k_global, double beta, double mu, const cplx *X,
+                       const cplx *C, const double *eps, const int *v, const cplx *T, const cplx *Ko,
+                       const cplx *S, int n_r, const int *r_idx, const double *kphase, cplx *dm_k, cplx *dmr)
+{
+    const size_t nn = (size_t)n * n, oo = (size_t)n_orb * n_orb;
+    cplx *Y = malloc(sizeof(cplx) * 3 * oo * (size_t)n_k);
+    cplx *a = malloc(sizeof(cplx) * oo), *b = malloc(sizeof(cplx) * oo), *E = malloc(sizeof(cplx) * oo);
+    cplx *Km = malloc(sizeof(cplx) * oo), *M = malloc(sizeof(cplx) * nn), *t = malloc(sizeof(cplx) * nn);
+    cplx *ch = malloc(sizeof(cplx) * nn);
+    int bad = 0;
+    for (int k = 0; k < n_k; ++k) {
+        cplx *DD = Y + (3 * (size_t)k) * oo, *DDir = DD + oo, *D = DDir + oo;
+        /* E = (T^H)^{-1} K_o */
+        adjoint(n_orb, T + k * oo, a);
+        bad |= orc_lu_inverse(n_orb, a, b);
+        matmul(n_orb, b, Ko + k * oo, E);
+        /* M = C^H X C */
+        adjoint(n, C + k * nn, ch);
+        matmul(n, X + k * nn, C + k * nn, t);
+        matmul(n, ch, t, M);
+        for (size_t e = 0; e < oo; ++e) Km[e] = 0.0;
+        for (int l = 0; l < n_orb; ++l) Km[IDX(l, l, n_orb)] = fermi_clamped(beta, eps[(size_t)k * n_orb + l], mu);
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) Km[IDX(v[(size_t)k * n + i], v[(size_t)k * n + j], n_orb)] = M[IDX(i, j, n)];
+        /* D = E K E^H */
+        matmul(n_orb, E, Km, a);
+        adjoint(n_orb, E, b);
+        matmul(n_orb, a, b, D);
+        if (dm_k) memcpy(dm_k + k * oo, D, sizeof(cplx) * oo);
+        if (S) {
+            bad |= orc_lu_inverse(n_orb, S + k * oo, b);
+            matmul(n_orb, b, D, DDir);
+            matmul(n_orb, DDir, b, DD);
+        } else {
+            memcpy(DDir, D, sizeof(cplx) * oo);
+            memcpy(DD, D, sizeof(cplx) * oo);
+        }
+    }
+    for (int r = 0; r < n_r; ++r) {
+        const int *q = r_idx + 5 * (size_t)r;
+        cplx s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        for (int k = 0; k < n_k; ++k) {
+            const double *p = kphase + 3 * (size_t)k;
+            const cplx ph = cexp(I * (p[0] * q[0] + p[1] * q[1] + p[2] * q[2]));
+            const cplx *DD = Y + (3 * (size_t)k) * oo;
+            s0 += DD[IDX(q[3], q[4], n_orb)] * ph;
+            s1 += DD[oo + IDX(q[3], q[4], n_orb)] * ph;
+            s2 += DD[2 * oo + IDX(q[3], q[4], n_orb)] * ph;
+        }
+        dmr[r] = s0 / (double)n_k_global;
+        dmr[(size_t)n_r + r] = s1 / (double)n_k_global;
+        dmr[2 * (size_t)n_r + r] = s2 / (double)n_k_global;
+    }
+    free(Y); free(a); free(b); free(E); free(Km); free(M); free(t); free(ch);
     return bad;
 }

@@ -56,6 +56,8 @@ def lib():
         _lib.orc_hk_from_hr.argtypes = [C.c_int, C.c_int, P, P, C.c_int, P, P]
         _lib.orc_greenftn_real.argtypes = [C.POINTER(OrcLattice), P, C.c_int, C.c_double, C.c_double, C.c_double,
                                            P, C.c_double, P, P, C.c_int]
+        _lib.orc_upfold_density.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, P, P, P,
+                                            P, P, P, P, C.c_int, P, P, P, P]
     return _lib
 
 
@@ -242,3 +244,26 @@ def greenftn_real(Hk, sigma_r, mu, e0, de, eta, nthreads=1):
     if rc:
         raise ValueError("singular matrix")
     return _from_cm(g), tr
+
+
+def upfold_density(dm_model, corr_base, ks_energy, val_to_ks, transform, ks_ortho, overlap, beta, mu, r_idx, kphase,
+                   n_k_global=None):
+    """upfolding_density with upfolding_k restated (downfolding == 1):
+    returns (D [nk][n_orb][n_orb], dmr [3][n_r]) with dmr = (dual-dual, dual-direct, direct-direct) DM(R).
+    overlap None means S = I."""
+    X = np.asarray(dm_model, dtype=np.complex128)
+    nk, n, _ = X.shape
+    eps = np.ascontiguousarray(np.asarray(ks_energy, dtype=np.float64))
+    n_orb = eps.shape[1]
+    v = np.ascontiguousarray(np.asarray(val_to_ks, dtype=np.int32))
+    ri = np.ascontiguousarray(np.asarray(r_idx, dtype=np.int32).reshape(-1, 5))
+    kp = np.ascontiguousarray(np.asarray(kphase, dtype=np.float64))
+    x, c, t, ko = _cm(X), _cm(corr_base), _cm(transform), _cm(ks_ortho)
+    s = None if overlap is None else _cm(overlap)
+    dm = np.zeros((nk, n_orb, n_orb), np.complex128)
+    dmr = np.zeros((3, ri.shape[0]), np.complex128)
+    rc = lib().orc_upfold_density(n, n_orb, nk, n_k_global or nk, beta, mu, _p(x), _p(c), _p(eps), _p(v), _p(t),
+                                  _p(ko), _p(s), ri.shape[0], _p(ri), _p(kp), _p(dm), _p(dmr))
+    if rc:
+        raise ValueError("singular transform or overlap")
+    return _from_cm(dm), dmr

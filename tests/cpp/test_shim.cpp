@@ -38,7 +38,15 @@ int orc_ft_w2tau(int, int, int, double, const cplx*, const cplx*, cplx*, int*, i
 int orc_ft_tau2w(int, int, int, double, const cplx*, cplx*, int*);
 int orc_hyb_delta(int, int, double, double, const cplx*, const cplx*, const cplx*, int, const int*, cplx*);
 void orc_hk_from_hr(int, int, const int*, const cplx*, int, const double*, cplx*);
+int orc_upfold_density(int, int, int, int, double, double, const cplx*, const cplx*, const double*, const int*,
+                       const cplx*, const cplx*, const cplx*, int, const int*, const double*, cplx*, cplx*);
 }
+
+struct IndexMat {  // the subset of Eigen::MatrixXi (H_Rindex) the adapter uses
+    std::vector<int> v;  // row-major [rows][5]
+    int rows() const { return (int)(v.size() / 5); }
+    int operator()(int r, int c) const { return v[5 * (size_t)r + c]; }
+};
 
 static unsigned long long rng_state = 12345;
 static double urand()
@@ -201,6 +209,77 @@ int main()
             for (int i = 0; i < no; ++i)
                 for (int j = 0; j < no; ++j) herm = std::fmax(herm, std::abs(Hb[k](i, j) - std::conj(Hb[k](j, i))));
         check("Construct_Hk Hermitian", herm, 1e-15);
+    }
+
+    // upfolding_density: 3 model bands inside 5 KS bands, 4 k points, a Hermitian
+    // positive definite overlap, every (R, i0, m0) with R in {-1,0,1} x {0} x {0}
+    {
+        const int n = 3, no = 5, nk = 4;
+        dmftgpu_shim::Globals gu;
+        gu.knum = nk;
+        gu.knum_mpiGlobal = nk;
+        gu.beta = 30.0;
+        const double mu = 0.1;
+        std::vector<Mat> dm(nk), cb(nk), ko(nk), tr(nk), so(nk);
+        std::vector<std::vector<double>> eps(nk, std::vector<double>(no));
+        std::vector<std::vector<int>> v2k(nk);
+        for (int k = 0; k < nk; ++k) {
+            dm[k].setZero(n, n);
+            cb[k].setZero(n, n);
+            for (int i = 0; i < n; ++i)
+                for (int j = 0; j <= i; ++j) {
+                    dm[k](i, j) = cplx(urand(), i == j ? 0.0 : urand());
+                    dm[k](j, i) = std::conj(dm[k](i, j));
+                }
+            for (int i = 0; i < n; ++i)
+                for (int j = 0; j < n; ++j) cb[k](i, j) = cplx(i == j ? 1.0 : 0.0) + 0.2 * cplx(urand(), urand());
+            ko[k].setZero(no, no);
+            tr[k].setZero(no, no);
+            so[k].setZero(no, no);
+            for (int i = 0; i < no; ++i)
+                for (int j = 0; j < no; ++j) {
+                    ko[k](i, j) = cplx(i == j ? 1.0 : 0.0) + 0.3 * cplx(urand(), urand());
+                    tr[k](i, j) = cplx(i == j ? 1.0 : 0.0) + 0.2 * cplx(urand(), urand());
+                }
+            for (int i = 0; i < no; ++i)
+                for (int j = 0; j < no; ++j) {
+                    cplx acc = i == j ? 1.0 : 0.0;
+                    for (int l = 0; l < no; ++l) acc += 0.1 * tr[k](i, l) * std::conj(tr[k](j, l));
+                    so[k](i, j) = acc;
+                }
+            for (int b = 0; b < no; ++b) eps[k][b] = 4.0 * urand() + (b == 0 ? -10.0 : 0.0);
+            v2k[k] = {(k + 1) % no, (k + 3) % no, (k + 4) % no};
+        }
+        std::vector<double> kp(3 * nk);
+        for (int k = 0; k < nk; ++k) kp[3 * k] = -PI + PI / 4 + 2 * PI * k / 4.0;
+        IndexMat hri;
+        for (int r = -1; r <= 1; ++r)
+            for (int i = 0; i < no; ++i)
+                for (int j = 0; j < no; ++j) hri.v.insert(hri.v.end(), {r, 0, 0, i, j});
+        const int nr = hri.rows();
+        // oracle on the same inputs
+        std::vector<cplx> X((size_t)nk * n * n), C((size_t)nk * n * n), T((size_t)nk * no * no),
+            K((size_t)nk * no * no), S((size_t)nk * no * no), od((size_t)nk * no * no), odmr(3 * (size_t)nr);
+        std::vector<double> e((size_t)nk * no);
+        std::vector<int> vv((size_t)nk * n);
+        for (int k = 0; k < nk; ++k) {
+            std::copy(dm[k].v.begin(), dm[k].v.end(), X.begin() + (size_t)k * n * n);
+            std::copy(cb[k].v.begin(), cb[k].v.end(), C.begin() + (size_t)k * n * n);
+            std::copy(tr[k].v.begin(), tr[k].v.end(), T.begin() + (size_t)k * no * no);
+            std::copy(ko[k].v.begin(), ko[k].v.end(), K.begin() + (size_t)k * no * no);
+            std::copy(so[k].v.begin(), so[k].v.end(), S.begin() + (size_t)k * no * no);
+            for (int b = 0; b < no; ++b) e[(size_t)k * no + b] = eps[k][b];
+            for (int i = 0; i < n; ++i) vv[(size_t)k * n + i] = v2k[k][i];
+        }
+        orc_upfold_density(n, no, nk, nk, gu.beta, mu, X.data(), C.data(), e.data(), vv.data(), T.data(), K.data(),
+                           S.data(), nr, hri.v.data(), kp.data(), od.data(), odmr.data());
+        std::vector<cplx> dmr[3];
+        dmftgpu_shim::upfolding_density(h, gu, dm, ko, hri, mu, so, tr, cb, eps, v2k, kp, dmr);
+        std::vector<cplx> gd((size_t)nk * no * no), gr(3 * (size_t)nr);
+        for (int k = 0; k < nk; ++k) std::copy(dm[k].v.begin(), dm[k].v.end(), gd.begin() + (size_t)k * no * no);
+        for (int x = 0; x < 3; ++x) std::copy(dmr[x].begin(), dmr[x].end(), gr.begin() + (size_t)x * nr);
+        check("upfolding_density D_k", relerr(gd.data(), od.data(), od.size()) + (dm[0].rows() == no ? 0 : 1), 1e-13);
+        check("upfolding_density DM(R)", relerr(gr.data(), odmr.data(), odmr.size()), 1e-13);
     }
 
     dmftgpu_destroy(h);

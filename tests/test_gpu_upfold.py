@@ -1,0 +1,137 @@
+"""GPU upfolding_density (upfolding.cpp:38-139) against the oracle.
+
+Bar: 1e-12 normwise relative on D_k and on the three R sums (the GPU sums k
+in tiles and inverts by Gauss-Jordan, the oracle sums k in order and
+inverts by LU; both pivot by column modulus).
+"""
+import numpy as np
+import pytest
+
+from dmftpack_amd import lattice as L
+from oracle import oracle as O
+
+BETA, MU = 40.0, 0.2
+
+
+def rel(x, ref):
+    return float(np.max(np.abs(np.asarray(x) - ref)) / max(np.max(np.abs(ref)), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    from dmftpack_amd import gpu as G
+    h = G.Handle()
+    yield G, h
+    h.close()
+
+
+def both(h, c, **kw):
+    args = (c["dm_model"], c["corr_base"], c["ks_energy"], c["val_to_ks"], c["transform"], c["ks_ortho"],
+            c["overlap"], BETA, MU, c["r_idx"], c["kphase"])
+    return h.upfold_density(*args, **kw), O.upfold_density(*args)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,n_orb,side,overlap", [(1, 1, 2, True), (3, 5, 3, True), (5, 9, 3, False),
+                                                   (14, 20, 3, True), (14, 37, 2, True), (20, 70, 2, True)])
+def test_upfold_vs_oracle(gpu, n, n_orb, side, overlap):
+    G, h = gpu
+    c = L.upfold_case(n, n_orb, side, 17 + n, overlap=overlap)
+    print(f"upfold n={n} n_orb={n_orb} nk={side ** 3} overlap={overlap}", flush=True)
+    (D, dmr), (Dr, dmr_r) = both(h, c)
+    assert rel(D, Dr) <= 1e-12
+    for x in range(3):
+        assert rel(dmr[x], dmr_r[x]) <= 1e-12, x
+
+
+@pytest.mark.gpu
+def test_upfold_round_trip_identity(gpu):
+    """Identity bases on an 8^3 mesh: the R sums of H(k) built on the GPU from a
+    14-orbital hopping list return that list's H(R)."""
+    G, h = gpu
+    n = 14
+    Rs, HR = L.random_tb_hr(n, 23)
+    hops = L.hopping_list(Rs, HR)
+    kp = L.kmesh_phases(8)
+    Hk = G.from_dev_cm(h.hk_from_hr(n, hops, kp))
+    c = L.upfold_case(n, n, 8, 3, overlap=False, identity=True)
+    c["dm_model"] = Hk
+    D, dmr = h.upfold_density(c["dm_model"], c["corr_base"], c["ks_energy"], c["val_to_ks"], c["transform"],
+                              c["ks_ortho"], None, BETA, MU, c["r_idx"], c["kphase"])
+    assert np.max(np.abs(D - Hk)) < 1e-13
+    want = {}
+    for (r1, r2, r3, i, j, t) in hops:
+        want[(r1, r2, r3, i, j)] = want.get((r1, r2, r3, i, j), 0.0) + t
+    ref = np.array([want.get(tuple(int(x) for x in row), 0.0) for row in c["r_idx"]])
+    for x in range(3):
+        assert np.max(np.abs(dmr[x] - ref)) < 1e-13
+
+
+@pytest.mark.gpu
+def test_upfold_chunks_and_options(gpu):
+    """k in chunks of 5 (several launches, R sums accumulated across them) equals one
+    chunk; dm_k = NULL gives the same sums; n_r = 0 gives dm_k only."""
+    G, h = gpu
+    c = L.upfold_case(5, 11, 3, 29)
+    args = (c["dm_model"], c["corr_base"], c["ks_energy"], c["val_to_ks"], c["transform"], c["ks_ortho"],
+            c["overlap"], BETA, MU)
+    D1, r1 = h.upfold_density(*args, c["r_idx"], c["kphase"])
+    h.set_option(G.capi.OPT_UPFOLD_CHUNK, 5)
+    D5, r5 = h.upfold_density(*args, c["r_idx"], c["kphase"])
+    _, r0 = h.upfold_density(*args, c["r_idx"], c["kphase"], want_dm_k=False)
+    h.set_option(G.capi.OPT_UPFOLD_CHUNK, 0)
+    assert np.max(np.abs(D5 - D1)) == 0.0
+    assert rel(r5, r1) <= 1e-14 and rel(r0, r5) == 0.0
+    Dn, rn = h.upfold_density(*args, np.zeros((0, 5), np.int32), c["kphase"])
+    assert np.max(np.abs(Dn - D1)) == 0.0 and rn.shape == (3, 0)
+
+
+@pytest.mark.gpu
+def test_upfold_from_lattice_density_matrix(gpu):
+    """The chain the reference runs after GreenFtn_w (TB.cpp:348): densityMatDFT of
+    the lattice path, upfolded; against the oracle on the same densityMatDFT."""
+    G, h = gpu
+    w = L.workload("dp14", seed=1, nk_side=3, n_freq=64)
+    mom = O.asymto_moments(w["sigma"], w["beta"])
+    lat = G.LatticeGF(w["Hk"], 5, 64, w["beta"], mom)
+    dm = lat.density_matrix(w["sigma"], 0.3)
+    lat.close()
+    c = L.upfold_case(14, 24, 3, 31)
+    c["dm_model"] = dm
+    (D, dmr), (Dr, dmr_r) = both(h, c)
+    assert rel(D, Dr) <= 1e-12
+    for x in range(3):
+        assert rel(dmr[x], dmr_r[x]) <= 1e-12
+    # the model-basis densityMatDFT is Hermitian, so is its upfolded D
+    assert np.max(np.abs(D - D.conj().transpose(0, 2, 1))) < 1e-12
+
+
+@pytest.mark.gpu
+def test_upfold_errors(gpu):
+    G, h = gpu
+    c = L.upfold_case(3, 5, 2, 37)
+    args = dict(dm_model=c["dm_model"], corr_base=c["corr_base"], ks_energy=c["ks_energy"], transform=c["transform"],
+                ks_ortho=c["ks_ortho"], overlap=c["overlap"], beta=BETA, mu=MU, kphase=c["kphase"])
+    bad = c["val_to_ks"].copy()
+    bad[0, 1] = bad[0, 0]  # a band twice in one window
+    with pytest.raises(G.capi.DmftGpuError):
+        h.upfold_density(val_to_ks=bad, r_idx=c["r_idx"], **args)
+    bad = c["val_to_ks"].copy()
+    bad[1, 0] = 5  # outside 0..n_orb-1
+    with pytest.raises(G.capi.DmftGpuError):
+        h.upfold_density(val_to_ks=bad, r_idx=c["r_idx"], **args)
+    r = c["r_idx"].copy()
+    r[3, 4] = 7
+    with pytest.raises(G.capi.DmftGpuError):
+        h.upfold_density(val_to_ks=c["val_to_ks"], r_idx=r, **args)
+    sing = dict(args)
+    T = c["transform"].copy()
+    T[2][:, 1] = 0.0  # a singular transformMatrix_k
+    sing["transform"] = T
+    with pytest.raises(G.capi.DmftGpuError):
+        h.upfold_density(val_to_ks=c["val_to_ks"], r_idx=c["r_idx"], **sing)
+    # the handle still works after the errors
+    D, _ = h.upfold_density(val_to_ks=c["val_to_ks"], r_idx=c["r_idx"], **args)
+    Dr, _ = O.upfold_density(c["dm_model"], c["corr_base"], c["ks_energy"], c["val_to_ks"], c["transform"],
+                             c["ks_ortho"], c["overlap"], BETA, MU, c["r_idx"], c["kphase"])
+    assert rel(D, Dr) <= 1e-12
