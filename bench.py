@@ -59,10 +59,14 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--method", default="schur_f32r", choices=["schur_f32r", "schur_f64", "plain_f64"])
-    ap.add_argument("--workload", default="dp14", choices=["dp14", "t2g", "large32", "batch64"],
+    ap.add_argument("--workload", default="dp14",
+                    choices=["dp14", "t2g", "large32", "batch64", "density_matrix", "real_axis", "ft", "hk",
+                             "upfold", "aux"],
                     help="dp14: the 14-orbital precision configuration (default, the headline); t2g: 3-orbital t2g, "
                          "Nk=16^3, N_freq=1024, beta=40; large32: 32 orbitals, two 5x5 correlated blocks, Nk=40^3, "
-                         "N_freq=4096, beta=100; batch64: 64 uneven lattices in one replayable step")
+                         "N_freq=4096, beta=100; batch64: 64 uneven lattices in one replayable step; the others time "
+                         "one path either side of the lattice loop (densityMatDFT, real axis, FT round trip, "
+                         "H(R)->H(k), upfolding, moments + hybridisation + mu search), one JSON line each")
     ap.add_argument("--nk-side", type=int, default=None)
     ap.add_argument("--n-freq", type=int, default=None)
     ap.add_argument("--refine", type=int, default=1)
@@ -96,6 +100,8 @@ def main():
 
     if args.workload == "batch64":
         return bench_batch(args, torch, capi, G, L)
+    if args.workload in PATHS:
+        return bench_path(args, torch, capi, G, L)
 
     # ---- synthetic workload -------------------------------------------------
     log(f"building workload {args.workload}")
@@ -368,6 +374,298 @@ def bench_batch(args, torch, capi, G, L):
     print(json.dumps(out), flush=True)
     b.close()
     h.close()
+
+
+# ---- the paths either side of the lattice hot path --------------------------------
+PATHS = {
+    "density_matrix": "densityMatDFT (GreenFtn.cpp:291-319): dp14 lattice, Nk=32^3, N_freq=2048, full n x n "
+                      "inverse per (k,iw)",
+    "real_axis": "retarded_GreenFtn2 k sums (qs_spectra.cpp:118-131): dp14 lattice, Nk=32^3, 1000 energies, "
+                 "eta=0.05",
+    "ft": "FourierTransform_ + FT_t_to_w round trip (FourierTransform.cpp:126-178,521-561): d=10, N_freq=4096, "
+          "N_tau=8192, beta=100",
+    "hk": "H(R) -> H(k) (ConstructModelHamiltonian.cpp:58-77): 32 orbitals, Nk=40^3, two cubic shells",
+    "upfold": "upfolding_density (upfolding.cpp:38-139): NBAND=14, NumOrbit=28, Nk=32^3, overlap, "
+              "R in {-1,0,1}^3 x all orbital pairs",
+    "aux": "getAsymto_moments (d=5, N_freq=2048) + Construct_hyb_delta (5x5, N_freq=2048) + "
+           "GreenFtn_w_adjust_mu (dp14 lattice, Newton on dN/dmu)",
+}
+
+
+def _events(torch, fn, steps, warmup):
+    """Per-call GPU time (HIP events on the current stream, the stream the
+    library's handle launches on) and wall time, after warmup calls."""
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(steps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / steps
+    return e0.elapsed_time(e1) / 1e3 / steps, wall
+
+
+def _cpu_sample(fn, total_units, units_of, budget, probe):
+    """Time the oracle on a bounded sample: a probe size first, then a sample
+    sized to about `budget` seconds; returns (units/s, sample size, seconds)."""
+    t0 = time.perf_counter()
+    fn(probe)
+    tp = max(time.perf_counter() - t0, 1e-4)
+    size = int(max(probe, min(total_units, probe * budget / tp)))
+    t0 = time.perf_counter()
+    fn(size)
+    tc = max(time.perf_counter() - t0, 1e-4)
+    return units_of(size) / tc, size, tc
+
+
+def _flop_roofline(achieved_tflops, kernel, note, flop_per_unit, units):
+    return dict(bound="mfma", achieved=achieved_tflops, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
+                frac=achieved_tflops / PEAK_FP64_TFLOPS, traffic=None, kernel=kernel, flop_per_unit=flop_per_unit,
+                units_per_launch=units, note=note)
+
+
+def bench_path(args, torch, capi, G, L):
+    from oracle import oracle as O
+    cores = len(os.sched_getaffinity(0))
+    budget = args.cpu_seconds
+    methods = {"plain_f64": G.PLAIN_F64, "schur_f64": G.SCHUR_F64, "schur_f32r": G.SCHUR_F32R}
+    wname = args.workload
+    cpu, extra, dtype = None, {}, "f64"
+
+    if wname in ("density_matrix", "real_axis", "aux"):
+        log("building the dp14 lattice")
+        w = L.workload("dp14", seed=1)
+        Hk, sig, beta, mom = w["Hk"], w["sigma"], w["beta"], w["exact_moments"]
+        nk, n = Hk.shape[0], Hk.shape[1]
+        nc, nf = sig.shape[1], sig.shape[0]
+        mu = 0.25
+        lat = G.LatticeGF(Hk, nc, nf, beta, mom)
+        s_dev = lat.sigma_dev(sig)
+
+    if wname == "density_matrix":
+        units = nk * 2 * nf
+        out = torch.empty((nk, n, n), dtype=torch.complex128, device="cuda")
+        fn = lambda: lat.check(lat.lib.dmftgpu_density_matrix(lat.h, G._ptr(s_dev), mu, G._ptr(out)))  # noqa: E731
+        log("timing density_matrix")
+        t_gpu, t_wall = _events(torch, fn, args.steps, args.warmup)
+        metric, unit, value = "densityMatDFT (k,w) inversions/s", "(k,w) inversions/s", units / t_wall
+        ach = f_fwd(n) * units / t_gpu / 1e12
+        roof = _flop_roofline(ach, "k_density_plain", "algorithmic 8n^3 FLOPs per (k,iw) x Nk*2N_freq / call time "
+                              "(events); full n x n inverse per (k,iw), plain FP64", f_fwd(n), units)
+        if not args.no_cpu:
+            def run(kk):
+                O.greenftn_w(Hk[:kk], sig, mom, mu, beta, density=True, nthreads=cores)
+            v, kk, tc = _cpu_sample(run, nk, lambda kk: kk * 2 * nf, budget, max(1, cores))
+            cpu = dict(value=v, unit=unit, cores=cores, kind="port",
+                       sample=f"oracle GreenFtn_w with densityMatDFT on {kk} of {nk} k x {2 * nf} frequencies, {tc:.1f} s")
+        cfg = dict(n=n, n_corr=nc, n_k=nk, n_freq=nf, beta=beta, units_per_call=units)
+    elif wname == "real_axis":
+        ne, e0, de, eta = 1000, -5.0, 0.01, 0.05
+        E = e0 + de * np.arange(ne)
+        rng = np.random.default_rng(1)
+        sig_r = np.zeros((ne, nc, nc), complex)
+        for _ in range(2):
+            B = rng.normal(size=(nc, nc)) + 1j * rng.normal(size=(nc, nc))
+            sig_r += (B @ B.conj().T / nc)[None] / (E - rng.uniform(-1, 1) + 0.3j)[:, None, None]
+        sr = G.to_dev_cm(sig_r)
+        g = torch.empty((ne, nc, nc), dtype=torch.complex128, device="cuda")
+        tr = torch.empty((ne,), dtype=torch.complex128, device="cuda")
+        units = nk * ne
+        per = {}
+        for name in ("schur_f64", "schur_f32r", "plain_f64"):
+            m = methods[name]
+            fn = lambda: lat.check(lat.lib.dmftgpu_greenftn_real(lat.h, m, ne, e0, de, eta, G._ptr(sr), mu,  # noqa: E731
+                                                                 G._ptr(g), G._ptr(tr)))
+            log(f"timing real axis {name}")
+            t_gpu, t_wall = _events(torch, fn, args.steps if name != "plain_f64" else 2, 1)
+            per[name] = dict(value=units / t_wall, ms_per_call=t_wall * 1e3, gpu_ms=t_gpu * 1e3)
+        head = args.method
+        metric, unit, value = "real-axis (k,E) inversions/s (k-summed G^R)", "(k,E) inversions/s", per[head]["value"]
+        dtype = "f32+f64" if head == "schur_f32r" else "f64"
+        ach = f_fwd(n) * units / (per[head]["gpu_ms"] / 1e3) / 1e12
+        roof = _flop_roofline(ach, "k_lattice_schur_cols<FWD> (real axis)" if head != "plain_f64" else
+                              "k_lattice_plain<FWD> (real axis)", "algorithmic 8n^3 FLOPs per (k,E) x Nk*n_E / call "
+                              "time (events)", f_fwd(n), units)
+        extra["methods"] = per
+        extra["speedup_vs_plain_fp64"] = per[head]["value"] / per["plain_f64"]["value"]
+        if not args.no_cpu:
+            def run(kk):
+                O.greenftn_real(Hk[:kk], sig_r, mu, e0, de, eta, nthreads=cores)
+            v, kk, tc = _cpu_sample(run, nk, lambda kk: kk * ne, budget, max(1, cores // 4))
+            cpu = dict(value=v, unit=unit, cores=cores, kind="port",
+                       sample=f"oracle retarded_GreenFtn2 k sums on {kk} of {nk} k x {ne} energies, {tc:.1f} s")
+        cfg = dict(n=n, n_corr=nc, n_k=nk, n_e=ne, eta=eta, method=head, units_per_call=units)
+    elif wname == "ft":
+        log("building the large32 self-energy")
+        sig, _ = L.pole_sigma(10, 4096, 100.0, 1, block=5)
+        beta, nf, ntau, d = 100.0, 4096, 8192, 10
+        h = G.Handle()
+        mom = h.get_asymto_moments(sig, beta)
+        f = G.to_dev_cm(sig)
+        m = G.to_dev_cm(mom)
+        gt = torch.empty((ntau + 1, d, d), dtype=torch.complex128, device="cuda")
+        fw = torch.empty((nf, d, d), dtype=torch.complex128, device="cuda")
+        pt, vi = capi.C.c_int(0), capi.C.c_int(0)
+
+        def fn():
+            h.check(h.lib.dmftgpu_ft_w2tau(h.h, d, nf, ntau, beta, G._ptr(f), G._ptr(m), G._ptr(gt), capi.C.byref(pt)))
+            h.check(h.lib.dmftgpu_ft_tau2w(h.h, d, nf, ntau, beta, G._ptr(gt), G._ptr(fw), capi.C.byref(vi)))
+        log("timing iw -> tau -> iw")
+        t_gpu, t_wall = _events(torch, fn, args.steps, args.warmup)
+        pairs = (ntau + 1) * nf
+        units = 2 * pairs
+        metric, unit, value = "(tau,iw) pairs/s, iw->tau->iw round trip", "(tau,w) pairs/s", units / t_wall
+        ach = 8.0 * d * d * units / t_gpu / 1e12
+        roof = _flop_roofline(ach, "k_dft<W2Tau> + k_dft<Tau2W>", "algorithmic 8 d^2 FLOPs (one complex MAC per "
+                              "element) per (tau,iw) pair and direction / call time (events); the twiddles and the "
+                              "tail are extra", 8.0 * d * d, units)
+        if not args.no_cpu:
+            sig_np, mom_np = sig, mom
+
+            def run(nt):
+                gts, _ = O.ft_w2tau(sig_np, mom_np, beta, nt, nthreads=cores)
+                O.ft_tau2w(gts, beta, nf)
+            v, nt, tc = _cpu_sample(run, ntau, lambda nt: 2 * (nt + 1) * nf, budget, 16)
+            cpu = dict(value=v, unit=unit, cores=cores, kind="port",
+                       sample=f"oracle FourierTransform_ (OpenMP over tau) + FT_t_to_w at N_tau={nt} of {ntau}, "
+                              f"N_freq={nf}, {tc:.1f} s")
+        cfg = dict(d=d, n_freq=nf, n_tau=ntau, beta=beta, units_per_call=units)
+        lat = h
+    elif wname == "hk":
+        n, side = 32, 40
+        Rs, HR = L.random_tb_hr(n, 1)
+        hops = L.hopping_list(Rs, HR)
+        kp_np = L.kmesh_phases(side)
+        nk = kp_np.shape[0]
+        h = G.Handle()
+        idx = np.ascontiguousarray(np.array([x[:5] for x in hops], dtype=np.int32).reshape(-1, 5))
+        tt = np.ascontiguousarray(np.array([complex(x[5]) for x in hops], dtype=np.complex128))
+        kp = torch.from_numpy(np.ascontiguousarray(kp_np)).to("cuda")
+        out = torch.empty((nk, n, n), dtype=torch.complex128, device="cuda")
+
+        def fn():
+            h.check(h.lib.dmftgpu_hk_from_hr(h.h, n, len(hops), idx.ctypes.data_as(capi.C.POINTER(capi.C.c_int)),
+                                             tt.ctypes.data_as(capi.C.POINTER(capi.C.c_double)), nk, G._ptr(kp),
+                                             G._ptr(out)))
+        log("timing H(R) -> H(k)")
+        t_gpu, t_wall = _events(torch, fn, args.steps, args.warmup)
+        units = nk
+        metric, unit, value = "H(k) matrices/s", "k points/s", units / t_wall
+        nbytes = nk * n * n * 16.0
+        ach = nbytes / t_gpu / 1e9
+        roof = dict(bound="hbm", achieved=ach, peak=PEAK_HBM_GBS, unit="GB/s", frac=ach / PEAK_HBM_GBS,
+                    traffic=None, kernel="k_hk_from_hr", bytes_per_unit=n * n * 16.0, units_per_launch=units,
+                    hops=len(hops),
+                    note="algorithmic bytes = the H(k) written once (n^2 x 16 B per k) / call time (events, incl. "
+                         "the host grouping of the hops); each element also takes one FP64 sincos per hop, "
+                         "which sets the time")
+        if not args.no_cpu:
+            def run(kk):
+                O.hk_from_hr(n, hops, kp_np[:kk])
+            v, kk, tc = _cpu_sample(run, nk, lambda kk: kk, budget, 16)
+            cpu = dict(value=v, unit=unit, cores=1, kind="port",
+                       sample=f"oracle H(k) sum on {kk} of {nk} k points, {len(hops)} hops, {tc:.1f} s")
+        cfg = dict(n=n, n_k=nk, hops=len(hops), units_per_call=units)
+        lat = h
+    elif wname == "upfold":
+        n, no, side = 14, 28, 32
+        log("building the upfolding inputs")
+        c = L.upfold_case(n, no, side, 1)
+        nk = c["kphase"].shape[0]
+        h = G.Handle()
+        X, Cb, T, Ko, S = (G.to_dev_cm(c[k]) for k in ("dm_model", "corr_base", "transform", "ks_ortho", "overlap"))
+        eps = torch.from_numpy(np.ascontiguousarray(c["ks_energy"])).to("cuda")
+        kp = torch.from_numpy(np.ascontiguousarray(c["kphase"])).to("cuda")
+        v = np.ascontiguousarray(c["val_to_ks"].astype(np.int32))
+        ri = np.ascontiguousarray(c["r_idx"].astype(np.int32))
+        nr = ri.shape[0]
+        D = torch.empty((nk, no, no), dtype=torch.complex128, device="cuda")
+        R = torch.empty((3, nr), dtype=torch.complex128, device="cuda")
+        PI_ = capi.C.POINTER(capi.C.c_int)
+
+        def fn():
+            h.check(h.lib.dmftgpu_upfold_density(h.h, n, no, nk, nk, 40.0, 0.2, G._ptr(X), G._ptr(Cb), G._ptr(eps),
+                                                 v.ctypes.data_as(PI_), G._ptr(T), G._ptr(Ko), G._ptr(S), nr,
+                                                 ri.ctypes.data_as(PI_), G._ptr(kp), G._ptr(D), G._ptr(R)))
+        log("timing upfolding")
+        t_gpu, t_wall = _events(torch, fn, args.steps, args.warmup)
+        units = nk
+        fl = 8.0 * (7 * no ** 3 + 2 * n ** 3) + 24.0 * nr
+        metric, unit, value = "upfolded k points/s (D_k and the three R sums)", "k points/s", units / t_wall
+        ach = fl * units / t_gpu / 1e12
+        roof = _flop_roofline(ach, "k_zgemm_sq / k_gj_inverse / k_dm_r", "algorithmic FLOPs per k: 8(7 n_orb^3 + "
+                              "2 n^3) for the products and inverses + 24 n_R for the R sums (sincos not counted) / "
+                              "call time (events)", fl, units)
+        if not args.no_cpu:
+            def run(kk):
+                O.upfold_density(c["dm_model"][:kk], c["corr_base"][:kk], c["ks_energy"][:kk], c["val_to_ks"][:kk],
+                                 c["transform"][:kk], c["ks_ortho"][:kk], c["overlap"][:kk], 40.0, 0.2, c["r_idx"],
+                                 c["kphase"][:kk], n_k_global=nk)
+            v_, kk, tc = _cpu_sample(run, nk, lambda kk: kk, budget, 8)
+            cpu = dict(value=v_, unit=unit, cores=1, kind="port",
+                       sample=f"oracle upfolding_density on {kk} of {nk} k points, {nr} R entries, {tc:.1f} s")
+        cfg = dict(n=n, n_orb=no, n_k=nk, n_r=nr, units_per_call=units)
+        lat = h
+    elif wname == "aux":
+        h = G.Handle()
+        f = G.to_dev_cm(sig)
+        mo = torch.empty((4, nc, nc), dtype=torch.complex128, device="cuda")
+        win = (capi.C.c_int * 4)()
+        fm = lambda: h.check(h.lib.dmftgpu_moments(h.h, nc, nf, beta, G._ptr(f), 0, G._ptr(mo), win))  # noqa: E731
+        log("timing moments")
+        tm_gpu, tm = _events(torch, fm, args.steps, args.warmup)
+        gl, _ = lat.greenftn_w_dev(s_dev, mu, G.SCHUR_F64)
+        himp = G.to_dev_cm(np.mean(Hk, axis=0)[:nc, :nc][None])[0].contiguous()
+        orb = (capi.C.c_int * nc)(*range(nc))
+        dl = torch.empty((nf, nc, nc), dtype=torch.complex128, device="cuda")
+        fh = lambda: h.check(h.lib.dmftgpu_hyb_delta(h.h, nc, nf, beta, mu, G._ptr(gl), G._ptr(s_dev),  # noqa: E731
+                                                     G._ptr(himp), nc, orb, G._ptr(dl)))
+        log("timing hybridisation")
+        th_gpu, th = _events(torch, fh, args.steps, args.warmup)
+        mu_out, nev = capi.C.c_double(0), capi.C.c_int(0)
+        fmu = lambda: lat.check(lat.lib.dmftgpu_adjust_mu(lat.h, G.SCHUR_F64, G._ptr(s_dev), 0.0, w["n_target"],  # noqa: E731
+                                                          1e-10, capi.C.byref(mu_out), capi.C.byref(nev)))
+        log("timing mu search")
+        tu_gpu, tu = _events(torch, fmu, max(1, args.steps // 2), 1)
+        units = nk * 2 * nf * nev.value
+        metric, unit, value = "GreenFtn_w_adjust_mu (k,iw) units/s", "(k,w) units/s", units / tu
+        ach = 8.0 * n ** 3 * units / tu_gpu / 1e12
+        roof = _flop_roofline(ach, "k_lattice_schur<DENS>", "algorithmic 8n^3 FLOPs per (k,iw) per N(mu) evaluation "
+                              "x evaluations / call time (events)", f_fwd(n), units)
+        extra["moments_ms"] = tm * 1e3
+        extra["hyb_delta_ms"] = th * 1e3
+        extra["adjust_mu"] = dict(ms=tu * 1e3, n_eval=nev.value, mu=mu_out.value)
+        if not args.no_cpu:
+            t0 = time.perf_counter()
+            O.asymto_moments(sig, beta)
+            t_om = time.perf_counter() - t0
+            gl_np = G.from_dev_cm(gl)
+            t0 = time.perf_counter()
+            O.hyb_delta(gl_np, sig, np.mean(Hk, axis=0)[:nc, :nc], list(range(nc)), beta, mu)
+            t_oh = time.perf_counter() - t0
+
+            def run(kk):
+                O.giw_to_nele(Hk[:kk], sig, mom, mu, beta, nthreads=cores)
+            v_, kk, tc = _cpu_sample(run, nk, lambda kk: kk * 2 * nf, budget, max(1, cores))
+            cpu = dict(value=v_, unit=unit, cores=cores, kind="port",
+                       sample=f"oracle GiwToNele (one N(mu) evaluation, eigenvalue form) on {kk} of {nk} k x "
+                              f"{2 * nf} frequencies, {tc:.1f} s; moments {t_om * 1e3:.1f} ms and hybridisation "
+                              f"{t_oh * 1e3:.1f} ms on 1 thread",
+                       moments_ms=t_om * 1e3, hyb_delta_ms=t_oh * 1e3)
+        cfg = dict(n=n, n_corr=nc, n_k=nk, n_freq=nf, beta=beta, mu_tol=1e-10, units_per_call=units)
+        h.close()
+    cfg["workload"] = PATHS[wname]
+    line = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": units / value * 1e3, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": cfg, "roofline": roof,
+            "cpu_baseline": cpu}
+    line.update(extra)
+    print(json.dumps(line), flush=True)
+    lat.close()
 
 
 if __name__ == "__main__":
