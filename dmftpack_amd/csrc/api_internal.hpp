@@ -50,6 +50,10 @@ struct dmftgpu_handle_s {
     dgi::Workspace prep;     // prepare-time data
     dgi::Workspace ws;       // per-call scratch
     dgi::Workspace ft;       // Fourier-transform scratch
+    dgi::Workspace mom_cache;  // moment fit: basis table + inverted normal matrices of one (n_freq, beta, branch)
+    bool mom_key_valid = false;
+    int mom_key_nf = 0, mom_key_green = 0;
+    double mom_key_beta = 0.0;
     dgi::Workspace stat;     // static weights wt/dt [nk][n] of the cached mu
     dgi::Workspace fused;    // tr G^2 partials shared by a fused forward and its adjoint
     dgi::Workspace cot;      // loss cotangents gbar [n_freq][nc][nc] + the loss sum

@@ -83,7 +83,17 @@ __device__ void lu_inverse4(const double (&A)[16], int n, double (&X)[16])
 // columns of the design matrix for row pair jj (Re row 2jj, Im row 2jj+1)
 // non-G: Re: [1, 0, -1/z^2, 0]   Im: [0, -1/z, 0, 1/z^3]   (:227-230)
 // G    : Re: [-1/z^2, 0]          Im: [0, 1/z^3]             (:366-367)
-__global__ void k_mom_ktk(int n_freq, int range, double beta, int green, double* __restrict__ kinv)
+// basis values per frequency, tab[wn] = {-1/z, -1/z^2, 1/z^3, 1/z}, the
+// factors the fit loops below multiply by (formed as the reference forms them)
+__global__ void k_mom_basis(int n_freq, double beta, double4* __restrict__ tab)
+{
+    const int wn = blockIdx.x * blockDim.x + threadIdx.x;
+    if (wn >= n_freq) return;
+    const double z = (2 * wn + 1) * kPi / beta;
+    tab[wn] = double4{-1. / pow(z, 1), -1. / pow(z, 2), 1. / pow(z, 3), 1.0 / z};
+}
+
+__global__ void k_mom_ktk(int n_freq, int range, int green, const double4* __restrict__ tab, double* __restrict__ kinv)
 {
     const int w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= range) return;
@@ -91,16 +101,16 @@ __global__ void k_mom_ktk(int n_freq, int range, double beta, int green, double*
     double K[16];
     for (int e = 0; e < 16; ++e) K[e] = 0.0;
     for (int wn = w; wn < n_freq; ++wn) {
-        const double z = (2 * wn + 1) * kPi / beta;
+        const double4 c = tab[wn];
         double re[4] = {0, 0, 0, 0}, im[4] = {0, 0, 0, 0};
         if (!green) {
             re[0] = 1.0;
-            im[1] = -1. / pow(z, 1);
-            re[2] = -1. / pow(z, 2);
-            im[3] = 1. / pow(z, 3);
+            im[1] = c.x;
+            re[2] = c.y;
+            im[3] = c.z;
         } else {
-            re[0] = -1. / pow(z, 2);
-            im[1] = 1. / pow(z, 3);
+            re[0] = c.y;
+            im[1] = c.z;
         }
         for (int b = 0; b < nm; ++b)
             for (int a = 0; a < nm; ++a) K[a + b * 4] += re[a] * re[b] + im[a] * im[b];
@@ -111,8 +121,8 @@ __global__ void k_mom_ktk(int n_freq, int range, double beta, int green, double*
 }
 
 // mt[a][w][e] = sum_b Kinv[w][a][b] (K^T g_e)_b     (:239-243)
-__global__ void k_mom_fit(const cd* __restrict__ fw, int d, int n_freq, int range, double beta, int green,
-                          const double* __restrict__ kinv, cd* __restrict__ mt)
+__global__ void k_mom_fit(const cd* __restrict__ fw, int d, int n_freq, int range, const double4* __restrict__ tab,
+                          int green, const double* __restrict__ kinv, cd* __restrict__ mt)
 {
     const int m2 = d * d;
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -122,7 +132,7 @@ __global__ void k_mom_fit(const cd* __restrict__ fw, int d, int n_freq, int rang
     const int nm = green ? 2 : 4;
     cd s[4] = {cd{0, 0}, cd{0, 0}, cd{0, 0}, cd{0, 0}};
     for (int wn = w; wn < n_freq; ++wn) {
-        const double z = (2 * wn + 1) * kPi / beta;
+        const double4 c = tab[wn];
         const cd fab = fw[(size_t)wn * m2 + al + be * d];
         const cd fba = fw[(size_t)wn * m2 + be + al * d];
         // Re = (F_ab + conj(F_ba))/2 ; Im = (F_ab - conj(F_ba))/(2i)
@@ -130,13 +140,13 @@ __global__ void k_mom_fit(const cd* __restrict__ fw, int d, int n_freq, int rang
         cd im = cd{0.5 * (fab.y + fba.y), -0.5 * (fab.x - fba.x)};
         if (!green) {
             s[0] = cadd(s[0], re);
-            s[1] = cadd(s[1], cscale(im, -1. / pow(z, 1)));
-            s[2] = cadd(s[2], cscale(re, -1. / pow(z, 2)));
-            s[3] = cadd(s[3], cscale(im, 1. / pow(z, 3)));
+            s[1] = cadd(s[1], cscale(im, c.x));
+            s[2] = cadd(s[2], cscale(re, c.y));
+            s[3] = cadd(s[3], cscale(im, c.z));
         } else {
-            if (al == be) im.x += 1.0 / z;  // (:373)
-            s[0] = cadd(s[0], cscale(re, -1. / pow(z, 2)));
-            s[1] = cadd(s[1], cscale(im, 1. / pow(z, 3)));
+            if (al == be) im.x += c.w;  // (:373)
+            s[0] = cadd(s[0], cscale(re, c.y));
+            s[1] = cadd(s[1], cscale(im, c.z));
         }
     }
     const double* X = kinv + (size_t)w * 16;
@@ -494,13 +504,21 @@ extern "C" int dmftgpu_moments(dmftgpu_handle h, int d, int n_freq, double beta,
     // the reference asserts on the window bounds (:321, :440)
     if (range < 1 || ncent < 1) return fail(h, DMFTGPU_ERR_ARG, "n_freq=%d too small for the window search", n_freq);
     const int m2 = d * d;
-    size_t need = (size_t)range * 16 * 8 + (size_t)nm * range * m2 * 16 + (size_t)nm * range * 8 +
+    // the basis table and the inverted normal matrices depend on (n_freq, beta,
+    // green_asym) only: kept in the handle and formed again when those change
+    const bool hit = h->mom_key_valid && h->mom_key_nf == n_freq && h->mom_key_beta == beta &&
+                     h->mom_key_green == green_asym;
+    hipError_t eg;
+    char* kbase = (char*)h->mom_cache.get((size_t)n_freq * sizeof(double4) + 256 + (size_t)range * 16 * 8, &eg);
+    if (!kbase) return fail(h, DMFTGPU_ERR_HIP, "moment cache: %s", hipGetErrorString(eg));
+    double4* tab = (double4*)kbase;
+    double* kinv = (double*)(kbase + ((size_t)n_freq * sizeof(double4) + 255) / 256 * 256);
+    size_t need = (size_t)nm * range * m2 * 16 + (size_t)nm * range * 8 +
                   (size_t)nm * ncent * m2 * 16 + (size_t)ncent * m2 * 16 * 6 + (size_t)ncent * d * 8 * 2 + 64 * 256;
     int rc = 0;
     char* base = ft_ws(h, need, &rc);
     if (!base) return rc;
     Carve cv{base};
-    double* kinv = cv.take<double>((size_t)range * 16);
     cd* mt = cv.take<cd>((size_t)nm * range * m2);
     double* trace = cv.take<double>((size_t)nm * range);
     cd* avg = cv.take<cd>((size_t)nm * ncent * m2);
@@ -510,9 +528,18 @@ extern "C" int dmftgpu_moments(dmftgpu_handle h, int d, int n_freq, double beta,
     cd* scr = cv.take<cd>((size_t)ncent * m2 * 3);
     double* wT = cv.take<double>((size_t)ncent * d);
     hipStream_t s = h->stream;
-    hipLaunchKernelGGL(k_mom_ktk, dim3(nblk(range)), dim3(256), 0, s, n_freq, range, beta, green_asym, kinv);
-    HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_mom_fit, dim3(nblk((size_t)range * m2)), dim3(256), 0, s, (const cd*)fw, d, n_freq, range, beta,
+    if (!hit) {
+        h->mom_key_valid = false;
+        hipLaunchKernelGGL(k_mom_basis, dim3(nblk(n_freq)), dim3(256), 0, s, n_freq, beta, tab);
+        HIPCHK(hipGetLastError());
+        hipLaunchKernelGGL(k_mom_ktk, dim3(nblk(range)), dim3(256), 0, s, n_freq, range, green_asym, tab, kinv);
+        HIPCHK(hipGetLastError());
+        h->mom_key_valid = true;
+        h->mom_key_nf = n_freq;
+        h->mom_key_beta = beta;
+        h->mom_key_green = green_asym;
+    }
+    hipLaunchKernelGGL(k_mom_fit, dim3(nblk((size_t)range * m2)), dim3(256), 0, s, (const cd*)fw, d, n_freq, range, tab,
                        green_asym, kinv, mt);
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(k_mom_trace, dim3(nblk((size_t)range * nm)), dim3(256), 0, s, mt, d, range, nm, trace);

@@ -79,6 +79,23 @@ def test_moments_match_oracle(H, kind):
 
 
 @pytest.mark.gpu
+def test_moments_cache_across_grids(H):
+    """The handle keeps the fit's normal matrices for the last (N_freq, beta, branch):
+    a sequence of calls that hits and misses it returns, bit for bit, what a
+    fresh handle returns for each call."""
+    from dmftpack_amd import gpu as G
+    seq = [(5, 512, 50.0, False), (3, 512, 50.0, False), (5, 256, 50.0, False), (5, 512, 40.0, False),
+           (4, 512, 40.0, True), (4, 512, 40.0, False), (5, 512, 50.0, False)]
+    for d, nf, beta, green in seq:
+        fw = three_pole_g(d, nf, beta, seed=d + nf)
+        got, wg = H.get_asymto_moments(fw, beta, green_asym=green, return_windows=True)
+        fresh = G.Handle()
+        want, ww = fresh.get_asymto_moments(fw, beta, green_asym=green, return_windows=True)
+        fresh.close()
+        assert wg == ww and np.array_equal(got, want), (d, nf, beta, green)
+
+
+@pytest.mark.gpu
 def test_moments_green_branch(H):
     fw = three_pole_g(4, 300, 40.0, seed=5)
     ref, wref = O.asymto_moments(fw, 40.0, green_asym=True, return_windows=True)
