@@ -296,6 +296,7 @@ __global__ void k_ft_subtract_tail(const cd* __restrict__ fw, int d, int n_freq,
 // out[j][e] = sum_i coef(j,i) in[i][e]  (NIN = 1)  or  sum_i ca in0 + cb in1 (NIN = 2)
 // workgroup: TJ consecutive rows j, all elements; inputs staged in LDS TI rows at a time.
 struct CoefW2Tau {
+    static constexpr bool PHASE_ROW = false;
     double beta, w0, dw, tstep;
     const double* taus;  // explicit tau values (FourierTransform_tau), or null for tau_j = j*tstep
     __device__ void operator()(int j, int i, cd* c) const
@@ -309,14 +310,24 @@ struct CoefW2Tau {
 };
 
 struct CoefTau2W {
+    // the phases exp(i w_n tau_t) of a tile's rows are formed once per (n, t)
+    // (k_dft stages them in LDS); segment t takes those at t and t + 1
+    static constexpr bool PHASE_ROW = true;
     double beta, dt;
-    __device__ void operator()(int n, int t, cd* c) const
+    __device__ cd phase(int n, int t) const
+    {
+        const double wn = (2 * n + 1) * kPi / beta;
+        const double tau = t * dt;
+        double s, c;
+        sincos(tau * wn, &s, &c);
+        return cd{c, s};
+    }
+    __device__ void operator()(int n, int t, cd* c) const { from_phases(n, t, phase(n, t), phase(n, t + 1), c); }
+    __device__ void from_phases(int n, int t, cd e0, cd e1, cd* c) const
     {
         const double wn = (2 * n + 1) * kPi / beta;
         const double tau0 = t * dt, tau1 = (t + 1) * dt;
-        double s0, c0, s1, c1;
-        sincos(tau0 * wn, &s0, &c0);
-        sincos(tau1 * wn, &s1, &c1);
+        const double s0 = e0.y, c0 = e0.x, s1 = e1.y, c1 = e1.x;
         const cd pI = cd{c1 - c0, s1 - s0};
         const cd pI2 = cd{tau1 * c1 - tau0 * c0, tau1 * s1 - tau0 * s0};
         // ca = pI/w^2 + pI2/(i w) ; cb = pI/(i w);  1/(i w) = -i/w
@@ -328,29 +339,50 @@ struct CoefTau2W {
     }
 };
 
+// out[j][e] = sum_i coef(j, i) in[i][e] over this workgroup's share of i:
+// blockIdx.y selects the i range [y*ichunk, min(ni, (y+1)*ichunk)); with more
+// than one range the results are partial sums, out + y*nj*m2, which
+// k_dft_sum_parts adds in range order.
 template <class Coef, int NIN, int OPT>
-__global__ __launch_bounds__(256) void k_dft(int nj, int ni, int m2, int TJ, int TI, const cd* __restrict__ in0,
-                                             const cd* __restrict__ in1, cd* __restrict__ out, Coef coef)
+__global__ __launch_bounds__(256) void k_dft(int nj, int ni, int m2, int TJ, int TI, int ichunk,
+                                             const cd* __restrict__ in0, const cd* __restrict__ in1,
+                                             cd* __restrict__ out, Coef coef)
 {
     extern __shared__ cd sm[];
     cd* sc = sm;                        // [NIN][TJ][TI]
     cd* sx = sm + NIN * TJ * TI;        // [NIN][TI][m2]
+    cd* sp = sx + NIN * TI * m2;        // PHASE_ROW: [TJ][TI + 1]
     const int j0 = blockIdx.x * TJ;
+    const int ib = blockIdx.y * ichunk;
+    const int ie = min(ni, ib + ichunk);
+    out += (size_t)blockIdx.y * nj * m2;
     const int nout = TJ * m2;
     cd acc[OPT];
 #pragma unroll
     for (int q = 0; q < OPT; ++q) acc[q] = cd{0, 0};
-    for (int i0 = 0; i0 < ni; i0 += TI) {
+    for (int i0 = ib; i0 < ie; i0 += TI) {
         __syncthreads();
+        if constexpr (Coef::PHASE_ROW) {
+            for (int t = threadIdx.x; t < TJ * (TI + 1); t += 256) {
+                const int jj = t / (TI + 1), ii = t % (TI + 1);
+                sp[t] = (j0 + jj < nj && i0 + ii <= ie) ? coef.phase(j0 + jj, i0 + ii) : cd{0, 0};
+            }
+            __syncthreads();
+        }
         for (int t = threadIdx.x; t < TJ * TI; t += 256) {
             const int jj = t / TI, ii = t % TI;
             cd c[2] = {cd{0, 0}, cd{0, 0}};
-            if (j0 + jj < nj && i0 + ii < ni) coef(j0 + jj, i0 + ii, c);
+            if (j0 + jj < nj && i0 + ii < ie) {
+                if constexpr (Coef::PHASE_ROW)
+                    coef.from_phases(j0 + jj, i0 + ii, sp[jj * (TI + 1) + ii], sp[jj * (TI + 1) + ii + 1], c);
+                else
+                    coef(j0 + jj, i0 + ii, c);
+            }
             for (int q = 0; q < NIN; ++q) sc[(q * TJ + jj) * TI + ii] = c[q];
         }
         for (int t = threadIdx.x; t < TI * m2; t += 256) {
             const int ii = t / m2, e = t % m2;
-            const bool ok = i0 + ii < ni;
+            const bool ok = i0 + ii < ie;
             sx[ii * m2 + e] = ok ? in0[(size_t)(i0 + ii) * m2 + e] : cd{0, 0};
             if (NIN == 2) sx[(TI + ii) * m2 + e] = ok ? in1[(size_t)(i0 + ii) * m2 + e] : cd{0, 0};
         }
@@ -431,24 +463,74 @@ __global__ void k_antiherm(const cd* __restrict__ f, int d, int count, cd* __res
     out[t] = cd{0.5 * dlt.y, -0.5 * dlt.x};  // dlt / (2i)
 }
 
-template <class Coef, int NIN>
-hipError_t launch_dft(int nj, int ni, int m2, const cd* in0, const cd* in1, cd* out, Coef coef, hipStream_t s)
+// parts[y][j][e] -> out[j][e], y ascending
+__global__ void k_dft_sum_parts(const cd* __restrict__ parts, int nparts, size_t count, cd* __restrict__ out)
 {
-    int TJ = 2048 / m2;
-    if (TJ > 32) TJ = 32;
-    if (TJ < 1) TJ = 1;
-    int TI = 4096 / (m2 * NIN);
-    if (TI > 32) TI = 32;
-    if (TI < 1) TI = 1;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    cd s = parts[t];
+    for (int y = 1; y < nparts; ++y) s = cadd(s, parts[(size_t)y * count + t]);
+    out[t] = s;
+}
+
+struct DftShape {
+    int TJ, TI, ks, ichunk;
+};
+
+// Tiles of the transform and the split of the summed index i over
+// workgroups: enough workgroups to fill the device (about 1024) when the
+// output alone gives fewer.
+inline DftShape dft_shape(int nj, int ni, int m2, int nin)
+{
+    DftShape d;
+    d.TJ = 2048 / m2;
+    if (d.TJ > 32) d.TJ = 32;
+    if (d.TJ < 1) d.TJ = 1;
+    d.TI = 2048 / (m2 * nin);
+    if (d.TI > 32) d.TI = 32;
+    if (d.TI < 1) d.TI = 1;
+    const int gx = (nj + d.TJ - 1) / d.TJ;
+    const int itiles = (ni + d.TI - 1) / d.TI;
+    int ks = (1024 + gx - 1) / gx;
+    if (ks > itiles) ks = itiles;
+    if (ks > 16) ks = 16;
+    if (ks < 1) ks = 1;
+    const int per = (itiles + ks - 1) / ks;
+    d.ichunk = per * d.TI;
+    d.ks = (ni + d.ichunk - 1) / d.ichunk;
+    return d;
+}
+
+// scratch (in cd) the split transform needs for its partial sums
+inline size_t dft_part_count(int nj, int ni, int m2, int nin)
+{
+    const DftShape d = dft_shape(nj, ni, m2, nin);
+    return d.ks > 1 ? (size_t)d.ks * nj * m2 : 0;
+}
+
+template <class Coef, int NIN>
+hipError_t launch_dft(int nj, int ni, int m2, const cd* in0, const cd* in1, cd* out, Coef coef, cd* parts,
+                      hipStream_t s)
+{
+    const DftShape d = dft_shape(nj, ni, m2, NIN);
+    const int TJ = d.TJ, TI = d.TI;
     const int opt = (TJ * m2 + 255) / 256;
-    size_t shm = sizeof(cd) * ((size_t)NIN * TJ * TI + (size_t)NIN * TI * m2);
-    dim3 grid((nj + TJ - 1) / TJ);
+    size_t shm = sizeof(cd) * ((size_t)NIN * TJ * TI + (size_t)NIN * TI * m2 +
+                               (Coef::PHASE_ROW ? (size_t)TJ * (TI + 1) : 0));
+    dim3 grid((nj + TJ - 1) / TJ, d.ks);
+    cd* dst = d.ks > 1 ? parts : out;
+    if (d.ks > 1 && !parts) return hipErrorInvalidValue;
 #define L_(O)                                                                                                     \
     {                                                                                                             \
         hipError_t e = hipFuncSetAttribute((const void*)k_dft<Coef, NIN, O>, hipFuncAttributeMaxDynamicSharedMemorySize, \
                                            (int)shm);                                                             \
         if (e != hipSuccess) return e;                                                                            \
-        hipLaunchKernelGGL((k_dft<Coef, NIN, O>), grid, dim3(256), shm, s, nj, ni, m2, TJ, TI, in0, in1, out, coef); \
+        hipLaunchKernelGGL((k_dft<Coef, NIN, O>), grid, dim3(256), shm, s, nj, ni, m2, TJ, TI, d.ichunk, in0, in1,  \
+                           dst, coef);                                                                            \
+        e = hipGetLastError();                                                                                    \
+        if (e != hipSuccess || d.ks == 1) return e;                                                               \
+        const size_t cnt = (size_t)nj * m2;                                                                       \
+        hipLaunchKernelGGL(k_dft_sum_parts, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, parts, d.ks, cnt, out); \
         return hipGetLastError();                                                                                 \
     }
     if (opt <= 1) L_(1)
@@ -642,14 +724,16 @@ extern "C" int dmftgpu_ft_w2tau(dmftgpu_handle h, int d, int n_freq, int n_tau, 
     std::vector<hl::cx> A1, A2;
     std::vector<double> e1, e2;
     tail_setup(d, m1, mm2, m3, A1, e1, A2, e2);
+    const size_t npart = dft_part_count(ntau1, n_freq, m2, 1);
     size_t need = (size_t)n_freq * m2 * 16 + (size_t)ntau1 * m2 * 16 + 4 * (size_t)m2 * 16 + 4 * (size_t)d * 8 +
-                  (size_t)ntau1 * d * 8 + 16 * 256;
+                  (size_t)ntau1 * d * 8 + npart * 16 + 16 * 256;
     int rc = 0;
     char* base = ft_ws(h, need, &rc);
     if (!base) return rc;
     Carve cv{base};
     cd* D = cv.take<cd>((size_t)n_freq * m2);
     cd* X = cv.take<cd>((size_t)ntau1 * m2);
+    cd* parts = npart ? cv.take<cd>(npart) : nullptr;
     cd* dA1 = cv.take<cd>(m2);
     cd* dA2 = cv.take<cd>(m2);
     double* de1 = cv.take<double>(d);
@@ -663,7 +747,7 @@ extern "C" int dmftgpu_ft_w2tau(dmftgpu_handle h, int d, int n_freq, int n_tau, 
                        beta, dA1, de1, dA2, de2, D);
     HIPCHK(hipGetLastError());
     CoefW2Tau cw{beta, kPi / beta, (2 * kPi) / beta, beta / n_tau, nullptr};
-    HIPCHK((launch_dft<CoefW2Tau, 1>(ntau1, n_freq, m2, D, nullptr, X, cw, s)));
+    HIPCHK((launch_dft<CoefW2Tau, 1>(ntau1, n_freq, m2, D, nullptr, X, cw, parts, s)));
     cd* out = (cd*)gtau;
     hipLaunchKernelGGL(k_ft_finish, dim3(nblk((size_t)ntau1 * m2)), dim3(256), 0, s, X, d, ntau1, beta, beta / n_tau,
                        (const double*)nullptr, dA1, de1, dA2, de2, out);
@@ -721,7 +805,8 @@ extern "C" int dmftgpu_ft_tau2w(dmftgpu_handle h, int d, int n_freq, int n_tau, 
     const int m2 = d * d;
     const double dt = beta / n_tau;
     hipStream_t s = h->stream;
-    size_t need = 2 * (size_t)n_tau * m2 * 16 + (size_t)n_freq * m2 * 16 + (size_t)n_freq * d * 8 + 8 * 256;
+    const size_t npart = dft_part_count(n_freq, n_tau, m2, 2);
+    size_t need = 2 * (size_t)n_tau * m2 * 16 + (size_t)n_freq * m2 * 16 + (size_t)n_freq * d * 8 + npart * 16 + 8 * 256;
     int rc = 0;
     char* base = ft_ws(h, need, &rc);
     if (!base) return rc;
@@ -730,11 +815,12 @@ extern "C" int dmftgpu_ft_tau2w(dmftgpu_handle h, int d, int n_freq, int n_tau, 
     cd* bt = cv.take<cd>((size_t)n_tau * m2);
     cd* ah = cv.take<cd>((size_t)n_freq * m2);
     double* wmax = cv.take<double>((size_t)n_freq * d);
+    cd* parts = npart ? cv.take<cd>(npart) : nullptr;
     hipLaunchKernelGGL(k_ft_segments, dim3(nblk((size_t)n_tau * m2)), dim3(256), 0, s, (const cd*)gtau, m2, n_tau, dt,
                        at, bt);
     HIPCHK(hipGetLastError());
     CoefTau2W ct{beta, dt};
-    HIPCHK((launch_dft<CoefTau2W, 2>(n_freq, n_tau, m2, at, bt, (cd*)fw, ct, s)));
+    HIPCHK((launch_dft<CoefTau2W, 2>(n_freq, n_tau, m2, at, bt, (cd*)fw, ct, parts, s)));
     hipLaunchKernelGGL(k_antiherm, dim3(nblk((size_t)n_freq * m2)), dim3(256), 0, s, (const cd*)fw, d, n_freq, ah);
     HIPCHK(hipGetLastError());
     HIPCHK(dg::launch_herm_eig(ah, d, 0, d, nullptr, 0, wmax, nullptr, n_freq, s));
@@ -768,14 +854,16 @@ extern "C" int dmftgpu_ft_w2tau_points(dmftgpu_handle h, int d, int n_freq, doub
     std::vector<hl::cx> A1, A2;
     std::vector<double> e1, e2;
     tail_setup(d, m1, mm2, m3, A1, e1, A2, e2);
+    const size_t npart = dft_part_count(n_points, n_freq, m2, 1);
     size_t need = (size_t)n_freq * m2 * 16 + (size_t)n_points * m2 * 16 + 4 * (size_t)m2 * 16 + 4 * (size_t)d * 8 +
-                  (size_t)n_points * 8 + 16 * 256;
+                  (size_t)n_points * 8 + npart * 16 + 16 * 256;
     int rc = 0;
     char* base = ft_ws(h, need, &rc);
     if (!base) return rc;
     Carve cv{base};
     cd* D = cv.take<cd>((size_t)n_freq * m2);
     cd* X = cv.take<cd>((size_t)n_points * m2);
+    cd* parts = npart ? cv.take<cd>(npart) : nullptr;
     cd* dA1 = cv.take<cd>(m2);
     cd* dA2 = cv.take<cd>(m2);
     double* de1 = cv.take<double>(d);
@@ -790,7 +878,7 @@ extern "C" int dmftgpu_ft_w2tau_points(dmftgpu_handle h, int d, int n_freq, doub
                        beta, dA1, de1, dA2, de2, D);
     HIPCHK(hipGetLastError());
     CoefW2Tau cw{beta, kPi / beta, (2 * kPi) / beta, 0.0, dtau};
-    HIPCHK((launch_dft<CoefW2Tau, 1>(n_points, n_freq, m2, D, nullptr, X, cw, s)));
+    HIPCHK((launch_dft<CoefW2Tau, 1>(n_points, n_freq, m2, D, nullptr, X, cw, parts, s)));
     hipLaunchKernelGGL(k_ft_finish, dim3(nblk((size_t)n_points * m2)), dim3(256), 0, s, X, d, n_points, beta, 0.0,
                        (const double*)dtau, dA1, de1, dA2, de2, (cd*)out);
     HIPCHK(hipGetLastError());

@@ -147,7 +147,7 @@ def test_errors_are_reported(gpu):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["t2g", "dp14", "n_9", "large32"])
+@pytest.mark.parametrize("name", ["t2g", "dp14", "n_2", "n_9", "n_20", "n_27", "large32"])
 def test_density_matrix(gpu, name):
     """densityMatDFT (GreenFtn.cpp:291-319), full n x n per k."""
     Hk, sig, beta, ncl, mu = small_case(name)
