@@ -9,8 +9,8 @@ synthetic, generated deterministically, and resident in HBM before timing.
 
 One step = what a Newton-Krylov DMFT update needs per call (H(k) is bound
 once, with its Schur data eig(H_dd(k)), W = H_cd V_d, before timing):
-  moments   static eigen-decomposition of H(k) + P m0 P^H for the step's Sigma
-            tail (GreenFtn.cpp:197-215)
+  moments   fit of the Sigma tail moments (getAsymto_moments, GreenFtn.cpp:189-195)
+            and the static eigen-decomposition of H(k) + P m0 P^H (:197-215)
   forward   GreenFtn_w: G_loc, NumMatrix and total N over Nk x 2 N_freq (k, iw)
   adjoint   VJP of L = sum_w ||G_loc - G_tgt||_F^2 + (N - N0)^2 -> dL/dSigma, dL/dmu
 value = Nk * 2 N_freq / step time, for the FP32 + FP64-refinement Schur method;
@@ -140,8 +140,12 @@ def main():
     nbuf = torch.empty((nc, nc), dtype=torch.complex128, device=dev)
     sbar = torch.empty_like(s_dev)
 
+    win = (capi.C.c_int * 4)()
+
     def step(method):
-        # a new Sigma brings new tail moments: static eigen-decomposition of H(k) + P m0 P^H
+        # a new Sigma brings new tail moments: the fit of getAsymto_moments (GreenFtn.cpp:189-195), then the
+        # static eigen-decomposition of H(k) + P m0 P^H
+        lat.check(lat.lib.dmftgpu_moments(lat.h, nc, nf, beta, G._ptr(s_dev), 0, G._ptr(mom_dev), win))
         lat.check(lat.lib.dmftgpu_update_moments(lat.h, G._ptr(mom_dev)))
         # forward (G_loc, NumMatrix, N), loss cotangents and adjoint (dmftgpu_gloc_loss_grad)
         _, _, N, _, _, mb = lat.loss_grad_dev(s_dev, mu, gtgt, N0, method, gloc=gbuf, num=nbuf, out=sbar)
@@ -296,8 +300,9 @@ def main():
         "config": {"workload": WORKLOADS[args.workload],
                    "n": n, "n_corr": nc, "n_k": nk, "n_freq": nf, "beta": beta, "method": args.method,
                    "refine_steps": args.refine, "schur_layout": args.layout, "units_per_step": units,
-                   "step": "static eig for the Sigma tail + GreenFtn_w forward (G_loc, NumMatrix, N) + loss cotangents "
-                           "+ VJP (dL/dSigma, dL/dmu), one dmftgpu_gloc_loss_grad call after dmftgpu_update_moments"},
+                   "step": "tail-moment fit of Sigma + static eig for the tail + GreenFtn_w forward (G_loc, NumMatrix, "
+                           "N) + loss cotangents + VJP (dL/dSigma, dL/dmu): dmftgpu_moments, dmftgpu_update_moments, "
+                           "one dmftgpu_gloc_loss_grad call"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "host_to_device": {"hk_bytes": int(Hk.nbytes), "hk_upload_ms": upload_hk_ms,
