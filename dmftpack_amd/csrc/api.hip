@@ -362,7 +362,7 @@ int run_lattice(dmftgpu_handle h, int method, int mode, const double* sigma, dou
     const int n_freq = ra ? ra->n_e : L.n_freq;
     int slots;
     if (method == DMFTGPU_PLAIN_F64) slots = dg::plain_slots(n, mode);
-    else slots = dg::schur_slots(nc, mode, method == DMFTGPU_SCHUR_F32R, h->schur_layout == 1 || ra);
+    else slots = dg::schur_slots(nc, mode, method == DMFTGPU_SCHUR_F32R, h->schur_layout == 1 || (ra && mode != dg::MODE_FWD));
     Plan P = pk ? Plan{nk, 1} : make_plan(nk, nw_tot, slots, h->target_wg);
     if (fused_tr2 && P.n_chunks != fused_chunks)  // both halves of a fused pair must chunk k alike
         return fail(h, DMFTGPU_ERR_STATE, "fused partials laid out for %d k chunks, this call uses %d", fused_chunks,
@@ -451,7 +451,7 @@ size_t lattice_ws_bytes(dmftgpu_handle h, int method, int mode, const RealAxis* 
     const size_t n_freq = ra ? (size_t)ra->n_e : (size_t)L.n_freq;
     int slots = method == DMFTGPU_PLAIN_F64 ? dg::plain_slots(L.n, mode)
                                             : dg::schur_slots(nc, mode, method == DMFTGPU_SCHUR_F32R,
-                                                              h->schur_layout == 1 || ra);
+                                                              h->schur_layout == 1 || (ra && mode != dg::MODE_FWD));
     Plan P = make_plan(L.n_k, (int)nw_tot, slots, h->target_wg);
     size_t C = P.n_chunks + 1;  // +1 for the reduced copy
     size_t b = (C + 1) * nw_tot * 16 * (2 + (size_t)m2) + C * n_freq * 16 * (m2 + 1) + 64 * 256;

@@ -16,6 +16,9 @@ constexpr int MODE_FWD_HERM = 3;
 // adjoint follows at the same (Sigma, mu), the frequencies beyond N_freq are done
 // once, in the forward, instead of once there and once in the adjoint.
 constexpr int MODE_DENS_HERM = 4;
+// One-matrix-per-lane kernels only: the forward on the real axis (retarded_GreenFtn2),
+// frequency point nw being z = e0 + nw de + i eta (freq_point)
+constexpr int MODE_FWD_REAL = 5;
 
 // Data read through the constant address space: the compiler treats it as
 // invariant for the kernel's duration, so wave-uniform loads become scalar

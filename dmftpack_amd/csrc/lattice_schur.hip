@@ -63,7 +63,8 @@ hipError_t launch_lattice_schur(const LatArgs& a, int mode, bool fp32, hipStream
     if (fused && !schur_can_fuse(a.nc, cols, a.real_axis)) return hipErrorInvalidValue;
     hipError_t e = launch_sigma_all(a, s);
     if (e != hipSuccess) return e;
-    if (a.nc > 6 || cols || a.real_axis) {  // the per-lane kernels take Matsubara points only
+    if (a.real_axis && mode == MODE_FWD && a.nc <= 6 && !cols) return dispatch_nc(a, MODE_FWD_REAL, fp32, a.pSigAll, s, wg, nullptr);
+    if (a.nc > 6 || cols || a.real_axis) {  // the per-lane kernels take the real axis in the forward only
         switch (cols_bucket(a.nc)) {
             case 4: return launch_cols_4(a, mode, fp32, a.pSigAll, s);
             case 5: return launch_cols_5(a, mode, fp32, a.pSigAll, s);

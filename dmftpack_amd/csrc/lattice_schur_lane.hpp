@@ -101,7 +101,7 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
     constexpr int NP = NC * (NC + 1) / 2;
     constexpr int M2 = NC * NC;
     constexpr bool HERM = MODE == MODE_FWD_HERM || MODE == MODE_DENS_HERM;
-    constexpr bool FWDLIKE = MODE == MODE_FWD || MODE == MODE_FWD_HERM;
+    constexpr bool FWDLIKE = MODE == MODE_FWD || MODE == MODE_FWD_HERM || MODE == MODE_FWD_REAL;
     // accumulators that do not fit next to the working matrices live in LDS
     constexpr bool ACC_LDS = (MODE == MODE_VJP) || (MODE == MODE_DENS_HERM) || (FWDLIKE && F32R);
     const int nd = a.n - NC;
@@ -115,8 +115,14 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
     const int nwc = wvalid ? nw : nw_hi - 1;  // clamp for loads
     const int k0 = chunk * a.k_per_chunk;
     const int k1 = min(a.nk, k0 + a.k_per_chunk);
-    const double w = matsubara(nwc, a.beta);
-    const double mu = a.mu;
+    // z + mu = (mu + shift) + i w: the Matsubara point, or on the real axis E + mu + i eta
+    double mu, w;
+    if constexpr (MODE == MODE_FWD_REAL) {
+        freq_point(a, nwc, mu, w);
+    } else {
+        w = matsubara(nwc, a.beta);
+        mu = a.mu;
+    }
     const bool vjp_here = (MODE == MODE_VJP) && wvalid && nw < a.n_freq;
     const cd* sig = sig_all + (size_t)nwc * M2;
     const cd* gb = (MODE == MODE_VJP) ? a.gbar + (size_t)(nw < a.n_freq ? nw : 0) * M2 : nullptr;
@@ -524,7 +530,7 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
                     q += 2;
                 }
             }
-    } else if (MODE == MODE_FWD) {
+    } else if (MODE == MODE_FWD || MODE == MODE_FWD_REAL) {
         a.pTr[ch * a.nw_tot + nw] = trA;
         if (ACC_LDS) {
             for (int e = 0; e < M2; ++e) a.pG[(ch * a.nw_tot + nw) * M2 + e] = sAcc[e * WG + lane];
@@ -582,7 +588,7 @@ constexpr size_t schur_shm()
 {
     if constexpr (MODE == MODE_FWD_HERM) return F32R ? sizeof(double) * (size_t)NC * NC * WG : 0;
     if constexpr (MODE == MODE_DENS_HERM) return sizeof(double) * (size_t)NC * NC * WG;
-    constexpr bool ACC_LDS = (MODE == MODE_VJP) || (MODE == MODE_FWD && F32R);
+    constexpr bool ACC_LDS = (MODE == MODE_VJP) || ((MODE == MODE_FWD || MODE == MODE_FWD_REAL) && F32R);
     return ACC_LDS ? sizeof(cd) * (size_t)NC * NC * WG : 0;
 }
 
@@ -610,8 +616,13 @@ hipError_t launch_nc_w(const LatArgs& a, const cd* sig_all, hipStream_t s, const
 template <int NC, int MODE, bool F32R, bool TRACE, int REFINE>
 hipError_t launch_nc_r(const LatArgs& a, const cd* sig_all, hipStream_t s, int wg, const SchurMulti* m)
 {
-    if (wg == 128) return launch_nc_w<NC, MODE, F32R, TRACE, REFINE, 128>(a, sig_all, s, m);
-    return launch_nc_w<NC, MODE, F32R, TRACE, REFINE, 64>(a, sig_all, s, m);
+    if constexpr (MODE == MODE_FWD_REAL) {  // one width on the real axis
+        (void)wg;
+        return launch_nc_w<NC, MODE, F32R, TRACE, REFINE, 128>(a, sig_all, s, m);
+    } else {
+        if (wg == 128) return launch_nc_w<NC, MODE, F32R, TRACE, REFINE, 128>(a, sig_all, s, m);
+        return launch_nc_w<NC, MODE, F32R, TRACE, REFINE, 64>(a, sig_all, s, m);
+    }
 }
 
 template <int NC, int MODE, bool F32R, bool TRACE>
@@ -643,6 +654,13 @@ hipError_t launch_lane_nc(const LatArgs& a, int mode, bool fp32, const cd* sig_a
                         : launch_nc<NC, MODE_FWD_HERM, false, true>(a, sig_all, s, wg, m);
         return fp32 ? launch_nc<NC, MODE_FWD_HERM, true, false>(a, sig_all, s, wg, m)
                     : launch_nc<NC, MODE_FWD_HERM, false, false>(a, sig_all, s, wg, m);
+    }
+    if (mode == MODE_FWD_REAL) {
+        if (a.want_trace)
+            return fp32 ? launch_nc<NC, MODE_FWD_REAL, true, true>(a, sig_all, s, wg, m)
+                        : launch_nc<NC, MODE_FWD_REAL, false, true>(a, sig_all, s, wg, m);
+        return fp32 ? launch_nc<NC, MODE_FWD_REAL, true, false>(a, sig_all, s, wg, m)
+                    : launch_nc<NC, MODE_FWD_REAL, false, false>(a, sig_all, s, wg, m);
     }
     if (mode == MODE_DENS)
         return fp32 ? launch_nc<NC, MODE_DENS, true, false>(a, sig_all, s, wg, m)

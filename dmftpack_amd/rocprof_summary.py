@@ -75,7 +75,8 @@ def lattice_mode(name):
         return None
     args = [x.strip() for x in m.group(2).split(",")]
     idx = 2 if m.group(1) == "plain" else 1
-    return {"0": "FWD", "1": "DENS", "2": "VJP", "3": "FWD_HERM", "4": "DENS_HERM"}.get(args[idx] if len(args) > idx else "")
+    return {"0": "FWD", "1": "DENS", "2": "VJP", "3": "FWD_HERM", "4": "DENS_HERM", "5": "FWD_REAL"}.get(
+        args[idx] if len(args) > idx else "")
 
 
 def add_calls(d, keys):

@@ -121,7 +121,8 @@ int dmftgpu_greenftn_w(dmftgpu_handle h, int method, const double* sigma, double
  *   trace   [n_e] complex, optional (NULL): (1/n_k_global) sum_k tr G^R over
  *           the model space (-Im trace / pi is the total spectral weight)
  * Uses the H(k) bound by dmftgpu_prepare.  The Schur methods run the
- * column-per-lane kernels here. */
+ * one-matrix-per-lane kernels for n_corr <= 6 (the column-per-lane ones with
+ * DMFTGPU_OPT_SCHUR_LAYOUT = 1, and always above 6). */
 int dmftgpu_greenftn_real(dmftgpu_handle h, int method, int n_e, double e0, double de, double eta,
                           const double* sigma_r, double mu, double* gloc_r, double* trace);
 

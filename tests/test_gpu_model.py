@@ -129,6 +129,38 @@ def test_greenftn_real(gpu, name, method, tol):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("n,nc", [(3, 3), (14, 5), (9, 2), (12, 6)])
+@pytest.mark.parametrize("method,tol", [(1, 1e-12), (2, 1e-9)])
+def test_greenftn_real_layouts(gpu, n, nc, method, tol):
+    """The real axis through both Schur layouts for n_c <= 6 (one matrix per lane,
+    the default, and one column per lane), k sums and per-k spectra, against the
+    oracle."""
+    G, _ = gpu
+    Hk = L.t2g_hk(4)[0] if n == 3 else L.random_tb_hk(n, 3, 5)
+    ne, e0, de, eta, mu = 130, -4.0, 0.06, 0.05, 0.2
+    rng = np.random.default_rng(7)
+    E = e0 + de * np.arange(ne)
+    B = rng.normal(size=(nc, nc)) + 1j * rng.normal(size=(nc, nc))
+    sig = (B @ B.conj().T / nc)[None] / (E - 0.3 + 0.5j)[:, None, None]
+    ref, tr_ref = O.greenftn_real(Hk, sig, mu, e0, de, eta, nthreads=8)
+    print(f"greenftn_real layouts n={n} nc={nc} method {method}", flush=True)
+    lat = G.LatticeGF(Hk, nc, 16, 40.0, np.zeros((4, nc, nc), complex))
+    outs = []
+    for layout in (0, 1):
+        lat.set_option(G.capi.OPT_SCHUR_LAYOUT, layout)
+        got, tr = lat.greenftn_real(sig, mu, e0, de, eta, method=method)
+        assert np.max(np.abs(got - ref)) <= tol * max(1.0, np.max(np.abs(ref))), layout
+        assert np.max(np.abs(tr - tr_ref)) <= tol * max(1.0, np.max(np.abs(tr_ref))), layout
+        gk, trk = lat.greenftn_real_k(sig, mu, e0, de, eta, method=method)
+        assert np.max(np.abs(gk.mean(axis=0) - got)) <= 1e-13 * max(1.0, np.max(np.abs(got)))
+        outs.append(gk)
+    ref0, _ = O.greenftn_real(Hk[:1], sig, mu, e0, de, eta)
+    for gk in outs:
+        assert np.max(np.abs(gk[0] - ref0)) <= tol * max(1.0, np.max(np.abs(ref0)))
+    lat.close()
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("name,method,tol", [("t2g", 0, 1e-12), ("dp14", 1, 1e-12), ("dp14", 2, 1e-9),
                                              ("large32", 1, 1e-12)])
 def test_greenftn_real_per_k(gpu, name, method, tol):
