@@ -140,12 +140,10 @@ def main():
     nbuf = torch.empty((nc, nc), dtype=torch.complex128, device=dev)
     sbar = torch.empty_like(s_dev)
 
-    win = (capi.C.c_int * 4)()
-
     def step(method):
         # a new Sigma brings new tail moments: the fit of getAsymto_moments (GreenFtn.cpp:189-195), then the
         # static eigen-decomposition of H(k) + P m0 P^H
-        lat.check(lat.lib.dmftgpu_moments(lat.h, nc, nf, beta, G._ptr(s_dev), 0, G._ptr(mom_dev), win))
+        lat.check(lat.lib.dmftgpu_moments(lat.h, nc, nf, beta, G._ptr(s_dev), 0, G._ptr(mom_dev), None))
         lat.check(lat.lib.dmftgpu_update_moments(lat.h, G._ptr(mom_dev)))
         # forward (G_loc, NumMatrix, N), loss cotangents and adjoint (dmftgpu_gloc_loss_grad)
         _, _, N, _, _, mb = lat.loss_grad_dev(s_dev, mu, gtgt, N0, method, gloc=gbuf, num=nbuf, out=sbar)
@@ -619,8 +617,7 @@ def bench_path(args, torch, capi, G, L):
         h = G.Handle()
         f = G.to_dev_cm(sig)
         mo = torch.empty((4, nc, nc), dtype=torch.complex128, device="cuda")
-        win = (capi.C.c_int * 4)()
-        fm = lambda: h.check(h.lib.dmftgpu_moments(h.h, nc, nf, beta, G._ptr(f), 0, G._ptr(mo), win))  # noqa: E731
+        fm = lambda: h.check(h.lib.dmftgpu_moments(h.h, nc, nf, beta, G._ptr(f), 0, G._ptr(mo), None))  # noqa: E731
         log("timing moments")
         tm_gpu, tm = _events(torch, fm, args.steps, args.warmup)
         gl, _ = lat.greenftn_w_dev(s_dev, mu, G.SCHUR_F64)

@@ -252,12 +252,84 @@ __global__ void k_mom_posmat(const cd* __restrict__ avg, int d, int ncent, int g
 }
 
 // final moment: mean over the selected window of (mt_ab + conj(mt_ba))/2, then Hermitised (:319-330, :458-461)
-__global__ void k_mom_final(const cd* __restrict__ mt, int d, int range, int mom_idx, int cen, int Nv,
-                            cd* __restrict__ out)
+// Window choice of getAsymto_moments (:289-315) for moment a = blockIdx.x:
+// the spread of the moment's trace over the 2 Nv + 1 windows around each
+// centre (mean, then the squared deviations, in the reference's order), and
+// the scan "minPos = ncent-1; for i: if ok[i] && var[i] < var[minPos] then
+// minPos = i", i.e. the first admissible centre of least spread if that
+// spread is below the last centre's, else the last centre.  ok[i] is the
+// positivity test on the window-averaged moments (:301-312, :432).
+// cen[target] = minPos + Nv, target = a (a + 2 for the G branch).
+__global__ __launch_bounds__(256) void k_mom_select(const double* __restrict__ trace, const double* __restrict__ w1,
+                                                    const double* __restrict__ wT, int d, int range, int ncent, int Nv,
+                                                    int green, int* __restrict__ cen)
+{
+    __shared__ double bv[256];
+    __shared__ int bi[256];
+    const int a = blockIdx.x, t = threadIdx.x;
+    auto spread = [&](int i) {
+        const double* v = trace + (size_t)a * range + i;  // centre i + Nv, window [i, i + 2 Nv]
+        double mean = 0.0;
+        for (int j = 0; j < 2 * Nv + 1; ++j) mean += v[j];
+        mean /= (2 * Nv + 1);
+        double ss = 0.0;
+        for (int j = 0; j < 2 * Nv + 1; ++j) ss += (v[j] - mean) * (v[j] - mean);
+        return sqrt(ss / (2 * Nv + 1));
+    };
+    double best = 0.0;
+    int besti = -1;
+    for (int i = t; i < ncent; i += 256) {
+        const bool ok = green ? wT[(size_t)i * d] > 1 : (w1[(size_t)i * d] > 0 && wT[(size_t)i * d] > 0);
+        if (!ok) continue;
+        const double v = spread(i);
+        if (besti < 0 || v < best) {  // ascending i within a thread: the first of equals stays
+            best = v;
+            besti = i;
+        }
+    }
+    bv[t] = best;
+    bi[t] = besti;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+            const int oi = bi[t + w];
+            const double ov = bv[t + w];
+            if (oi >= 0 && (bi[t] < 0 || ov < bv[t] || (ov == bv[t] && oi < bi[t]))) {
+                bv[t] = ov;
+                bi[t] = oi;
+            }
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        const int last = ncent - 1;
+        const int minPos = (bi[0] >= 0 && bv[0] < spread(last)) ? bi[0] : last;
+        cen[green ? a + 2 : a] = minPos + Nv;
+    }
+}
+
+// moments[4][d][d] = 0 and windows = -1, then m1 = I on the G branch (:451-452)
+__global__ void k_mom_init(cd* __restrict__ mom, int d, int green, int* __restrict__ cen)
+{
+    const int m2 = d * d;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < 4) cen[t] = -1;
+    if (t >= 4 * m2) return;
+    const int q = t / m2, e = t % m2;
+    mom[t] = cd{(green && q == 1 && e % d == e / d) ? 1.0 : 0.0, 0.0};
+}
+
+// final moment of the chosen window: blockIdx.y = moment index
+__global__ void k_mom_final(const cd* __restrict__ mt, int d, int range, int green, const int* __restrict__ cen_all,
+                            int Nv, cd* __restrict__ mom)
 {
     const int m2 = d * d;
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= m2) return;
+    const int mom_idx = blockIdx.y;
+    const int target = green ? mom_idx + 2 : mom_idx;
+    const int cen = cen_all[target];
+    cd* out = mom + (size_t)target * m2;
     const int al = e % d, be = e / d;
     cd s = cd{0, 0}, st = cd{0, 0};
     for (int w = cen - Nv; w <= cen + Nv; ++w) {
@@ -609,6 +681,7 @@ extern "C" int dmftgpu_moments(dmftgpu_handle h, int d, int n_freq, double beta,
     cd* T = cv.take<cd>((size_t)ncent * m2);
     cd* scr = cv.take<cd>((size_t)ncent * m2 * 3);
     double* wT = cv.take<double>((size_t)ncent * d);
+    int* cen = cv.take<int>(4);
     hipStream_t s = h->stream;
     if (!hit) {
         h->mom_key_valid = false;
@@ -632,47 +705,20 @@ extern "C" int dmftgpu_moments(dmftgpu_handle h, int d, int n_freq, double beta,
     hipLaunchKernelGGL(k_mom_posmat, dim3(nblk(ncent)), dim3(256), 0, s, avg, d, ncent, green_asym, w1, U1, T, scr);
     HIPCHK(hipGetLastError());
     HIPCHK(dg::launch_herm_eig(T, d, 0, d, nullptr, 0, wT, nullptr, ncent, s));
-    std::vector<double> htr((size_t)nm * range), hw1(green_asym ? 0 : (size_t)ncent * d), hwT((size_t)ncent * d);
-    HIPCHK(hipMemcpyAsync(htr.data(), trace, sizeof(double) * htr.size(), hipMemcpyDeviceToHost, s));
-    if (!green_asym) HIPCHK(hipMemcpyAsync(hw1.data(), w1, sizeof(double) * hw1.size(), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(hwT.data(), wT, sizeof(double) * hwT.size(), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    std::vector<char> ok(ncent);
-    for (int i = 0; i < ncent; ++i) {
-        if (!green_asym) ok[i] = hw1[(size_t)i * d] > 0 && hwT[(size_t)i * d] > 0;  // (:301-312)
-        else ok[i] = hwT[(size_t)i * d] > 1;                                         // (:432)
-    }
+    // window choice and final moments on the device: no host round trip
     cd* dm = (cd*)moments;
-    HIPCHK(hipMemsetAsync(dm, 0, sizeof(cd) * 4 * m2, s));
-    int win[4] = {-1, -1, -1, -1};
-    for (int mom = 0; mom < nm; ++mom) {
-        std::vector<double> var(ncent);
-        for (int c = Nv; c < range - Nv; ++c) {  // (:289-293)
-            const double* v = &htr[(size_t)mom * range + c - Nv];
-            double mean = 0.0;
-            for (int j = 0; j < 2 * Nv + 1; ++j) mean += v[j];
-            mean /= (2 * Nv + 1);
-            double ss = 0.0;
-            for (int j = 0; j < 2 * Nv + 1; ++j) ss += (v[j] - mean) * (v[j] - mean);
-            var[c - Nv] = std::sqrt(ss / (2 * Nv + 1));
-        }
-        int minPos = ncent - 1;  // (:295-315)
-        for (int i = 0; i < ncent; ++i)
-            if (ok[i] && var[i] < var[minPos]) minPos = i;
-        const int cen = minPos + Nv;
-        const int target = green_asym ? mom + 2 : mom;
-        win[target] = cen;
-        hipLaunchKernelGGL(k_mom_final, dim3(nblk(m2)), dim3(256), 0, s, mt, d, range, mom, cen, Nv,
-                           dm + (size_t)target * m2);
-        HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_mom_init, dim3(nblk((size_t)4 * m2)), dim3(256), 0, s, dm, d, green_asym, cen);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_mom_select, dim3(nm), dim3(256), 0, s, trace, w1, wT, d, range, ncent, Nv, green_asym, cen);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_mom_final, dim3(nblk(m2), nm), dim3(256), 0, s, mt, d, range, green_asym, cen, Nv, dm);
+    HIPCHK(hipGetLastError());
+    if (windows) {  // the only wait of the call
+        int win[4];
+        HIPCHK(hipMemcpyAsync(win, cen, sizeof(win), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        std::memcpy(windows, win, sizeof(win));
     }
-    if (green_asym) {  // m0 = 0, m1 = I (:451-452)
-        std::vector<cd> I1(m2, cd{0, 0});
-        for (int i = 0; i < d; ++i) I1[i + i * d] = cd{1.0, 0.0};
-        HIPCHK(hipMemcpyAsync(dm + m2, I1.data(), sizeof(cd) * m2, hipMemcpyHostToDevice, s));
-    }
-    HIPCHK(hipStreamSynchronize(s));
-    if (windows) std::memcpy(windows, win, sizeof(win));
     return DMFTGPU_OK;
 }
 

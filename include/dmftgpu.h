@@ -214,7 +214,10 @@ int dmftgpu_batch_destroy(dmftgpu_batch b);
 
 /* getAsymto_moments (FourierTransform.cpp:184-465): fw [n_freq][d][d] ->
  * moments [4][d][d].  green_asym != 0 selects the G-like branch (m0=0, m1=I).
- * windows (HOST, optional, 4 ints) receives the selected window centres. */
+ * windows (HOST, optional, 4 ints) receives the selected window centres
+ * (-1 for a moment the branch does not fit).  The window choice runs on the
+ * device: without windows the call only enqueues work on the handle's stream
+ * (moments is ready once the stream has passed it); with windows it waits. */
 int dmftgpu_moments(dmftgpu_handle h, int d, int n_freq, double beta, const double* fw, int green_asym,
                     double* moments, int* windows);
 
