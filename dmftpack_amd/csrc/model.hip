@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <vector>
 
@@ -108,6 +109,47 @@ __global__ void k_hk_from_hr(const int* __restrict__ start, const int4* __restri
     Hk[gid] = acc;
 }
 
+// The same sum with the phase taken apart per axis, exp(-i phi.R) =
+// ex[R1] ey[R2] ez[R3] with ea[r] = exp(-i phi_a r), |r| <= rmax, and a tile of
+// KT k points per workgroup: the tile's axis tables are formed once into LDS,
+// and each thread takes one matrix element for all KT k, so each hop is read
+// once per tile instead of once per k.
+template <int KT>
+__global__ __launch_bounds__(256) void k_hk_from_hr_axes(const int* __restrict__ start, const int4* __restrict__ hr,
+                                                         const cd* __restrict__ t, const double* __restrict__ kphase,
+                                                         int n, int nk, int rmax, cd* __restrict__ Hk)
+{
+    extern __shared__ cd tab[];  // [KT][3][2 rmax + 1]
+    const int k0 = blockIdx.y * KT;
+    const int nr = 2 * rmax + 1, per = 3 * nr;
+    for (int q = threadIdx.x; q < KT * per; q += blockDim.x) {
+        const int kk = q / per, ax = (q % per) / nr, r = q % nr - rmax;
+        const int k = min(k0 + kk, nk - 1);
+        double sn, cs;
+        sincos(kphase[3 * (size_t)k + ax] * r, &sn, &cs);
+        tab[q] = cd{cs, -sn};
+    }
+    __syncthreads();
+    const int nn = n * n;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nn) return;
+    cd acc[KT];
+#pragma unroll
+    for (int kk = 0; kk < KT; ++kk) acc[kk] = cd{0.0, 0.0};
+    for (int q = start[e]; q < start[e + 1]; ++q) {
+        const int4 r = hr[q];
+        const cd tq = t[q];
+#pragma unroll
+        for (int kk = 0; kk < KT; ++kk) {
+            const cd* p = tab + kk * per + rmax;
+            acc[kk] = cfma(tq, cmul(cmul(p[r.x], p[nr + r.y]), p[2 * nr + r.z]), acc[kk]);
+        }
+    }
+#pragma unroll
+    for (int kk = 0; kk < KT; ++kk)
+        if (k0 + kk < nk) Hk[(size_t)(k0 + kk) * nn + e] = acc[kk];
+}
+
 }  // namespace
 
 // Construct_hyb_delta (Construct_hyb_delta.cpp:47-119), off-diagonal form (:96-117)
@@ -154,6 +196,9 @@ extern "C" int dmftgpu_hk_from_hr(dmftgpu_handle h, int n, int n_hop, const int*
     }
     for (size_t e = 0; e < nn; ++e) start[e + 1] += start[e];
     std::vector<int> fill(start.begin(), start.end() - 1);
+    int rmax = 0;
+    for (int q = 0; q < n_hop; ++q)
+        for (int c = 0; c < 3; ++c) rmax = std::max(rmax, std::abs(hop_idx[5 * q + c]));
     std::vector<int> rq(4 * (size_t)(n_hop > 0 ? n_hop : 1), 0);
     std::vector<double> tq(2 * (size_t)(n_hop > 0 ? n_hop : 1), 0.0);
     for (int q = 0; q < n_hop; ++q) {
@@ -177,9 +222,17 @@ extern "C" int dmftgpu_hk_from_hr(dmftgpu_handle h, int n, int n_hop, const int*
     HIPCHK(hipMemcpyAsync(dstart, start.data(), (nn + 1) * sizeof(int), hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipMemcpyAsync(dr, rq.data(), rq.size() * sizeof(int), hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipMemcpyAsync(dt, tq.data(), b_t, hipMemcpyHostToDevice, h->stream));
-    const size_t total = nn * (size_t)n_k;
-    hipLaunchKernelGGL(k_hk_from_hr, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, h->stream, dstart, dr, dt,
-                       kphase, n, n_k, (cd*)Hk);
+    if (rmax <= 64 && (n_k + 7) / 8 <= 65535) {  // per-axis phase tables in LDS, 8 k points per workgroup
+        constexpr int KT = 8;
+        const unsigned wg = nn >= 256 ? 256u : (unsigned)((nn + 63) / 64 * 64);
+        const dim3 grid((unsigned)((nn + wg - 1) / wg), (unsigned)((n_k + KT - 1) / KT));
+        hipLaunchKernelGGL(k_hk_from_hr_axes<KT>, grid, dim3(wg), sizeof(cd) * KT * 3 * (2 * rmax + 1), h->stream,
+                           dstart, dr, dt, kphase, n, n_k, rmax, (cd*)Hk);
+    } else {
+        const size_t total = nn * (size_t)n_k;
+        hipLaunchKernelGGL(k_hk_from_hr, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, h->stream, dstart, dr, dt,
+                           kphase, n, n_k, (cd*)Hk);
+    }
     HIPCHK(hipGetLastError());
     // the host vectors die here: wait for the copies that read them
     HIPCHK(hipStreamSynchronize(h->stream));

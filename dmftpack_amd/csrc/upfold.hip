@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <vector>
 
@@ -236,6 +237,68 @@ __global__ __launch_bounds__(256) void k_dm_r(const cd* __restrict__ Y0, const c
     *d2 = cadd(*d2, cscale(a2, inv_nk));
 }
 
+// exp(+i phi_a r), |r| <= rmax, per k of the chunk: tab[k][a][r + rmax]
+__global__ void k_axis_phases(const double* __restrict__ kphase, int kc, int rmax, cd* __restrict__ tab)
+{
+    const int nr = 2 * rmax + 1;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)kc * 3 * nr) return;
+    const int k = (int)(t / (3 * nr)), q = (int)(t % (3 * nr));
+    const int ax = q / nr, r = q % nr - rmax;
+    double sn, cs;
+    sincos(kphase[3 * (size_t)k + ax] * r, &sn, &cs);
+    tab[t] = cd{cs, sn};
+}
+
+// k_dm_r with the phase taken apart per axis, exp(+i phi.R) = ex[R1] ey[R2] ez[R3]:
+// a tile of kt k values stages the three matrix elements and the k's axis
+// tables in LDS; two complex products per (R, k) instead of a sincos.
+__global__ __launch_bounds__(256) void k_dm_r_axes(const cd* __restrict__ Y0, const cd* __restrict__ Y1,
+                                                   const cd* __restrict__ Y2, const cd* __restrict__ tab, int rmax,
+                                                   int kc, int kt, size_t oo, const RItem* __restrict__ items,
+                                                   const int4* __restrict__ rs, int n_r, double inv_nk,
+                                                   cd* __restrict__ dmr)
+{
+    extern __shared__ cd sm[];
+    const int nr = 2 * rmax + 1, per = 3 * nr;
+    cd* y = sm;              // [3][kt]
+    cd* ph = sm + 3 * kt;    // [kt][3][nr]
+    const RItem it = items[blockIdx.x];
+    const int t = threadIdx.x;
+    const int r = it.r0 + t;
+    const bool live = r < it.r1;
+    const int4 R = rs[live ? r : it.r0];
+    const int ox = R.x + rmax, oy = nr + R.y + rmax, oz = 2 * nr + R.z + rmax;
+    cd a0 = cd{0.0, 0.0}, a1 = a0, a2 = a0;
+    for (int k0 = 0; k0 < kc; k0 += kt) {
+        const int kn = min(kt, kc - k0);
+        for (int q = t; q < kn; q += 256) {
+            const size_t o = (size_t)(k0 + q) * oo + it.e;
+            y[q] = Y0[o];
+            y[kt + q] = Y1[o];
+            y[2 * kt + q] = Y2[o];
+        }
+        for (int q = t; q < kn * per; q += 256) ph[q] = tab[(size_t)k0 * per + q];
+        __syncthreads();
+        for (int q = 0; q < kn; ++q) {
+            const cd* p = ph + q * per;
+            const cd z = cmul(cmul(p[ox], p[oy]), p[oz]);
+            a0 = cfma(y[q], z, a0);
+            a1 = cfma(y[kt + q], z, a1);
+            a2 = cfma(y[2 * kt + q], z, a2);
+        }
+        __syncthreads();
+    }
+    if (!live) return;
+    const int o = R.w;
+    cd* d0 = dmr + o;
+    cd* d1 = dmr + (size_t)n_r + o;
+    cd* d2 = dmr + 2 * (size_t)n_r + o;
+    *d0 = cadd(*d0, cscale(a0, inv_nk));
+    *d1 = cadd(*d1, cscale(a1, inv_nk));
+    *d2 = cadd(*d2, cscale(a2, inv_nk));
+}
+
 __global__ void k_zero(cd* __restrict__ p, size_t count)
 {
     const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -296,6 +359,12 @@ extern "C" int dmftgpu_upfold_density(dmftgpu_handle h, int n, int n_orb, int n_
         for (int a = q; a < q1; a += 256) items.push_back(RItem{(int)e, a, std::min(q1, a + 256), 0});
         q = q1;
     }
+    int rmax = 0;  // the R sums take the per-axis phase tables when |R_a| <= 16
+    for (int r = 0; r < n_r; ++r)
+        for (int c = 0; c < 3; ++c) rmax = std::max(rmax, std::abs(r_idx[5 * (size_t)r + c]));
+    const bool axes = n_r > 0 && rmax <= 16;
+    const int nrr = 2 * rmax + 1;
+    const int kt = std::max(16, std::min(256, (48 * 1024 / (int)sizeof(cd)) / (3 + 3 * nrr)));
     for (int q = 0; q < n_r; ++q) {
         const int r = order[q];
         rs[4 * (size_t)q] = r_idx[5 * (size_t)r];
@@ -303,8 +372,8 @@ extern "C" int dmftgpu_upfold_density(dmftgpu_handle h, int n, int n_orb, int n_
         rs[4 * (size_t)q + 2] = r_idx[5 * (size_t)r + 2];
         rs[4 * (size_t)q + 3] = r;
     }
-    // scratch: maps + per-chunk W1, W2, E (+ D) [oo] and T1, M [nn] per k
-    const size_t per_k = (4 * oo + 2 * nn) * sizeof(cd);
+    // scratch: maps + per-chunk W1, W2, E (+ D) [oo], T1, M [nn] and the axis phases per k
+    const size_t per_k = (4 * oo + 2 * nn + (axes ? 3 * (size_t)nrr : 0)) * sizeof(cd);
     size_t kc_cap = std::min<size_t>({(size_t)n_k, UPF_CHUNK_BYTES / per_k, 65535});
     if (h->upfold_chunk > 0) kc_cap = std::min<size_t>(kc_cap, (size_t)h->upfold_chunk);
     const int kc_max = (int)std::max<size_t>(1, kc_cap);
@@ -326,6 +395,7 @@ extern "C" int dmftgpu_upfold_density(dmftgpu_handle h, int n, int n_orb, int n_
     cd* Dw = Eb + oo * kc_max;
     cd* T1 = Dw + oo * kc_max;
     cd* Mb = T1 + nn * kc_max;
+    cd* Ptab = Mb + nn * kc_max;  // axes: [kc][3][2 rmax + 1]
     hipStream_t s = h->stream;
     HIPCHK(hipMemcpyAsync(d_inv, inv.data(), inv.size() * sizeof(int), hipMemcpyHostToDevice, s));
     if (n_r > 0) {
@@ -369,7 +439,16 @@ extern "C" int dmftgpu_upfold_density(dmftgpu_handle h, int n, int n_orb, int n_
             DDir = W2;
             DD = Eb;
         }
-        if (n_r > 0) {
+        if (axes) {
+            const size_t np = (size_t)kc * 3 * nrr;
+            hipLaunchKernelGGL(k_axis_phases, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s,
+                               kphase + 3 * (size_t)k0, kc, rmax, Ptab);
+            HIPCHK(hipGetLastError());
+            const size_t shm = sizeof(cd) * ((size_t)3 * kt + (size_t)kt * 3 * nrr);
+            hipLaunchKernelGGL(k_dm_r_axes, dim3((unsigned)items.size()), dim3(256), shm, s, DD, DDir, (const cd*)D,
+                               (const cd*)Ptab, rmax, kc, kt, oo, d_items, d_rs, n_r, inv_nk, (cd*)dmr);
+            HIPCHK(hipGetLastError());
+        } else if (n_r > 0) {
             hipLaunchKernelGGL(k_dm_r, dim3((unsigned)items.size()), dim3(256), 0, s, DD, DDir, (const cd*)D,
                                kphase + 3 * (size_t)k0, kc, oo, d_items, d_rs, n_r, inv_nk, (cd*)dmr);
             HIPCHK(hipGetLastError());
