@@ -535,6 +535,80 @@ __global__ void k_antiherm(const cd* __restrict__ f, int d, int count, cd* __res
     out[t] = cd{0.5 * dlt.y, -0.5 * dlt.x};  // dlt / (2i)
 }
 
+// Register-tiled form for m2 <= 256 (used for tau -> iw, two inputs per i):
+// thread (jg, e), jg < jgroups, keeps output
+// element e of the rj consecutive rows jg*rj .. jg*rj + rj - 1, so one read of
+// in[i][e] from LDS serves rj complex FMAs and the coefficient reads are the
+// same address across the lanes of a row group (broadcast).  Tiles, i ranges
+// and the order of the sum over i are those of k_dft.
+template <class Coef, int NIN, int RJ>
+__global__ __launch_bounds__(256) void k_dft_rt(int nj, int ni, int m2, int TJ, int TI, int ichunk, int jgroups,
+                                                int rj, const cd* __restrict__ in0, const cd* __restrict__ in1,
+                                                cd* __restrict__ out, Coef coef)
+{
+    extern __shared__ cd sm[];
+    cd* sc = sm;                        // [NIN][TJ][TI]
+    cd* sx = sm + NIN * TJ * TI;        // [NIN][TI][m2]
+    cd* sp = sx + NIN * TI * m2;        // PHASE_ROW: [TJ][TI + 1]
+    const int j0 = blockIdx.x * TJ;
+    const int ib = blockIdx.y * ichunk;
+    const int ie = min(ni, ib + ichunk);
+    out += (size_t)blockIdx.y * nj * m2;
+    const int e = threadIdx.x % m2, jg = threadIdx.x / m2;
+    const bool act = jg < jgroups;
+    cd acc[RJ];
+#pragma unroll
+    for (int r = 0; r < RJ; ++r) acc[r] = cd{0, 0};
+    for (int i0 = ib; i0 < ie; i0 += TI) {
+        __syncthreads();
+        if constexpr (Coef::PHASE_ROW) {
+            for (int t = threadIdx.x; t < TJ * (TI + 1); t += 256) {
+                const int jj = t / (TI + 1), ii = t % (TI + 1);
+                sp[t] = (j0 + jj < nj && i0 + ii <= ie) ? coef.phase(j0 + jj, i0 + ii) : cd{0, 0};
+            }
+            __syncthreads();
+        }
+        for (int t = threadIdx.x; t < TJ * TI; t += 256) {
+            const int jj = t / TI, ii = t % TI;
+            cd c[2] = {cd{0, 0}, cd{0, 0}};
+            if (j0 + jj < nj && i0 + ii < ie) {
+                if constexpr (Coef::PHASE_ROW)
+                    coef.from_phases(j0 + jj, i0 + ii, sp[jj * (TI + 1) + ii], sp[jj * (TI + 1) + ii + 1], c);
+                else
+                    coef(j0 + jj, i0 + ii, c);
+            }
+            for (int q = 0; q < NIN; ++q) sc[(q * TJ + jj) * TI + ii] = c[q];
+        }
+        for (int t = threadIdx.x; t < TI * m2; t += 256) {
+            const int ii = t / m2, ee = t % m2;
+            const bool ok = i0 + ii < ie;
+            sx[ii * m2 + ee] = ok ? in0[(size_t)(i0 + ii) * m2 + ee] : cd{0, 0};
+            if (NIN == 2) sx[(TI + ii) * m2 + ee] = ok ? in1[(size_t)(i0 + ii) * m2 + ee] : cd{0, 0};
+        }
+        __syncthreads();
+        if (act) {
+            const int jr = jg * rj;
+            for (int ii = 0; ii < TI; ++ii) {
+                const cd x0 = sx[ii * m2 + e];
+                const cd x1 = NIN == 2 ? sx[(TI + ii) * m2 + e] : cd{0, 0};
+#pragma unroll
+                for (int r = 0; r < RJ; ++r) {
+                    if (r < rj) {
+                        acc[r] = cfma(sc[(jr + r) * TI + ii], x0, acc[r]);
+                        if (NIN == 2) acc[r] = cfma(sc[(TJ + jr + r) * TI + ii], x1, acc[r]);
+                    }
+                }
+            }
+        }
+    }
+    if (!act) return;
+#pragma unroll
+    for (int r = 0; r < RJ; ++r) {
+        const int jj = jg * rj + r;
+        if (r < rj && j0 + jj < nj) out[(size_t)(j0 + jj) * m2 + e] = acc[r];
+    }
+}
+
 // parts[y][j][e] -> out[j][e], y ascending
 __global__ void k_dft_sum_parts(const cd* __restrict__ parts, int nparts, size_t count, cd* __restrict__ out)
 {
@@ -546,21 +620,33 @@ __global__ void k_dft_sum_parts(const cd* __restrict__ parts, int nparts, size_t
 }
 
 struct DftShape {
-    int TJ, TI, ks, ichunk;
+    int TJ, TI, ks, ichunk, jgroups, rj;  // jgroups > 0: the register-tiled kernel
 };
 
 // Tiles of the transform and the split of the summed index i over
 // workgroups: enough workgroups to fill the device (about 1024) when the
 // output alone gives fewer.
-inline DftShape dft_shape(int nj, int ni, int m2, int nin)
+inline DftShape dft_shape(int nj, int ni, int m2, int nin, bool phase_row)
 {
     DftShape d;
-    d.TJ = 2048 / m2;
-    if (d.TJ > 32) d.TJ = 32;
-    if (d.TJ < 1) d.TJ = 1;
-    d.TI = 2048 / (m2 * nin);
-    if (d.TI > 32) d.TI = 32;
-    if (d.TI < 1) d.TI = 1;
+    if (m2 <= 256 && nin == 2) {  // register-tiled (tau -> iw; measured slower for iw -> tau): TJ = jgroups * rj rows
+        d.jgroups = 256 / m2;
+        d.rj = 256 / d.jgroups;
+        if (d.rj > 8) d.rj = 8;
+        d.TJ = d.jgroups * d.rj;
+        d.TI = 3072 / (nin * (d.TJ + m2) + (phase_row ? d.TJ + 1 : 0));
+        if (d.TI > 32) d.TI = 32;
+        if (d.TI < 1) d.TI = 1;
+    } else {
+        d.jgroups = 0;
+        d.rj = 0;
+        d.TJ = 2048 / m2;
+        if (d.TJ > 32) d.TJ = 32;
+        if (d.TJ < 1) d.TJ = 1;
+        d.TI = 2048 / (m2 * nin);
+        if (d.TI > 32) d.TI = 32;
+        if (d.TI < 1) d.TI = 1;
+    }
     const int gx = (nj + d.TJ - 1) / d.TJ;
     const int itiles = (ni + d.TI - 1) / d.TI;
     int ks = (1024 + gx - 1) / gx;
@@ -574,9 +660,9 @@ inline DftShape dft_shape(int nj, int ni, int m2, int nin)
 }
 
 // scratch (in cd) the split transform needs for its partial sums
-inline size_t dft_part_count(int nj, int ni, int m2, int nin)
+inline size_t dft_part_count(int nj, int ni, int m2, int nin, bool phase_row)
 {
-    const DftShape d = dft_shape(nj, ni, m2, nin);
+    const DftShape d = dft_shape(nj, ni, m2, nin, phase_row);
     return d.ks > 1 ? (size_t)d.ks * nj * m2 : 0;
 }
 
@@ -584,7 +670,7 @@ template <class Coef, int NIN>
 hipError_t launch_dft(int nj, int ni, int m2, const cd* in0, const cd* in1, cd* out, Coef coef, cd* parts,
                       hipStream_t s)
 {
-    const DftShape d = dft_shape(nj, ni, m2, NIN);
+    const DftShape d = dft_shape(nj, ni, m2, NIN, Coef::PHASE_ROW);
     const int TJ = d.TJ, TI = d.TI;
     const int opt = (TJ * m2 + 255) / 256;
     size_t shm = sizeof(cd) * ((size_t)NIN * TJ * TI + (size_t)NIN * TI * m2 +
@@ -592,6 +678,18 @@ hipError_t launch_dft(int nj, int ni, int m2, const cd* in0, const cd* in1, cd* 
     dim3 grid((nj + TJ - 1) / TJ, d.ks);
     cd* dst = d.ks > 1 ? parts : out;
     if (d.ks > 1 && !parts) return hipErrorInvalidValue;
+    if (d.jgroups > 0) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_dft_rt<Coef, NIN, 8>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((k_dft_rt<Coef, NIN, 8>), grid, dim3(256), shm, s, nj, ni, m2, TJ, TI, d.ichunk, d.jgroups,
+                           d.rj, in0, in1, dst, coef);
+        e = hipGetLastError();
+        if (e != hipSuccess || d.ks == 1) return e;
+        const size_t cnt = (size_t)nj * m2;
+        hipLaunchKernelGGL(k_dft_sum_parts, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, parts, d.ks, cnt, out);
+        return hipGetLastError();
+    }
 #define L_(O)                                                                                                     \
     {                                                                                                             \
         hipError_t e = hipFuncSetAttribute((const void*)k_dft<Coef, NIN, O>, hipFuncAttributeMaxDynamicSharedMemorySize, \
@@ -770,7 +868,7 @@ extern "C" int dmftgpu_ft_w2tau(dmftgpu_handle h, int d, int n_freq, int n_tau, 
     std::vector<hl::cx> A1, A2;
     std::vector<double> e1, e2;
     tail_setup(d, m1, mm2, m3, A1, e1, A2, e2);
-    const size_t npart = dft_part_count(ntau1, n_freq, m2, 1);
+    const size_t npart = dft_part_count(ntau1, n_freq, m2, 1, CoefW2Tau::PHASE_ROW);
     size_t need = (size_t)n_freq * m2 * 16 + (size_t)ntau1 * m2 * 16 + 4 * (size_t)m2 * 16 + 4 * (size_t)d * 8 +
                   (size_t)ntau1 * d * 8 + npart * 16 + 16 * 256;
     int rc = 0;
@@ -851,7 +949,7 @@ extern "C" int dmftgpu_ft_tau2w(dmftgpu_handle h, int d, int n_freq, int n_tau, 
     const int m2 = d * d;
     const double dt = beta / n_tau;
     hipStream_t s = h->stream;
-    const size_t npart = dft_part_count(n_freq, n_tau, m2, 2);
+    const size_t npart = dft_part_count(n_freq, n_tau, m2, 2, CoefTau2W::PHASE_ROW);
     size_t need = 2 * (size_t)n_tau * m2 * 16 + (size_t)n_freq * m2 * 16 + (size_t)n_freq * d * 8 + npart * 16 + 8 * 256;
     int rc = 0;
     char* base = ft_ws(h, need, &rc);
@@ -900,7 +998,7 @@ extern "C" int dmftgpu_ft_w2tau_points(dmftgpu_handle h, int d, int n_freq, doub
     std::vector<hl::cx> A1, A2;
     std::vector<double> e1, e2;
     tail_setup(d, m1, mm2, m3, A1, e1, A2, e2);
-    const size_t npart = dft_part_count(n_points, n_freq, m2, 1);
+    const size_t npart = dft_part_count(n_points, n_freq, m2, 1, CoefW2Tau::PHASE_ROW);
     size_t need = (size_t)n_freq * m2 * 16 + (size_t)n_points * m2 * 16 + 4 * (size_t)m2 * 16 + 4 * (size_t)d * 8 +
                   (size_t)n_points * 8 + npart * 16 + 16 * 256;
     int rc = 0;
