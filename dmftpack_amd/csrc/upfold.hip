@@ -334,6 +334,7 @@ extern "C" int dmftgpu_upfold_density(dmftgpu_handle h, int n, int n_orb, int n_
     for (int k = 0; k < n_k; ++k)
         for (int i = 0; i < n; ++i) {
             const int b = val_to_ks[(size_t)k * n + i];
+            if (b == -1) continue;  // a padded band (NBAND[k] < n): no KS band
             if (b < 0 || b >= n_orb) return fail(h, DMFTGPU_ERR_ARG, "val_to_ks[%d][%d]=%d outside 0..%d", k, i, b, n_orb - 1);
             if (inv[(size_t)k * n_orb + b] >= 0)
                 return fail(h, DMFTGPU_ERR_ARG, "val_to_ks[%d] names band %d twice", k, b);

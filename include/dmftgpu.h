@@ -281,7 +281,9 @@ int dmftgpu_hk_from_hr(dmftgpu_handle h, int n, int n_hop, const int* hop_idx, c
  *   dm_model   [n_k][n][n]          densityMatDFT in the model basis
  *   corr_base  [n_k][n][n]          DF_CorrBase
  *   ks_energy  [n_k][n_orb]         KS_eigenEnergy (double)
- *   val_to_ks  [n_k][n]   HOST int  FromValToKS, distinct bands per k
+ *   val_to_ks  [n_k][n]   HOST int  FromValToKS, distinct bands per k; -1 marks a padded
+ *                                   band (NBAND[k] < n), whose rows and columns of
+ *                                   dm_model must be zero
  *   transform  [n_k][n_orb][n_orb]  transformMatrix_k
  *   ks_ortho   [n_k][n_orb][n_orb]  KS_eigenVectors_orthoBasis
  *   overlap    [n_k][n_orb][n_orb]  S_overlap, or NULL for S = I

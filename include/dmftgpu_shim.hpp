@@ -22,6 +22,7 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
 #include <complex>
 #include <cstdio>
 #include <cstdlib>
@@ -83,6 +84,35 @@ std::vector<cplx> pack(const std::vector<Matrix>& mats, size_t count, int rows)
     return out;
 }
 
+// NBAND[k] may differ between k points (the energy window, downfolding.cpp:162-175).
+// H(k) goes to the library at the largest NBAND: each smaller H(k) is padded with
+// decoupled bands at kPadEnergy on the diagonal.  A decoupled band has G = G0
+// at every frequency and a Fermi factor of exactly 0 (beta (E - mu) > 100), so
+// G_loc, NumMatrix, N and the model block of densityMatDFT are those of the
+// unpadded H(k).
+constexpr double kPadEnergy = 1.0e4;
+
+template <class Matrix>
+int max_rows(const std::vector<Matrix>& mats, size_t count)
+{
+    int n = 0;
+    for (size_t m = 0; m < count; ++m) n = std::max(n, (int)mats[m].rows());
+    return n;
+}
+
+template <class Matrix>
+std::vector<cplx> pack_hk(const std::vector<Matrix>& H_k, size_t count, int n)
+{
+    std::vector<cplx> out(count * n * n, cplx(0.0, 0.0));
+    for (size_t k = 0; k < count; ++k) {
+        const int nb = (int)H_k[k].rows();
+        cplx* dst = &out[k * n * n];
+        for (int j = 0; j < nb; ++j) std::memcpy(dst + (size_t)j * n, H_k[k].data() + (size_t)j * nb, sizeof(cplx) * nb);
+        for (int i = nb; i < n; ++i) dst[i + (size_t)i * n] = kPadEnergy;
+    }
+    return out;
+}
+
 template <class SelfEnergy>
 std::vector<cplx> pack_sigma(SelfEnergy& S, int n_freq, int nc)
 {
@@ -139,10 +169,10 @@ void GreenFtn_w(dmftgpu_handle h, const Globals& g, int NumCluster, int NumHartr
                 std::vector<Matrix>& densityMatDFT, Matrix& num_matrix)
 {
     const int nc = NumCluster * NumHartrOrbit_per_cluster;
-    const int n = (int)H_k[0].rows();
+    const int n = max_rows(H_k, g.knum);  // NBAND[k] may vary: padded to the largest
     const dmftgpu_lattice L = lattice_of(g, n, nc, NumCluster);
     DevBuf hk((size_t)g.knum * n * n), sig((size_t)g.N_freq * nc * nc), gl((size_t)g.N_freq * nc * nc), nm((size_t)nc * nc);
-    hk.upload(pack(H_k, g.knum, n).data());
+    hk.upload(pack_hk(H_k, g.knum, n).data());
     sig.upload(pack_sigma(SelfE_w, g.N_freq, nc).data());
     std::vector<cplx> mom = moments_dev(h, nc, g.N_freq, g.beta, sig, false);  // :189-195
     DevBuf dmom(mom.size());
@@ -169,9 +199,12 @@ void GreenFtn_w(dmftgpu_handle h, const Globals& g, int NumCluster, int NumHartr
         die(h, dmftgpu_density_matrix(h, (const double*)sig.p, mu, (double*)dm.p), "densityMatDFT");
         std::vector<cplx> hdm(dm.n);
         dm.download(hdm.data());
-        for (int k = 0; k < g.knum; ++k) {
-            densityMatDFT[k].setZero(n, n);
-            std::memcpy(densityMatDFT[k].data(), &hdm[(size_t)k * n * n], sizeof(cplx) * n * n);
+        for (int k = 0; k < g.knum; ++k) {  // the NBAND[k] x NBAND[k] block of each k
+            const int nb = (int)H_k[k].rows();
+            densityMatDFT[k].setZero(nb, nb);
+            for (int j = 0; j < nb; ++j)
+                std::memcpy(densityMatDFT[k].data() + (size_t)j * nb, &hdm[(size_t)k * n * n + (size_t)j * n],
+                            sizeof(cplx) * nb);
         }
     }
 }
@@ -185,10 +218,10 @@ double GreenFtn_w_adjust_mu(dmftgpu_handle h, const Globals& g, int NumCluster, 
                             int /*mu_adjust*/)
 {
     const int nc = NumCluster * NumHartrOrbit_per_cluster;
-    const int n = (int)H_k[0].rows();
+    const int n = max_rows(H_k, g.knum);  // NBAND[k] may vary: padded to the largest
     dmftgpu_lattice L = lattice_of(g, n, nc, NumCluster);
     DevBuf hk((size_t)g.knum * n * n), sig((size_t)g.N_freq * nc * nc);
-    hk.upload(pack(H_k, g.knum, n).data());
+    hk.upload(pack_hk(H_k, g.knum, n).data());
     std::vector<cplx> hs = pack_sigma(SelfE_w, g.N_freq, nc);
     sig.upload(hs.data());
     std::vector<cplx> mom = moments_dev(h, nc, g.N_freq, g.beta, sig, false);  // GreenFtn.cpp:523
@@ -451,13 +484,28 @@ void upfolding_density(dmftgpu_handle h, const Globals& g, std::vector<Matrix>& 
                        std::vector<cplx> (&dmr)[3])
 {
     const int nk = g.knum;
-    const int n = (int)densityMatDFT[0].rows();
+    // NBAND[k] may vary: the model blocks are padded to the largest, the padded
+    // bands marked -1 in the band map (zero density, identity basis block)
+    const int n = max_rows(densityMatDFT, nk);
     const int no = (int)KS_eigenVectors_orthoBasis[0].rows();
     const int nr = (int)H_Rindex.rows();
     DevBuf X((size_t)nk * n * n), Cb((size_t)nk * n * n), T((size_t)nk * no * no), Ko((size_t)nk * no * no),
         S(S_overlap.empty() ? 1 : (size_t)nk * no * no), D((size_t)nk * no * no), R(3 * (size_t)(nr > 0 ? nr : 1));
-    X.upload(pack(densityMatDFT, nk, n).data());
-    Cb.upload(pack(DF_CorrBase, nk, n).data());
+    {
+        std::vector<cplx> hx((size_t)nk * n * n, cplx(0.0, 0.0)), hc((size_t)nk * n * n, cplx(0.0, 0.0));
+        for (int k = 0; k < nk; ++k) {
+            const int nb = (int)densityMatDFT[k].rows();
+            for (int j = 0; j < nb; ++j) {
+                std::memcpy(&hx[(size_t)k * n * n + (size_t)j * n], densityMatDFT[k].data() + (size_t)j * nb,
+                            sizeof(cplx) * nb);
+                std::memcpy(&hc[(size_t)k * n * n + (size_t)j * n], DF_CorrBase[k].data() + (size_t)j * nb,
+                            sizeof(cplx) * nb);
+            }
+            for (int i = nb; i < n; ++i) hc[(size_t)k * n * n + i + (size_t)i * n] = 1.0;
+        }
+        X.upload(hx.data());
+        Cb.upload(hc.data());
+    }
     T.upload(pack(transformMatrix_k, nk, no).data());
     Ko.upload(pack(KS_eigenVectors_orthoBasis, nk, no).data());
     if (!S_overlap.empty()) S.upload(pack(S_overlap, nk, no).data());
@@ -465,7 +513,8 @@ void upfolding_density(dmftgpu_handle h, const Globals& g, std::vector<Matrix>& 
     std::vector<int> v((size_t)nk * n), ridx(5 * (size_t)(nr > 0 ? nr : 1));
     for (int k = 0; k < nk; ++k) {
         for (int b = 0; b < no; ++b) eps[(size_t)k * no + b] = KS_eigenEnergy[k][b];
-        for (int i = 0; i < n; ++i) v[(size_t)k * n + i] = FromValToKS[k][i];
+        const int nb = (int)densityMatDFT[k].rows();
+        for (int i = 0; i < n; ++i) v[(size_t)k * n + i] = i < nb ? FromValToKS[k][i] : -1;
     }
     for (int r = 0; r < nr; ++r)
         for (int c = 0; c < 5; ++c) ridx[5 * (size_t)r + c] = H_Rindex(r, c);

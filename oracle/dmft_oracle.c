@@ -1127,7 +1127,10 @@ int orc_upfold_density(int n, int n_orb, int n_k, int n_k_global, double beta, d
         for (size_t e = 0; e < oo; ++e) Km[e] = 0.0;
         for (int l = 0; l < n_orb; ++l) Km[IDX(l, l, n_orb)] = fermi_clamped(beta, eps[(size_t)k * n_orb + l], mu);
         for (int j = 0; j < n; ++j)
-            for (int i = 0; i < n; ++i) Km[IDX(v[(size_t)k * n + i], v[(size_t)k * n + j], n_orb)] = M[IDX(i, j, n)];
+            for (int i = 0; i < n; ++i) {
+                const int vi = v[(size_t)k * n + i], vj = v[(size_t)k * n + j];
+                if (vi >= 0 && vj >= 0) Km[IDX(vi, vj, n_orb)] = M[IDX(i, j, n)];  /* -1: padded band */
+            }
         /* D = E K E^H */
         matmul(n_orb, E, Km, a);
         adjoint(n_orb, E, b);

@@ -132,6 +132,46 @@ int main()
     orc_asymto_moments(nc, nf, beta, sg.data(), 0, om.data(), nullptr);
     check("getAsymto_moments", relerr(mm.data(), om.data(), om.size()), 1e-8);
 
+    // NBAND[k] varying with k: the first half of the k points keep 5 of the 6 bands
+    // (the last band and its couplings dropped); the adapter pads them, and the
+    // oracle runs each band count on its own k points with the global k count
+    {
+        std::vector<Mat> Hr(nk);
+        for (int k = 0; k < nk; ++k) {
+            const int nb = k < nk / 2 ? n - 1 : n;
+            Hr[k].setZero(nb, nb);
+            for (int j = 0; j < nb; ++j)
+                for (int i = 0; i < nb; ++i) Hr[k](i, j) = Hk[k](i, j);
+        }
+        std::vector<Mat> Gr, densr(nk);
+        Mat numr;
+        dmftgpu_shim::GreenFtn_w(h, g, ncl, npc, Hr, S, Gr, mu, densr, numr);
+        std::vector<cplx> glr((size_t)nf * nc * nc, 0.0), nmr((size_t)nc * nc, 0.0);
+        double dens_err = 0.0, dens_scale = 0.0;
+        for (int part = 0; part < 2; ++part) {
+            const int nb = part == 0 ? n - 1 : n, k0 = part == 0 ? 0 : nk / 2, kn = nk / 2;
+            std::vector<cplx> hh((size_t)kn * nb * nb), g1(glr.size()), h1(glr.size()), n1(nmr.size()),
+                d1((size_t)kn * nb * nb);
+            for (int k = 0; k < kn; ++k) std::copy(Hr[k0 + k].v.begin(), Hr[k0 + k].v.end(), hh.begin() + (size_t)k * nb * nb);
+            orc_lattice Lp{nb, nc, ncl, kn, nk, nf, 2, beta};
+            orc_greenftn_w(&Lp, hh.data(), sg.data(), mm.data(), mu, g1.data(), h1.data(), n1.data(), d1.data(), 1);
+            for (size_t e = 0; e < glr.size(); ++e) glr[e] += g1[e];
+            for (size_t e = 0; e < nmr.size(); ++e) nmr[e] += n1[e];
+            for (int k = 0; k < kn; ++k) {
+                if (densr[k0 + k].rows() != nb) dens_err = 1.0;
+                for (int e = 0; e < nb * nb; ++e) {
+                    dens_err = std::fmax(dens_err, std::abs(densr[k0 + k].v[e] - d1[(size_t)k * nb * nb + e]));
+                    dens_scale = std::fmax(dens_scale, std::abs(d1[(size_t)k * nb * nb + e]));
+                }
+            }
+        }
+        std::vector<cplx> gotr((size_t)nf * nc * nc);
+        for (int w = 0; w < nf; ++w) std::copy(Gr[w].v.begin(), Gr[w].v.end(), gotr.begin() + (size_t)w * nc * nc);
+        check("ragged NBAND Gw", relerr(gotr.data(), glr.data(), glr.size()), 1e-12);
+        check("ragged NBAND NumMatrix", relerr(numr.v.data(), nmr.data(), nmr.size()), 1e-12);
+        check("ragged NBAND densityMatDFT", dens_err / (dens_scale > 0 ? dens_scale : 1.0), 1e-12);
+    }
+
     // mu search: N(mu) = NumberOfElectron
     double mu_new = dmftgpu_shim::GreenFtn_w_adjust_mu(h, g, ncl, npc, Hk, S, 0.0, 1.0, 1);
     std::printf("adjusted mu %.10f\n", mu_new);
@@ -280,6 +320,46 @@ int main()
         for (int x = 0; x < 3; ++x) std::copy(dmr[x].begin(), dmr[x].end(), gr.begin() + (size_t)x * nr);
         check("upfolding_density D_k", relerr(gd.data(), od.data(), od.size()) + (dm[0].rows() == no ? 0 : 1), 1e-13);
         check("upfolding_density DM(R)", relerr(gr.data(), odmr.data(), odmr.size()), 1e-13);
+
+        // NBAND[k] varying: k = 0, 1 keep 2 of the 3 model bands; the oracle runs
+        // each band count on its own k points with the global k count
+        std::vector<Mat> dmr2(nk), cbr(nk);
+        std::vector<std::vector<int>> vr(nk);
+        for (int k = 0; k < nk; ++k) {
+            const int nb = k < 2 ? n - 1 : n;
+            dmr2[k].setZero(nb, nb);
+            cbr[k].setZero(nb, nb);
+            for (int j = 0; j < nb; ++j)
+                for (int i = 0; i < nb; ++i) {
+                    dmr2[k](i, j) = X[(size_t)k * n * n + i + (size_t)j * n];
+                    cbr[k](i, j) = C[(size_t)k * n * n + i + (size_t)j * n];
+                }
+            vr[k].assign(v2k[k].begin(), v2k[k].begin() + nb);
+        }
+        const std::vector<Mat> dmr2_in = dmr2;  // the adapter overwrites densityMatDFT in place (DFT basis)
+        std::vector<cplx> rdmr[3];
+        dmftgpu_shim::upfolding_density(h, gu, dmr2, ko, hri, mu, so, tr, cbr, eps, vr, kp, rdmr);
+        std::vector<cplx> osum(3 * (size_t)nr, 0.0), od2((size_t)nk * no * no);
+        for (int part = 0; part < 2; ++part) {
+            const int nb = part == 0 ? n - 1 : n, k0 = 2 * part, kn = 2;
+            std::vector<cplx> Xp((size_t)kn * nb * nb), Cp((size_t)kn * nb * nb), part_dmr(3 * (size_t)nr);
+            std::vector<int> vp((size_t)kn * nb);
+            for (int k = 0; k < kn; ++k) {
+                std::copy(dmr2_in[k0 + k].v.begin(), dmr2_in[k0 + k].v.end(), Xp.begin() + (size_t)k * nb * nb);
+                std::copy(cbr[k0 + k].v.begin(), cbr[k0 + k].v.end(), Cp.begin() + (size_t)k * nb * nb);
+                for (int i = 0; i < nb; ++i) vp[(size_t)k * nb + i] = vr[k0 + k][i];
+            }
+            orc_upfold_density(nb, no, kn, nk, gu.beta, mu, Xp.data(), Cp.data(), e.data() + (size_t)k0 * no, vp.data(),
+                               T.data() + (size_t)k0 * no * no, K.data() + (size_t)k0 * no * no,
+                               S.data() + (size_t)k0 * no * no, nr, hri.v.data(), kp.data() + 3 * (size_t)k0,
+                               od2.data() + (size_t)k0 * no * no, part_dmr.data());
+            for (size_t q = 0; q < osum.size(); ++q) osum[q] += part_dmr[q];
+        }
+        std::vector<cplx> gd2((size_t)nk * no * no), gr2(3 * (size_t)nr);
+        for (int k = 0; k < nk; ++k) std::copy(dmr2[k].v.begin(), dmr2[k].v.end(), gd2.begin() + (size_t)k * no * no);
+        for (int x = 0; x < 3; ++x) std::copy(rdmr[x].begin(), rdmr[x].end(), gr2.begin() + (size_t)x * nr);
+        check("ragged upfolding D_k", relerr(gd2.data(), od2.data(), od2.size()), 1e-13);
+        check("ragged upfolding DM(R)", relerr(gr2.data(), osum.data(), osum.size()), 1e-13);
     }
 
     dmftgpu_destroy(h);
