@@ -95,15 +95,24 @@ hipError_t zgemm_sq(int N, int batch, const cd* A, const cd* B, cd* C, hipStream
 // largest |A(i, j)|, i >= j, the first of equals; the row exchanges are undone
 // on the columns at the end.  One workgroup per matrix; the matrix stays in
 // global memory (the workgroup's barriers order its own loads and stores).
-// A zero pivot sets *singular.
+// A zero pivot sets *singular.  IN_LDS: the matrix is staged in LDS for the
+// elimination (N <= 64), the same steps in the same order.
+template <bool IN_LDS>
 __global__ __launch_bounds__(256) void k_gj_inverse(int N, cd* A, int* singular)
 {
     __shared__ cd col[UPF_MAX_ORB];
     __shared__ int piv[UPF_MAX_ORB];
     __shared__ double rmax[256];
     __shared__ int ridx[256];
+    extern __shared__ cd mlds[];
     const int t = threadIdx.x;
-    cd* M = A + (size_t)blockIdx.x * N * N;
+    cd* G = A + (size_t)blockIdx.x * N * N;
+    cd* M = G;
+    if constexpr (IN_LDS) {
+        for (int e = t; e < N * N; e += 256) mlds[e] = G[e];
+        M = mlds;
+        __syncthreads();
+    }
     for (int j = 0; j < N; ++j) {
         double best = -1.0;
         int bi = j;
@@ -160,6 +169,21 @@ __global__ __launch_bounds__(256) void k_gj_inverse(int N, cd* A, int* singular)
             }
         __syncthreads();
     }
+    if constexpr (IN_LDS)
+        for (int e = t; e < N * N; e += 256) G[e] = mlds[e];
+}
+
+hipError_t gj_inverse(int N, int batch, cd* A, int* singular, hipStream_t s)
+{
+    if (N <= 64) {  // up to 64 KiB of matrix next to the 13 KiB of static arrays
+        const size_t shm = sizeof(cd) * (size_t)N * N;
+        const hipError_t e = hipFuncSetAttribute((const void*)k_gj_inverse<true>,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_gj_inverse<true>, dim3((unsigned)batch), dim3(256), shm, s, N, A, singular);
+    } else
+        hipLaunchKernelGGL(k_gj_inverse<false>, dim3((unsigned)batch), dim3(256), 0, s, N, A, singular);
+    return hipGetLastError();
 }
 
 // K(a, b) of upfolding_k in the KS basis: the model block M(inv_a, inv_b)
@@ -417,8 +441,7 @@ extern "C" int dmftgpu_upfold_density(dmftgpu_handle h, int n, int n_orb, int n_
         cd* D = dm_k ? (cd*)dm_k + ko : Dw;
         // E = (T^{-1})^H K_o
         HIPCHK(hipMemcpyAsync(W1, Tm + ko, oo * kc * sizeof(cd), hipMemcpyDeviceToDevice, s));
-        hipLaunchKernelGGL(k_gj_inverse, dim3((unsigned)kc), dim3(256), 0, s, n_orb, W1, d_flag);
-        HIPCHK(hipGetLastError());
+        HIPCHK(gj_inverse(n_orb, kc, W1, d_flag, s));
         HIPCHK((zgemm_sq<true, false>(n_orb, kc, W1, Ko + ko, Eb, s)));
         // M = C^H (X C)
         HIPCHK((zgemm_sq<false, false>(n, kc, X + kn, Cb + kn, T1, s)));
@@ -433,8 +456,7 @@ extern "C" int dmftgpu_upfold_density(dmftgpu_handle h, int n, int n_orb, int n_
         const cd *DD = D, *DDir = D;
         if (Sm) {  // DDir = S^{-1} D (W2), DD = DDir S^{-1} (Eb)
             HIPCHK(hipMemcpyAsync(W1, Sm + ko, oo * kc * sizeof(cd), hipMemcpyDeviceToDevice, s));
-            hipLaunchKernelGGL(k_gj_inverse, dim3((unsigned)kc), dim3(256), 0, s, n_orb, W1, d_flag);
-            HIPCHK(hipGetLastError());
+            HIPCHK(gj_inverse(n_orb, kc, W1, d_flag, s));
             HIPCHK((zgemm_sq<false, false>(n_orb, kc, W1, D, W2, s)));
             HIPCHK((zgemm_sq<false, false>(n_orb, kc, W2, W1, Eb, s)));
             DDir = W2;
