@@ -37,7 +37,7 @@ struct dmftgpu_handle_s {
     dg::cd* Vd = nullptr;    // [nk][nd][nd]
     dg::cd* W = nullptr;     // [nk][nc][nd] H_cd V_d
     dg::cd* kdata = nullptr; // [nk][H_cc | P_d pairs | e_d] of the one-matrix-per-lane Schur kernels
-    int target_wg = 8192;    // workgroups per lattice launch (k chunking)
+    int target_wg = 0;       // work items per lattice launch (k chunking); 0: lattice_target's default
     // static-part totals for the last mu (valid until the next prepare)
     unsigned prep_gen = 0, static_gen = ~0u;
     double static_mu = 0.0;

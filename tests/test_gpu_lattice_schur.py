@@ -212,3 +212,28 @@ def test_loss_grad(gpu, name, ncl, mname, method, tol):
     assert rel(r["sigma_bar"], sb_ref) <= tol
     assert abs(r["mu_bar"] - mb_ref) <= tol * max(1.0, abs(mb_ref))
     lat.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("method", [1, 2])
+def test_k_chunking_does_not_change_results(gpu, method):
+    """The k chunking (DMFTGPU_OPT_TARGET_WORKGROUPS: 0 = default, 7 = one chunk,
+    10^6 = one k per chunk) changes only the order of the k sums: the fused loss
+    and gradient agree to rounding."""
+    w = L.workload("dp14", seed=5, nk_side=5, n_freq=96)
+    Hk, sig, beta = w["Hk"], w["sigma"], w["beta"]
+    mom = w["exact_moments"]
+    lat = gpu.LatticeGF(Hk, 5, 96, beta, mom)
+    tgt = lat.greenftn_w(L.perturbed(sig, seed=3), 0.2, method=gpu.PLAIN_F64)["gloc"]
+    out = []
+    for target in (0, 7, 1000000):
+        lat.set_option(gpu.capi.OPT_TARGET_WORKGROUPS, target)
+        out.append(lat.loss_grad(sig, 0.2, tgt, 6.5, method=method))
+    lat.set_option(gpu.capi.OPT_TARGET_WORKGROUPS, 0)
+    lat.close()
+    for r in out[1:]:
+        assert rel(r["gloc"], out[0]["gloc"]) <= 1e-13
+        assert rel(r["num_matrix"], out[0]["num_matrix"]) <= 1e-13
+        assert abs(r["n_total"] - out[0]["n_total"]) <= 1e-12
+        assert rel(r["sigma_bar"], out[0]["sigma_bar"]) <= 1e-12
+        assert abs(r["mu_bar"] - out[0]["mu_bar"]) <= 1e-12 * max(1.0, abs(out[0]["mu_bar"]))
