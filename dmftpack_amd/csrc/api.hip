@@ -276,11 +276,16 @@ struct Plan {
 // or by default 32768 for the one-matrix-per-lane Schur kernels (their
 // forward is capped by LDS or registers at 6-8 wavefronts per CU, and shorter
 // k walks over more workgroups keep the CUs fuller: 24.0 -> 21.4 ms for the
-// config-2 forward call) and 8192 for the others.
-int lattice_target(dmftgpu_handle h, int method, int nc, bool cols)
+// config-2 forward call), but no fewer than 64 k per chunk (the per-chunk
+// partial sums grow with the chunk count: with 1000 real-axis energies 16 k
+// per chunk made the call 25% slower), and 8192 for the others.
+int lattice_target(dmftgpu_handle h, int method, int nc, bool cols, int nk, int nw_tot, int slots)
 {
     if (h->target_wg > 0) return h->target_wg;
-    return (method != DMFTGPU_PLAIN_F64 && !cols && nc <= 6) ? 32768 : 8192;
+    if (method == DMFTGPU_PLAIN_F64 || cols || nc > 6) return 8192;
+    const long n_wt = (nw_tot + slots - 1) / slots;
+    const long cap = n_wt * std::max(1, nk / 64);
+    return (int)std::max(8192L, std::min(32768L, cap));
 }
 
 Plan make_plan(int nk, int nw_tot, int slots, int target_wg)
@@ -375,7 +380,7 @@ int run_lattice(dmftgpu_handle h, int method, int mode, const double* sigma, dou
     if (method == DMFTGPU_PLAIN_F64) slots = dg::plain_slots(n, mode);
     else slots = dg::schur_slots(nc, mode, method == DMFTGPU_SCHUR_F32R, h->schur_layout == 1 || (ra && mode != dg::MODE_FWD));
     const bool cols = h->schur_layout == 1 || (ra && mode != dg::MODE_FWD);
-    Plan P = pk ? Plan{nk, 1} : make_plan(nk, nw_tot, slots, lattice_target(h, method, nc, cols));
+    Plan P = pk ? Plan{nk, 1} : make_plan(nk, nw_tot, slots, lattice_target(h, method, nc, cols, nk, nw_tot, slots));
     if (fused_tr2 && P.n_chunks != fused_chunks)  // both halves of a fused pair must chunk k alike
         return fail(h, DMFTGPU_ERR_STATE, "fused partials laid out for %d k chunks, this call uses %d", fused_chunks,
                     P.n_chunks);
@@ -465,7 +470,8 @@ size_t lattice_ws_bytes(dmftgpu_handle h, int method, int mode, const RealAxis* 
                                             : dg::schur_slots(nc, mode, method == DMFTGPU_SCHUR_F32R,
                                                               h->schur_layout == 1 || (ra && mode != dg::MODE_FWD));
     Plan P = make_plan(L.n_k, (int)nw_tot, slots,
-                       lattice_target(h, method, nc, h->schur_layout == 1 || (ra && mode != dg::MODE_FWD)));
+                       lattice_target(h, method, nc, h->schur_layout == 1 || (ra && mode != dg::MODE_FWD), L.n_k,
+                                      (int)nw_tot, slots));
     size_t C = P.n_chunks + 1;  // +1 for the reduced copy
     size_t b = (C + 1) * nw_tot * 16 * (2 + (size_t)m2) + C * n_freq * 16 * (m2 + 1) + 64 * 256;
     return b;
@@ -792,7 +798,7 @@ extern "C" int dmftgpu_gloc_loss_grad(dmftgpu_handle h, int method, const double
     int fchunks = 0;
     if (method != DMFTGPU_PLAIN_F64 && dg::schur_can_fuse(nc, h->schur_layout == 1, false) && nw_tot > L.n_freq) {
         const int slots = dg::schur_slots(nc, dg::MODE_FWD, method == DMFTGPU_SCHUR_F32R, false);
-        const Plan P = make_plan(L.n_k, nw_tot, slots, lattice_target(h, method, nc, false));
+        const Plan P = make_plan(L.n_k, nw_tot, slots, lattice_target(h, method, nc, false, L.n_k, nw_tot, slots));
         fchunks = P.n_chunks;
         ftr2 = (cd*)h->fused.get((size_t)P.n_chunks * nw_tot * sizeof(cd), &e);
         if (!ftr2) return fail(h, DMFTGPU_ERR_HIP, "fused partials: %s", hipGetErrorString(e));
