@@ -466,6 +466,51 @@ void Construct_Hk(dmftgpu_handle h, const HoppingList& hr, const std::vector<dou
     }
 }
 
+// The band loop of qs_spectra (qs_spectra.cpp:197-221) on a k path whose H(k)
+// the handle holds (dmftgpu_prepare with n_k = the path's length and zero
+// moments).  retarded_GreenFtn2 (GreenFtn.cpp:425-466) upfolds G^R to the full
+// DFT basis, diag 1/(z - eps_b) over all NumOrbit KS bands plus
+// C^H (G - G0) C on the window (:450-458), so its trace is
+//   tr G^R_full = sum_b 1/(z - eps_kb) + tr G(k, z) - tr G0(k, z),  z = E_n + mu + i eta,
+// G0 the model inverse without Sigma (DF_CorrBase drops out of the trace).
+// tr G0 comes from a second call with Sigma = 0.
+//   spectralWeight[k][n]            = -Im tr G^R_full / pi                    (:209-213)
+//   partial_spectralWeight[k][n][i] = -Im G^R(i, i) / pi, i < n_corr          (:215-220)
+// sigma_r: Sigma^R(E_n) [n_e] n_corr x n_corr; KS_eigenEnergy[k][b], b < NumOrbit.
+template <class Matrix, class Energies>
+void qs_band_spectra(dmftgpu_handle h, int method, int n_corr, int knum, int n_e, double E0, double dE,
+                     double infinitesimal, double mu, const std::vector<Matrix>& sigma_r,
+                     const Energies& KS_eigenEnergy, int NumOrbit, std::vector<std::vector<double>>& spectralWeight,
+                     std::vector<std::vector<std::vector<double>>>& partial_spectralWeight)
+{
+    const int nc = n_corr;
+    DevBuf sig((size_t)n_e * nc * nc), zero((size_t)n_e * nc * nc), gk((size_t)knum * n_e * nc * nc),
+        tk((size_t)knum * n_e), t0((size_t)knum * n_e);
+    sig.upload(pack(sigma_r, n_e, nc).data());
+    std::vector<cplx> z0(zero.n, cplx(0.0, 0.0));
+    zero.upload(z0.data());
+    die(h, dmftgpu_greenftn_real_k(h, method, n_e, E0, dE, infinitesimal, (const double*)sig.p, mu, (double*)gk.p,
+                                   (double*)tk.p), "qs_spectra (G)");
+    die(h, dmftgpu_greenftn_real_k(h, method, n_e, E0, dE, infinitesimal, (const double*)zero.p, mu, nullptr,
+                                   (double*)t0.p), "qs_spectra (G0)");
+    std::vector<cplx> hg(gk.n), ht(tk.n), h0(t0.n);
+    gk.download(hg.data());
+    tk.download(ht.data());
+    t0.download(h0.data());
+    const double pi = 3.1415926535897932384626;  // dmft_common.h:43
+    spectralWeight.assign(knum, std::vector<double>(n_e, 0.0));
+    partial_spectralWeight.assign(knum, std::vector<std::vector<double>>(n_e, std::vector<double>(nc, 0.0)));
+    for (int k = 0; k < knum; ++k)
+        for (int n = 0; n < n_e; ++n) {
+            const cplx z(E0 + n * dE + mu, infinitesimal);
+            cplx tr = ht[(size_t)k * n_e + n] - h0[(size_t)k * n_e + n];
+            for (int b = 0; b < NumOrbit; ++b) tr += 1.0 / (z - KS_eigenEnergy[k][b]);
+            spectralWeight[k][n] = -tr.imag() / pi;
+            for (int i = 0; i < nc; ++i)
+                partial_spectralWeight[k][n][i] = -hg[((size_t)k * n_e + n) * nc * nc + i + (size_t)i * nc].imag() / pi;
+        }
+}
+
 // upfolding_density (upfolding.cpp:38-139).  densityMatDFT[k] is taken to the
 // DFT basis in place (upfolding_k, :10-35), as the reference leaves it; the
 // three R sums come back in H_Rindex row order (DM_DFT_DMFT.HWR,

@@ -40,6 +40,8 @@ int orc_hyb_delta(int, int, double, double, const cplx*, const cplx*, const cplx
 void orc_hk_from_hr(int, int, const int*, const cplx*, int, const double*, cplx*);
 int orc_upfold_density(int, int, int, int, double, double, const cplx*, const cplx*, const double*, const int*,
                        const cplx*, const cplx*, const cplx*, int, const int*, const double*, cplx*, cplx*);
+int orc_greenftn_real(const orc_lattice*, const cplx*, int, double, double, double, const cplx*, double, cplx*, cplx*,
+                      int);
 }
 
 struct IndexMat {  // the subset of Eigen::MatrixXi (H_Rindex) the adapter uses
@@ -360,6 +362,79 @@ int main()
         for (int x = 0; x < 3; ++x) std::copy(rdmr[x].begin(), rdmr[x].end(), gr2.begin() + (size_t)x * nr);
         check("ragged upfolding D_k", relerr(gd2.data(), od2.data(), od2.size()), 1e-13);
         check("ragged upfolding DM(R)", relerr(gr2.data(), osum.data(), osum.size()), 1e-13);
+    }
+
+    // qs_spectra band loop: 3 k points of a path, a 4-band model (2 correlated) inside 6 KS bands;
+    // H(k) = C diag(eps of the window bands) C^H with C unitary (a Householder reflection), as the
+    // downfolded model's eigenvalues are the window's KS energies
+    {
+        const int nb = 4, ncq = 2, no = 6, nkp = 3, ne = 40;
+        const double e0q = -3.0, deq = 0.15, eta = 0.05, muq = 0.1;
+        std::vector<std::vector<double>> eks(nkp, std::vector<double>(no));
+        std::vector<cplx> hq((size_t)nkp * nb * nb);
+        for (int k = 0; k < nkp; ++k) {
+            for (int b = 0; b < no; ++b) eks[k][b] = -4.0 + 8.0 * (b + 0.5 * (urand() + 0.5)) / no;
+            std::vector<cplx> u(nb);
+            double uu = 0;
+            for (int i = 0; i < nb; ++i) { u[i] = cplx(urand(), urand()); uu += std::norm(u[i]); }
+            for (int j = 0; j < nb; ++j)
+                for (int i = 0; i < nb; ++i) {
+                    cplx s = 0.0;
+                    for (int l = 0; l < nb; ++l) {  // C = I - 2 u u^H / |u|^2; H = C diag(eps_{1..4}) C^H
+                        const cplx cil = (i == l ? 1.0 : 0.0) - 2.0 * u[i] * std::conj(u[l]) / uu;
+                        const cplx cjl = (j == l ? 1.0 : 0.0) - 2.0 * u[j] * std::conj(u[l]) / uu;
+                        s += cil * eks[k][1 + l] * std::conj(cjl);
+                    }
+                    hq[(size_t)k * nb * nb + i + (size_t)j * nb] = s;
+                }
+        }
+        std::vector<Mat> sr(ne), s0(ne);
+        std::vector<cplx> hs((size_t)ne * ncq * ncq), h0((size_t)ne * ncq * ncq, 0.0);
+        for (int n = 0; n < ne; ++n) {
+            sr[n].setZero(ncq, ncq);
+            s0[n].setZero(ncq, ncq);
+            const cplx E(e0q + n * deq, 0.0);
+            sr[n](0, 0) = 0.4 / (E - 0.3 + cplx(0.0, 0.6));
+            sr[n](1, 1) = 0.3 / (E + 0.5 + cplx(0.0, 0.4));
+            sr[n](0, 1) = sr[n](1, 0) = 0.1 / (E + cplx(0.0, 0.5));
+            std::copy(sr[n].v.begin(), sr[n].v.end(), hs.begin() + (size_t)n * ncq * ncq);
+        }
+        dmftgpu_shim::DevBuf dh(hq.size()), dm(4 * (size_t)ncq * ncq);
+        dh.upload(hq.data());
+        std::vector<cplx> zm(dm.n, 0.0);
+        dm.upload(zm.data());
+        dmftgpu_lattice Lq{nb, ncq, 1, nkp, nkp, 16, 2, 40.0};
+        dmftgpu_shim::die(h, dmftgpu_prepare(h, &Lq, (const double*)dh.p, (const double*)dm.p), "prepare path");
+        double err0 = 0, errs = 0, errp = 0, scale = 0;
+        for (int pass = 0; pass < 2; ++pass) {
+            std::vector<std::vector<double>> sw;
+            std::vector<std::vector<std::vector<double>>> pw;
+            dmftgpu_shim::qs_band_spectra(h, DMFTGPU_SCHUR_F64, ncq, nkp, ne, e0q, deq, eta, muq, pass ? sr : s0, eks,
+                                          no, sw, pw);
+            for (int k = 0; k < nkp; ++k) {
+                std::vector<cplx> gr((size_t)ne * ncq * ncq), tr(ne), g0((size_t)ne * ncq * ncq), t0(ne);
+                orc_lattice L1{nb, ncq, 1, 1, 1, ne, 1, 1.0};
+                orc_greenftn_real(&L1, hq.data() + (size_t)k * nb * nb, ne, e0q, deq, eta, pass ? hs.data() : h0.data(),
+                                  muq, gr.data(), tr.data(), 1);
+                orc_greenftn_real(&L1, hq.data() + (size_t)k * nb * nb, ne, e0q, deq, eta, h0.data(), muq, g0.data(),
+                                  t0.data(), 1);
+                for (int n = 0; n < ne; ++n) {
+                    const cplx z(e0q + n * deq + muq, eta);
+                    cplx full = 0.0;
+                    for (int b = 0; b < no; ++b) full += 1.0 / (z - eks[k][b]);
+                    const double ref_free = -full.imag() / PI;                          // Sigma = 0: closed form
+                    const double ref = -(full + tr[n] - t0[n]).imag() / PI;
+                    scale = std::fmax(scale, std::fabs(ref));
+                    if (!pass) err0 = std::fmax(err0, std::fabs(sw[k][n] - ref_free));
+                    else errs = std::fmax(errs, std::fabs(sw[k][n] - ref));
+                    for (int i = 0; i < ncq; ++i)
+                        errp = std::fmax(errp, std::fabs(pw[k][n][i] + gr[(size_t)n * ncq * ncq + i + (size_t)i * ncq].imag() / PI));
+                }
+            }
+        }
+        check("qs_spectra bands, Sigma=0", err0 / scale, 1e-13);
+        check("qs_spectra bands", errs / scale, 1e-12);
+        check("qs_spectra partial weights", errp / scale, 1e-12);
     }
 
     dmftgpu_destroy(h);
