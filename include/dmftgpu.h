@@ -63,6 +63,9 @@ typedef struct dmftgpu_lattice {
 typedef struct dmftgpu_handle_s* dmftgpu_handle;
 
 /* ---- handle ---------------------------------------------------------------- */
+/* No reference counterpart: the reference holds this state in globals and in
+ * GreenFtn_w's locals.  A handle owns the device workspaces, the bound H(k)
+ * and its static eigen-decomposition; hip_stream NULL = the default stream. */
 int dmftgpu_create(dmftgpu_handle* h, void* hip_stream);
 int dmftgpu_destroy(dmftgpu_handle h);
 int dmftgpu_set_stream(dmftgpu_handle h, void* hip_stream);
@@ -101,7 +104,8 @@ int dmftgpu_prepare(dmftgpu_handle h, const dmftgpu_lattice* lat, const double* 
  * (GreenFtn.cpp:197-215); the Schur data of H(k) is kept. */
 int dmftgpu_update_moments(dmftgpu_handle h, const double* moments);
 
-/* Copies the static eigenpairs out: eps [n_k][n] ascending, U [n_k][n][n]. */
+/* Copies the static eigenpairs out: eps [n_k][n] ascending, U [n_k][n][n]
+ * (the reference keeps them in GreenFtn_w's locals, GreenFtn.cpp:197-215). */
 int dmftgpu_static_eig(dmftgpu_handle h, double* eps, double* U);
 
 /* GreenFtn_w (GreenFtn.cpp:142-422) for one mu.
