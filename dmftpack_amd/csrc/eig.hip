@@ -113,25 +113,29 @@ __global__ __launch_bounds__(64) void k_herm_eig(const cd* __restrict__ X, int l
                 const double c = rc[6 * k], sn = rc[6 * k + 1];
                 const cd ph = cd{rc[6 * k + 2], rc[6 * k + 3]};
                 cd xp = A[p + j * m], xq = A[q + j * m];
-                A[p + j * m] = csub(cscale(xp, c), cscale(cmul(ph, xq), sn));
-                A[q + j * m] = cadd(cscale(xp, sn), cscale(cmul(ph, xq), c));
-            }
-            __syncthreads();
-            if (lane < np) {
-                const int q = (int)rc[6 * lane + 5];
-                if (q >= 0 && rc[6 * lane + 1] != 0.0) {
-                    const int p = (int)rc[6 * lane + 4];
-                    A[p + q * m] = cd{0.0, 0.0};
-                    A[q + p * m] = cd{0.0, 0.0};
-                    A[p + p * m].y = 0.0;
-                    A[q + q * m].y = 0.0;
+                cd yp = csub(cscale(xp, c), cscale(cmul(ph, xq), sn));
+                cd yq = cadd(cscale(xp, sn), cscale(cmul(ph, xq), c));
+                // a rotated pair's off-diagonal entries are zero and its
+                // diagonal real by construction: store them exactly
+                if (sn != 0.0) {
+                    if (j == q) {
+                        yp = cd{0.0, 0.0};
+                        yq.y = 0.0;
+                    } else if (j == p) {
+                        yq = cd{0.0, 0.0};
+                        yp.y = 0.0;
+                    }
                 }
+                A[p + j * m] = yp;
+                A[q + j * m] = yq;
             }
             __syncthreads();
         }
     }
     __syncthreads();
-    // rank sort (stable, ascending)
+    // rank sort (stable, ascending); perm[r] = column of Vs with rank r (the
+    // pair table is free now and holds the m <= 32 ints)
+    int* perm = (int*)rc;
     for (int i = lane; i < m; i += 64) {
         double wi = A[i + i * m].x;
         int r = 0;
@@ -140,8 +144,15 @@ __global__ __launch_bounds__(64) void k_herm_eig(const cd* __restrict__ X, int l
             r += (wj < wi) || (wj == wi && j < i);
         }
         w[(size_t)b * m + r] = wi;
-        if (V)
-            for (int e = 0; e < m; ++e) V[(size_t)b * m * m + e + (size_t)r * m] = Vs[e + i * m];
+        perm[r] = i;
+    }
+    if (V) {
+        __syncthreads();
+        for (int e = lane; e < m * m; e += 64) {
+            int c = perm[e / m];
+            if ((unsigned)c >= (unsigned)m) c = e / m;  // NaN eigenvalues leave ranks unassigned
+            V[(size_t)b * m * m + e] = Vs[e % m + c * m];
+        }
     }
 }
 
