@@ -190,40 +190,58 @@ hipError_t launch_gloc_hf(const double* ev, const cd* U, int n, int nc, int nk, 
     return hipGetLastError();
 }
 
-__global__ __launch_bounds__(256) void k_reduce_chunks(const cd* __restrict__ part, int n_chunks, size_t len,
-                                                       cd* __restrict__ out)
+// Sum of n_chunks partial arrays part[c * len + i] (one per k chunk).  A
+// workgroup covers 32 consecutive entries; each half-wavefront sums one of 8
+// contiguous ranges of chunks, in chunk order, and the 8 range sums are added
+// in range order.  The result is deterministic for a given (n_chunks, len),
+// and the chunk loop is 8 times shorter than a one-thread-per-entry sum, which
+// matters for the short per-frequency arrays (traces, dL/dmu) whose grids
+// would otherwise hold a few workgroups walking hundreds of chunks.
+__device__ __forceinline__ double radd(double a, double b) { return a + b; }
+__device__ __forceinline__ cd radd(cd a, cd b) { return cadd(a, b); }
+
+constexpr int kRedX = 32, kRedY = 8;
+
+template <class T>
+__global__ __launch_bounds__(kRedX* kRedY) void k_reduce_chunks(const T* __restrict__ part, int n_chunks, size_t len,
+                                                                 T* __restrict__ out)
 {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= len) return;
-    cd acc = cd{0.0, 0.0};
-    for (int c = 0; c < n_chunks; ++c) acc = cadd(acc, part[(size_t)c * len + i]);
-    out[i] = acc;
+    __shared__ T red[kRedY][kRedX];
+    const int tx = threadIdx.x % kRedX, ty = threadIdx.x / kRedX;
+    const size_t i = (size_t)blockIdx.x * kRedX + tx;
+    const int c0 = (int)((long long)n_chunks * ty / kRedY);
+    const int c1 = (int)((long long)n_chunks * (ty + 1) / kRedY);
+    T acc{};
+    if (i < len) {
+#pragma unroll 4
+        for (int c = c0; c < c1; ++c) acc = radd(acc, part[(size_t)c * len + i]);
+    }
+    red[ty][tx] = acc;
+    __syncthreads();
+    if (ty == 0 && i < len) {
+        T sum = red[0][tx];
+        for (int y = 1; y < kRedY; ++y) sum = radd(sum, red[y][tx]);
+        out[i] = sum;
+    }
 }
 
-__global__ __launch_bounds__(256) void k_reduce_chunks_d(const double* __restrict__ part, int n_chunks, size_t len,
-                                                         double* __restrict__ out)
+template <class T>
+static hipError_t launch_reduce(const T* part, int n_chunks, size_t len, T* out, hipStream_t s)
 {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= len) return;
-    double acc = 0.0;
-    for (int c = 0; c < n_chunks; ++c) acc += part[(size_t)c * len + i];
-    out[i] = acc;
+    if (len == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_reduce_chunks<T>, dim3((unsigned)((len + kRedX - 1) / kRedX)), dim3(kRedX * kRedY), 0, s,
+                       part, n_chunks, len, out);
+    return hipGetLastError();
 }
 
 hipError_t launch_reduce_chunks(const cd* part, int n_chunks, size_t len, cd* out, hipStream_t s)
 {
-    if (len == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_reduce_chunks, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, part, n_chunks, len,
-                       out);
-    return hipGetLastError();
+    return launch_reduce(part, n_chunks, len, out, s);
 }
 
 hipError_t launch_reduce_chunks_d(const double* part, int n_chunks, size_t len, double* out, hipStream_t s)
 {
-    if (len == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_reduce_chunks_d, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, part, n_chunks, len,
-                       out);
-    return hipGetLastError();
+    return launch_reduce(part, n_chunks, len, out, s);
 }
 
 // One workgroup per output quantity q: q < m2 -> element e=q of
