@@ -329,16 +329,21 @@ hipError_t launch_scale(const cd* in, size_t len, double scale, cd* out, hipStre
 // Loss cotangents of L = sum ||G_loc - G_tgt||^2: gbar = 2 (G_loc - G_tgt) on the
 // entries inside a cluster block, 0 across clusters (G_loc is returned
 // block-diagonal, GreenFtn.cpp:371-383, so those entries do not depend on Sigma);
-// *loss = sum |G_loc - G_tgt|^2 over all entries.  One workgroup, fixed-order sum.
-__global__ __launch_bounds__(256) void k_loss_cotangent(const cd* __restrict__ gloc, const cd* __restrict__ gtgt,
-                                                        int n_freq, int nc, int npc, cd* __restrict__ gbar,
-                                                        double* __restrict__ loss)
+// *loss = sum |G_loc - G_tgt|^2 over all entries.  One workgroup of 1024
+// threads (the whole array is n_freq * nc^2 entries, tens of thousands),
+// fixed-order tree sum.
+constexpr int kLossThreads = 1024;
+
+__global__ __launch_bounds__(kLossThreads) void k_loss_cotangent(const cd* __restrict__ gloc,
+                                                                 const cd* __restrict__ gtgt, int n_freq, int nc,
+                                                                 int npc, cd* __restrict__ gbar,
+                                                                 double* __restrict__ loss)
 {
-    __shared__ double red[256];
+    __shared__ double red[kLossThreads];
     const int m2 = nc * nc;
     const size_t len = (size_t)n_freq * m2;
     double acc = 0.0;
-    for (size_t t = threadIdx.x; t < len; t += 256) {
+    for (size_t t = threadIdx.x; t < len; t += kLossThreads) {
         const cd d = csub(gloc[t], gtgt[t]);
         acc = fma(d.x, d.x, fma(d.y, d.y, acc));
         const int e = (int)(t % m2), i = e % nc, j = e / nc;
@@ -346,7 +351,7 @@ __global__ __launch_bounds__(256) void k_loss_cotangent(const cd* __restrict__ g
     }
     red[threadIdx.x] = acc;
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
+    for (int o = kLossThreads / 2; o > 0; o >>= 1) {
         if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
         __syncthreads();
     }
@@ -356,7 +361,7 @@ __global__ __launch_bounds__(256) void k_loss_cotangent(const cd* __restrict__ g
 hipError_t launch_loss_cotangent(const cd* gloc, const cd* gtgt, int n_freq, int nc, int npc, cd* gbar, double* loss,
                                  hipStream_t s)
 {
-    hipLaunchKernelGGL(k_loss_cotangent, dim3(1), dim3(256), 0, s, gloc, gtgt, n_freq, nc, npc, gbar, loss);
+    hipLaunchKernelGGL(k_loss_cotangent, dim3(1), dim3(kLossThreads), 0, s, gloc, gtgt, n_freq, nc, npc, gbar, loss);
     return hipGetLastError();
 }
 
