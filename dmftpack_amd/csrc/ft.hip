@@ -251,7 +251,10 @@ __global__ void k_mom_posmat(const cd* __restrict__ avg, int d, int ncent, int g
         }
 }
 
-// final moment: mean over the selected window of (mt_ab + conj(mt_ba))/2, then Hermitised (:319-330, :458-461)
+// threads of the window-selection workgroup (one workgroup per moment, the
+// window centres spread over its threads)
+constexpr int kSelThreads = 1024;
+
 // Window choice of getAsymto_moments (:289-315) for moment a = blockIdx.x:
 // the spread of the moment's trace over the 2 Nv + 1 windows around each
 // centre (mean, then the squared deviations, in the reference's order), and
@@ -260,12 +263,12 @@ __global__ void k_mom_posmat(const cd* __restrict__ avg, int d, int ncent, int g
 // spread is below the last centre's, else the last centre.  ok[i] is the
 // positivity test on the window-averaged moments (:301-312, :432).
 // cen[target] = minPos + Nv, target = a (a + 2 for the G branch).
-__global__ __launch_bounds__(256) void k_mom_select(const double* __restrict__ trace, const double* __restrict__ w1,
+__global__ __launch_bounds__(kSelThreads) void k_mom_select(const double* __restrict__ trace, const double* __restrict__ w1,
                                                     const double* __restrict__ wT, int d, int range, int ncent, int Nv,
                                                     int green, int* __restrict__ cen)
 {
-    __shared__ double bv[256];
-    __shared__ int bi[256];
+    __shared__ double bv[kSelThreads];
+    __shared__ int bi[kSelThreads];
     const int a = blockIdx.x, t = threadIdx.x;
     auto spread = [&](int i) {
         const double* v = trace + (size_t)a * range + i;  // centre i + Nv, window [i, i + 2 Nv]
@@ -278,7 +281,7 @@ __global__ __launch_bounds__(256) void k_mom_select(const double* __restrict__ t
     };
     double best = 0.0;
     int besti = -1;
-    for (int i = t; i < ncent; i += 256) {
+    for (int i = t; i < ncent; i += kSelThreads) {
         const bool ok = green ? wT[(size_t)i * d] > 1 : (w1[(size_t)i * d] > 0 && wT[(size_t)i * d] > 0);
         if (!ok) continue;
         const double v = spread(i);
@@ -290,7 +293,7 @@ __global__ __launch_bounds__(256) void k_mom_select(const double* __restrict__ t
     bv[t] = best;
     bi[t] = besti;
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
+    for (int w = kSelThreads / 2; w > 0; w >>= 1) {
         if (t < w) {
             const int oi = bi[t + w];
             const double ov = bv[t + w];
@@ -319,7 +322,8 @@ __global__ void k_mom_init(cd* __restrict__ mom, int d, int green, int* __restri
     mom[t] = cd{(green && q == 1 && e % d == e / d) ? 1.0 : 0.0, 0.0};
 }
 
-// final moment of the chosen window: blockIdx.y = moment index
+// final moment of the chosen window: mean over the window of (mt_ab + conj(mt_ba))/2,
+// then Hermitised (:319-330, :458-461); blockIdx.y = moment index
 __global__ void k_mom_final(const cd* __restrict__ mt, int d, int range, int green, const int* __restrict__ cen_all,
                             int Nv, cd* __restrict__ mom)
 {
@@ -807,7 +811,7 @@ extern "C" int dmftgpu_moments(dmftgpu_handle h, int d, int n_freq, double beta,
     cd* dm = (cd*)moments;
     hipLaunchKernelGGL(k_mom_init, dim3(nblk((size_t)4 * m2)), dim3(256), 0, s, dm, d, green_asym, cen);
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_mom_select, dim3(nm), dim3(256), 0, s, trace, w1, wT, d, range, ncent, Nv, green_asym, cen);
+    hipLaunchKernelGGL(k_mom_select, dim3(nm), dim3(kSelThreads), 0, s, trace, w1, wT, d, range, ncent, Nv, green_asym, cen);
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(k_mom_final, dim3(nblk(m2), nm), dim3(256), 0, s, mt, d, range, green_asym, cen, Nv, dm);
     HIPCHK(hipGetLastError());
