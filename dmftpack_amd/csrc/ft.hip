@@ -185,70 +185,65 @@ __global__ void k_mom_avg(const cd* __restrict__ mt, int m2, int range, int nm, 
 // positivity matrix per window centre:
 //   non-G: T = Vi m3 Vi - (Vi m2 Vi)^2, Vi = (m1)^{-1/2}   (:298-308)
 //   G    : T = m1 - m0^2                                   (:423-424)
-// (w1, U1) = eig(m1avg) for the non-G branch.  scratch: 3 d*d per centre.
-__global__ void k_mom_posmat(const cd* __restrict__ avg, int d, int ncent, int green, const double* __restrict__ w1,
+// (w1, U1) = eig(m1avg) for the non-G branch.  scratch: 3 d*d per centre
+// (Vi, P2 = Vi m2 Vi, tmp).  One thread per (centre, entry); the non-G branch
+// runs as five launches (stage 0: Vi, 1: tmp = Vi m2, 2: P2 = tmp Vi,
+// 3: tmp = Vi m3, 4: T = tmp Vi - P2 P2), stage -1 is the G branch.  A centre
+// whose m1 is not positive gets T = -1 (the window is rejected anyway).
+__global__ void k_mom_posmat(const cd* __restrict__ avg, int d, int ncent, int stage, const double* __restrict__ w1,
                              const cd* __restrict__ U1, cd* __restrict__ T, cd* __restrict__ scr)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncent) return;
     const int m2 = d * d;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)ncent * m2) return;
+    const int c = (int)(t / m2), e = (int)(t % m2), i = e % d, j = e / d;
     auto A = [&](int a) { return avg + ((size_t)a * ncent + c) * m2; };
     cd* out = T + (size_t)c * m2;
-    if (green) {
+    if (stage < 0) {
         const cd* m0 = A(0);
-        const cd* m1 = A(1);
-        for (int j = 0; j < d; ++j)
-            for (int i = 0; i < d; ++i) {
-                cd s = m1[i + j * d];
-                for (int l = 0; l < d; ++l) s = cfms(m0[i + l * d], m0[l + j * d], s);
-                out[i + j * d] = s;
-            }
+        cd s = A(1)[e];
+        for (int l = 0; l < d; ++l) s = cfms(m0[i + l * d], m0[l + j * d], s);
+        out[e] = s;
         return;
     }
     const double* w = w1 + (size_t)c * d;
-    const cd* U = U1 + (size_t)c * m2;
-    if (!(w[0] > 0.0)) {  // m1 not positive: the window is rejected anyway
-        for (int e = 0; e < m2; ++e) out[e] = cd{-1.0, 0.0};
+    if (!(w[0] > 0.0)) {
+        if (stage == 4) out[e] = cd{-1.0, 0.0};
         return;
     }
     cd* Vi = scr + (size_t)c * 3 * m2;
     cd* P2 = Vi + m2;
     cd* tmp = P2 + m2;
-    for (int j = 0; j < d; ++j)
-        for (int i = 0; i < d; ++i) {
-            cd s = cd{0, 0};
-            for (int l = 0; l < d; ++l) s = cadd(s, cscale(cmulc(U[i + l * d], U[j + l * d]), 1.0 / sqrt(w[l])));
-            Vi[i + j * d] = s;
-        }
-    const cd* m2a = A(2);
-    const cd* m3a = A(3);
-    // P2 = Vi m2 Vi
-    for (int j = 0; j < d; ++j)
-        for (int i = 0; i < d; ++i) {
-            cd s = cd{0, 0};
-            for (int l = 0; l < d; ++l) s = cfma(Vi[i + l * d], m2a[l + j * d], s);
-            tmp[i + j * d] = s;
-        }
-    for (int j = 0; j < d; ++j)
-        for (int i = 0; i < d; ++i) {
-            cd s = cd{0, 0};
-            for (int l = 0; l < d; ++l) s = cfma(tmp[i + l * d], Vi[l + j * d], s);
-            P2[i + j * d] = s;
-        }
-    // out = Vi m3 Vi
-    for (int j = 0; j < d; ++j)
-        for (int i = 0; i < d; ++i) {
-            cd s = cd{0, 0};
-            for (int l = 0; l < d; ++l) s = cfma(Vi[i + l * d], m3a[l + j * d], s);
-            tmp[i + j * d] = s;
-        }
-    for (int j = 0; j < d; ++j)
-        for (int i = 0; i < d; ++i) {
-            cd s = cd{0, 0};
-            for (int l = 0; l < d; ++l) s = cfma(tmp[i + l * d], Vi[l + j * d], s);
-            for (int l = 0; l < d; ++l) s = cfms(P2[i + l * d], P2[l + j * d], s);
-            out[i + j * d] = s;
-        }
+    cd s = cd{0, 0};
+    switch (stage) {
+    case 0: {
+        const cd* U = U1 + (size_t)c * m2;
+        for (int l = 0; l < d; ++l) s = cadd(s, cscale(cmulc(U[i + l * d], U[j + l * d]), 1.0 / sqrt(w[l])));
+        Vi[e] = s;
+        break;
+    }
+    case 1: {
+        const cd* m2a = A(2);
+        for (int l = 0; l < d; ++l) s = cfma(Vi[i + l * d], m2a[l + j * d], s);
+        tmp[e] = s;
+        break;
+    }
+    case 2:
+        for (int l = 0; l < d; ++l) s = cfma(tmp[i + l * d], Vi[l + j * d], s);
+        P2[e] = s;
+        break;
+    case 3: {
+        const cd* m3a = A(3);
+        for (int l = 0; l < d; ++l) s = cfma(Vi[i + l * d], m3a[l + j * d], s);
+        tmp[e] = s;
+        break;
+    }
+    default:
+        for (int l = 0; l < d; ++l) s = cfma(tmp[i + l * d], Vi[l + j * d], s);
+        for (int l = 0; l < d; ++l) s = cfms(P2[i + l * d], P2[l + j * d], s);
+        out[e] = s;
+        break;
+    }
 }
 
 // threads of the window-selection workgroup (one workgroup per moment, the
@@ -804,8 +799,11 @@ extern "C" int dmftgpu_moments(dmftgpu_handle h, int d, int n_freq, double beta,
     hipLaunchKernelGGL(k_mom_avg, dim3(nblk((size_t)nm * ncent * m2)), dim3(256), 0, s, mt, m2, range, nm, Nv, avg);
     HIPCHK(hipGetLastError());
     if (!green_asym) HIPCHK(dg::launch_herm_eig(avg + (size_t)1 * ncent * m2, d, 0, d, nullptr, 0, w1, U1, ncent, s));
-    hipLaunchKernelGGL(k_mom_posmat, dim3(nblk(ncent)), dim3(256), 0, s, avg, d, ncent, green_asym, w1, U1, T, scr);
-    HIPCHK(hipGetLastError());
+    for (int stage = green_asym ? -1 : 0; stage <= (green_asym ? -1 : 4); ++stage) {
+        hipLaunchKernelGGL(k_mom_posmat, dim3(nblk((size_t)ncent * m2)), dim3(256), 0, s, avg, d, ncent, stage, w1, U1,
+                           T, scr);
+        HIPCHK(hipGetLastError());
+    }
     HIPCHK(dg::launch_herm_eig(T, d, 0, d, nullptr, 0, wT, nullptr, ncent, s));
     // window choice and final moments on the device: no host round trip
     cd* dm = (cd*)moments;
