@@ -60,6 +60,7 @@ _SIGS = {
     "dmftgpu_greenftn_w": ([P, I, P, D, P, P, P, PD], I),
     "dmftgpu_density_matrix": ([P, P, D, P], I),
     "dmftgpu_density": ([P, I, P, D, PD, PD], I),
+    "dmftgpu_set_core_bands": ([P, I, P, PI], I),
     "dmftgpu_adjust_mu": ([P, I, P, D, D, D, PD, PI], I),
     "dmftgpu_gloc_vjp": ([P, I, P, D, P, D, P, PD], I),
     "dmftgpu_gloc_loss_grad": ([P, I, P, D, P, D, P, P, PD, PD, P, PD], I),

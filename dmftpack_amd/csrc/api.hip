@@ -110,6 +110,7 @@ extern "C" int dmftgpu_destroy(dmftgpu_handle h)
     h->static_cache.release();
     h->fused.release();
     h->cot.release();
+    h->core.release();
     delete h;
     return DMFTGPU_OK;
 }
@@ -234,6 +235,7 @@ extern "C" int dmftgpu_prepare(dmftgpu_handle h, const dmftgpu_lattice* L, const
         if (lane_kernels) HIPCHK(dg::launch_schur_kdata(h->Hk, h->W, h->ed, n, nc, nk, h->kdata, h->stream));
         h->schur_ready = true;
     }
+    h->core_n_orb = 0;  // Kohn-Sham bands belong to the previous H(k)
     h->prepared = true;
     ++h->prep_gen;
     return DMFTGPU_OK;
@@ -252,6 +254,47 @@ extern "C" int dmftgpu_update_moments(dmftgpu_handle h, const double* moments)
     HIPCHK(hipMemcpyAsync(h->mom, moments, sizeof(cd) * 4 * nc * nc, hipMemcpyDefault, h->stream));
     HIPCHK(dg::launch_herm_eig(h->Hk, n, 0, n, h->mom, nc, h->ev, h->U, nk, h->stream));
     ++h->prep_gen;
+    return DMFTGPU_OK;
+}
+
+// Kohn-Sham bands for the core-band term of GiwToNele (GreenFtn.cpp:622-631):
+// ks_energy [n_k][n_orb] (host or device), val_to_ks [n_k][n] HOST, the KS
+// band of each window band (FromValToKS), -1 for a padded band.  The bands of a
+// k point that no window band names are the ones the term counts.
+extern "C" int dmftgpu_set_core_bands(dmftgpu_handle h, int n_orb, const double* ks_energy, const int* val_to_ks)
+{
+    (void)hipGetLastError();  // drop an error left by an earlier failed call
+    if (!h) return DMFTGPU_ERR_ARG;
+    if (!h->prepared) return fail(h, DMFTGPU_ERR_STATE, "dmftgpu_prepare has not been called");
+    ++h->prep_gen;  // the cached static totals include the core term
+    if (n_orb == 0) {
+        h->core_n_orb = 0;
+        return DMFTGPU_OK;
+    }
+    const int n = h->lat.n, nk = h->lat.n_k;
+    if (n_orb < 0 || n_orb > 4096 || !ks_energy || !val_to_ks)
+        return fail(h, DMFTGPU_ERR_ARG, "n_orb=%d (1..4096) needs ks_energy and val_to_ks", n_orb);
+    std::vector<unsigned char> out((size_t)nk * n_orb, 1);
+    for (int k = 0; k < nk; ++k)
+        for (int i = 0; i < n; ++i) {
+            const int b = val_to_ks[(size_t)k * n + i];
+            if (b == -1) continue;
+            if (b < 0 || b >= n_orb)
+                return fail(h, DMFTGPU_ERR_ARG, "val_to_ks[%d][%d] = %d outside 0..%d", k, i, b, n_orb - 1);
+            if (!out[(size_t)k * n_orb + b])
+                return fail(h, DMFTGPU_ERR_ARG, "k point %d names KS band %d twice", k, b);
+            out[(size_t)k * n_orb + b] = 0;
+        }
+    const size_t eb = ((size_t)nk * n_orb * sizeof(double) + 255) / 256 * 256;
+    hipError_t e;
+    char* base = (char*)h->core.get(eb + out.size(), &e);
+    if (!base) return fail(h, DMFTGPU_ERR_HIP, "core bands: %s", hipGetErrorString(e));
+    h->core_eps = (double*)base;
+    h->core_out = (unsigned char*)(base + eb);
+    HIPCHK(hipMemcpyAsync(h->core_eps, ks_energy, sizeof(double) * (size_t)nk * n_orb, hipMemcpyDefault, h->stream));
+    HIPCHK(hipMemcpyAsync(h->core_out, out.data(), out.size(), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));  // out is a host temporary
+    h->core_n_orb = n_orb;
     return DMFTGPU_OK;
 }
 
@@ -300,49 +343,88 @@ Plan make_plan(int nk, int nw_tot, int slots, int target_wg)
 }
 
 // Static weights and their per-k reduction: fills st_out[m2] (complex) and
-// st_scalars[2] = {sum w, sum dN terms} (device), all unnormalised.  The
-// result depends only on (mu, prepared lattice) and is cached per handle.
-int run_static(dmftgpu_handle h, double mu, cd* st_out, double* st_scalars, char* ws, size_t& off)
+// st_scalars[2] = {sum w, sum dN terms} (device), all unnormalised.  With
+// Kohn-Sham bands bound by dmftgpu_set_core_bands the scalars include the
+// core-band term of GiwToNele (GreenFtn.cpp:622-631) and its mu-derivative.
+// The result depends only on (mu, prepared lattice, core bands) and is kept
+// per handle, in the handle's workspaces; with from_ws every buffer is carved
+// from ws instead and nothing is kept.  wt_out (optional) receives the
+// per-level weights w_kl [nk][n].
+int run_static(dmftgpu_handle h, double mu, cd* st_out, double* st_scalars, char* ws, size_t& off,
+               bool from_ws = false, const double** wt_out = nullptr)
 {
     const dmftgpu_lattice& L = h->lat;
     const int n = L.n, nc = L.n_corr, nk = L.n_k, m2 = nc * nc;
     const int nw_tot = L.n_freq * L.high_freq;
+    const size_t wbytes = ((size_t)nk * n * sizeof(double) + 255) / 256 * 256;
     hipError_t e;
-    char* cache = (char*)h->static_cache.get((size_t)m2 * sizeof(cd) + 256, &e);
-    if (!cache) return fail(h, DMFTGPU_ERR_HIP, "static cache: %s", hipGetErrorString(e));
-    char* wbuf = (char*)h->stat.get(2 * (((size_t)nk * n * sizeof(double) + 255) / 256 * 256), &e);
-    if (!wbuf) return fail(h, DMFTGPU_ERR_HIP, "static weights: %s", hipGetErrorString(e));
+    char *cache, *wbuf;
+    if (from_ws) {
+        cache = ws + off;
+        off += ((size_t)m2 * sizeof(cd) + 256 + 255) / 256 * 256;
+        wbuf = ws + off;
+        off += 2 * wbytes;
+    } else {
+        cache = (char*)h->static_cache.get((size_t)m2 * sizeof(cd) + 256, &e);
+        if (!cache) return fail(h, DMFTGPU_ERR_HIP, "static cache: %s", hipGetErrorString(e));
+        wbuf = (char*)h->stat.get(2 * wbytes, &e);
+        if (!wbuf) return fail(h, DMFTGPU_ERR_HIP, "static weights: %s", hipGetErrorString(e));
+    }
     double* wt = (double*)wbuf;
-    double* dt = (double*)(wbuf + ((size_t)nk * n * sizeof(double) + 255) / 256 * 256);
-    if (!(h->static_cache_on && h->static_gen == h->prep_gen && h->static_mu == mu)) {
+    double* dt = (double*)(wbuf + wbytes);
+    if (wt_out) *wt_out = wt;
+    if (from_ws || !(h->static_cache_on && h->static_gen == h->prep_gen && h->static_mu == mu)) {
         const int kpc = (nk + 255) / 256;
         const int chunks = (nk + kpc - 1) / kpc;
+        const int cchunks = h->core_n_orb > 0 ? chunks : 0;  // core-band partials follow the static ones
+        const int tot = chunks + cchunks;
         cd* pNum = (cd*)(ws + off);
         off += ((size_t)chunks * m2 * sizeof(cd) + 255) / 256 * 256;
         double* pN = (double*)(ws + off);
-        off += ((size_t)chunks * 2 * sizeof(double) + 255) / 256 * 256;
+        off += ((size_t)tot * 2 * sizeof(double) + 255) / 256 * 256;
+        double* pdN = pN + tot;
         HIPCHK(dg::launch_static_terms(h->ev, n, nk, nw_tot, L.beta, mu, wt, dt, h->stream));
-        HIPCHK(dg::launch_static_reduce(wt, dt, h->U, n, nc, nk, kpc, chunks, pNum, pN, pN + chunks, h->stream));
+        HIPCHK(dg::launch_static_reduce(wt, dt, h->U, n, nc, nk, kpc, chunks, pNum, pN, pdN, h->stream));
+        if (cchunks)
+            HIPCHK(dg::launch_core_bands(h->core_eps, h->core_out, h->core_n_orb, nk, kpc, cchunks, L.beta, mu,
+                                         pN + chunks, pdN + chunks, h->stream));
         HIPCHK(dg::launch_reduce_chunks(pNum, chunks, m2, (cd*)cache, h->stream));
-        HIPCHK(dg::launch_reduce_chunks_d(pN, chunks, 1, (double*)(cache + (size_t)m2 * sizeof(cd)), h->stream));
-        HIPCHK(dg::launch_reduce_chunks_d(pN + chunks, chunks, 1, (double*)(cache + (size_t)m2 * sizeof(cd)) + 1, h->stream));
-        h->static_gen = h->prep_gen;
-        h->static_mu = mu;
+        HIPCHK(dg::launch_reduce_chunks_d(pN, tot, 1, (double*)(cache + (size_t)m2 * sizeof(cd)), h->stream));
+        HIPCHK(dg::launch_reduce_chunks_d(pdN, tot, 1, (double*)(cache + (size_t)m2 * sizeof(cd)) + 1, h->stream));
+        if (!from_ws) {
+            h->static_gen = h->prep_gen;
+            h->static_mu = mu;
+        }
     }
     HIPCHK(hipMemcpyAsync(st_out, cache, sizeof(cd) * m2, hipMemcpyDeviceToDevice, h->stream));
     HIPCHK(hipMemcpyAsync(st_scalars, cache + (size_t)m2 * sizeof(cd), 2 * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
     return DMFTGPU_OK;
 }
 
-size_t static_ws_bytes(const dmftgpu_lattice& L)
+// workspace run_static takes from ws (from_ws: its cache and weights too)
+size_t static_ws_bytes(const dmftgpu_lattice& L, bool from_ws = false)
 {
     const int kpc = (L.n_k + 255) / 256;
     const int chunks = (L.n_k + kpc - 1) / kpc;
-    return 2 * (((size_t)L.n_k * L.n * 8 + 255) / 256 * 256) + ((size_t)chunks * L.n_corr * L.n_corr * 16 + 255) / 256 * 256 +
-           ((size_t)chunks * 16 + 255) / 256 * 256 + 4096;
+    size_t b = ((size_t)chunks * L.n_corr * L.n_corr * 16 + 255) / 256 * 256 + ((size_t)2 * chunks * 16 + 255) / 256 * 256 +
+               4096;
+    if (from_ws)
+        b += 2 * (((size_t)L.n_k * L.n * 8 + 255) / 256 * 256) + ((size_t)L.n_corr * L.n_corr * 16 + 256 + 255) / 256 * 256;
+    return b;
 }
 
 bool valid_method(int m) { return m == DMFTGPU_PLAIN_F64 || m == DMFTGPU_SCHUR_F64 || m == DMFTGPU_SCHUR_F32R; }
+
+bool method_ok(dmftgpu_handle h, int method, int* rc)
+{
+    *rc = DMFTGPU_OK;
+    if (!h->prepared) *rc = fail(h, DMFTGPU_ERR_STATE, "dmftgpu_prepare has not been called");
+    else if (!valid_method(method)) *rc = fail(h, DMFTGPU_ERR_ARG, "unknown method %d", method);
+    else if (method != DMFTGPU_PLAIN_F64 && !h->schur_ready)
+        *rc = fail(h, DMFTGPU_ERR_UNSUPPORTED, "Schur method needs n_corr <= 16 (n=%d, n_corr=%d)", h->lat.n,
+                   h->lat.n_corr);
+    return *rc == DMFTGPU_OK;
+}
 
 // Runs the per-(k, iw) kernel of the chosen method in the given mode and
 // reduces the per-chunk partials.  Outputs (device, in ws):
@@ -675,9 +757,10 @@ extern "C" int dmftgpu_adjust_mu(dmftgpu_handle h, int method, const double* sig
 {
     (void)hipGetLastError();  // drop an error left by an earlier failed call
     if (!h) return DMFTGPU_ERR_ARG;
-    if (!h->prepared) return fail(h, DMFTGPU_ERR_STATE, "dmftgpu_prepare has not been called");
-    if (!valid_method(method)) return fail(h, DMFTGPU_ERR_ARG, "unknown method %d", method);
+    int rc0;
+    if (!method_ok(h, method, &rc0)) return rc0;
     if (!sigma || !mu_out) return fail(h, DMFTGPU_ERR_ARG, "sigma and mu_out are required");
+    if (!std::isfinite(mu0) || !std::isfinite(n_target)) return fail(h, DMFTGPU_ERR_ARG, "mu0 or n_target is not finite");
     if (!(tol > 0.0)) tol = 1e-5;
     double mu = mu0, N, dN;
     int evals = 0;
@@ -750,16 +833,6 @@ int vjp_impl(dmftgpu_handle h, int method, const double* sigma, double mu, const
     return DMFTGPU_OK;
 }
 
-bool method_ok(dmftgpu_handle h, int method, int* rc)
-{
-    *rc = DMFTGPU_OK;
-    if (!h->prepared) *rc = fail(h, DMFTGPU_ERR_STATE, "dmftgpu_prepare has not been called");
-    else if (!valid_method(method)) *rc = fail(h, DMFTGPU_ERR_ARG, "unknown method %d", method);
-    else if (method != DMFTGPU_PLAIN_F64 && !h->schur_ready)
-        *rc = fail(h, DMFTGPU_ERR_UNSUPPORTED, "Schur method needs n_corr <= 16 (n=%d, n_corr=%d)", h->lat.n,
-                   h->lat.n_corr);
-    return *rc == DMFTGPU_OK;
-}
 
 }  // namespace
 
@@ -845,7 +918,8 @@ extern "C" int dmftgpu_density_matrix(dmftgpu_handle h, const double* sigma, dou
     off += ((size_t)m2 * sizeof(cd) + 255) / 256 * 256;
     double* sts = (double*)(ws + off);
     off += 256;
-    int rc = run_static(h, mu, st, sts, ws, off);  // fills the static weights for this mu
+    const double* wt = nullptr;
+    int rc = run_static(h, mu, st, sts, ws, off, false, &wt);  // the static weights for this mu
     if (rc) return rc;
     dg::LatArgs a{};
     a.Hk = h->Hk;
@@ -858,7 +932,6 @@ extern "C" int dmftgpu_density_matrix(dmftgpu_handle h, const double* sigma, dou
     a.nw_tot = L.n_freq * L.high_freq;
     a.beta = L.beta;
     a.mu = mu;
-    const double* wt = (const double*)h->stat.ptr;
     HIPCHK(dg::launch_density_matrix(a, h->U, wt, (cd*)dens, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     return DMFTGPU_OK;

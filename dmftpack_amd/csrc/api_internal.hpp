@@ -55,6 +55,13 @@ struct dmftgpu_handle_s {
     int mom_key_nf = 0, mom_key_green = 0;
     double mom_key_beta = 0.0;
     dgi::Workspace stat;     // static weights wt/dt [nk][n] of the cached mu
+    // Kohn-Sham bands outside the model window (core-band term of GiwToNele,
+    // GreenFtn.cpp:622-631), bound by dmftgpu_set_core_bands: energies
+    // [nk][n_orb] and a flag per band, 1 = not one of the k point's window bands
+    int core_n_orb = 0;
+    double* core_eps = nullptr;
+    unsigned char* core_out = nullptr;
+    dgi::Workspace core;
     dgi::Workspace fused;    // tr G^2 partials shared by a fused forward and its adjoint
     dgi::Workspace cot;      // loss cotangents gbar [n_freq][nc][nc] + the loss sum
     // optional kernel timing (dmftgpu_enable_timing): events around the main

@@ -34,8 +34,15 @@ using namespace dg;
 // ---------------------------------------------------------------------------
 // getAsymto_moments kernels
 // ---------------------------------------------------------------------------
+// Partial-pivot LU inverse of the real n x n (n <= 4) normal matrix, the
+// steps of Eigen's PartialPivLU::inverse: pivot = the first largest |entry|
+// of the column, multipliers by the reciprocal pivot, fused updates, forward
+// then backward substitution per unit column.  Every multiply-add is an
+// explicit fma and nothing else is contracted, so the oracle's restatement
+// (which does the same) gives the same bits.
 __device__ void lu_inverse4(const double (&A)[16], int n, double (&X)[16])
 {
+#pragma clang fp contract(off)
     double lu[16];
     int perm[4];
     for (int e = 0; e < 16; ++e) lu[e] = A[e];
@@ -64,16 +71,16 @@ __device__ void lu_inverse4(const double (&A)[16], int n, double (&X)[16])
         }
         for (int j = k + 1; j < n; ++j) {
             const double u = lu[k + j * 4];
-            for (int i = k + 1; i < n; ++i) lu[i + j * 4] -= lu[i + k * 4] * u;
+            for (int i = k + 1; i < n; ++i) lu[i + j * 4] = fma(-lu[i + k * 4], u, lu[i + j * 4]);
         }
     }
     for (int c = 0; c < n; ++c) {
         double x[4];
         for (int i = 0; i < n; ++i) x[i] = (perm[i] == c) ? 1.0 : 0.0;
         for (int i = 0; i < n; ++i)
-            for (int j = 0; j < i; ++j) x[i] -= lu[i + j * 4] * x[j];
+            for (int j = 0; j < i; ++j) x[i] = fma(-lu[i + j * 4], x[j], x[i]);
         for (int i = n - 1; i >= 0; --i) {
-            for (int j = i + 1; j < n; ++j) x[i] -= lu[i + j * 4] * x[j];
+            for (int j = i + 1; j < n; ++j) x[i] = fma(-lu[i + j * 4], x[j], x[i]);
             x[i] /= lu[i + i * 4];
         }
         for (int i = 0; i < n; ++i) X[i + c * 4] = x[i];
@@ -83,23 +90,18 @@ __device__ void lu_inverse4(const double (&A)[16], int n, double (&X)[16])
 // columns of the design matrix for row pair jj (Re row 2jj, Im row 2jj+1)
 // non-G: Re: [1, 0, -1/z^2, 0]   Im: [0, -1/z, 0, 1/z^3]   (:227-230)
 // G    : Re: [-1/z^2, 0]          Im: [0, 1/z^3]             (:366-367)
-// basis values per frequency, tab[wn] = {-1/z, -1/z^2, 1/z^3, 1/z}, the
-// factors the fit loops below multiply by (formed as the reference forms them)
-__global__ void k_mom_basis(int n_freq, double beta, double4* __restrict__ tab)
-{
-    const int wn = blockIdx.x * blockDim.x + threadIdx.x;
-    if (wn >= n_freq) return;
-    const double z = (2 * wn + 1) * kPi / beta;
-    tab[wn] = double4{-1. / pow(z, 1), -1. / pow(z, 2), 1. / pow(z, 3), 1.0 / z};
-}
-
+// basis values per frequency, tab[wn] = {-1/z, -1/z^2, 1/z^3, 1/z}, formed on
+// the host with std::pow as the reference forms them (dmftgpu_moments)
 __global__ void k_mom_ktk(int n_freq, int range, int green, const double4* __restrict__ tab, double* __restrict__ kinv)
 {
+#pragma clang fp contract(off)
     const int w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= range) return;
     const int nm = green ? 2 : 4;
     double K[16];
     for (int e = 0; e < 16; ++e) K[e] = 0.0;
+    // K^T K summed over the rows in order, Re row then Im row of each
+    // frequency, one fma per row (zero entries included)
     for (int wn = w; wn < n_freq; ++wn) {
         const double4 c = tab[wn];
         double re[4] = {0, 0, 0, 0}, im[4] = {0, 0, 0, 0};
@@ -113,23 +115,38 @@ __global__ void k_mom_ktk(int n_freq, int range, int green, const double4* __res
             im[1] = c.z;
         }
         for (int b = 0; b < nm; ++b)
-            for (int a = 0; a < nm; ++a) K[a + b * 4] += re[a] * re[b] + im[a] * im[b];
+            for (int a = 0; a < nm; ++a) {
+                K[a + b * 4] = fma(re[a], re[b], K[a + b * 4]);
+                K[a + b * 4] = fma(im[a], im[b], K[a + b * 4]);
+            }
     }
     double X[16];
     lu_inverse4(K, nm, X);
     for (int e = 0; e < 16; ++e) kinv[(size_t)w * 16 + e] = X[e];
 }
 
-// mt[a][w][e] = sum_b Kinv[w][a][b] (K^T g_e)_b     (:239-243)
+// mt[a][w][e] = sum_rows KM_solve[w][a][row] g_e[row], with the rows of
+// KM_solve = (K^T K)^{-1} K^T formed first, as the reference associates the
+// product (FourierTransform.cpp:239-243): per frequency the Re row of the
+// design matrix contributes sum_b Kinv[a][b] Re_b and the Im row
+// sum_b Kinv[a][b] Im_b (the rows' zero entries left out), each taken times
+// its half of g and added, Re row first, in ascending frequency order.
 __global__ void k_mom_fit(const cd* __restrict__ fw, int d, int n_freq, int range, const double4* __restrict__ tab,
                           int green, const double* __restrict__ kinv, cd* __restrict__ mt)
 {
+#pragma clang fp contract(off)
     const int m2 = d * d;
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (size_t)range * m2) return;
     const int w = (int)(t / m2), e = (int)(t % m2);
     const int al = e % d, be = e / d;
     const int nm = green ? 2 : 4;
+    const double* X = kinv + (size_t)w * 16;  // Kinv(a, b) at X[a + 4 b]
+    double xk[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) xk[a][b] = X[a + 4 * b];
     cd s[4] = {cd{0, 0}, cd{0, 0}, cd{0, 0}, cd{0, 0}};
     for (int wn = w; wn < n_freq; ++wn) {
         const double4 c = tab[wn];
@@ -139,22 +156,25 @@ __global__ void k_mom_fit(const cd* __restrict__ fw, int d, int n_freq, int rang
         const cd re = cd{0.5 * (fab.x + fba.x), 0.5 * (fab.y - fba.y)};
         cd im = cd{0.5 * (fab.y + fba.y), -0.5 * (fab.x - fba.x)};
         if (!green) {
-            s[0] = cadd(s[0], re);
-            s[1] = cadd(s[1], cscale(im, c.x));
-            s[2] = cadd(s[2], cscale(re, c.y));
-            s[3] = cadd(s[3], cscale(im, c.z));
+            // Re row [1, 0, -1/z^2, 0], Im row [0, -1/z, 0, 1/z^3]   (:227-230)
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const double kr = fma(xk[a][2], c.y, xk[a][0]);
+                const double ki = fma(xk[a][3], c.z, xk[a][1] * c.x);
+                s[a] = cadd(s[a], cd{fma(ki, im.x, kr * re.x), fma(ki, im.y, kr * re.y)});
+            }
         } else {
+            // Re row [-1/z^2, 0], Im row [0, 1/z^3]   (:366-367)
             if (al == be) im.x += c.w;  // (:373)
-            s[0] = cadd(s[0], cscale(re, c.y));
-            s[1] = cadd(s[1], cscale(im, c.z));
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                const double kr = xk[a][0] * c.y;
+                const double ki = xk[a][1] * c.z;
+                s[a] = cadd(s[a], cd{fma(ki, im.x, kr * re.x), fma(ki, im.y, kr * re.y)});
+            }
         }
     }
-    const double* X = kinv + (size_t)w * 16;
-    for (int a = 0; a < nm; ++a) {
-        cd m = cd{0, 0};
-        for (int b = 0; b < nm; ++b) m = cadd(m, cscale(s[b], X[a + b * 4]));
-        mt[((size_t)a * range + w) * m2 + e] = m;
-    }
+    for (int a = 0; a < nm; ++a) mt[((size_t)a * range + w) * m2 + e] = s[a];
 }
 
 // trace[a][w] = sum_alpha Re mt[a][w][alpha alpha]     (:244-247)
@@ -179,7 +199,8 @@ __global__ void k_mom_avg(const cd* __restrict__ mt, int m2, int range, int nm, 
     const int a = (int)(t / ((size_t)m2 * ncent));
     cd s = cd{0, 0};
     for (int iw = c; iw <= c + 2 * Nv; ++iw) s = cadd(s, mt[((size_t)a * range + iw) * m2 + e]);
-    avg[t] = cscale(s, 1.0 / (2 * Nv + 1));
+    const double cnt = 2 * Nv + 1;
+    avg[t] = cd{s.x / cnt, s.y / cnt};
 }
 
 // positivity matrix per window centre:
@@ -266,12 +287,16 @@ __global__ __launch_bounds__(kSelThreads) void k_mom_select(const double* __rest
     __shared__ int bi[kSelThreads];
     const int a = blockIdx.x, t = threadIdx.x;
     auto spread = [&](int i) {
+#pragma clang fp contract(off)
         const double* v = trace + (size_t)a * range + i;  // centre i + Nv, window [i, i + 2 Nv]
         double mean = 0.0;
         for (int j = 0; j < 2 * Nv + 1; ++j) mean += v[j];
         mean /= (2 * Nv + 1);
         double ss = 0.0;
-        for (int j = 0; j < 2 * Nv + 1; ++j) ss += (v[j] - mean) * (v[j] - mean);
+        for (int j = 0; j < 2 * Nv + 1; ++j) {
+            const double dv = v[j] - mean;
+            ss += dv * dv;
+        }
         return sqrt(ss / (2 * Nv + 1));
     };
     double best = 0.0;
@@ -336,8 +361,9 @@ __global__ void k_mom_final(const cd* __restrict__ mt, int d, int range, int gre
         s = cadd(s, cscale(cadd(m[al + be * d], cconj(m[be + al * d])), 0.5));
         st = cadd(st, cscale(cadd(m[be + al * d], cconj(m[al + be * d])), 0.5));
     }
-    s = cscale(s, 1.0 / (2 * Nv + 1));
-    st = cscale(st, 1.0 / (2 * Nv + 1));
+    const double cnt = 2 * Nv + 1;
+    s = cd{s.x / cnt, s.y / cnt};
+    st = cd{st.x / cnt, st.y / cnt};
     out[e] = cscale(cadd(s, cconj(st)), 0.5);
 }
 
@@ -782,8 +808,13 @@ extern "C" int dmftgpu_moments(dmftgpu_handle h, int d, int n_freq, double beta,
     hipStream_t s = h->stream;
     if (!hit) {
         h->mom_key_valid = false;
-        hipLaunchKernelGGL(k_mom_basis, dim3(nblk(n_freq)), dim3(256), 0, s, n_freq, beta, tab);
-        HIPCHK(hipGetLastError());
+        std::vector<double4> htab(n_freq);
+        for (int wn = 0; wn < n_freq; ++wn) {
+            const double z = (2 * wn + 1) * kPi / beta;  // (:224, :364)
+            htab[wn] = double4{-1. / std::pow(z, 1), -1. / std::pow(z, 2), 1. / std::pow(z, 3), 1.0 / z};
+        }
+        HIPCHK(hipMemcpyAsync(tab, htab.data(), sizeof(double4) * n_freq, hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));  // htab is a host temporary
         hipLaunchKernelGGL(k_mom_ktk, dim3(nblk(range)), dim3(256), 0, s, n_freq, range, green_asym, tab, kinv);
         HIPCHK(hipGetLastError());
         h->mom_key_valid = true;

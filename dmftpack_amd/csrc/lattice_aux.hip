@@ -154,6 +154,52 @@ hipError_t launch_static_reduce(const double* wt, const double* dt, const cd* U,
     return hipGetLastError();
 }
 
+// Core-band term of GiwToNele (GreenFtn.cpp:622-631): the reference adds
+// f(eps_kl - mu) over all NumOrbit Kohn-Sham bands and subtracts it over the
+// k point's window bands, i.e. it counts the bands outside the window.  Here
+// those bands are flagged (out[k][l] = 1) and summed directly, f unclamped as
+// in the reference, with the mu-derivative beta f (1 - f).  One workgroup per
+// chunk of k_per_chunk k points; fixed-order block sums into pN[ch], pdN[ch].
+__global__ __launch_bounds__(256) void k_core_bands(const double* __restrict__ eps,
+                                                    const unsigned char* __restrict__ out, int n_orb, int nk,
+                                                    int k_per_chunk, double beta, double mu, double* __restrict__ pN,
+                                                    double* __restrict__ pdN)
+{
+    __shared__ double red[2][256];
+    const int ch = blockIdx.x;
+    const size_t t0 = (size_t)ch * k_per_chunk * n_orb;
+    const size_t t1 = (size_t)min(nk, (ch + 1) * k_per_chunk) * n_orb;
+    double sN = 0.0, sD = 0.0;
+    for (size_t t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
+        if (!out[t]) continue;
+        const double f = 1.0 / (1.0 + exp(beta * (eps[t] - mu)));
+        sN += f;
+        sD += beta * f * (1.0 - f);
+    }
+    red[0][threadIdx.x] = sN;
+    red[1][threadIdx.x] = sD;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + o];
+            red[1][threadIdx.x] += red[1][threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        pN[ch] = red[0][0];
+        pdN[ch] = red[1][0];
+    }
+}
+
+hipError_t launch_core_bands(const double* eps, const unsigned char* out, int n_orb, int nk, int k_per_chunk,
+                             int n_chunks, double beta, double mu, double* pN, double* pdN, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_core_bands, dim3(n_chunks), dim3(256), 0, s, eps, out, n_orb, nk, k_per_chunk, beta, mu, pN,
+                       pdN);
+    return hipGetLastError();
+}
+
 // GwHF: thread per (nw, e); loops over all k and levels.  Optional output,
 // written only to files by the reference (GreenFtn.cpp:394-407).
 __global__ __launch_bounds__(256) void k_gloc_hf(const double* __restrict__ ev, const cd* __restrict__ U, int n,

@@ -155,6 +155,11 @@ hipError_t launch_static_terms(const double* ev, int n, int nk, int nw_tot, doub
 hipError_t launch_static_reduce(const double* wt, const double* dt, const cd* U, int n, int nc, int nk,
                                 int k_per_chunk, int n_chunks, cd* pNum, double* pN, double* pdN, hipStream_t s);
 
+// Core-band term of GiwToNele: per-chunk sums of f(eps - mu) and beta f (1 - f)
+// over the Kohn-Sham bands flagged out[k][l] = 1 (outside the model window)
+hipError_t launch_core_bands(const double* eps, const unsigned char* out, int n_orb, int nk, int k_per_chunk,
+                             int n_chunks, double beta, double mu, double* pN, double* pdN, hipStream_t s);
+
 // loss cotangents gbar = 2 (G_loc - G_tgt) inside cluster blocks (npc orbitals per
 // cluster), 0 across them; *loss = sum |G_loc - G_tgt|^2 (device scalar)
 hipError_t launch_loss_cotangent(const cd* gloc, const cd* gtgt, int n_freq, int nc, int npc, cd* gbar, double* loss,

@@ -198,6 +198,17 @@ class LatticeGF(Handle):
         s._dmft_colmajor = True
         return s
 
+    def set_core_bands(self, ks_energy, val_to_ks):
+        """KS_eigenEnergy [nk][NumOrbit] and FromValToKS [nk][n] (-1 = padded band) for the
+        core-band term of GiwToNele (GreenFtn.cpp:622-631); None removes them."""
+        if ks_energy is None:
+            self.check(self.lib.dmftgpu_set_core_bands(self.h, 0, None, None))
+            return
+        eps = np.ascontiguousarray(np.asarray(ks_energy, dtype=np.float64))
+        v2k = np.ascontiguousarray(np.asarray(val_to_ks, dtype=np.int32))
+        self.check(self.lib.dmftgpu_set_core_bands(self.h, eps.shape[1], eps.ctypes.data_as(C.c_void_p),
+                                                   v2k.ctypes.data_as(C.POINTER(C.c_int))))
+
     def static_eig(self):
         ev = torch.zeros((self.nk, self.n), dtype=torch.float64, device="cuda")
         U = torch.zeros((self.nk, self.n, self.n), dtype=torch.complex128, device="cuda")

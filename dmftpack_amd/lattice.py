@@ -332,3 +332,42 @@ def _all_r_idx(n_orb):
                     for j in range(n_orb):
                         rows.append((r1, r2, r3, i, j))
     return np.array(rows, dtype=np.int32)
+
+
+def windowed_case(n, n_orb, nk_side, seed, n_below=2, padded=0):
+    """A downfolded lattice with Kohn-Sham bands outside the model window
+    (downfolding.cpp:162-175, GiwToNele's core-band term GreenFtn.cpp:622-631):
+    ks_energy [nk][n_orb] ascending per k, the window being bands
+    n_below .. n_below+n-1 (val_to_ks [nk][n], the window band order shuffled
+    per k); H(k) = U diag(window energies) U^H with a seeded unitary U, so
+    eig H(k) are the window's KS energies.  The bands below the window lie in
+    [-1.6, -0.4], those above in [0.6, 1.8], close enough to mu ~ 0 that their
+    Fermi factors and derivatives matter at beta ~ 10.  padded > 0 marks the
+    last `padded` window bands of every other k as absent (val_to_ks = -1, the
+    shim's padding of a smaller NBAND[k]): their H(k) rows and columns become
+    decoupled levels at 1e4."""
+    kp = kmesh_phases(nk_side)
+    nk = kp.shape[0]
+    n_above = n_orb - n - n_below
+    assert n_above >= 0
+    u = uniform(seed, 601, nk * n_orb).reshape(nk, n_orb)
+    eps = np.empty((nk, n_orb))
+    eps[:, :n_below] = -1.6 + 1.2 * u[:, :n_below]
+    eps[:, n_below:n_below + n] = -0.8 + 1.6 * u[:, n_below:n_below + n]
+    eps[:, n_below + n:] = 0.6 + 1.2 * u[:, n_below + n:]
+    eps = np.sort(eps, axis=1)
+    perm = np.argsort(uniform(seed, 602, nk * n).reshape(nk, n), axis=1)
+    v2k = (n_below + perm).astype(np.int32)
+    U = _unitary(seed, 603, nk, n)
+    win = np.take_along_axis(eps, v2k, axis=1)
+    if padded:
+        for k in range(1, nk, 2):
+            v2k[k, n - padded:] = -1
+            win[k, n - padded:] = 1.0e4
+            U[k, :, n - padded:] = 0.0
+            U[k, n - padded:, :] = 0.0
+            U[k, n - padded:, n - padded:] = np.eye(padded)
+            q, r = np.linalg.qr(U[k, :n - padded, :n - padded])
+            U[k, :n - padded, :n - padded] = q
+    Hk = (U * win[:, None, :]) @ U.conj().transpose(0, 2, 1)
+    return dict(Hk=0.5 * (Hk + Hk.conj().transpose(0, 2, 1)), ks_energy=eps, val_to_ks=v2k, kphase=kp)

@@ -146,14 +146,31 @@ int dmftgpu_greenftn_real_k(dmftgpu_handle h, int method, int n_e, double e0, do
  * Full n x n inverse, DMFTGPU_PLAIN_F64 only. */
 int dmftgpu_density_matrix(dmftgpu_handle h, const double* sigma, double mu, double* dens);
 
+/* Kohn-Sham bands of the bound H(k) for the core-band term of GiwToNele
+ * (GreenFtn.cpp:622-631), which counts the Fermi weight of the KS bands that
+ * lie outside the model window:
+ *     (1/knum_mpiGlobal) sum_k [ sum_{l<NumOrbit} f(eps_kl - mu) - sum_{l0<NBAND} f(eps_k,FromValToKS[k][l0] - mu) ]
+ *   n_orb      NumOrbit (TB.h:7), 1..4096; 0 removes the bands again
+ *   ks_energy  [n_k][n_orb] KS_eigenEnergy (TB.h:19), host or device
+ *   val_to_ks  [n_k][n] HOST FromValToKS (TB.h:31): the KS band of each window
+ *              band, distinct per k; -1 marks a padded band (NBAND[k] < n)
+ * Once bound, N(mu) of dmftgpu_density, dmftgpu_adjust_mu and dmftgpu_greenftn_w
+ * (n_total) is the total electron number of GiwToNele, and dN/dmu, mu_bar and
+ * the loss of dmftgpu_gloc_loss_grad include the term.  dmftgpu_prepare
+ * removes the binding (the bands belong to the H(k) it replaces). */
+int dmftgpu_set_core_bands(dmftgpu_handle h, int n_orb, const double* ks_energy, const int* val_to_ks);
+
 /* Total electron number N(mu) of GiwToNele (GreenFtn.cpp:513-635, via the
- * trace identity sum_l 1/(z+mu-lambda_l) = tr G) and its analytic dN/dmu.
+ * trace identity sum_l 1/(z+mu-lambda_l) = tr G) and its analytic dN/dmu,
+ * with the core-band term when Kohn-Sham bands are bound (dmftgpu_set_core_bands).
  * n_out, dn_dmu_out are HOST pointers (either may be NULL). */
 int dmftgpu_density(dmftgpu_handle h, int method, const double* sigma, double mu, double* n_out, double* dn_dmu_out);
 
-/* GreenFtn_w_adjust_mu (GreenFtn.cpp:66-139): mu with N(mu) = n_target.
- * Safeguarded Newton on the analytic dN/dmu (bracket + bisection fallback),
- * stopping when |dmu| <= tol.  mu0 initial guess; mu_out, n_eval HOST. */
+/* GreenFtn_w_adjust_mu (GreenFtn.cpp:66-139): mu with N(mu) = n_target
+ * (NumberOfElectron: with bound Kohn-Sham bands the total count, core bands
+ * included, as in the reference).  Safeguarded Newton on the analytic dN/dmu
+ * (bracket + bisection fallback), stopping when |dmu| <= tol.  mu0 initial
+ * guess; mu_out, n_eval HOST. */
 int dmftgpu_adjust_mu(dmftgpu_handle h, int method, const double* sigma, double mu0, double n_target, double tol,
                       double* mu_out, int* n_eval);
 

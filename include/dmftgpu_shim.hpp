@@ -26,8 +26,10 @@
 #include <complex>
 #include <cstdio>
 #include <cstdlib>
+#include <cmath>
 #include <cstring>
 #include <functional>
+#include <string>
 #include <vector>
 
 #include "dmftgpu.h"
@@ -46,6 +48,8 @@ struct Globals {
     double NumberOfElectron = 0.0;    // tight_common.h:64
     int method = DMFTGPU_PLAIN_F64;   // per-(k, iw) solver
     double mu_tolerance = 1e-5;       // |dmu| stop of GreenFtn_w_adjust_mu (GreenFtn.cpp:96)
+    int mpi_rank = 0;                 // dmft_common.h:44; rank 0 prints and writes files (ifroot)
+    bool write_files = false;         // GreenFtn_w writes Gw_loc*.dat / GwHF_loc*.dat (GreenFtn.cpp:387-421)
     // optional cross-rank SUM of a complex128 buffer (identity when unset)
     std::function<void(cplx*, size_t)> allreduce;
 };
@@ -159,39 +163,153 @@ inline std::vector<cplx> moments_dev(dmftgpu_handle h, int d, int n_freq, double
     return out;
 }
 
+namespace detail {
+// Eigenvalues of a small Hermitian matrix (column-major): the real symmetric
+// embedding [[Re A, -Im A], [Im A, Re A]] has each eigenvalue of A twice; it is
+// diagonalised by cyclic Jacobi rotations (tan of the angle from the stable
+// root of t^2 + 2 theta t - 1 = 0) until the off-diagonal mass is below 1e-30
+// of the total.  Host only; used for the NumMatrix check of GreenFtn_w.
+inline std::vector<double> herm_eigvals(const cplx* A, int n)
+{
+    const int m = 2 * n;
+    std::vector<double> B((size_t)m * m);
+    auto b = [&](int i, int j) -> double& { return B[i + (size_t)j * m]; };
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+            const cplx v = A[i + (size_t)j * n];
+            b(i, j) = b(i + n, j + n) = v.real();
+            b(i + n, j) = v.imag();
+            b(i, j + n) = -v.imag();
+        }
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0, tot = 0.0;
+        for (int j = 0; j < m; ++j)
+            for (int i = 0; i < m; ++i) (i == j ? tot : off) += b(i, j) * b(i, j);
+        if (off <= 1e-30 * (tot + off)) break;
+        for (int p = 0; p < m - 1; ++p)
+            for (int q = p + 1; q < m; ++q) {
+                if (b(p, q) == 0.0) continue;
+                const double theta = (b(q, q) - b(p, p)) / (2.0 * b(p, q));
+                const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
+                for (int k = 0; k < m; ++k) {
+                    const double bkp = b(k, p), bkq = b(k, q);
+                    b(k, p) = c * bkp - sn * bkq;
+                    b(k, q) = sn * bkp + c * bkq;
+                }
+                for (int k = 0; k < m; ++k) {
+                    const double bpk = b(p, k), bqk = b(q, k);
+                    b(p, k) = c * bpk - sn * bqk;
+                    b(q, k) = sn * bpk + c * bqk;
+                }
+            }
+    }
+    std::vector<double> d(m);
+    for (int i = 0; i < m; ++i) d[i] = b(i, i);
+    std::sort(d.begin(), d.end());
+    std::vector<double> w(n);
+    for (int i = 0; i < n; ++i) w[i] = 0.5 * (d[2 * i] + d[2 * i + 1]);
+    return w;
+}
+
+// The per-cluster files of GreenFtn_w (GreenFtn.cpp:387-421):
+//   Gw_loc.dat<cl>, GwHF_loc.dat<cl>: "%0.5f" of w_n, then "    %e    %e" per diagonal entry
+//   Gw_loc.full.dat<cl>: "%d %d %d %e    %e" per entry with |G| > 1e-8
+//   GwHF_loc.full.dat<cl>: the same line for every entry
+// (the solver loop reads the .full files back, dmft_main.cpp:394,398)
+template <class Matrix>
+void write_gw_loc_files(const Globals& g, int NumCluster, int npc, const std::vector<Matrix>& Gw,
+                        const std::vector<Matrix>& GwHF)
+{
+    const double pi = 3.1415926535897932384626;  // dmft_common.h:43
+    for (int at = 0; at < NumCluster; ++at) {
+        const std::string sfx = std::to_string(at);
+        std::FILE* f2 = std::fopen(("Gw_loc.dat" + sfx).c_str(), "w");
+        std::FILE* f3 = std::fopen(("Gw_loc.full.dat" + sfx).c_str(), "w");
+        std::FILE* h2 = std::fopen(("GwHF_loc.dat" + sfx).c_str(), "w");
+        std::FILE* h3 = std::fopen(("GwHF_loc.full.dat" + sfx).c_str(), "w");
+        if (!f2 || !f3 || !h2 || !h3) {
+            std::fprintf(stderr, "[dmftgpu] GreenFtn_w: cannot write Gw_loc files\n");
+            std::exit(1);
+        }
+        for (int n = 0; n < g.N_freq; ++n) {
+            const double wn = pi * (2 * n + 1) / g.beta;
+            std::fprintf(f2, "%0.5f", wn);
+            std::fprintf(h2, "%0.5f", wn);
+            for (int a = 0; a < npc; ++a) {
+                const int aF = at * npc + a;
+                const cplx ga = Gw[n](aF, aF), ha = GwHF[n](aF, aF);
+                std::fprintf(f2, "    %e    %e", ga.real(), ga.imag());
+                std::fprintf(h2, "    %e    %e", ha.real(), ha.imag());
+                for (int b = 0; b < npc; ++b) {
+                    const int bF = at * npc + b;
+                    const cplx gab = Gw[n](aF, bF), hab = GwHF[n](aF, bF);
+                    if (std::abs(gab) > 1e-8) std::fprintf(f3, "%d %d %d %e    %e\n", n, a, b, gab.real(), gab.imag());
+                    std::fprintf(h3, "%d %d %d %e    %e\n", n, a, b, hab.real(), hab.imag());
+                }
+            }
+            std::fprintf(f2, "\n");
+            std::fprintf(h2, "\n");
+        }
+        std::fclose(f2);
+        std::fclose(f3);
+        std::fclose(h2);
+        std::fclose(h3);
+        std::printf("FILEOUT:Gw_loc.dat%d\nFILEOUT:Gw_loc.full.dat%d\n", at, at);
+    }
+}
+}  // namespace detail
+
 // void GreenFtn_w(int NumCluster, int NumHartrOrbit_per_cluster, vector<MatrixXcd>& H_k,
 //                 ImgFreqFtn& SelfE_w, vector<MatrixXcd>& Gw, double mu, vector<MatrixXcd>& densityMatDFT)
-// (GreenFtn.cpp:142).  NumMatrix (the global written at :356) is returned through num_matrix.
+// (GreenFtn.cpp:142).  NumMatrix (the global written at :356) is returned through num_matrix,
+// with the reference's warning when an eigenvalue leaves [0, 1 + 1e-10] (:358-365).
+// GwHF (the static-part Green's function of :277, which the reference only writes to
+// GwHF_loc*.dat) is returned when GwHF is given or Globals::write_files is set; with
+// write_files rank 0 writes the four per-cluster files of :387-421.
 // densityMatDFT is filled when it is non-empty on entry (the caller pre-sizes it, TB.cpp:296-297).
 template <class Matrix, class SelfEnergy>
 void GreenFtn_w(dmftgpu_handle h, const Globals& g, int NumCluster, int NumHartrOrbit_per_cluster,
                 std::vector<Matrix>& H_k, SelfEnergy& SelfE_w, std::vector<Matrix>& Gw, double mu,
-                std::vector<Matrix>& densityMatDFT, Matrix& num_matrix)
+                std::vector<Matrix>& densityMatDFT, Matrix& num_matrix, std::vector<Matrix>* GwHF = nullptr)
 {
     const int nc = NumCluster * NumHartrOrbit_per_cluster;
     const int n = max_rows(H_k, g.knum);  // NBAND[k] may vary: padded to the largest
     const dmftgpu_lattice L = lattice_of(g, n, nc, NumCluster);
-    DevBuf hk((size_t)g.knum * n * n), sig((size_t)g.N_freq * nc * nc), gl((size_t)g.N_freq * nc * nc), nm((size_t)nc * nc);
+    const bool want_hf = GwHF != nullptr || g.write_files;
+    DevBuf hk((size_t)g.knum * n * n), sig((size_t)g.N_freq * nc * nc), gl((size_t)g.N_freq * nc * nc), nm((size_t)nc * nc),
+        ghf(want_hf ? (size_t)g.N_freq * nc * nc : 1);
     hk.upload(pack_hk(H_k, g.knum, n).data());
     sig.upload(pack_sigma(SelfE_w, g.N_freq, nc).data());
     std::vector<cplx> mom = moments_dev(h, nc, g.N_freq, g.beta, sig, false);  // :189-195
     DevBuf dmom(mom.size());
     dmom.upload(mom.data());
     die(h, dmftgpu_prepare(h, &L, (const double*)hk.p, (const double*)dmom.p), "prepare");
-    die(h, dmftgpu_greenftn_w(h, g.method, (const double*)sig.p, mu, (double*)gl.p, nullptr, (double*)nm.p, nullptr),
+    die(h, dmftgpu_greenftn_w(h, g.method, (const double*)sig.p, mu, (double*)gl.p, want_hf ? (double*)ghf.p : nullptr,
+                              (double*)nm.p, nullptr),
         "GreenFtn_w");
-    std::vector<cplx> hgl(gl.n), hnm(nm.n);
+    std::vector<cplx> hgl(gl.n), hnm(nm.n), hhf(want_hf ? ghf.n : 0);
     gl.download(hgl.data());
     nm.download(hnm.data());
-    if (g.allreduce) {  // :356, :368
+    if (want_hf) ghf.download(hhf.data());
+    if (g.allreduce) {  // :356, :368-369
         g.allreduce(hnm.data(), hnm.size());
         g.allreduce(hgl.data(), hgl.size());
+        if (want_hf) g.allreduce(hhf.data(), hhf.size());
     }
-    Gw.resize(g.N_freq);
-    for (int w = 0; w < g.N_freq; ++w) {
-        Gw[w].setZero(nc, nc);
-        std::memcpy(Gw[w].data(), &hgl[(size_t)w * nc * nc], sizeof(cplx) * nc * nc);
-    }
+    for (double ev : detail::herm_eigvals(hnm.data(), nc))  // :358-365 (a warning, the run goes on)
+        if ((ev < 0 || ev > 1 + 1e-10) && g.mpi_rank == 0) std::printf("[GreenFtn]: NumMatrix is unphysical, %g\n", ev);
+    auto unpack = [&](const std::vector<cplx>& src, std::vector<Matrix>& dst) {
+        dst.resize(g.N_freq);
+        for (int w = 0; w < g.N_freq; ++w) {
+            dst[w].setZero(nc, nc);
+            std::memcpy(dst[w].data(), &src[(size_t)w * nc * nc], sizeof(cplx) * nc * nc);
+        }
+    };
+    unpack(hgl, Gw);
+    std::vector<Matrix> hf_local;
+    std::vector<Matrix>& hf = GwHF ? *GwHF : hf_local;
+    if (want_hf) unpack(hhf, hf);
     num_matrix.setZero(nc, nc);
     std::memcpy(num_matrix.data(), hnm.data(), sizeof(cplx) * nc * nc);
     if (!densityMatDFT.empty()) {  // :291-319
@@ -207,15 +325,21 @@ void GreenFtn_w(dmftgpu_handle h, const Globals& g, int NumCluster, int NumHartr
                             sizeof(cplx) * nb);
         }
     }
+    if (g.write_files && g.mpi_rank == 0) detail::write_gw_loc_files(g, NumCluster, NumHartrOrbit_per_cluster, Gw, hf);
 }
 
 // double GreenFtn_w_adjust_mu(vector<MatrixXcd>& H_k, ImgFreqFtn& SelfE_w, double mu, double dmu, int mu_adjust)
 // (GreenFtn.cpp:66).  Sigma and m0 enter through their intra-cluster blocks, as in
 // valenceHAMILTONIAN.  Newton on the analytic dN/dmu replaces the bracketing of :93-121.
-template <class Matrix, class SelfEnergy>
+// The target is Globals::NumberOfElectron, the total electron count: GiwToNele adds the
+// Fermi weight of the Kohn-Sham bands outside the window (:622-631), from the globals
+// KS_eigenEnergy[k][l] (l < NumOrbit) and FromValToKS[k][l0] (l0 < NBAND[k]) that arrive
+// here as arguments (TB.h:7,19,31).
+template <class Matrix, class SelfEnergy, class Energies>
 double GreenFtn_w_adjust_mu(dmftgpu_handle h, const Globals& g, int NumCluster, int NumHartrOrbit_per_cluster,
                             std::vector<Matrix>& H_k, SelfEnergy& SelfE_w, double mu, double /*dmu*/,
-                            int /*mu_adjust*/)
+                            int /*mu_adjust*/, const Energies& KS_eigenEnergy,
+                            const std::vector<std::vector<int>>& FromValToKS, int NumOrbit)
 {
     const int nc = NumCluster * NumHartrOrbit_per_cluster;
     const int n = max_rows(H_k, g.knum);  // NBAND[k] may vary: padded to the largest
@@ -231,11 +355,33 @@ double GreenFtn_w_adjust_mu(dmftgpu_handle h, const Globals& g, int NumCluster, 
     DevBuf dmom(mom.size());
     dmom.upload(mom.data());
     die(h, dmftgpu_prepare(h, &L, (const double*)hk.p, (const double*)dmom.p), "prepare");
+    if (NumOrbit > 0) {  // :622-631
+        std::vector<double> eps((size_t)g.knum * NumOrbit);
+        std::vector<int> v2k((size_t)g.knum * n, -1);  // -1: a padded band (l0 >= NBAND[k])
+        for (int k = 0; k < g.knum; ++k) {
+            for (int l = 0; l < NumOrbit; ++l) eps[(size_t)k * NumOrbit + l] = KS_eigenEnergy[k][l];
+            const int nb = (int)H_k[k].rows();
+            for (int l0 = 0; l0 < nb; ++l0) v2k[(size_t)k * n + l0] = FromValToKS[k][l0];
+        }
+        die(h, dmftgpu_set_core_bands(h, NumOrbit, eps.data(), v2k.data()), "GreenFtn_w_adjust_mu (core bands)");
+    }
     double mu_out = mu;
     int ne = 0;
     die(h, dmftgpu_adjust_mu(h, g.method, (const double*)sig.p, mu, g.NumberOfElectron, g.mu_tolerance, &mu_out, &ne),
         "GreenFtn_w_adjust_mu");
+    if (g.mpi_rank == 0) std::printf("Adjusted Chemical potential : From %2.5f to %2.5f  (RD= %+2.5f )\n", mu, mu_out, mu_out - mu);
     return mu_out;
+}
+
+// Model-space-only form (no Kohn-Sham bands outside the window: NumOrbit = NBAND and
+// FromValToKS the identity, whose core-band term is zero).
+template <class Matrix, class SelfEnergy>
+double GreenFtn_w_adjust_mu(dmftgpu_handle h, const Globals& g, int NumCluster, int NumHartrOrbit_per_cluster,
+                            std::vector<Matrix>& H_k, SelfEnergy& SelfE_w, double mu, double dmu, int mu_adjust)
+{
+    const std::vector<std::vector<double>> none;
+    return GreenFtn_w_adjust_mu(h, g, NumCluster, NumHartrOrbit_per_cluster, H_k, SelfE_w, mu, dmu, mu_adjust, none,
+                                std::vector<std::vector<int>>(), 0);
 }
 
 // void getAsymto_moments(vector<MatrixXcd>& moments, MatrixXcd* delta_w, bool greenFtnAsym) (tight_common.h:141)

@@ -26,7 +26,17 @@
  *   orc_giw_to_nele      eigenvalue formula of GreenFtn.cpp:600-634 evaluated
  *                        with numpy.linalg.eigvals (two independent codings)
  *   orc_adjust_mu        N(mu_out) = target within the bisection tolerance
- *   orc_asymto_moments   exact recovery of polynomial tails, single pole
+ *   orc_asymto_moments   exact recovery of polynomial tails, single pole;
+ *                        the fallback to the last window when no window passes
+ *                        the positivity tests (m1 or Vi m3 Vi - (Vi m2 Vi)^2
+ *                        not positive definite); the window against an
+ *                        independent numpy lstsq scan where the spread
+ *                        minimum is clear
+ *   orc_select_window    constructed spreads / flags: first strict minimum,
+ *                        positivity filter, last-centre fallback
+ *   orc_causality_patch  constructed G(tau): one and two positive interior
+ *                        points (exact linear interpolants), no negative
+ *                        continuation and positive endpoints (-1e-10 I)
  *   orc_ft_w2tau         two-pole closed form, G(0)+G(beta) = -m1
  *   orc_ft_tau2w         exact transform of piecewise-linear G(tau)
  *   orc_gloc_vjp         central finite differences of orc_density/orc_loss
@@ -66,6 +76,9 @@ typedef struct orc_lattice {
 } orc_lattice;
 
 #define IDX(i, j, ld) ((size_t)(i) + (size_t)(j) * (size_t)(ld))
+
+int orc_select_window(int ncent, const double *var, const int *ok);
+int orc_causality_patch(int d, int n_tau, double beta, cplx *gt);
 
 /* ------------------------------------------------------------------------- */
 /* Small dense kernels (Eigen restatements)                                   */
@@ -461,8 +474,43 @@ int orc_greenftn_w(const orc_lattice *L, const cplx *Hk, const cplx *sigma, cons
 /* equals it for any diagonalisable H+Sigma; lambda_inf are the (real)         */
 /* eigenvalues of the Hermitian H + P m0 P^H.  Sigma and m0 enter through      */
 /* their intra-cluster blocks only (:32-39).  The core-band term (:622-631)    */
-/* is zero for a model-space-only Hamiltonian and is omitted.                  */
+/* is added when core != NULL (orc_core_bands).                                */
 /* ------------------------------------------------------------------------- */
+/* Kohn-Sham bands of the core-band term: eps [n_k][n_orb] (KS_eigenEnergy),
+ * val_to_ks [n_k][n] (FromValToKS; -1 marks a padded band, NBAND[k] < n). */
+typedef struct orc_core {
+    int n_orb;
+    const double *eps;
+    const int *val_to_ks;
+} orc_core;
+
+/* Core-band term of GiwToNele (GreenFtn.cpp:622-631), unnormalised sum over
+ * this rank's k: f over all NumOrbit KS bands, minus f over the window bands
+ * FromValToKS[k][l0], l0 < NBAND[k], in the reference's order.  *dn_out gets
+ * the mu-derivative of the same expression (beta f (1-f) per term). */
+double orc_core_bands(const orc_lattice *L, const orc_core *c, double mu, double *dn_out)
+{
+    const double beta = L->beta;
+    double s = 0.0, ds = 0.0;
+    for (int k = 0; k < L->n_k; ++k) {
+        const double *e = c->eps + (size_t)k * c->n_orb;
+        for (int l = 0; l < c->n_orb; ++l) {
+            double f = 1. / (1 + exp(beta * (e[l] - mu)));
+            s += f;
+            ds += beta * f * (1.0 - f);
+        }
+        for (int l0 = 0; l0 < L->n; ++l0) {
+            int l = c->val_to_ks[(size_t)k * L->n + l0];
+            if (l < 0) continue; /* padded band: l0 >= NBAND[k] */
+            double f = 1. / (1 + exp(beta * (e[l] - mu)));
+            s -= f;
+            ds -= beta * f * (1.0 - f);
+        }
+    }
+    if (dn_out) *dn_out = ds;
+    return s;
+}
+
 static void cluster_mask(const orc_lattice *L, const cplx *in, cplx *out)
 {
     const int nc = L->n_corr, npc = nc / L->n_cluster;
@@ -470,7 +518,8 @@ static void cluster_mask(const orc_lattice *L, const cplx *in, cplx *out)
         for (int i = 0; i < nc; ++i) out[IDX(i, j, nc)] = (i / npc == j / npc) ? in[IDX(i, j, nc)] : 0.0;
 }
 
-double orc_giw_to_nele(const orc_lattice *L, const cplx *Hk, const cplx *sigma, const cplx *mom, double mu, int nthreads)
+double orc_giw_to_nele(const orc_lattice *L, const cplx *Hk, const cplx *sigma, const cplx *mom, double mu,
+                       const orc_core *core, int nthreads)
 {
     const int n = L->n, nc = L->n_corr;
     const size_t n2 = (size_t)n * n, m2 = (size_t)nc * nc;
@@ -515,17 +564,18 @@ double orc_giw_to_nele(const orc_lattice *L, const cplx *Hk, const cplx *sigma, 
         free(A); free(G); free(sw); free(swm); free(linf);
     }
     free(m0);
+    if (core) total += orc_core_bands(L, core, mu, NULL); /* :622-631 */
     return total / (double)L->n_k_global; /* :632 */
 }
 
 /* GreenFtn_w_adjust_mu (GreenFtn.cpp:66-139): bracket by steps of dmu, then
  * bisect until |dmu| <= 1e-5.  Returns mu; *n_eval counts N(mu) evaluations. */
 double orc_adjust_mu(const orc_lattice *L, const cplx *Hk, const cplx *sigma, const cplx *mom, double mu, double dmu,
-                     double n_target, int nthreads, int *n_eval)
+                     double n_target, const orc_core *core, int nthreads, int *n_eval)
 {
     double nele[2];
     int evals = 0;
-    nele[0] = orc_giw_to_nele(L, Hk, sigma, mom, mu, nthreads);
+    nele[0] = orc_giw_to_nele(L, Hk, sigma, mom, mu, core, nthreads);
     nele[1] = -1;
     ++evals;
     double muUB = 0, muLB = 0, goal = n_target;
@@ -535,7 +585,7 @@ double orc_adjust_mu(const orc_lattice *L, const cplx *Hk, const cplx *sigma, co
     while (nele[0] != goal && fabs(dmu) > 1e-5) {
         mu += dmu;
         nele[1] = nele[0];
-        nele[0] = orc_giw_to_nele(L, Hk, sigma, mom, mu, nthreads);
+        nele[0] = orc_giw_to_nele(L, Hk, sigma, mom, mu, core, nthreads);
         ++evals;
         if (nearAns == 0) {
             if (nele[0] < goal) {
@@ -566,6 +616,62 @@ double orc_adjust_mu(const orc_lattice *L, const cplx *Hk, const cplx *sigma, co
 /* fw [n_freq][d][d]; moments out [4][d][d].                                   */
 /* Returns the selected window centre per moment in win_out[4] (may be NULL).  */
 /* ------------------------------------------------------------------------- */
+/* The window scan of getAsymto_moments (FourierTransform.cpp:289-315, the G
+ * branch :412-436): minPos starts at the last centre and moves to centre i
+ * when i passes the positivity tests (ok[i]) and its spread is below the
+ * current minPos's, scanning i upwards.  Returns minPos (the centre is
+ * minPos + Nv). */
+int orc_select_window(int ncent, const double *var, const int *ok)
+{
+    int minPos = ncent - 1;
+    for (int i = 0; i < ncent; ++i)
+        if (ok[i] && var[i] < var[minPos]) minPos = i;
+    return minPos;
+}
+
+/* Real partial-pivot LU inverse (n <= 4) for the moment fit's normal matrix:
+ * the steps of orc_lu_inverse on real data, multiply-adds as explicit fma. */
+static void lu_inverse_real(int n, const double *A, double *Ainv)
+{
+    double lu[16];
+    int perm[4];
+    memcpy(lu, A, sizeof(double) * (size_t)n * n);
+    for (int i = 0; i < n; ++i) perm[i] = i;
+    for (int k = 0; k < n; ++k) {
+        int p = k;
+        double best = fabs(lu[IDX(k, k, n)]);
+        for (int i = k + 1; i < n; ++i)
+            if (fabs(lu[IDX(i, k, n)]) > best) { best = fabs(lu[IDX(i, k, n)]); p = i; }
+        if (p != k) {
+            for (int j = 0; j < n; ++j) {
+                double t = lu[IDX(k, j, n)];
+                lu[IDX(k, j, n)] = lu[IDX(p, j, n)];
+                lu[IDX(p, j, n)] = t;
+            }
+            int t = perm[k]; perm[k] = perm[p]; perm[p] = t;
+        }
+        if (best != 0.0) {
+            double inv = 1.0 / lu[IDX(k, k, n)];
+            for (int i = k + 1; i < n; ++i) lu[IDX(i, k, n)] *= inv;
+        }
+        for (int j = k + 1; j < n; ++j) {
+            double u = lu[IDX(k, j, n)];
+            for (int i = k + 1; i < n; ++i) lu[IDX(i, j, n)] = fma(-lu[IDX(i, k, n)], u, lu[IDX(i, j, n)]);
+        }
+    }
+    for (int c = 0; c < n; ++c) {
+        double x[4];
+        for (int i = 0; i < n; ++i) x[i] = (perm[i] == c) ? 1.0 : 0.0;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < i; ++j) x[i] = fma(-lu[IDX(i, j, n)], x[j], x[i]);
+        for (int i = n - 1; i >= 0; --i) {
+            for (int j = i + 1; j < n; ++j) x[i] = fma(-lu[IDX(i, j, n)], x[j], x[i]);
+            x[i] /= lu[IDX(i, i, n)];
+        }
+        for (int i = 0; i < n; ++i) Ainv[i + (size_t)c * n] = x[i];
+    }
+}
+
 int orc_asymto_moments(int d, int n_freq, double beta, const cplx *fw, int green_asym, cplx *moments, int *win_out)
 {
     const size_t m2 = (size_t)d * d;
@@ -592,20 +698,25 @@ int orc_asymto_moments(int d, int n_freq, double beta, const cplx *fw, int green
                 KM[(2 * jj + 1) + (size_t)1 * 2 * len] = 1. / pow(z, 3);
             }
         }
-        cplx KtK[16], KtKinv[16];
+        /* The normal matrix is real: K^T K by rows in order, its inverse by a
+         * real partial-pivot LU (Eigen's PartialPivLU steps), KM_solve formed
+         * before it meets g (:239-243).  Every multiply-add below is an explicit
+         * fma and the file is built without contraction, which fixes the
+         * rounding of this ill-conditioned fit to one operation sequence. */
+        double KtK[16], KtKinv[16];
         for (int b = 0; b < nmom; ++b)
             for (int a = 0; a < nmom; ++a) {
                 double s = 0;
-                for (int r = 0; r < 2 * len; ++r) s += KM[r + (size_t)a * 2 * len] * KM[r + (size_t)b * 2 * len];
+                for (int r = 0; r < 2 * len; ++r) s = fma(KM[r + (size_t)a * 2 * len], KM[r + (size_t)b * 2 * len], s);
                 KtK[a + b * nmom] = s;
             }
-        orc_lu_inverse(nmom, KtK, KtKinv);
+        lu_inverse_real(nmom, KtK, KtKinv);
         /* KM_solve (nmom x 2len) */
-        cplx *KS = (cplx *)malloc(sizeof(cplx) * (size_t)nmom * 2 * len);
+        double *KS = (double *)malloc(sizeof(double) * (size_t)nmom * 2 * len);
         for (int r = 0; r < 2 * len; ++r)
             for (int a = 0; a < nmom; ++a) {
-                cplx s = 0;
-                for (int b = 0; b < nmom; ++b) s += KtKinv[a + b * nmom] * KM[r + (size_t)b * 2 * len];
+                double s = 0;
+                for (int b = 0; b < nmom; ++b) s = fma(KtKinv[a + b * nmom], KM[r + (size_t)b * 2 * len], s);
                 KS[a + (size_t)r * nmom] = s;
             }
         for (int al = 0; al < d; ++al) {
@@ -617,10 +728,13 @@ int orc_asymto_moments(int d, int n_freq, double beta, const cplx *fw, int green
                     cplx fab = fw[(size_t)wn * m2 + IDX(al, be, d)];
                     cplx fba = fw[(size_t)wn * m2 + IDX(be, al, d)];
                     cplx re = (fab + conj(fba)) / 2.0;
-                    cplx im = (fab - conj(fba)) / (2.0 * I);
+                    /* (fab - conj(fba)) / (2i), written out (:231) */
+                    cplx im = CMPLX(0.5 * (cimag(fab) + cimag(fba)), -0.5 * (creal(fab) - creal(fba)));
                     if (green_asym && al == be) im += 1.0 / z; /* :373 */
-                    for (int a = 0; a < nmom; ++a)
-                        acc[a] += KS[a + (size_t)(2 * jj) * nmom] * re + KS[a + (size_t)(2 * jj + 1) * nmom] * im;
+                    for (int a = 0; a < nmom; ++a) {
+                        const double kr = KS[a + (size_t)(2 * jj) * nmom], ki = KS[a + (size_t)(2 * jj + 1) * nmom];
+                        acc[a] += CMPLX(fma(ki, creal(im), kr * creal(re)), fma(ki, cimag(im), kr * cimag(re)));
+                    }
                 }
                 for (int a = 0; a < nmom; ++a) mt[((size_t)w * m2 + IDX(al, be, d)) * nmom + a] = acc[a];
             }
@@ -683,10 +797,7 @@ int orc_asymto_moments(int d, int n_freq, double beta, const cplx *fw, int green
             }
             var[c - Nv] = sqrt(ss / (2 * Nv + 1));
         }
-        int minPos = ncent - 1;
-        for (int i = 0; i < ncent; ++i)
-            if (ok[i] && var[i] < var[minPos]) minPos = i;
-        int cen = minPos + Nv;
+        int cen = orc_select_window(ncent, var, ok) + Nv;
         int target = green_asym ? mom + 2 : mom;
         cplx *out = moments + (size_t)target * m2;
         memset(out, 0, sizeof(cplx) * m2);
@@ -795,6 +906,21 @@ int orc_ft_w2tau(int d, int n_freq, int n_tau, double beta, const cplx *fw, cons
     if (nthreads < 1) nthreads = 1;
 #pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads)
     for (int j = 0; j <= n_tau; ++j) orc_ft_tau(d, n_freq, beta, fw, mom, j * beta / n_tau, gt + (size_t)j * m2);
+    int check = orc_causality_patch(d, n_tau, beta, gt);
+    if (patched) *patched = check;
+    return 0;
+}
+
+/* The causality patch of FourierTransform_ (FourierTransform.cpp:145-175) on
+ * gt [n_tau+1][d][d], in place, in the reference's order: an interior tau_t
+ * whose G(tau_t) has no negative eigenvalue is replaced by the linear
+ * interpolation between G(tau_{t-1}) (already patched) and G(tau_t1) for
+ * t1 = t+1, t+2, ... until the interpolant has a negative eigenvalue; a point
+ * still without one (endpoints included, which are never interpolated)
+ * becomes -1e-10 I.  Returns CHECK (1 when any point was touched). */
+int orc_causality_patch(int d, int n_tau, double beta, cplx *gt)
+{
+    const size_t m2 = (size_t)d * d;
     int check = 0;
     double mine11 = 0;
     for (int t = 0; t <= n_tau; ++t) {
@@ -818,8 +944,7 @@ int orc_ft_w2tau(int d, int n_freq, int n_tau, double beta, const cplx *fw, cons
             for (int l = 0; l < d; ++l) gt[(size_t)t * m2 + IDX(l, l, d)] = -1e-10;
         }
     }
-    if (patched) *patched = check;
-    return 0;
+    return check;
 }
 
 /* FT_t_to_w (FourierTransform.cpp:521-561): exact integral of the piecewise
@@ -871,7 +996,7 @@ int orc_ft_tau2w(int d, int n_freq, int n_tau, double beta, const cplx *gt, cplx
 /* trace of the densityMatDFT of GreenFtn.cpp:291-319 (Fermi unclamped).       */
 /* ------------------------------------------------------------------------- */
 int orc_density(const orc_lattice *L, const cplx *Hk, const cplx *sigma, const cplx *mom, double mu, double *n_out,
-                double *dn_out, int nthreads)
+                double *dn_out, const orc_core *core, int nthreads)
 {
     const int n = L->n, nc = L->n_corr;
     const size_t n2 = (size_t)n * n, m2 = (size_t)nc * nc;
@@ -924,6 +1049,11 @@ int orc_density(const orc_lattice *L, const cplx *Hk, const cplx *sigma, const c
         free(A); free(G); free(sw);
     }
     free(ev); free(U);
+    if (core) {
+        double dc = 0.0;
+        Nt += orc_core_bands(L, core, mu, &dc);
+        dN += dc;
+    }
     if (n_out) *n_out = Nt / L->n_k_global;
     if (dn_out) *dn_out = dN / L->n_k_global;
     return 0;
@@ -946,7 +1076,7 @@ int orc_gloc_vjp(const orc_lattice *L, const cplx *Hk, const cplx *sigma, const 
     const double beta = L->beta;
     if (nthreads < 1) nthreads = 1;
     double dNdmu = 0;
-    orc_density(L, Hk, sigma, mom, mu, NULL, &dNdmu, nthreads);
+    orc_density(L, Hk, sigma, mom, mu, NULL, &dNdmu, NULL, nthreads);
     memset(sigma_bar, 0, sizeof(cplx) * (size_t)L->n_freq * m2);
     double mub = 0;
 #pragma omp parallel num_threads(nthreads)

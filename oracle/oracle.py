@@ -24,6 +24,22 @@ class OrcLattice(C.Structure):
                 ("n_k_global", C.c_int), ("n_freq", C.c_int), ("high_freq", C.c_int), ("beta", C.c_double)]
 
 
+class OrcCore(C.Structure):
+    _fields_ = [("n_orb", C.c_int), ("eps", C.c_void_p), ("val_to_ks", C.c_void_p)]
+
+
+def _core(core, nk, n):
+    """core = (ks_energy [nk][n_orb] float, val_to_ks [nk][n] int, -1 = padded band) or None.
+    Returns (ctypes pointer or None, arrays kept alive)."""
+    if core is None:
+        return None, None
+    eps = np.ascontiguousarray(core[0], dtype=np.float64)
+    v2k = np.ascontiguousarray(core[1], dtype=np.int32)
+    assert eps.shape[0] == nk and v2k.shape == (nk, n)
+    c = OrcCore(eps.shape[1], eps.ctypes.data_as(C.c_void_p), v2k.ctypes.data_as(C.c_void_p))
+    return C.byref(c), (c, eps, v2k)
+
+
 def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
@@ -36,13 +52,18 @@ def lib():
         _lib = C.CDLL(_LIB_PATH)
         P = C.c_void_p
         _lib.orc_greenftn_w.argtypes = [C.POINTER(OrcLattice), P, P, P, C.c_double, P, P, P, P, C.c_int]
-        _lib.orc_giw_to_nele.argtypes = [C.POINTER(OrcLattice), P, P, P, C.c_double, C.c_int]
+        _lib.orc_giw_to_nele.argtypes = [C.POINTER(OrcLattice), P, P, P, C.c_double, C.POINTER(OrcCore), C.c_int]
         _lib.orc_giw_to_nele.restype = C.c_double
-        _lib.orc_adjust_mu.argtypes = [C.POINTER(OrcLattice), P, P, P, C.c_double, C.c_double, C.c_double, C.c_int,
-                                       C.POINTER(C.c_int)]
+        _lib.orc_adjust_mu.argtypes = [C.POINTER(OrcLattice), P, P, P, C.c_double, C.c_double, C.c_double,
+                                       C.POINTER(OrcCore), C.c_int, C.POINTER(C.c_int)]
         _lib.orc_adjust_mu.restype = C.c_double
         _lib.orc_density.argtypes = [C.POINTER(OrcLattice), P, P, P, C.c_double, C.POINTER(C.c_double),
-                                     C.POINTER(C.c_double), C.c_int]
+                                     C.POINTER(C.c_double), C.POINTER(OrcCore), C.c_int]
+        _lib.orc_core_bands.argtypes = [C.POINTER(OrcLattice), C.POINTER(OrcCore), C.c_double,
+                                        C.POINTER(C.c_double)]
+        _lib.orc_core_bands.restype = C.c_double
+        _lib.orc_select_window.argtypes = [C.c_int, P, P]
+        _lib.orc_causality_patch.argtypes = [C.c_int, C.c_int, C.c_double, P]
         _lib.orc_gloc_vjp.argtypes = [C.POINTER(OrcLattice), P, P, P, C.c_double, P, C.c_double, P,
                                       C.POINTER(C.c_double), C.c_int]
         _lib.orc_asymto_moments.argtypes = [C.c_int, C.c_int, C.c_double, P, C.c_int, P, P]
@@ -120,35 +141,52 @@ def greenftn_w(Hk, sigma, moments, mu, beta, n_cluster=1, n_k_global=None, densi
     return out
 
 
-def giw_to_nele(Hk, sigma, moments, mu, beta, n_cluster=1, nthreads=1, high_freq=2):
+def giw_to_nele(Hk, sigma, moments, mu, beta, n_cluster=1, nthreads=1, high_freq=2, core=None):
+    """GiwToNele restated; core = (KS_eigenEnergy [nk][NumOrbit], FromValToKS [nk][n]) adds the
+    core-band term (GreenFtn.cpp:622-631)."""
     Hk = np.asarray(Hk, dtype=np.complex128)
     nk, n, _ = Hk.shape
     nf, nc, _ = sigma.shape
     L = _lat(n, nc, n_cluster, nk, nk, nf, beta, high_freq)
     h, s, m = _cm(Hk), _cm(sigma), _cm(moments)
-    return lib().orc_giw_to_nele(C.byref(L), _p(h), _p(s), _p(m), mu, nthreads)
+    cp, _keep = _core(core, nk, n)
+    return lib().orc_giw_to_nele(C.byref(L), _p(h), _p(s), _p(m), mu, cp, nthreads)
 
 
-def adjust_mu(Hk, sigma, moments, mu, beta, n_target, dmu=1.0, n_cluster=1, nthreads=1):
+def adjust_mu(Hk, sigma, moments, mu, beta, n_target, dmu=1.0, n_cluster=1, nthreads=1, core=None):
     Hk = np.asarray(Hk, dtype=np.complex128)
     nk, n, _ = Hk.shape
     nf, nc, _ = sigma.shape
     L = _lat(n, nc, n_cluster, nk, nk, nf, beta)
     h, s, m = _cm(Hk), _cm(sigma), _cm(moments)
+    cp, _keep = _core(core, nk, n)
     ne = C.c_int(0)
-    mu_out = lib().orc_adjust_mu(C.byref(L), _p(h), _p(s), _p(m), mu, dmu, n_target, nthreads, C.byref(ne))
+    mu_out = lib().orc_adjust_mu(C.byref(L), _p(h), _p(s), _p(m), mu, dmu, n_target, cp, nthreads, C.byref(ne))
     return mu_out, ne.value
 
 
-def density(Hk, sigma, moments, mu, beta, nthreads=1, high_freq=2):
+def density(Hk, sigma, moments, mu, beta, nthreads=1, high_freq=2, core=None):
     Hk = np.asarray(Hk, dtype=np.complex128)
     nk, n, _ = Hk.shape
     nf, nc, _ = sigma.shape
     L = _lat(n, nc, 1, nk, nk, nf, beta, high_freq)
     h, s, m = _cm(Hk), _cm(sigma), _cm(moments)
+    cp, _keep = _core(core, nk, n)
     N, dN = C.c_double(0), C.c_double(0)
-    lib().orc_density(C.byref(L), _p(h), _p(s), _p(m), mu, C.byref(N), C.byref(dN), nthreads)
+    lib().orc_density(C.byref(L), _p(h), _p(s), _p(m), mu, C.byref(N), C.byref(dN), cp, nthreads)
     return N.value, dN.value
+
+
+def core_bands(ks_energy, val_to_ks, mu, beta, n_k_global=None):
+    """Core-band term of GiwToNele alone, normalised by n_k_global; returns (term, d term/dmu)."""
+    eps = np.asarray(ks_energy, dtype=np.float64)
+    v2k = np.asarray(val_to_ks, dtype=np.int32)
+    nk, n = v2k.shape
+    L = _lat(n, 1, 1, nk, n_k_global or nk, 1, beta)
+    cp, _keep = _core((eps, v2k), nk, n)
+    d = C.c_double(0)
+    s = lib().orc_core_bands(C.byref(L), cp, mu, C.byref(d))
+    return s / L.n_k_global, d.value / L.n_k_global
 
 
 def gloc_vjp(Hk, sigma, moments, mu, beta, gbar, nbar, nthreads=1):
@@ -267,3 +305,17 @@ def upfold_density(dm_model, corr_base, ks_energy, val_to_ks, transform, ks_orth
     if rc:
         raise ValueError("singular transform or overlap")
     return _from_cm(dm), dmr
+
+
+def select_window(var, ok):
+    """The window scan of getAsymto_moments on given spreads and positivity flags: returns minPos."""
+    v = np.ascontiguousarray(var, dtype=np.float64)
+    o = np.ascontiguousarray(ok, dtype=np.int32)
+    return lib().orc_select_window(v.size, _p(v), _p(o))
+
+
+def causality_patch(gt, beta):
+    """FourierTransform_'s causality patch on gt [n_tau+1][d][d]: returns (patched copy, CHECK)."""
+    g = _cm(gt)
+    check = lib().orc_causality_patch(g.shape[1], g.shape[0] - 1, beta, _p(g))
+    return _from_cm(g), check

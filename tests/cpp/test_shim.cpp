@@ -4,6 +4,9 @@
 #include <cmath>
 #include <complex>
 #include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <unistd.h>
 #include <vector>
 
 #include "dmftgpu.h"
@@ -36,12 +39,17 @@ int orc_greenftn_w(const orc_lattice*, const cplx*, const cplx*, const cplx*, do
 int orc_asymto_moments(int, int, double, const cplx*, int, cplx*, int*);
 int orc_ft_w2tau(int, int, int, double, const cplx*, const cplx*, cplx*, int*, int);
 int orc_ft_tau2w(int, int, int, double, const cplx*, cplx*, int*);
+int orc_ft_tau(int, int, double, const cplx*, const cplx*, double, cplx*);
 int orc_hyb_delta(int, int, double, double, const cplx*, const cplx*, const cplx*, int, const int*, cplx*);
 void orc_hk_from_hr(int, int, const int*, const cplx*, int, const double*, cplx*);
 int orc_upfold_density(int, int, int, int, double, double, const cplx*, const cplx*, const double*, const int*,
                        const cplx*, const cplx*, const cplx*, int, const int*, const double*, cplx*, cplx*);
 int orc_greenftn_real(const orc_lattice*, const cplx*, int, double, double, double, const cplx*, double, cplx*, cplx*,
                       int);
+struct orc_core { int n_orb; const double* eps; const int* val_to_ks; };
+double orc_giw_to_nele(const orc_lattice*, const cplx*, const cplx*, const cplx*, double, const orc_core*, int);
+int orc_density(const orc_lattice*, const cplx*, const cplx*, const cplx*, double, double*, double*, const orc_core*,
+                int);
 }
 
 struct IndexMat {  // the subset of Eigen::MatrixXi (H_Rindex) the adapter uses
@@ -132,7 +140,7 @@ int main()
     // the moments against the oracle's own fit
     std::vector<cplx> om(mm.size());
     orc_asymto_moments(nc, nf, beta, sg.data(), 0, om.data(), nullptr);
-    check("getAsymto_moments", relerr(mm.data(), om.data(), om.size()), 1e-8);
+    check("getAsymto_moments", relerr(mm.data(), om.data(), om.size()), 1e-12);
 
     // NBAND[k] varying with k: the first half of the k points keep 5 of the 6 bands
     // (the last band and its couplings dropped); the adapter pads them, and the
@@ -174,10 +182,79 @@ int main()
         check("ragged NBAND densityMatDFT", dens_err / (dens_scale > 0 ? dens_scale : 1.0), 1e-12);
     }
 
-    // mu search: N(mu) = NumberOfElectron
+    // GwHF (GreenFtn.cpp:277) and the per-cluster files of :387-421
+    {
+        std::vector<Mat> Gw2, hf, none;
+        Mat num2;
+        char tmpl[] = "/tmp/dmftgpu_shim_XXXXXX";
+        const char* dir = mkdtemp(tmpl);
+        char cwd[4096];
+        if (!dir || !getcwd(cwd, sizeof(cwd)) || chdir(dir) != 0) { std::printf("no temp dir\n"); return 2; }
+        dmftgpu_shim::Globals gf = g;
+        gf.write_files = true;
+        dmftgpu_shim::GreenFtn_w(h, gf, ncl, npc, Hk, S, Gw2, mu, none, num2, &hf);
+        std::vector<cplx> ghf_got((size_t)nf * nc * nc);
+        for (int w = 0; w < nf; ++w) std::copy(hf[w].v.begin(), hf[w].v.end(), ghf_got.begin() + (size_t)w * nc * nc);
+        check("GreenFtn_w GwHF", relerr(ghf_got.data(), ghf.data(), ghf.size()), 1e-12);
+        // GwHF_loc.full.dat<cl>: "n h1 h2 re im" for every entry of the cluster block, %e (7 digits)
+        double ferr = 0.0;
+        int lines = 0;
+        for (int at = 0; at < ncl; ++at) {
+            std::FILE* f = std::fopen(("GwHF_loc.full.dat" + std::to_string(at)).c_str(), "r");
+            int w, a, b;
+            double re, im;
+            while (f && std::fscanf(f, "%d %d %d %lf %lf", &w, &a, &b, &re, &im) == 5) {
+                const cplx ref = ghf[(size_t)w * nc * nc + (at * npc + a) + (size_t)(at * npc + b) * nc];
+                ferr = std::fmax(ferr, std::abs(cplx(re, im) - ref) / std::fmax(std::abs(ref), 1e-300));
+                ++lines;
+            }
+            if (f) std::fclose(f);
+            std::FILE* q = std::fopen(("Gw_loc.dat" + std::to_string(at)).c_str(), "r");
+            if (!q) ferr = 1.0;
+            else std::fclose(q);
+        }
+        if (lines != ncl * nf * npc * npc) ferr = 1.0;
+        check("GwHF_loc.full.dat values", ferr, 1e-6);
+        if (chdir(cwd) != 0) ++failures;
+    }
+
+    // mu search: N(mu) = NumberOfElectron, checked with the oracle's GiwToNele at mu_out
     double mu_new = dmftgpu_shim::GreenFtn_w_adjust_mu(h, g, ncl, npc, Hk, S, 0.0, 1.0, 1);
-    std::printf("adjusted mu %.10f\n", mu_new);
-    if (!std::isfinite(mu_new)) ++failures;
+    {
+        const double Nn = orc_giw_to_nele(&L, hk.data(), sg.data(), mm.data(), mu_new, nullptr, 1);
+        double Nd = 0, dN = 0;
+        orc_density(&L, hk.data(), sg.data(), mm.data(), mu_new, &Nd, &dN, nullptr, 1);
+        std::printf("adjusted mu %.10f, N(mu) %.12f, dN/dmu %.6f\n", mu_new, Nn, dN);
+        check("adjust_mu N(mu_out) - target", std::fabs(Nn - g.NumberOfElectron), 1e-5 * dN);
+    }
+    // with Kohn-Sham bands outside the window (GiwToNele's core-band term,
+    // GreenFtn.cpp:622-631): NumOrbit = n + 3, two bands below the window
+    // (KS bands 0, 1) and one above; the target is the total count
+    {
+        const int no = n + 3;
+        std::vector<std::vector<double>> ks(nk, std::vector<double>(no));
+        std::vector<std::vector<int>> v2k(nk, std::vector<int>(n));
+        std::vector<double> eps_flat((size_t)nk * no);
+        std::vector<int> v_flat((size_t)nk * n);
+        for (int k = 0; k < nk; ++k) {
+            ks[k][0] = -1.0 + 0.3 * urand();
+            ks[k][1] = -0.6 + 0.3 * urand();
+            for (int l = 2; l < no - 1; ++l) ks[k][l] = urand();
+            ks[k][no - 1] = 0.9 + 0.3 * urand();
+            for (int l0 = 0; l0 < n; ++l0) v2k[k][l0] = 2 + (l0 + k) % n;
+            for (int l = 0; l < no; ++l) eps_flat[(size_t)k * no + l] = ks[k][l];
+            for (int l0 = 0; l0 < n; ++l0) v_flat[(size_t)k * n + l0] = v2k[k][l0];
+        }
+        dmftgpu_shim::Globals gc = g;
+        gc.NumberOfElectron = 2.0 + 2.5;
+        const double mu_c = dmftgpu_shim::GreenFtn_w_adjust_mu(h, gc, ncl, npc, Hk, S, 0.0, 1.0, 1, ks, v2k, no);
+        orc_core core{no, eps_flat.data(), v_flat.data()};
+        const double Nn = orc_giw_to_nele(&L, hk.data(), sg.data(), mm.data(), mu_c, &core, 1);
+        double Nd = 0, dN = 0;
+        orc_density(&L, hk.data(), sg.data(), mm.data(), mu_c, &Nd, &dN, &core, 1);
+        std::printf("adjusted mu with core bands %.10f, N(mu) %.12f, dN/dmu %.6f\n", mu_c, Nn, dN);
+        check("core bands N(mu_out) - target", std::fabs(Nn - gc.NumberOfElectron), 1e-5 * dN);
+    }
 
     // FourierTransform (G-like input) and FT_t_to_w round trip
     std::vector<Mat> gt(ntau + 1), gw2(nf);
@@ -188,10 +265,44 @@ int main()
     orc_ft_w2tau(nc, nf, ntau, beta, got.data(), gm.data(), ogt.data(), &patched, 1);
     std::vector<cplx> ggt(ogt.size());
     for (int t = 0; t <= ntau; ++t) std::copy(gt[t].v.begin(), gt[t].v.end(), ggt.begin() + (size_t)t * nc * nc);
-    check("FourierTransform", relerr(ggt.data(), ogt.data(), ogt.size()), 1e-8);
+    check("FourierTransform", relerr(ggt.data(), ogt.data(), ogt.size()), 1e-12);
     Mat one;
     dmftgpu_shim::FourierTransform(h, g, Gw.data(), one, 3 * beta / ntau, true);
     check("FourierTransform(tau)", relerr(one.v.data(), &ggt[(size_t)3 * nc * nc], (size_t)nc * nc), 1e-12);
+    // the single-tau form at a tau off the grid, against the oracle's FourierTransform_tau
+    {
+        const double tau = 0.3719 * beta;
+        dmftgpu_shim::FourierTransform(h, g, Gw.data(), one, tau, true);
+        std::vector<cplx> o1((size_t)nc * nc);
+        orc_ft_tau(nc, nf, beta, got.data(), gm.data(), tau, o1.data());
+        check("FourierTransform(tau) off grid", relerr(one.v.data(), o1.data(), o1.size()), 1e-12);
+    }
+    // the moments1 overloads (FourierTransform.cpp:475-485, 501-511): the fitted moments
+    // with m0 = 0 and m1 = moments1, for the Sigma-like input (greenFtnAsym = false)
+    {
+        Mat m1;
+        m1.setZero(nc, nc);
+        for (int j = 0; j < nc; ++j)
+            for (int i = 0; i < nc; ++i) m1(i, j) = (i == j ? 1.3 + 0.1 * i : 0.05 * (i + 1) * (j + 1)) + cplx(0, 0.02 * (i - j));
+        std::vector<Mat> st(ntau + 1);
+        dmftgpu_shim::FourierTransform(h, g, S.w.data(), st.data(), m1, false);
+        std::vector<cplx> sm(4 * (size_t)nc * nc), ost((size_t)(ntau + 1) * nc * nc), gst(ost.size());
+        orc_asymto_moments(nc, nf, beta, sg.data(), 0, sm.data(), nullptr);
+        for (int e = 0; e < nc * nc; ++e) {
+            sm[e] = 0.0;
+            sm[(size_t)nc * nc + e] = m1.v[e];
+        }
+        int pt = 0;
+        orc_ft_w2tau(nc, nf, ntau, beta, sg.data(), sm.data(), ost.data(), &pt, 1);
+        for (int t = 0; t <= ntau; ++t) std::copy(st[t].v.begin(), st[t].v.end(), gst.begin() + (size_t)t * nc * nc);
+        check("FourierTransform(moments1)", relerr(gst.data(), ost.data(), ost.size()), 1e-12);
+        const double tau = 0.6113 * beta;
+        Mat s1;
+        dmftgpu_shim::FourierTransform(h, g, S.w.data(), s1, tau, m1, false);
+        std::vector<cplx> o1((size_t)nc * nc);
+        orc_ft_tau(nc, nf, beta, sg.data(), sm.data(), tau, o1.data());
+        check("FourierTransform(tau, moments1)", relerr(s1.v.data(), o1.data(), o1.size()), 1e-12);
+    }
     dmftgpu_shim::FT_t_to_w(h, g, gw2.data(), gt.data(), nf);
     std::vector<cplx> ow((size_t)nf * nc * nc), gw((size_t)nf * nc * nc);
     int viol = 0;
@@ -221,7 +332,7 @@ int main()
         int pt = 0;
         orc_ft_w2tau(2, nf, ntau, beta, od.data(), dm4.data(), odt.data(), &pt, 1);
         for (int t = 0; t <= ntau; ++t) std::copy(dt[t].v.begin(), dt[t].v.end(), gdt.begin() + (size_t)t * 4);
-        check("write_hyb_t", relerr(gdt.data(), odt.data(), odt.size()), 1e-8);
+        check("write_hyb_t", relerr(gdt.data(), odt.data(), odt.size()), 1e-12);
     }
 
     // read_HR of a two-atom Hk.HWR file, then Construct_Hk on a small mesh

@@ -4,9 +4,12 @@ Transforms: 1e-12 normwise relative (both sides form the same FP64 phase
 w*tau; only the sin/cos implementations differ).  Moments: the least-squares
 normal equations are ill-conditioned (columns scale as 1, 1/w, 1/w^2, 1/w^3;
 the condition number of the 2x2 (1/w, 1/w^3) block reaches ~1e9 over a
-window), and the GPU forms Kinv (K^T g) where the reference forms
-(Kinv K^T) g, so m0, m1 agree to 1e-9 and m2, m3 to 1e-7 relative; where the
-window spread has a clear minimum the selected windows must be identical.
+window), so a rounding difference in K^T K would move m2, m3 by ~1e-7.  Both
+sides therefore run one operation sequence: the basis values from the same
+std::pow, K^T K and its LU inverse with explicit fma, the rows of
+KM_solve = (K^T K)^{-1} K^T formed before they meet g, as the reference
+associates the product (FourierTransform.cpp:239-243), and no other
+contraction.  The windows must be identical and the moments agree to 1e-12.
 """
 import os
 
@@ -68,14 +71,28 @@ def test_moments_match_oracle(H, kind):
         beta = 40.0
     ref, wref = O.asymto_moments(fw, beta, return_windows=True)
     got, wgot = H.get_asymto_moments(fw, beta, return_windows=True)
-    if kind in ("pole5", "pole10"):
-        # a clear minimum of the window spread: the same window must be chosen
-        assert wgot == wref
-    # where every window fits the tail exactly (polynomial input; the constant
-    # real part of the t2g Sigma for m0) the spread is rounding noise and the
-    # chosen window may differ -- the moments may not
-    for q, tol in enumerate((1e-9, 1e-9, 1e-7, 1e-7)):
-        assert rel(got[q], ref[q]) < tol or np.max(np.abs(got[q] - ref[q])) < 1e-12, (q, wgot, wref)
+    print(f"moments {kind}: windows GPU {wgot} oracle {wref}, rel err " +
+          " ".join(f"{rel(got[q], ref[q]):.1e}" for q in range(4)), flush=True)
+    assert wgot == wref
+    for q in range(4):
+        assert rel(got[q], ref[q]) <= 1e-12 or np.max(np.abs(got[q] - ref[q])) < 1e-15, (q, wgot, wref)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["all_rejected", "filtered"])
+def test_moment_windows_constructed(H, case):
+    """The constructed window cases of tests/test_cpu_oracle_choices.py: every window
+    rejected by the positivity tests (the last centre for all moments), and the
+    lowest-spread windows rejected with a clear admissible minimum elsewhere."""
+    from tests.moment_cases import noisy_two_pole
+    fw = noisy_two_pole(200, 50.0, 1.2, 3.0, 1e-4, seed=1) if case == "all_rejected" else \
+        noisy_two_pole(200, 50.0, 0.8, 1.0, 3e-4, seed=2)
+    ref, wref = O.asymto_moments(fw, 50.0, return_windows=True)
+    got, wgot = H.get_asymto_moments(fw, 50.0, return_windows=True)
+    print(f"windows {case}: GPU {wgot} oracle {wref}", flush=True)
+    assert wgot == wref
+    for q in range(4):
+        assert rel(got[q], ref[q]) <= 1e-12, q
 
 
 @pytest.mark.gpu
@@ -103,7 +120,7 @@ def test_moments_green_branch(H):
     assert wgot == wref
     assert np.array_equal(got[1], np.eye(4)) and np.max(np.abs(got[0])) == 0
     for q in (2, 3):
-        assert rel(got[q], ref[q]) < 1e-7, q
+        assert rel(got[q], ref[q]) <= 1e-12, q
 
 
 @pytest.mark.gpu
