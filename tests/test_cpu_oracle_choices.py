@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
-from tests.moment_cases import brute_scan, noisy_two_pole, scan_rule
+from moment_cases import brute_scan, noisy_two_pole, scan_rule
 
 PI = 3.1415926535897932384626
 

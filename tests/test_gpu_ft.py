@@ -84,7 +84,7 @@ def test_moment_windows_constructed(H, case):
     """The constructed window cases of tests/test_cpu_oracle_choices.py: every window
     rejected by the positivity tests (the last centre for all moments), and the
     lowest-spread windows rejected with a clear admissible minimum elsewhere."""
-    from tests.moment_cases import noisy_two_pole
+    from moment_cases import noisy_two_pole
     fw = noisy_two_pole(200, 50.0, 1.2, 3.0, 1e-4, seed=1) if case == "all_rejected" else \
         noisy_two_pole(200, 50.0, 0.8, 1.0, 3e-4, seed=2)
     ref, wref = O.asymto_moments(fw, 50.0, return_windows=True)
