@@ -58,6 +58,7 @@ _SIGS = {
     "dmftgpu_prepare": ([P, C.POINTER(Lattice), P, P], I),
     "dmftgpu_static_eig": ([P, P, P], I),
     "dmftgpu_greenftn_w": ([P, I, P, D, P, P, P, PD], I),
+    "dmftgpu_greenftn_w_noalloc": ([P, I, P, D, P, P, P, PD], I),
     "dmftgpu_density_matrix": ([P, P, D, P], I),
     "dmftgpu_density": ([P, I, P, D, PD, PD], I),
     "dmftgpu_set_core_bands": ([P, I, P, PI], I),

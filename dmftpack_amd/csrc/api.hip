@@ -3,8 +3,10 @@
 // the reference routine it replaces; the per-(k, iw) work runs on the GPU.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -88,6 +90,12 @@ extern "C" int dmftgpu_create(dmftgpu_handle* out, void* stream)
     if (hipGetDevice(&h->device) != hipSuccess) {
         delete h;
         return DMFTGPU_ERR_HIP;
+    }
+    // the no-alloc scratch of this device (dmftgpu_greenftn_w_noalloc); without
+    // it only that entry point is unavailable
+    if (dgi::scratch_address(h->device, &h->scratch) != hipSuccess) {
+        (void)hipGetLastError();
+        h->scratch = nullptr;
     }
     *out = h;
     return DMFTGPU_OK;
@@ -351,7 +359,7 @@ Plan make_plan(int nk, int nw_tot, int slots, int target_wg)
 // from ws instead and nothing is kept.  wt_out (optional) receives the
 // per-level weights w_kl [nk][n].
 int run_static(dmftgpu_handle h, double mu, cd* st_out, double* st_scalars, char* ws, size_t& off,
-               bool from_ws = false, const double** wt_out = nullptr)
+               bool from_ws = false, const double** wt_out = nullptr, size_t cap = SIZE_MAX)
 {
     const dmftgpu_lattice& L = h->lat;
     const int n = L.n, nc = L.n_corr, nk = L.n_k, m2 = nc * nc;
@@ -383,6 +391,7 @@ int run_static(dmftgpu_handle h, double mu, cd* st_out, double* st_scalars, char
         double* pN = (double*)(ws + off);
         off += ((size_t)tot * 2 * sizeof(double) + 255) / 256 * 256;
         double* pdN = pN + tot;
+        if (off > cap) return fail(h, DMFTGPU_ERR_STATE, "static-part scratch needs %zu of %zu bytes", off, cap);
         HIPCHK(dg::launch_static_terms(h->ev, n, nk, nw_tot, L.beta, mu, wt, dt, h->stream));
         HIPCHK(dg::launch_static_reduce(wt, dt, h->U, n, nc, nk, kpc, chunks, pNum, pN, pdN, h->stream));
         if (cchunks)
@@ -452,7 +461,8 @@ struct PerK {
 // (dmftgpu_gloc_loss_grad), which fills the rest and reduces them.
 int run_lattice(dmftgpu_handle h, int method, int mode, const double* sigma, double mu, const double* gbar,
                 double nbar, char* ws, size_t& off, Sums& S, bool want_trace = false, const RealAxis* ra = nullptr,
-                const PerK* pk = nullptr, cd* fused_tr2 = nullptr, int fused_chunks = 0)
+                const PerK* pk = nullptr, cd* fused_tr2 = nullptr, int fused_chunks = 0, int max_chunks = 0,
+                size_t cap = SIZE_MAX)
 {
     const dmftgpu_lattice& L = h->lat;
     const int n = L.n, nc = L.n_corr, nk = L.n_k, m2 = nc * nc;
@@ -463,6 +473,10 @@ int run_lattice(dmftgpu_handle h, int method, int mode, const double* sigma, dou
     else slots = dg::schur_slots(nc, mode, method == DMFTGPU_SCHUR_F32R, h->schur_layout == 1 || (ra && mode != dg::MODE_FWD));
     const bool cols = h->schur_layout == 1 || (ra && mode != dg::MODE_FWD);
     Plan P = pk ? Plan{nk, 1} : make_plan(nk, nw_tot, slots, lattice_target(h, method, nc, cols, nk, nw_tot, slots));
+    if (max_chunks > 0 && P.n_chunks > max_chunks) {  // partial sums held to a fixed scratch size
+        const int kpc = (nk + max_chunks - 1) / max_chunks;
+        P = Plan{(nk + kpc - 1) / kpc, kpc};
+    }
     if (fused_tr2 && P.n_chunks != fused_chunks)  // both halves of a fused pair must chunk k alike
         return fail(h, DMFTGPU_ERR_STATE, "fused partials laid out for %d k chunks, this call uses %d", fused_chunks,
                     P.n_chunks);
@@ -512,6 +526,17 @@ int run_lattice(dmftgpu_handle h, int method, int mode, const double* sigma, dou
         a.pMu = (cd*)take(C * n_freq * sizeof(cd));
     }
     if (method != DMFTGPU_PLAIN_F64) a.pSigAll = (cd*)take((size_t)nw_tot * m2 * sizeof(cd));
+    // the reduced sums: every region is carved before any kernel runs
+    if (!pk) {
+        S.tr = (cd*)take(nw_tot * sizeof(cd));
+        if (mode == dg::MODE_FWD) S.Gsum = (cd*)take((size_t)nw_tot * m2 * sizeof(cd));
+        else S.tr2 = (cd*)take(nw_tot * sizeof(cd));
+        if (mode == dg::MODE_VJP) {
+            S.Sb = (cd*)take((size_t)n_freq * m2 * sizeof(cd));
+            S.mu = (cd*)take(n_freq * sizeof(cd));
+        }
+    }
+    if (off > cap) return fail(h, DMFTGPU_ERR_STATE, "lattice scratch needs %zu of %zu bytes", off, cap);
     if (h->timing) HIPCHK(hipEventRecord(h->ev0[mode], h->stream));
     if (method == DMFTGPU_PLAIN_F64) {
         HIPCHK(dg::launch_lattice_plain(a, mode, h->stream, nullptr));
@@ -524,18 +549,13 @@ int run_lattice(dmftgpu_handle h, int method, int mode, const double* sigma, dou
         h->ev_used[mode] = true;
     }
     if (pk) return DMFTGPU_OK;
-    S.tr = (cd*)take(nw_tot * sizeof(cd));
     HIPCHK(dg::launch_reduce_chunks(a.pTr, P.n_chunks, nw_tot, S.tr, h->stream));
     if (mode == dg::MODE_FWD) {
-        S.Gsum = (cd*)take((size_t)nw_tot * m2 * sizeof(cd));
         HIPCHK(dg::launch_reduce_chunks(a.pG, P.n_chunks, (size_t)nw_tot * m2, S.Gsum, h->stream));
     } else {
-        S.tr2 = (cd*)take(nw_tot * sizeof(cd));
         HIPCHK(dg::launch_reduce_chunks(a.pTr2, P.n_chunks, nw_tot, S.tr2, h->stream));
     }
     if (mode == dg::MODE_VJP) {
-        S.Sb = (cd*)take((size_t)n_freq * m2 * sizeof(cd));
-        S.mu = (cd*)take(n_freq * sizeof(cd));
         HIPCHK(dg::launch_reduce_chunks(a.pSb, P.n_chunks, (size_t)n_freq * m2, S.Sb, h->stream));
         HIPCHK(dg::launch_reduce_chunks(a.pMu, P.n_chunks, n_freq, S.mu, h->stream));
     }
@@ -564,28 +584,52 @@ size_t lattice_ws_bytes(dmftgpu_handle h, int method, int mode, const RealAxis* 
 // GreenFtn_w (GreenFtn.cpp:142-422)
 namespace {
 
+// ext_ws (optional): caller-provided scratch of ext_cap bytes used instead of
+// the handle's workspaces (the static part then keeps no cache); the k
+// chunking is reduced until the partial sums fit.
 int forward_impl(dmftgpu_handle h, int method, const double* sigma, double mu, double* gloc, double* gloc_hf,
-                 double* num_matrix, double* n_total, cd* fused_tr2, int fused_chunks)
+                 double* num_matrix, double* n_total, cd* fused_tr2, int fused_chunks, char* ext_ws = nullptr,
+                 size_t ext_cap = 0)
 {
     const dmftgpu_lattice& L = h->lat;
     const int nc = L.n_corr, m2 = nc * nc;
     const int nw_tot = L.n_freq * L.high_freq;
-    hipError_t e;
-    size_t need = lattice_ws_bytes(h, method, dg::MODE_FWD) + static_ws_bytes(L) + (size_t)(m2 + 8) * 32 + 8192;
-    char* ws = (char*)h->ws.get(need, &e);
-    if (!ws) return fail(h, DMFTGPU_ERR_HIP, "workspace %zu bytes: %s", need, hipGetErrorString(e));
+    char* ws;
+    int max_chunks = 0;
+    if (ext_ws) {
+        // bytes: fixed part (Sigma table, reduced sums, totals, static part) plus
+        // per k chunk the partials of tr G and G_cc, each region 256-aligned
+        const size_t fixed = 3 * (((size_t)nw_tot * m2 * 16 + 255) / 256 * 256) + ((size_t)nw_tot * 16 + 255) / 256 * 256 +
+                             ((size_t)(m2 + 3) * 16 + 255) / 256 * 256 + ((size_t)m2 * 16 + 255) / 256 * 256 + 256 +
+                             static_ws_bytes(L, true);
+        const size_t per_chunk = (size_t)nw_tot * 16 * (m2 + 1);
+        if (ext_cap < fixed + per_chunk + 512)
+            return fail(h, DMFTGPU_ERR_UNSUPPORTED,
+                        "lattice needs at least %zu bytes of scratch, the no-alloc maximum is %zu", fixed + per_chunk + 512,
+                        ext_cap);
+        max_chunks = (int)std::min<size_t>((ext_cap - fixed - 512) / per_chunk, 1 << 20);
+        ws = ext_ws;
+    } else {
+        hipError_t e;
+        size_t need = lattice_ws_bytes(h, method, dg::MODE_FWD) + static_ws_bytes(L) + (size_t)(m2 + 8) * 32 + 8192;
+        ws = (char*)h->ws.get(need, &e);
+        if (!ws) return fail(h, DMFTGPU_ERR_HIP, "workspace %zu bytes: %s", need, hipGetErrorString(e));
+    }
     size_t off = 0;
     Sums S;
-    int rc = run_lattice(h, method, dg::MODE_FWD, sigma, mu, nullptr, 0.0, ws, off, S, n_total != nullptr, nullptr,
-                         nullptr, fused_tr2, fused_chunks);
-    if (rc) return rc;
+    // totals and the static part first, then the lattice: each checks the
+    // capacity before it launches anything
     cd* tot = (cd*)(ws + off);
     off += ((size_t)(m2 + 3) * sizeof(cd) + 255) / 256 * 256;
     cd* st = (cd*)(ws + off);
     off += ((size_t)m2 * sizeof(cd) + 255) / 256 * 256;
     double* sts = (double*)(ws + off);
     off += 256;
-    rc = run_static(h, mu, st, sts, ws, off);
+    const size_t cap = ext_ws ? ext_cap : SIZE_MAX;
+    int rc = run_static(h, mu, st, sts, ws, off, ext_ws != nullptr, nullptr, cap);
+    if (rc) return rc;
+    rc = run_lattice(h, method, dg::MODE_FWD, sigma, mu, nullptr, 0.0, ws, off, S, n_total != nullptr, nullptr,
+                     nullptr, fused_tr2, fused_chunks, max_chunks, cap);
     if (rc) return rc;
     HIPCHK(dg::launch_freq_totals(S.Gsum, m2, nc, S.tr, nullptr, nw_tot, nullptr, 0, tot, h->stream));
     const double inv_nk = 1.0 / (double)L.n_k_global;
@@ -594,24 +638,24 @@ int forward_impl(dmftgpu_handle h, int method, const double* sigma, double mu, d
         HIPCHK(dg::launch_gloc_hf(h->ev, h->U, L.n, nc, L.n_k, L.n_freq, L.beta, mu, inv_nk, L.n_cluster, (cd*)gloc_hf,
                                   h->stream));
     // NumMatrix = [ (1/beta) sum_n sum_k (G_cc + G_cc^H) + sum_k (U w U^H)_cc ] / knum_mpiGlobal
-    std::vector<cd> htot(m2 + 3), hst(m2);
+    // (host arrays on the stack: n_corr <= 32)
+    cd htot[32 * 32 + 3], hst[32 * 32], num[32 * 32];
     double hsts[2];
-    HIPCHK(hipMemcpyAsync(htot.data(), tot, sizeof(cd) * (m2 + 3), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipMemcpyAsync(hst.data(), st, sizeof(cd) * m2, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(htot, tot, sizeof(cd) * (m2 + 3), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(hst, st, sizeof(cd) * m2, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipMemcpyAsync(hsts, sts, sizeof(double) * 2, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     if (n_total) {
         *n_total = (htot[m2].x / L.beta + hsts[0]) / (double)L.n_k_global;
         if (!std::isfinite(*n_total)) return fail(h, DMFTGPU_ERR_NUMERIC, "N(mu) is not finite");
     }
-    std::vector<cd> num(m2);
     for (int q = 0; q < m2; ++q) {
         num[q].x = (htot[q].x / L.beta + hst[q].x) * inv_nk;
         num[q].y = (htot[q].y / L.beta + hst[q].y) * inv_nk;
         if (!std::isfinite(num[q].x) || !std::isfinite(num[q].y))
             return fail(h, DMFTGPU_ERR_NUMERIC, "NumMatrix is not finite");
     }
-    HIPCHK(hipMemcpyAsync(num_matrix, num.data(), sizeof(cd) * m2, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(num_matrix, num, sizeof(cd) * m2, hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     return DMFTGPU_OK;
 }
@@ -630,6 +674,24 @@ extern "C" int dmftgpu_greenftn_w(dmftgpu_handle h, int method, const double* si
     if (!sigma || !gloc || !num_matrix) return fail(h, DMFTGPU_ERR_ARG, "sigma, gloc and num_matrix are required");
     if (!std::isfinite(mu)) return fail(h, DMFTGPU_ERR_ARG, "mu is not finite");
     return forward_impl(h, method, sigma, mu, gloc, gloc_hf, num_matrix, n_total, nullptr, 0);
+}
+
+// dmftgpu_greenftn_w without device allocation: every scratch buffer is carved
+// from this device's fixed scratch array (noalloc.hip), the k chunking reduced
+// until the partial sums fit; the static part is recomputed each call (no
+// cached copy can live in a shared array).  No hipMalloc / hipFree, no host heap.
+extern "C" int dmftgpu_greenftn_w_noalloc(dmftgpu_handle h, int method, const double* sigma, double mu, double* gloc,
+                                          double* gloc_hf, double* num_matrix, double* n_total)
+{
+    (void)hipGetLastError();  // drop an error left by an earlier failed call
+    if (!h) return DMFTGPU_ERR_ARG;
+    int rc;
+    if (!method_ok(h, method, &rc)) return rc;
+    if (!sigma || !gloc || !num_matrix) return fail(h, DMFTGPU_ERR_ARG, "sigma, gloc and num_matrix are required");
+    if (!std::isfinite(mu)) return fail(h, DMFTGPU_ERR_ARG, "mu is not finite");
+    if (!h->scratch) return fail(h, DMFTGPU_ERR_HIP, "no-alloc scratch unavailable on device %d", h->device);
+    return forward_impl(h, method, sigma, mu, gloc, gloc_hf, num_matrix, n_total, nullptr, 0, (char*)h->scratch,
+                        dgi::kNoAllocScratchBytes);
 }
 
 // retarded_GreenFtn2 (GreenFtn.cpp:425-466) summed over k, as qs_spectra

@@ -19,11 +19,18 @@ struct Workspace {
 
 int fail(dmftgpu_handle h, int code, const char* fmt, ...);
 
+// Scratch of dmftgpu_greenftn_w_noalloc: a file-scope device array of this
+// size per device (noalloc.hip); scratch_address returns its address on
+// `device`, which must be current (the lookup runs once per device).
+constexpr size_t kNoAllocScratchBytes = (size_t)128 << 20;
+hipError_t scratch_address(int device, void** out);
+
 }  // namespace dgi
 
 struct dmftgpu_handle_s {
     hipStream_t stream = nullptr;
     int device = 0;
+    void* scratch = nullptr;  // this device's no-alloc scratch (dgi::scratch_address), or null
     std::string err;
     // lattice bound by dmftgpu_prepare
     bool prepared = false;

@@ -118,6 +118,21 @@ int dmftgpu_static_eig(dmftgpu_handle h, double* eps, double* U);
 int dmftgpu_greenftn_w(dmftgpu_handle h, int method, const double* sigma, double mu, double* gloc, double* gloc_hf,
                        double* num_matrix, double* n_total);
 
+/* No reference counterpart: dmftgpu_greenftn_w for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments and results; it calls no
+ * hipMalloc / hipFree and uses no host heap.  Its scratch (k-chunk partial
+ * sums, the Sigma table, the static weights) is one fixed array of 128 MiB per
+ * device, loaded with the library and held until the process ends; the k
+ * chunking is reduced until the partial sums fit, and a lattice whose
+ * scratch exceeds 128 MiB even as one chunk (roughly n_freq*high_freq*n_corr^2
+ * * 48 B plus 16 B per k point and band) is rejected with
+ * DMFTGPU_ERR_UNSUPPORTED.  dmftgpu_create takes the array's address and
+ * dmftgpu_prepare (which allocates) must have run.  Every call on one device
+ * shares the array: calls that overlap on a device race. */
+int dmftgpu_greenftn_w_noalloc(dmftgpu_handle h, int method, const double* sigma, double mu, double* gloc,
+                               double* gloc_hf, double* num_matrix, double* n_total);
+
 /* Real-axis Green's function of retarded_GreenFtn2 (GreenFtn.cpp:425-466),
  * summed over k as qs_spectra accumulates it (qs_spectra.cpp:118-131):
  *   G^R(k, E_j) = [(E_j + mu + i eta) I - H(k) - P Sigma^R(E_j) P^H]^{-1},
