@@ -208,22 +208,22 @@ def main():
             extra["speedup_vs_plain_fp64"] = value / others["plain_f64"]["value"]
 
     # ---- roofline of the dominant kernel --------------------------------------
-    t_f, t_v = float(np.mean(kf)) / 1e3, float(np.mean(kv)) / 1e3
     # On the one-matrix-per-lane Schur path dmftgpu_gloc_loss_grad is fused: the
     # forward call is two launches (forward below N_freq; Hermitian-part forward +
-    # tr G^2 above), the adjoint call one launch over the first N_freq frequencies.
-    # The library's events time each call as a whole.
+    # tr G^2 above), the adjoint call one launch over the first N_freq frequencies,
+    # which is the longest single launch of the step.  The library's events time
+    # each call's main launches on its own stream.
+    t_f, t_v = float(np.mean(kf)) / 1e3, float(np.mean(kv)) / 1e3
     fused = head != G.PLAIN_F64 and nc <= 6 and args.layout == 0
-    if t_v >= t_f:
+    if fused or t_v >= t_f:
         kname = "k_lattice_schur<VJP> (first N_freq)" if fused else "k_lattice_schur<VJP>"
-        t_k, flop, pkey = t_v, f_adj(n, nc), "VJP_CALL"
+        t_k, flop, pkey = t_v, f_adj(n, nc), ("VJP" if fused else "VJP_CALL")
         units_k = nk * nf if fused else units
     else:
-        kname = "k_lattice_schur<FWD>+<DENS_HERM>" if fused else "k_lattice_schur<FWD>"
+        kname = "k_lattice_schur<FWD>"
         t_k, flop, pkey, units_k = t_f, f_fwd(n), "FWD_CALL", units
     if head == G.PLAIN_F64:
         kname = kname.replace("schur", "plain")
-    achieved = flop * units_k / t_k / 1e12
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     if os.path.exists(tpath):
@@ -231,26 +231,37 @@ def main():
             traffic = json.load(open(tpath)).get(args.method, {}).get(pkey)
         except Exception:
             traffic = None
-    # executed VALU FLOPs of the same kernel (rocprofv3 SQ_INSTS_VALU_FLOPS_* pass summarised in profiles/)
+    # FLOPs the kernel executes per launch: rocprofv3 SQ_INSTS_VALU_FLOPS_FP64 +
+    # SQ_INSTS_VALU_FLOPS_FP32 (x 64 lanes) of the same kernel on the same workload,
+    # summarised in profiles/pmc_flops.json; FP32 is not packed in this build, so
+    # both issue at the FP64 VALU rate and share its 78.6 TFLOP/s peak
     executed = None
     fpath = os.path.join(ROOT, "profiles", "pmc_flops.json")
-    if os.path.exists(fpath):
+    if os.path.exists(fpath) and args.workload == "dp14":
         try:
             ex = json.load(open(fpath)).get(args.method, {}).get(pkey)
             if ex:
-                fl = ex["fp64_flops"] + ex["fp32_flops"]
-                executed = dict(flop_per_launch=fl, tflops=fl / t_k / 1e12, frac=fl / t_k / 1e12 / PEAK_FP64_TFLOPS,
-                                source="profiles/valu_counters_dp14_step_" + args.method + ".txt")
+                executed = dict(flop_per_launch=ex["fp64_flops"] + ex["fp32_flops"], fp64=ex["fp64_flops"],
+                                fp32=ex["fp32_flops"], counter_kernel=ex["kernel"],
+                                counter_launch_ms=ex["launch_s"] * 1e3)
         except Exception:
             executed = None
-    roofline = dict(bound="mfma", achieved=achieved, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
-                    frac=achieved / PEAK_FP64_TFLOPS, traffic=traffic, kernel=kname,
-                    kernel_ms=t_k * 1e3, fwd_kernel_ms=t_f * 1e3, vjp_kernel_ms=t_v * 1e3,
-                    flop_per_unit=flop, units_per_launch=units_k, executed=executed,
-                    note="achieved = algorithmic 8n^3-based FLOPs (adjoint: 8n^3+16nc^3+8n nc^2+8n^2) per (k,iw) "
-                         "x units / measured kernel time; the Schur methods execute fewer FLOPs than this "
-                         "(executed: counted VALU FLOPs of the same kernel over the same live time). "
-                         "Bound: FP64 VALU, whose MI355X peak equals the FP64 matrix peak (78.6 TFLOP/s)")
+    plain_eq = dict(flop_per_unit=flop, units_per_launch=units_k,
+                    tflops=flop * units_k / t_k / 1e12, frac_of_fp64_peak=flop * units_k / t_k / 1e12 / PEAK_FP64_TFLOPS,
+                    note="the plain algorithm's work (8n^3 per (k,iw); adjoint 8n^3+16nc^3+8n nc^2+8n^2) over this "
+                         "kernel's time: a speed figure, not a fraction of peak, since the Schur kernels do less work")
+    achieved = executed["flop_per_launch"] / t_k / 1e12 if executed else None
+    roofline = dict(bound="fp64_valu", achieved=achieved, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
+                    frac=None if achieved is None else achieved / PEAK_FP64_TFLOPS, traffic=traffic, kernel=kname,
+                    kernel_ms=t_k * 1e3, fwd_call_ms=t_f * 1e3, vjp_call_ms=t_v * 1e3,
+                    flop_per_launch=None if executed is None else executed["flop_per_launch"],
+                    flop_source=None if executed is None else
+                    "profiles/pmc_flops.json: rocprofv3 SQ_INSTS_VALU_FLOPS_FP64 + _FP32 x 64 lanes of "
+                    + executed["counter_kernel"] + f" ({executed['counter_launch_ms']:.2f} ms under the profiler)",
+                    plain_equivalent=plain_eq,
+                    note="achieved = FLOPs the dominant kernel executes per launch (hardware counters, same kernel "
+                         "and workload) / its live duration (HIP events on the library's stream); the kernels are "
+                         "FP64/FP32 VALU code (no MFMA), bound by the VALU issue rate, 78.6 TFLOP/s")
 
     # ---- CPU baseline: the oracle's GreenFtn_w on a k subset ---------------------
     cpu = None
@@ -426,7 +437,7 @@ def _cpu_sample(fn, total_units, units_of, budget, probe):
 
 
 def _flop_roofline(achieved_tflops, kernel, note, flop_per_unit, units):
-    return dict(bound="mfma", achieved=achieved_tflops, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
+    return dict(bound="fp64_valu", achieved=achieved_tflops, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
                 frac=achieved_tflops / PEAK_FP64_TFLOPS, traffic=None, kernel=kernel, flop_per_unit=flop_per_unit,
                 units_per_launch=units, note=note)
 

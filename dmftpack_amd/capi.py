@@ -64,6 +64,7 @@ _SIGS = {
     "dmftgpu_set_core_bands": ([P, I, P, PI], I),
     "dmftgpu_adjust_mu": ([P, I, P, D, D, D, PD, PI], I),
     "dmftgpu_gloc_vjp": ([P, I, P, D, P, D, P, PD], I),
+    "dmftgpu_gloc_jvp": ([P, I, P, D, P, D, P, PD], I),
     "dmftgpu_gloc_loss_grad": ([P, I, P, D, P, D, P, P, PD, PD, P, PD], I),
     "dmftgpu_moments": ([P, I, I, D, P, I, P, PI], I),
     "dmftgpu_batch_create": ([P, I, P, I, I, C.POINTER(P)], I),

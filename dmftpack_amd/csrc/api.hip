@@ -910,6 +910,68 @@ extern "C" int dmftgpu_gloc_vjp(dmftgpu_handle h, int method, const double* sigm
     return vjp_impl(h, method, sigma, mu, gloc_bar, n_bar, sigma_bar, mu_bar, nullptr, 0);
 }
 
+// Jacobian-vector product of (G_loc, N) in (Sigma(iw_n), n < N_freq; mu), moments
+// held fixed.  Per (k, iw_n):  dG_cc = G (dSigma - dmu S') G  with (G^2)_cc = G S' G,
+// and dN = (1/Nk) [ (2/beta) Re sum_{n<N_freq} tr(dSigma_n (G^2)_cc) ] + dmu dN/dmu.
+// G X G is the conjugate transpose of G^H X^H G^H, the product the adjoint forms
+// with V = gbar + c_n S'^H (c_n = 2 nbar / beta): with gbar = dSigma^H and
+// c_n = -dmu the adjoint kernels accumulate exactly dG_cc^H, and their per-frequency
+// mu terms tr(gbar^H (G^2)_cc) = tr(dSigma (G^2)_cc).  So the same kernels run with
+// those inputs, and the results are transposed back.
+static int jvp_impl(dmftgpu_handle h, int method, const double* sigma, double mu, const double* dsigma,
+                    double dmu, double* dgloc, double* dn)
+{
+    const dmftgpu_lattice& L = h->lat;
+    const int nc = L.n_corr, m2 = nc * nc;
+    const int nw_tot = L.n_freq * L.high_freq;
+    hipError_t e;
+    const size_t gb_bytes = ((size_t)L.n_freq * m2 * sizeof(cd) + 255) / 256 * 256;
+    size_t need = lattice_ws_bytes(h, method, dg::MODE_VJP) + static_ws_bytes(L) + (size_t)(m2 + 8) * 32 + 8192 +
+                  gb_bytes;
+    char* ws = (char*)h->ws.get(need, &e);
+    if (!ws) return fail(h, DMFTGPU_ERR_HIP, "workspace %zu bytes: %s", need, hipGetErrorString(e));
+    size_t off = 0;
+    cd* gbarH = (cd*)ws;  // dSigma^H
+    off += gb_bytes;
+    HIPCHK(dg::launch_adjoint_stack((const cd*)dsigma, (size_t)L.n_freq, nc, 1.0, gbarH, h->stream));
+    Sums S;
+    int rc = run_lattice(h, method, dg::MODE_VJP, sigma, mu, (const double*)gbarH, -0.5 * dmu * L.beta, ws, off, S);
+    if (rc) return rc;
+    cd* tot = (cd*)(ws + off);
+    off += ((size_t)(m2 + 3) * sizeof(cd) + 255) / 256 * 256;
+    cd* st = (cd*)(ws + off);
+    off += ((size_t)m2 * sizeof(cd) + 255) / 256 * 256;
+    double* sts = (double*)(ws + off);
+    off += 256;
+    rc = run_static(h, mu, st, sts, ws, off);
+    if (rc) return rc;
+    const double inv_nk = 1.0 / (double)L.n_k_global;
+    HIPCHK(dg::launch_adjoint_stack(S.Sb, (size_t)L.n_freq, nc, inv_nk, (cd*)dgloc, h->stream));
+    HIPCHK(dg::launch_freq_totals(nullptr, m2, nc, nullptr, S.tr2, nw_tot, S.mu, L.n_freq, tot, h->stream));
+    cd htot[32 * 32 + 3];
+    double hsts[2];
+    HIPCHK(hipMemcpyAsync(htot, tot, sizeof(cd) * (m2 + 3), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(hsts, sts, sizeof(double) * 2, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const double dNdmu = (-htot[m2 + 1].x / L.beta + hsts[1]) * inv_nk;
+    const double d = 2.0 / L.beta * htot[m2 + 2].x * inv_nk + dmu * dNdmu;
+    if (!std::isfinite(d)) return fail(h, DMFTGPU_ERR_NUMERIC, "dN is not finite");
+    if (dn) *dn = d;
+    return DMFTGPU_OK;
+}
+
+extern "C" int dmftgpu_gloc_jvp(dmftgpu_handle h, int method, const double* sigma, double mu, const double* dsigma,
+                                double dmu, double* dgloc, double* dn)
+{
+    (void)hipGetLastError();  // drop an error left by an earlier failed call
+    if (!h) return DMFTGPU_ERR_ARG;
+    int rc;
+    if (!method_ok(h, method, &rc)) return rc;
+    if (!sigma || !dsigma || !dgloc) return fail(h, DMFTGPU_ERR_ARG, "sigma, dsigma and dgloc are required");
+    if (!std::isfinite(mu) || !std::isfinite(dmu)) return fail(h, DMFTGPU_ERR_ARG, "mu or dmu is not finite");
+    return jvp_impl(h, method, sigma, mu, dsigma, dmu, dgloc, dn);
+}
+
 // Loss L = sum_{n<N_freq} ||G_loc(iw_n) - G_tgt(iw_n)||_F^2 + (N - N_tgt)^2 and its
 // gradient: forward, cotangents on the device, adjoint.  On the one-matrix-per-lane
 // Schur path the forward also forms tr G^2 beyond N_freq (fused), so the adjoint

@@ -358,6 +358,27 @@ hipError_t launch_gloc_finalize(const cd* Gsum, int n_freq, int nc, int n_cluste
     return hipGetLastError();
 }
 
+// out[m] = scale * in[m]^H for a stack of d x d matrices (column-major)
+__global__ __launch_bounds__(256) void k_adjoint_stack(const cd* __restrict__ in, size_t count, int d, double scale,
+                                                       cd* __restrict__ out)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t m2 = (size_t)d * d;
+    if (t >= count * m2) return;
+    const size_t m = t / m2;
+    const int e = (int)(t % m2), i = e % d, j = e / d;
+    out[t] = cscale(cconj(in[m * m2 + j + (size_t)i * d]), scale);
+}
+
+hipError_t launch_adjoint_stack(const cd* in, size_t count, int d, double scale, cd* out, hipStream_t s)
+{
+    const size_t total = count * d * d;
+    if (total == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_adjoint_stack, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, in, count, d, scale,
+                       out);
+    return hipGetLastError();
+}
+
 __global__ __launch_bounds__(256) void k_scale(const cd* __restrict__ in, size_t len, double scale,
                                                cd* __restrict__ out)
 {

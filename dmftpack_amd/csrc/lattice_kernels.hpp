@@ -184,6 +184,8 @@ hipError_t launch_freq_totals(const cd* G, int m2, int nc, const cd* tr, const c
 // gloc[nw][e] = Gsum[nw][e] * scale for nw < n_freq, zero outside cluster blocks
 hipError_t launch_gloc_finalize(const cd* Gsum, int n_freq, int nc, int n_cluster, double scale, cd* gloc,
                                 hipStream_t s);
+// out[m] = scale * in[m]^H over a stack of count d x d matrices (in != out)
+hipError_t launch_adjoint_stack(const cd* in, size_t count, int d, double scale, cd* out, hipStream_t s);
 // out[i] = in[i] * scale
 hipError_t launch_scale(const cd* in, size_t len, double scale, cd* out, hipStream_t s);
 

@@ -304,6 +304,19 @@ class LatticeGF(Handle):
         return dict(gloc=from_dev_cm(gl), num_matrix=from_dev_cm(num), n_total=N, loss=loss,
                     sigma_bar=from_dev_cm(sb), mu_bar=mb)
 
+    def jvp_dev(self, sigma_dev, mu, dsig_dev, dmu, method=PLAIN_F64, out=None):
+        if out is None:
+            out = torch.empty_like(sigma_dev)
+        dn = C.c_double(0)
+        self.check(self.lib.dmftgpu_gloc_jvp(self.h, method, _ptr(sigma_dev), mu, _ptr(dsig_dev), dmu, _ptr(out),
+                                             C.byref(dn)))
+        return out, dn.value
+
+    def jvp(self, sigma, mu, dsigma, dmu, method=PLAIN_F64):
+        """(dG_loc, dN) for the tangent (dsigma, dmu): Jacobian-vector product, moments fixed."""
+        out, dn = self.jvp_dev(self.sigma_dev(sigma), mu, to_dev_cm(dsigma), dmu, method)
+        return from_dev_cm(out), dn
+
     def vjp(self, sigma, mu, gbar, nbar, method=PLAIN_F64):
         """(sigma_bar, mu_bar) for cotangents (gbar of G_loc, nbar of N); dL/dRe + i dL/dIm."""
         s = self.sigma_dev(sigma)

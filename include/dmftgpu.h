@@ -199,6 +199,18 @@ int dmftgpu_adjust_mu(dmftgpu_handle h, int method, const double* sigma, double 
 int dmftgpu_gloc_vjp(dmftgpu_handle h, int method, const double* sigma, double mu, const double* gloc_bar,
                      double n_bar, double* sigma_bar, double* mu_bar);
 
+/* Jacobian-vector product of (G_loc, N) in (Sigma(iw_n), n < N_freq) and mu,
+ * moments held fixed, G recomputed -- the J v of a Newton-Krylov update of
+ * the DMFT self-consistency (the reference mixes with Pulay, pulay.cpp:48-167):
+ *   dsigma [n_freq][nc][nc], dmu   tangent of (Sigma, mu)
+ *   dgloc  [n_freq][nc][nc]        output: dG_loc = (1/Nk) sum_k G_cc (dSigma - dmu S') G_cc,
+ *                                  the full nc x nc block (as the cotangent of dmftgpu_gloc_vjp)
+ *   dn     HOST output (may be NULL): dN
+ * The adjoint of dmftgpu_gloc_vjp: Re<gloc_bar, dgloc> + n_bar dn =
+ * Re<sigma_bar, dsigma> + mu_bar dmu.  No reference counterpart. */
+int dmftgpu_gloc_jvp(dmftgpu_handle h, int method, const double* sigma, double mu, const double* dsigma, double dmu,
+                     double* dgloc, double* dn);
+
 /* Loss and gradient in one call, for fitting / Newton-Krylov updates:
  *   L = sum_{n<N_freq} ||G_loc(iw_n) - G_tgt(iw_n)||_F^2 + (N - n_target)^2
  * with G_loc as dmftgpu_greenftn_w returns it (cluster-block-diagonal).
