@@ -534,9 +534,13 @@ def bench_path(args, torch, capi, G, L):
         units = 2 * pairs
         metric, unit, value = "(tau,iw) pairs/s, iw->tau->iw round trip", "(tau,w) pairs/s", units / t_wall
         ach = 8.0 * d * d * units / t_gpu / 1e12
-        roof = _flop_roofline(ach, "k_dft<W2Tau> + k_dft<Tau2W>", "algorithmic 8 d^2 FLOPs (one complex MAC per "
-                              "element) per (tau,iw) pair and direction / call time (events); the twiddles and the "
-                              "tail are extra", 8.0 * d * d, units)
+        roof = _flop_roofline(ach, "k_dft_mfma<W2Tau> + k_dft_mfma<Tau2W>",
+                              "algorithmic 8 d^2 FLOPs (one complex MAC per element) per (tau,iw) pair and direction "
+                              "/ call time (events, both calls: moment-free transform kernels, tail, causality "
+                              "eigensolves); the two transforms are FP64 MFMA complex GEMMs (4 real "
+                              "v_mfma_f64_16x16x4 per complex step, d^2 padded to 16-element tiles)",
+                              8.0 * d * d, units)
+        roof["bound"] = "mfma"
         if not args.no_cpu:
             sig_np, mom_np = sig, mom
 

@@ -111,7 +111,7 @@ extern "C" int dmftgpu_destroy(dmftgpu_handle h)
         if (h->ev1[m]) (void)hipEventDestroy(h->ev1[m]);
     }
     h->ft.release();
-    h->mom_cache.release();
+    for (auto& m : h->mom_cache) m.buf.release();
     h->ws.release();
     h->prep.release();
     h->stat.release();

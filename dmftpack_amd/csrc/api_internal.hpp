@@ -57,10 +57,18 @@ struct dmftgpu_handle_s {
     dgi::Workspace prep;     // prepare-time data
     dgi::Workspace ws;       // per-call scratch
     dgi::Workspace ft;       // Fourier-transform scratch
-    dgi::Workspace mom_cache;  // moment fit: basis table + inverted normal matrices of one (n_freq, beta, branch)
-    bool mom_key_valid = false;
-    int mom_key_nf = 0, mom_key_green = 0;
-    double mom_key_beta = 0.0;
+    // moment fit: basis table + inverted normal matrices per (n_freq, beta, branch);
+    // kMomCache grids are kept (a DMFT iteration alternates the Sigma-like and
+    // G-like fits), the least recently used one is formed again
+    static constexpr int kMomCache = 4;
+    struct MomEntry {
+        dgi::Workspace buf;
+        bool valid = false;
+        int nf = 0, green = 0;
+        double beta = 0.0;
+        unsigned long long used = 0;
+    } mom_cache[kMomCache];
+    unsigned long long mom_clock = 0;
     dgi::Workspace stat;     // static weights wt/dt [nk][n] of the cached mu
     // Kohn-Sham bands outside the model window (core-band term of GiwToNele,
     // GreenFtn.cpp:622-631), bound by dmftgpu_set_core_bands: energies

@@ -92,16 +92,24 @@ __device__ void lu_inverse4(const double (&A)[16], int n, double (&X)[16])
 // G    : Re: [-1/z^2, 0]          Im: [0, 1/z^3]             (:366-367)
 // basis values per frequency, tab[wn] = {-1/z, -1/z^2, 1/z^3, 1/z}, formed on
 // the host with std::pow as the reference forms them (dmftgpu_moments)
-__global__ void k_mom_ktk(int n_freq, int range, int green, const double4* __restrict__ tab, double* __restrict__ kinv)
+// K^T K of window w, entry (a, b) with a <= b (the matrix is symmetric): one
+// thread per (window, entry), summing the rows in order, Re row then Im row of
+// each frequency, one fma per row (zero entries included).
+__global__ void k_mom_ktk(int n_freq, int range, int green, const double4* __restrict__ tab, double* __restrict__ ktk)
 {
 #pragma clang fp contract(off)
-    const int w = blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= range) return;
     const int nm = green ? 2 : 4;
-    double K[16];
-    for (int e = 0; e < 16; ++e) K[e] = 0.0;
-    // K^T K summed over the rows in order, Re row then Im row of each
-    // frequency, one fma per row (zero entries included)
+    const int npair = nm * (nm + 1) / 2;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= range * npair) return;
+    const int w = t / npair;
+    int q = t % npair, b = 0;
+    while (q > b) {  // pair index q -> (a, b), a <= b, column by column
+        q -= b + 1;
+        ++b;
+    }
+    const int a = q;
+    double K = 0.0;
     for (int wn = w; wn < n_freq; ++wn) {
         const double4 c = tab[wn];
         double re[4] = {0, 0, 0, 0}, im[4] = {0, 0, 0, 0};
@@ -114,13 +122,21 @@ __global__ void k_mom_ktk(int n_freq, int range, int green, const double4* __res
             re[0] = c.y;
             im[1] = c.z;
         }
-        for (int b = 0; b < nm; ++b)
-            for (int a = 0; a < nm; ++a) {
-                K[a + b * 4] = fma(re[a], re[b], K[a + b * 4]);
-                K[a + b * 4] = fma(im[a], im[b], K[a + b * 4]);
-            }
+        K = fma(re[a], re[b], K);
+        K = fma(im[a], im[b], K);
     }
-    double X[16];
+    ktk[(size_t)w * 16 + a + b * 4] = K;
+    ktk[(size_t)w * 16 + b + a * 4] = K;
+}
+
+// inverse of each window's normal matrix, in place (one thread per window)
+__global__ void k_mom_kinv(int range, int green, double* __restrict__ kinv)
+{
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= range) return;
+    const int nm = green ? 2 : 4;
+    double K[16], X[16];
+    for (int e = 0; e < 16; ++e) K[e] = kinv[(size_t)w * 16 + e];
     lu_inverse4(K, nm, X);
     for (int e = 0; e < 16; ++e) kinv[(size_t)w * 16 + e] = X[e];
 }
@@ -392,8 +408,18 @@ __global__ void k_ft_subtract_tail(const cd* __restrict__ fw, int d, int n_freq,
 
 // out[j][e] = sum_i coef(j,i) in[i][e]  (NIN = 1)  or  sum_i ca in0 + cb in1 (NIN = 2)
 // workgroup: TJ consecutive rows j, all elements; inputs staged in LDS TI rows at a time.
+// Phase recurrence of the matrix-core transform (k_dft_mfma, iw -> tau): a lane
+// walks its row j through i, i + 4, i + 8, ...; the phase factor at i + 4 is the
+// one at i times a per-row constant, and every 16th step it is formed again
+// exactly from the FP64 phase, so the rounding of the products stays at ~1e-15.
+struct PhaseWalk {
+    cd p;   // phase factor at the current i
+    cd m4;  // factor from i to i + 4
+};
+
 struct CoefW2Tau {
     static constexpr bool PHASE_ROW = false;
+    static constexpr bool WALK = true;  // coefficients by the phase recurrence in k_dft_mfma
     double beta, w0, dw, tstep;
     const double* taus;  // explicit tau values (FourierTransform_tau), or null for tau_j = j*tstep
     __device__ void operator()(int j, int i, cd* c) const
@@ -404,6 +430,16 @@ struct CoefW2Tau {
         sincos(w * tau, &s, &co);
         c[0] = cd{co, -s};  // exp(-i w tau)
     }
+    __device__ void walk_start(int j, int i, PhaseWalk& st) const
+    {
+        const double tau = taus ? taus[j] : j * tstep;
+        double s, co;
+        sincos((w0 + i * dw) * tau, &s, &co);
+        st.p = cd{co, -s};
+        sincos(4.0 * dw * tau, &s, &co);
+        st.m4 = cd{co, -s};
+    }
+    __device__ void walk_coef(const PhaseWalk& st, int, int, cd* c) const { c[0] = st.p; }
 };
 
 struct CoefTau2W {
@@ -419,7 +455,13 @@ struct CoefTau2W {
         sincos(tau * wn, &s, &c);
         return cd{c, s};
     }
+    // The coefficients take differences of neighbouring phase factors, which cancel
+    // at low frequency, so every phase is formed exactly (no recurrence): rounding
+    // in them would be amplified by 1/(w dt).
+    static constexpr bool WALK = false;
     __device__ void operator()(int n, int t, cd* c) const { from_phases(n, t, phase(n, t), phase(n, t + 1), c); }
+    __device__ void walk_start(int, int, PhaseWalk&) const {}
+    __device__ void walk_coef(const PhaseWalk&, int n, int t, cd* c) const { (*this)(n, t, c); }
     __device__ void from_phases(int n, int t, cd e0, cd e1, cd* c) const
     {
         const double wn = (2 * n + 1) * kPi / beta;
@@ -644,6 +686,78 @@ __global__ void k_dft_sum_parts(const cd* __restrict__ parts, int nparts, size_t
     out[t] = s;
 }
 
+// The same transform on the matrix cores (FP64 MFMA), for m2 <= 16 * NE:
+// out[j][e] = sum_i sum_q coef_q(j, i) in_q[i][e] as complex products split
+// into real v_mfma_f64_16x16x4 (D = A B + D, A 16 x 4, B 4 x 16).  A wavefront
+// owns 16 consecutive rows j and every e tile (NE tiles of 16 elements), so
+// each coefficient it forms serves NE tiles.  Per step of 4 summed indices,
+// lane l = r + 16 k forms coef(j0 + r, i0 + k), the A operand's (r, k) entry
+// (by the phase recurrence of PhaseWalk),
+// and loads in[i0 + k][16 t + r], the B operand's (k, r) entry of tile t; the
+// accumulator of tile t holds out[j0 + c][16 t + r'] at lane r' + 16 (c % 4),
+// register c / 4.  blockIdx.y selects the range of i, as in k_dft.
+typedef double dmfma4 __attribute__((ext_vector_type(4)));
+
+template <class Coef, int NIN, int NE>
+__global__ __launch_bounds__(256) void k_dft_mfma(int nj, int ni, int m2, int ichunk, const cd* __restrict__ in0,
+                                                  const cd* __restrict__ in1, cd* __restrict__ out, Coef coef)
+{
+    const int lane = threadIdx.x & 63;
+    const int r = lane & 15, k = lane >> 4;
+    const int j0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;
+    const int ib = blockIdx.y * ichunk;
+    const int ie = min(ni, ib + ichunk);
+    out += (size_t)blockIdx.y * nj * m2;
+    dmfma4 accR[NE], accI[NE];
+#pragma unroll
+    for (int t = 0; t < NE; ++t) {
+        accR[t] = dmfma4{0.0, 0.0, 0.0, 0.0};
+        accI[t] = dmfma4{0.0, 0.0, 0.0, 0.0};
+    }
+    const int j = j0 + r;
+    const bool jok = j < nj;
+    PhaseWalk st{};
+    for (int i0 = ib, step = 0; i0 < ie; i0 += 4, ++step) {
+        const int i = i0 + k;
+        if constexpr (Coef::WALK) {
+            if ((step & 15) == 0) coef.walk_start(jok ? j : 0, i, st);
+            else st.p = cmul(st.p, st.m4);
+        }
+        cd c[2] = {cd{0, 0}, cd{0, 0}};
+        if (i < ie && jok) coef.walk_coef(st, j, i, c);
+#pragma unroll
+        for (int q = 0; q < NIN; ++q) {
+            const cd* in = q == 0 ? in0 : in1;
+            cd b[NE];
+#pragma unroll
+            for (int t = 0; t < NE; ++t) {
+                const int e = 16 * t + r;
+                b[t] = (i < ie && e < m2) ? in[(size_t)i * m2 + e] : cd{0, 0};
+            }
+            // four passes over the tiles: consecutive MFMAs update different accumulators
+            const double ar = c[q].x, ai = c[q].y;
+#pragma unroll
+            for (int t = 0; t < NE; ++t) accR[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, b[t].x, accR[t], 0, 0, 0);
+#pragma unroll
+            for (int t = 0; t < NE; ++t) accI[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, b[t].y, accI[t], 0, 0, 0);
+#pragma unroll
+            for (int t = 0; t < NE; ++t) accR[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, b[t].y, accR[t], 0, 0, 0);
+#pragma unroll
+            for (int t = 0; t < NE; ++t) accI[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, b[t].x, accI[t], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < NE; ++t) {
+        const int e = 16 * t + r;
+        if (e >= m2) continue;
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const int jj = j0 + k + 4 * g;
+            if (jj < nj) out[(size_t)jj * m2 + e] = cd{accR[t][g], accI[t][g]};
+        }
+    }
+}
+
 struct DftShape {
     int TJ, TI, ks, ichunk, jgroups, rj;  // jgroups > 0: the register-tiled kernel
 };
@@ -651,8 +765,31 @@ struct DftShape {
 // Tiles of the transform and the split of the summed index i over
 // workgroups: enough workgroups to fill the device (about 1024) when the
 // output alone gives fewer.
+// The matrix-core form (k_dft_mfma) for m2 <= 128: rows in tiles of 64 per
+// workgroup (16 per wavefront), the summed index split so that the launch has
+// about 4096 wavefronts (4 per SIMD), in multiples of 4.
+constexpr int kDftMfmaMaxM2 = 128;
+
+inline DftShape dft_shape_mfma(int nj, int ni)
+{
+    DftShape d{};
+    d.TJ = 64;
+    d.TI = 4;
+    const int gx = (nj + d.TJ - 1) / d.TJ;
+    const int steps = (ni + 3) / 4;
+    int ks = (1024 + gx - 1) / gx;
+    if (ks > steps) ks = steps;
+    if (ks > 32) ks = 32;
+    if (ks < 1) ks = 1;
+    d.ichunk = (steps + ks - 1) / ks * 4;
+    d.ks = (ni + d.ichunk - 1) / d.ichunk;
+    d.jgroups = -1;
+    return d;
+}
+
 inline DftShape dft_shape(int nj, int ni, int m2, int nin, bool phase_row)
 {
+    if (m2 <= kDftMfmaMaxM2) return dft_shape_mfma(nj, ni);
     DftShape d;
     if (m2 <= 256 && nin == 2) {  // register-tiled (tau -> iw; measured slower for iw -> tau): TJ = jgroups * rj rows
         d.jgroups = 256 / m2;
@@ -696,6 +833,23 @@ hipError_t launch_dft(int nj, int ni, int m2, const cd* in0, const cd* in1, cd* 
                       hipStream_t s)
 {
     const DftShape d = dft_shape(nj, ni, m2, NIN, Coef::PHASE_ROW);
+    if (d.jgroups < 0) {  // matrix cores
+        if (d.ks > 1 && !parts) return hipErrorInvalidValue;
+        cd* dstm = d.ks > 1 ? parts : out;
+        dim3 gridm((nj + d.TJ - 1) / d.TJ, d.ks);
+#define M_(NE)                                                                                                    \
+    if (m2 <= 16 * NE) {                                                                                          \
+        hipLaunchKernelGGL((k_dft_mfma<Coef, NIN, NE>), gridm, dim3(256), 0, s, nj, ni, m2, d.ichunk, in0, in1, dstm, \
+                           coef);                                                                                 \
+    } else
+        M_(1) M_(2) M_(4) M_(6) M_(7) M_(8) return hipErrorInvalidValue;
+#undef M_
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess || d.ks == 1) return e;
+        const size_t cnt = (size_t)nj * m2;
+        hipLaunchKernelGGL(k_dft_sum_parts, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, parts, d.ks, cnt, out);
+        return hipGetLastError();
+    }
     const int TJ = d.TJ, TI = d.TI;
     const int opt = (TJ * m2 + 255) / 256;
     size_t shm = sizeof(cd) * ((size_t)NIN * TJ * TI + (size_t)NIN * TI * m2 +
@@ -783,10 +937,21 @@ extern "C" int dmftgpu_moments(dmftgpu_handle h, int d, int n_freq, double beta,
     const int m2 = d * d;
     // the basis table and the inverted normal matrices depend on (n_freq, beta,
     // green_asym) only: kept in the handle and formed again when those change
-    const bool hit = h->mom_key_valid && h->mom_key_nf == n_freq && h->mom_key_beta == beta &&
-                     h->mom_key_green == green_asym;
+    dmftgpu_handle_s::MomEntry* ent = nullptr;
+    for (auto& m : h->mom_cache)
+        if (m.valid && m.nf == n_freq && m.beta == beta && m.green == green_asym) ent = &m;
+    const bool hit = ent != nullptr;
+    if (!hit) {  // the least recently used entry (an unused one first)
+        ent = &h->mom_cache[0];
+        for (auto& m : h->mom_cache)
+            if (!m.valid || m.used < ent->used) {
+                ent = &m;
+                if (!m.valid) break;
+            }
+    }
+    ent->used = ++h->mom_clock;
     hipError_t eg;
-    char* kbase = (char*)h->mom_cache.get((size_t)n_freq * sizeof(double4) + 256 + (size_t)range * 16 * 8, &eg);
+    char* kbase = (char*)ent->buf.get((size_t)n_freq * sizeof(double4) + 256 + (size_t)range * 16 * 8, &eg);
     if (!kbase) return fail(h, DMFTGPU_ERR_HIP, "moment cache: %s", hipGetErrorString(eg));
     double4* tab = (double4*)kbase;
     double* kinv = (double*)(kbase + ((size_t)n_freq * sizeof(double4) + 255) / 256 * 256);
@@ -807,7 +972,7 @@ extern "C" int dmftgpu_moments(dmftgpu_handle h, int d, int n_freq, double beta,
     int* cen = cv.take<int>(4);
     hipStream_t s = h->stream;
     if (!hit) {
-        h->mom_key_valid = false;
+        ent->valid = false;
         std::vector<double4> htab(n_freq);
         for (int wn = 0; wn < n_freq; ++wn) {
             const double z = (2 * wn + 1) * kPi / beta;  // (:224, :364)
@@ -815,12 +980,16 @@ extern "C" int dmftgpu_moments(dmftgpu_handle h, int d, int n_freq, double beta,
         }
         HIPCHK(hipMemcpyAsync(tab, htab.data(), sizeof(double4) * n_freq, hipMemcpyHostToDevice, s));
         HIPCHK(hipStreamSynchronize(s));  // htab is a host temporary
-        hipLaunchKernelGGL(k_mom_ktk, dim3(nblk(range)), dim3(256), 0, s, n_freq, range, green_asym, tab, kinv);
+        const int npair = green_asym ? 3 : 10;
+        hipLaunchKernelGGL(k_mom_ktk, dim3(nblk((size_t)range * npair)), dim3(256), 0, s, n_freq, range, green_asym, tab,
+                           kinv);
         HIPCHK(hipGetLastError());
-        h->mom_key_valid = true;
-        h->mom_key_nf = n_freq;
-        h->mom_key_beta = beta;
-        h->mom_key_green = green_asym;
+        hipLaunchKernelGGL(k_mom_kinv, dim3(nblk(range)), dim3(256), 0, s, range, green_asym, kinv);
+        HIPCHK(hipGetLastError());
+        ent->valid = true;
+        ent->nf = n_freq;
+        ent->beta = beta;
+        ent->green = green_asym;
     }
     hipLaunchKernelGGL(k_mom_fit, dim3(nblk((size_t)range * m2)), dim3(256), 0, s, (const cd*)fw, d, n_freq, range, tab,
                        green_asym, kinv, mt);
