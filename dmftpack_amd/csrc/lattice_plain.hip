@@ -95,7 +95,8 @@ __global__ __launch_bounds__(WG) void k_lattice_plain(LatArgs a)
             for (int i = 0; i < N; ++i)
                 if (i < nc) col[i] = csub(col[i], sSig[slot * m2 + i + c * nc]);
         }
-        gj_columns<N>(col, bc, c, n);
+        if constexpr (L == 16) gj_columns_row16<N>(col, c, n);
+        else gj_columns<N>(col, bc, c, n);
         // lane c < n now holds column c of G
         cd gcc = cd{0.0, 0.0};
 #pragma unroll
@@ -235,7 +236,8 @@ __global__ __launch_bounds__(256) void k_density_plain(LatArgs a, const cd* __re
             for (int i = 0; i < N; ++i)
                 if (i < nc) col[i] = csub(col[i], sigma_elem(a.sigma, a.mom, nc, a.n_freq, a.beta, nwc, i, c));
         }
-        gj_columns<N>(col, bc, c, n);
+        if constexpr (L == 16) gj_columns_row16<N>(col, c, n);
+        else gj_columns<N>(col, bc, c, n);
         if (wvalid && c < n) {
 #pragma unroll
             for (int i = 0; i < N; ++i) acc[i] = cadd(acc[i], col[i]);
