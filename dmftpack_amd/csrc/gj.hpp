@@ -77,31 +77,157 @@ __device__ __forceinline__ double row_bcast_j(double x, int j)
     }
 }
 
+// Rank-1 update c_i -= b_i t of one Gauss-Jordan step where b = lane J's
+// column, taken by the FMA itself: v_fmac_f64 with a DPP64 row_newbcast:J
+// source operand, so no separate broadcast move.  Per element the four FMAs
+// follow cfms's rounding order,
+//   A  x += b.y * t.y     B  y += b.y * (-t.x)
+//   C  x += b.x * (-t.x)  D  y += b.x * (-t.y)
+// issued phase by phase over a group of elements.  A DPP operand must not
+// have been written by either of the two preceding VALU instructions; in this
+// order every operand read by DPP was written at least two instructions
+// earlier, and "s_nop 1" at both ends of a block covers the compiler's own
+// instructions on either side, whose hazard tracking does not look inside the
+// asm.  The pivot lane has t = 0, so its registers -- the DPP sources of the
+// whole slot -- keep their values through the four phases.
+template <int J>
+__device__ __forceinline__ void fms_bcast4(cd& c0, cd& c1, cd& c2, cd& c3, double ty, double mtx, double mty)
+{
+    asm("s_nop 1\n\t"
+        "v_fmac_f64_dpp %0, %1, %8 row_newbcast:%c11\n\t"
+        "v_fmac_f64_dpp %2, %3, %8 row_newbcast:%c11\n\t"
+        "v_fmac_f64_dpp %4, %5, %8 row_newbcast:%c11\n\t"
+        "v_fmac_f64_dpp %6, %7, %8 row_newbcast:%c11\n\t"
+        "v_fmac_f64_dpp %1, %1, %9 row_newbcast:%c11\n\t"
+        "v_fmac_f64_dpp %3, %3, %9 row_newbcast:%c11\n\t"
+        "v_fmac_f64_dpp %5, %5, %9 row_newbcast:%c11\n\t"
+        "v_fmac_f64_dpp %7, %7, %9 row_newbcast:%c11\n\t"
+        "v_fmac_f64_dpp %0, %0, %9 row_newbcast:%c11\n\t"
+        "v_fmac_f64_dpp %2, %2, %9 row_newbcast:%c11\n\t"
+        "v_fmac_f64_dpp %4, %4, %9 row_newbcast:%c11\n\t"
+        "v_fmac_f64_dpp %6, %6, %9 row_newbcast:%c11\n\t"
+        "v_fmac_f64_dpp %1, %0, %10 row_newbcast:%c11\n\t"
+        "v_fmac_f64_dpp %3, %2, %10 row_newbcast:%c11\n\t"
+        "v_fmac_f64_dpp %5, %4, %10 row_newbcast:%c11\n\t"
+        "v_fmac_f64_dpp %7, %6, %10 row_newbcast:%c11\n\t"
+        "s_nop 1"
+        : "+v"(c0.x), "+v"(c0.y), "+v"(c1.x), "+v"(c1.y), "+v"(c2.x), "+v"(c2.y), "+v"(c3.x), "+v"(c3.y)
+        : "v"(ty), "v"(mtx), "v"(mty), "n"(J));
+}
+
+template <int J>
+__device__ __forceinline__ void fms_bcast3(cd& c0, cd& c1, cd& c2, double ty, double mtx, double mty)
+{
+    asm("s_nop 1\n\t"
+        "v_fmac_f64_dpp %0, %1, %6 row_newbcast:%c9\n\t"
+        "v_fmac_f64_dpp %2, %3, %6 row_newbcast:%c9\n\t"
+        "v_fmac_f64_dpp %4, %5, %6 row_newbcast:%c9\n\t"
+        "v_fmac_f64_dpp %1, %1, %7 row_newbcast:%c9\n\t"
+        "v_fmac_f64_dpp %3, %3, %7 row_newbcast:%c9\n\t"
+        "v_fmac_f64_dpp %5, %5, %7 row_newbcast:%c9\n\t"
+        "v_fmac_f64_dpp %0, %0, %7 row_newbcast:%c9\n\t"
+        "v_fmac_f64_dpp %2, %2, %7 row_newbcast:%c9\n\t"
+        "v_fmac_f64_dpp %4, %4, %7 row_newbcast:%c9\n\t"
+        "v_fmac_f64_dpp %1, %0, %8 row_newbcast:%c9\n\t"
+        "v_fmac_f64_dpp %3, %2, %8 row_newbcast:%c9\n\t"
+        "v_fmac_f64_dpp %5, %4, %8 row_newbcast:%c9\n\t"
+        "s_nop 1"
+        : "+v"(c0.x), "+v"(c0.y), "+v"(c1.x), "+v"(c1.y), "+v"(c2.x), "+v"(c2.y)
+        : "v"(ty), "v"(mtx), "v"(mty), "n"(J));
+}
+
+// two elements: one wait state between C1 and D0 is padded to two
+template <int J>
+__device__ __forceinline__ void fms_bcast2(cd& c0, cd& c1, double ty, double mtx, double mty)
+{
+    asm("s_nop 1\n\t"
+        "v_fmac_f64_dpp %0, %1, %4 row_newbcast:%c7\n\t"
+        "v_fmac_f64_dpp %2, %3, %4 row_newbcast:%c7\n\t"
+        "v_fmac_f64_dpp %1, %1, %5 row_newbcast:%c7\n\t"
+        "v_fmac_f64_dpp %3, %3, %5 row_newbcast:%c7\n\t"
+        "v_fmac_f64_dpp %0, %0, %5 row_newbcast:%c7\n\t"
+        "v_fmac_f64_dpp %2, %2, %5 row_newbcast:%c7\n\t"
+        "s_nop 0\n\t"
+        "v_fmac_f64_dpp %1, %0, %6 row_newbcast:%c7\n\t"
+        "v_fmac_f64_dpp %3, %2, %6 row_newbcast:%c7\n\t"
+        "s_nop 1"
+        : "+v"(c0.x), "+v"(c0.y), "+v"(c1.x), "+v"(c1.y)
+        : "v"(ty), "v"(mtx), "v"(mty), "n"(J));
+}
+
+template <int J>
+__device__ __forceinline__ void fms_bcast1(cd& c0, double ty, double mtx, double mty)
+{
+    asm("s_nop 1\n\t"
+        "v_fmac_f64_dpp %0, %1, %2 row_newbcast:%c5\n\t"
+        "v_fmac_f64_dpp %1, %1, %3 row_newbcast:%c5\n\t"
+        "s_nop 0\n\t"
+        "v_fmac_f64_dpp %0, %0, %3 row_newbcast:%c5\n\t"
+        "s_nop 1\n\t"
+        "v_fmac_f64_dpp %1, %0, %4 row_newbcast:%c5\n\t"
+        "s_nop 1"
+        : "+v"(c0.x), "+v"(c0.y)
+        : "v"(ty), "v"(mtx), "v"(mty), "n"(J));
+}
+
+// acc += s_M w, where s_M is lane M's value of s (DPP64 row_newbcast:M) and w
+// is the lane's own; cfma's rounding order.  Nothing the block writes is read
+// through DPP, so only the leading s_nop is needed.
+template <int M>
+__device__ __forceinline__ void cfma_bcast(cd& acc, const cd& s, const cd& w, double mwy)
+{
+    asm("s_nop 1\n\t"
+        "v_fmac_f64_dpp %0, %3, %6 row_newbcast:%c7\n\t"
+        "v_fmac_f64_dpp %1, %3, %4 row_newbcast:%c7\n\t"
+        "v_fmac_f64_dpp %0, %2, %4 row_newbcast:%c7\n\t"
+        "v_fmac_f64_dpp %1, %2, %5 row_newbcast:%c7"
+        : "+v"(acc.x), "+v"(acc.y)
+        : "v"(s.x), "v"(s.y), "v"(w.x), "v"(w.y), "v"(mwy), "n"(M));
+}
+
+// c_i -= b_i t for all i < N (b = lane J's column): groups of four, then the rest
+template <int J, int N>
+__device__ __forceinline__ void fms_bcast_all(cd (&col)[N], double ty, double mtx, double mty)
+{
+    constexpr int Q = N / 4, R = N % 4;
+#pragma unroll
+    for (int g = 0; g < Q; ++g) fms_bcast4<J>(col[4 * g], col[4 * g + 1], col[4 * g + 2], col[4 * g + 3], ty, mtx, mty);
+    if constexpr (R == 3) fms_bcast3<J>(col[4 * Q], col[4 * Q + 1], col[4 * Q + 2], ty, mtx, mty);
+    if constexpr (R == 2) fms_bcast2<J>(col[4 * Q], col[4 * Q + 1], ty, mtx, mty);
+    if constexpr (R == 1) fms_bcast1<J>(col[4 * Q], ty, mtx, mty);
+}
+
+// Steps J .. N-1 of the row-slot elimination; the step index is a template
+// argument (it is the DPP control immediate), so the recursion replaces an
+// unrolled loop.
+template <int N, int J>
+__device__ __forceinline__ void gj_row16_steps(cd (&col)[N], cd& scl, int c, int n)
+{
+    if constexpr (J < N) {
+        if (J < n) {
+            const cd bj = cd{row_bcast<J>(col[J].x), row_bcast<J>(col[J].y)};
+            const cd p = crcp(bj);
+            const bool isj = (c == J);
+            const cd t = isj ? cd{0.0, 0.0} : cmul(col[J], p);
+            fms_bcast_all<J, N>(col, t.y, -t.x, -t.y);
+            col[J] = isj ? cd{-1.0, 0.0} : t;
+            scl = isj ? cneg(p) : scl;
+            gj_row16_steps<N, J + 1>(col, scl, c, n);
+        }
+    }
+}
+
 // The same elimination for a slot that is one 16-lane row of the wavefront
-// (N <= 16, slot lanes 16 s .. 16 s + 15): the pivot column is broadcast from
-// lane j's registers by DPP instead of through LDS, so a step has no LDS
-// round trip and no wait.
+// (N <= 16, slot lanes 16 s .. 16 s + 15): the pivot column is read from
+// lane j's registers by the update FMAs themselves (DPP64 row_newbcast), so a
+// step has no LDS round trip, no wait and no broadcast moves.  The update of
+// element j is computed like the others and then overwritten.
 template <int N>
 __device__ __forceinline__ void gj_columns_row16(cd (&col)[N], int c, int n)
 {
     static_assert(N <= 16, "a slot of one 16-lane row");
     cd scl = cd{1.0, 0.0};
-#pragma unroll
-    for (int j = 0; j < N; ++j) {
-        if (j < n) {
-            cd b[N];
-#pragma unroll
-            for (int i = 0; i < N; ++i) b[i] = cd{row_bcast_j(col[i].x, j), row_bcast_j(col[i].y, j)};
-            const cd p = crcp(b[j]);
-            const bool isj = (c == j);
-            const cd t = isj ? cd{0.0, 0.0} : cmul(col[j], p);
-#pragma unroll
-            for (int i = 0; i < N; ++i)
-                if (i != j) col[i] = cfms(b[i], t, col[i]);
-            col[j] = isj ? cd{-1.0, 0.0} : t;
-            scl = isj ? cneg(p) : scl;
-        }
-    }
+    gj_row16_steps<N, 0>(col, scl, c, n);
 #pragma unroll
     for (int i = 0; i < N; ++i) col[i] = cmul(col[i], scl);
 }

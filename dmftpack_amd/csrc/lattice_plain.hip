@@ -31,10 +31,47 @@
 
 namespace dg {
 
+// Adjoint sums of a 16-lane slot, column c on lane c (lane M's column read by
+// DPP in the FMAs themselves):
+//   S2_:c += sum_{M < n} G_{:nc, M} G_{M c}     ((G^2)_cc, rows i < nc)
+template <int N, int NCR, int M>
+__device__ __forceinline__ void s2_from_lanes(const cd (&col)[N], cd (&S2)[NCR], int nc, int n)
+{
+    if constexpr (M < N) {
+        if (M < n) {
+            const double mwy = -col[M].y;
+#pragma unroll
+            for (int i = 0; i < NCR; ++i)
+                if (i < nc) cfma_bcast<M>(S2[i], col[i], col[M], mwy);
+            s2_from_lanes<N, NCR, M + 1>(col, S2, nc, n);
+        }
+    }
+}
+
+//   XH_:c += sum_{I < nc} G_{:nc, I} z_I      (z = gbar^H G_:c, so XH = G_cc gbar^H G_cc)
+template <int N, int NCR, int I>
+__device__ __forceinline__ void xh_from_lanes(const cd (&col)[N], const cd (&z)[NCR], cd (&XH)[NCR], int nc)
+{
+    if constexpr (I < NCR) {
+        if (I < nc) {
+            const double mzy = -z[I].y;
+#pragma unroll
+            for (int r = 0; r < NCR; ++r)
+                if (r < nc) cfma_bcast<I>(XH[r], col[r], z[I], mzy);
+            xh_from_lanes<N, NCR, I + 1>(col, z, XH, nc);
+        }
+    }
+}
+
 template <int N, int L, int MODE, int WG>
 __global__ __launch_bounds__(WG) void k_lattice_plain(LatArgs a)
 {
     constexpr int SLOTS = WG / L;
+    // 16-lane slots with n_c <= NCR keep the adjoint sums in registers: lane
+    // c < n_c holds column c of S2 = sum_k (G^2)_cc and of XH = sum_k G_cc
+    // gbar^H G_cc; the terms linear in (G^2)_cc (the N constraint, mu_bar)
+    // are applied once after the k loop
+    constexpr int NCR = (MODE == MODE_VJP && L == 16) ? (N < 6 ? N : 6) : 1;
     constexpr int BS = (MODE == MODE_FWD) ? 2 * N : (N * N > 2 * N ? N * N : 2 * N);
     constexpr int NN = N * N;
     constexpr int PF = (NN + WG - 1) / WG;
@@ -69,6 +106,10 @@ __global__ __launch_bounds__(WG) void k_lattice_plain(LatArgs a)
 
     cd trA = cd{0.0, 0.0}, tr2A = cd{0.0, 0.0}, muA = cd{0.0, 0.0};
     cd* bc = sB + slot * BS;
+    const bool vjp_reg = (MODE == MODE_VJP) && L == 16 && nc <= NCR;
+    cd S2[NCR], XH[NCR];
+#pragma unroll
+    for (int i = 0; i < NCR; ++i) S2[i] = XH[i] = cd{0.0, 0.0};
     __syncthreads();
 
     for (int k = k0; k < k1; ++k) {
@@ -124,7 +165,21 @@ __global__ __launch_bounds__(WG) void k_lattice_plain(LatArgs a)
                     if (m < n) s = cfma(bc[c + m * N], col[m], s);
                 if (wvalid) tr2A = cadd(tr2A, s);
             }
-            if (MODE == MODE_VJP) {
+            if constexpr (MODE == MODE_VJP && L == 16) {
+                if (vjp_reg && vjp_here) {  // every lane of the slot takes part
+                    s2_from_lanes<N, NCR, 0>(col, S2, nc, n);
+                    cd z[NCR];
+#pragma unroll
+                    for (int i = 0; i < NCR; ++i) {
+                        z[i] = cd{0.0, 0.0};
+#pragma unroll
+                        for (int j = 0; j < NCR; ++j)
+                            if (i < nc && j < nc) z[i] = cfmac(sGb[slot * m2 + j + i * nc], col[j], z[i]);
+                    }
+                    xh_from_lanes<N, NCR, 0>(col, z, XH, nc);
+                }
+            }
+            if (MODE == MODE_VJP && !vjp_reg) {
                 if (vjp_here && c < nc) {
                     // v_i = sum_j gbar_ij conj(G_cj)  (row c of G_cc)
                     for (int i = 0; i < nc; ++i) {
@@ -162,6 +217,21 @@ __global__ __launch_bounds__(WG) void k_lattice_plain(LatArgs a)
         __syncthreads();
     }
 
+    if (MODE == MODE_VJP && vjp_reg) {
+        // sigma_bar(c, r) = conj(XH_rc + cn S2_rc); mu_bar's tr(gbar^H S2) by columns
+        const double cn = 2.0 * (a.nbar_dev ? *a.nbar_dev : a.nbar) / a.beta;
+        cd mu_s = cd{0.0, 0.0};
+        if (c < nc) {
+#pragma unroll
+            for (int r = 0; r < NCR; ++r)
+                if (r < nc) {
+                    sAcc[slot * m2 + c + r * nc] = cconj(cadd(XH[r], cscale(S2[r], cn)));
+                    mu_s = cfmac(sGb[slot * m2 + r + c * nc], S2[r], mu_s);
+                }
+        }
+        muA = mu_s;
+        wave_sync();  // sAcc is stored below by other lanes of the slot
+    }
     // reduce lane partials of the slot (lanes of a slot are contiguous)
 #pragma unroll
     for (int o = L / 2; o > 0; o >>= 1) {
@@ -309,6 +379,11 @@ static hipError_t launch_n(const LatArgs& a, hipStream_t s, int* wtile)
     constexpr int L = N <= 4 ? 4 : N <= 8 ? 8 : N <= 16 ? 16 : 32;
     constexpr int WG = (MODE == MODE_FWD || N <= 8) ? 256 : 128;
     constexpr int SLOTS = WG / L;
+    // 16-lane slots with n_c <= NCR keep the adjoint sums in registers: lane
+    // c < n_c holds column c of S2 = sum_k (G^2)_cc and of XH = sum_k G_cc
+    // gbar^H G_cc; the terms linear in (G^2)_cc (the N constraint, mu_bar)
+    // are applied once after the k loop
+    constexpr int NCR = (MODE == MODE_VJP && L == 16) ? (N < 6 ? N : 6) : 1;
     constexpr int BS = (MODE == MODE_FWD) ? 2 * N : (N * N > 2 * N ? N * N : 2 * N);
     const int m2 = a.nc * a.nc;
     size_t shm = sizeof(cd) * (2 * (size_t)N * N + (size_t)SLOTS * BS + (size_t)SLOTS * m2 * (MODE == MODE_VJP ? 4 : 2));
