@@ -199,13 +199,16 @@ def main():
                 lat.set_option(capi.OPT_SCHUR_LAYOUT, layout)
                 steps = args.plain_steps if name == "plain_f64" else max(3, args.steps // 2)
                 log(f"timing {name} layout {layout}: {steps} steps")
-                ms, _, _ = timed(methods[name], steps, 1)
+                ms, pkf, pkv = timed(methods[name], steps, 1, kernel_times=(name == "plain_f64"))
                 key = name if name == "plain_f64" else f"{name}_layout{layout}"
                 others[key] = dict(value=units / ms, ms_per_step=ms * 1e3, steps=steps)
+                if name == "plain_f64":
+                    others[key].update(fwd_call_ms=float(np.mean(pkf)), vjp_call_ms=float(np.mean(pkv)))
         lat.set_option(capi.OPT_SCHUR_LAYOUT, args.layout)
         extra["other_methods"] = others
         if "plain_f64" in others:
             extra["speedup_vs_plain_fp64"] = value / others["plain_f64"]["value"]
+            extra["plain_fp64_executed"] = plain_executed(others["plain_f64"], args.workload)
 
     # ---- roofline of the dominant kernel --------------------------------------
     # On the one-matrix-per-lane Schur path dmftgpu_gloc_loss_grad is fused: the
@@ -327,6 +330,35 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(f"calibration copy: {args.calib_copy} MiB read + {args.calib_copy} MiB written per launch", flush=True)
+
+
+def plain_executed(rec, workload):
+    """Executed fraction of FP64 peak of the plain comparator's two lattice calls:
+    FLOPs per launch from rocprofv3 SQ_INSTS_VALU_FLOPS_FP64/_FP32 x 64 lanes of
+    k_lattice_plain on the same workload (profiles/pmc_flops.json, "plain_f64")
+    over this run's live call times (HIP events on the library's stream)."""
+    fpath = os.path.join(ROOT, "profiles", "pmc_flops.json")
+    if workload != "dp14" or not os.path.exists(fpath):
+        return None
+    try:
+        pf = json.load(open(fpath)).get("plain_f64", {})
+    except Exception:
+        return None
+    out = {}
+    for call, key in (("fwd_call", "FWD_CALL"), ("vjp_call", "VJP_CALL")):
+        ex, ms = pf.get(key), rec.get(call + "_ms")
+        if not ex or not ms:
+            continue
+        fl = ex["fp64_flops"] + ex["fp32_flops"]
+        out[call] = dict(ms=ms, flop_per_launch=fl, tflops=fl / (ms / 1e3) / 1e12,
+                         frac=fl / (ms / 1e3) / 1e12 / PEAK_FP64_TFLOPS, counter_kernel=ex["kernel"])
+    if not out:
+        return None
+    fl = sum(v["flop_per_launch"] for v in out.values())
+    ms = sum(v["ms"] for v in out.values())
+    out["both_calls_frac"] = fl / (ms / 1e3) / 1e12 / PEAK_FP64_TFLOPS
+    out["source"] = "profiles/pmc_flops.json (plain_f64) over live call times"
+    return out
 
 
 def bench_batch(args, torch, capi, G, L):
@@ -464,12 +496,28 @@ def bench_path(args, torch, capi, G, L):
         units = nk * 2 * nf
         out = torch.empty((nk, n, n), dtype=torch.complex128, device="cuda")
         fn = lambda: lat.check(lat.lib.dmftgpu_density_matrix(lat.h, G._ptr(s_dev), mu, G._ptr(out)))  # noqa: E731
-        log("timing density_matrix")
-        t_gpu, t_wall = _events(torch, fn, args.steps, args.warmup)
-        metric, unit, value = "densityMatDFT (k,w) inversions/s", "(k,w) inversions/s", units / t_wall
+        per = {}
+        for kern, kname in ((1, "full_inverse"), (0, "block_mfma")):
+            lat.set_option(capi.OPT_DENSITY_KERNEL, kern)
+            log(f"timing density_matrix ({kname})")
+            t_gpu, t_wall = _events(torch, fn, args.steps, args.warmup)
+            per[kname] = dict(value=units / t_wall, ms_per_call=t_wall * 1e3, gpu_ms=t_gpu * 1e3)
+            torch.cuda.synchronize()
+            per[kname]["out"] = out.clone()
+        diff = float(torch.max(torch.abs(per["block_mfma"]["out"] - per["full_inverse"]["out"])).item())
+        scale = float(torch.max(torch.abs(per["full_inverse"]["out"])).item())
+        for v in per.values():
+            del v["out"]
+        t_gpu = per["block_mfma"]["gpu_ms"] / 1e3
+        metric, unit, value = "densityMatDFT (k,w) inversions/s", "(k,w) inversions/s", per["block_mfma"]["value"]
+        extra["kernels"] = per
+        extra["speedup_block_vs_full_inverse"] = per["block_mfma"]["value"] / per["full_inverse"]["value"]
+        extra["max_abs_diff_block_vs_full_inverse"] = diff
+        extra["rel_diff_block_vs_full_inverse"] = diff / max(scale, 1e-300)
         ach = f_fwd(n) * units / t_gpu / 1e12
-        roof = _flop_roofline(ach, "k_density_plain", "algorithmic 8n^3 FLOPs per (k,iw) x Nk*2N_freq / call time "
-                              "(events); full n x n inverse per (k,iw), plain FP64", f_fwd(n), units)
+        roof = _flop_roofline(ach, "k_density_lr", "plain-equivalent 8n^3 FLOPs per (k,iw) x Nk*2N_freq / call time "
+                              "(events); the block form executes far fewer (Schur complement + rank-n_corr product "
+                              "on FP64 MFMA): a speed figure, not a fraction of peak", f_fwd(n), units)
         if not args.no_cpu:
             def run(kk):
                 O.greenftn_w(Hk[:kk], sig, mom, mu, beta, density=True, nthreads=cores)

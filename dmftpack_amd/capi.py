@@ -15,6 +15,7 @@ HEADER = os.path.join(HERE, "..", "include", "dmftgpu.h")
 OK, ERR_ARG, ERR_STATE, ERR_HIP, ERR_NUMERIC, ERR_UNSUPPORTED = 0, 1, 2, 3, 4, 5
 PLAIN_F64, SCHUR_F64, SCHUR_F32R = 0, 1, 2
 OPT_REFINE_STEPS, OPT_TARGET_WORKGROUPS, OPT_SCHUR_WG, OPT_SCHUR_LAYOUT, OPT_UPFOLD_CHUNK = 1, 2, 3, 4, 5
+OPT_DENSITY_KERNEL = 6
 MODE_FWD, MODE_DENS, MODE_VJP = 0, 1, 2
 
 

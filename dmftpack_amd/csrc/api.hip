@@ -181,6 +181,10 @@ extern "C" int dmftgpu_set_option(dmftgpu_handle h, int option, int value)
             if (value < 0) return fail(h, DMFTGPU_ERR_ARG, "upfolding chunk %d < 0", value);
             h->upfold_chunk = value;
             return DMFTGPU_OK;
+        case DMFTGPU_OPT_DENSITY_KERNEL:
+            if (value != 0 && value != 1) return fail(h, DMFTGPU_ERR_ARG, "density kernel %d not 0/1", value);
+            h->density_kernel = value;
+            return DMFTGPU_OK;
         default: return fail(h, DMFTGPU_ERR_ARG, "unknown option %d", option);
     }
 }
@@ -1034,7 +1038,10 @@ extern "C" int dmftgpu_density_matrix(dmftgpu_handle h, const double* sigma, dou
     const dmftgpu_lattice& L = h->lat;
     const int m2 = L.n_corr * L.n_corr;
     hipError_t e;
-    size_t need = static_ws_bytes(L) + (size_t)(m2 + 8) * 32 + 8192;
+    const size_t nw_tot = (size_t)L.n_freq * L.high_freq;
+    const bool block = h->density_kernel == 0 && h->schur_ready && h->kdata && dg::density_lr_supported(L.n, L.n_corr);
+    const size_t sig_bytes = block ? (nw_tot * m2 * sizeof(cd) + 255) / 256 * 256 : 0;
+    size_t need = static_ws_bytes(L) + (size_t)(m2 + 8) * 32 + 8192 + sig_bytes;
     char* ws = (char*)h->ws.get(need, &e);
     if (!ws) return fail(h, DMFTGPU_ERR_HIP, "workspace %zu bytes: %s", need, hipGetErrorString(e));
     size_t off = 0;
@@ -1042,6 +1049,8 @@ extern "C" int dmftgpu_density_matrix(dmftgpu_handle h, const double* sigma, dou
     off += ((size_t)m2 * sizeof(cd) + 255) / 256 * 256;
     double* sts = (double*)(ws + off);
     off += 256;
+    cd* sig_all = block ? (cd*)(ws + off) : nullptr;
+    off += sig_bytes;
     const double* wt = nullptr;
     int rc = run_static(h, mu, st, sts, ws, off, false, &wt);  // the static weights for this mu
     if (rc) return rc;
@@ -1056,7 +1065,14 @@ extern "C" int dmftgpu_density_matrix(dmftgpu_handle h, const double* sigma, dou
     a.nw_tot = L.n_freq * L.high_freq;
     a.beta = L.beta;
     a.mu = mu;
-    HIPCHK(dg::launch_density_matrix(a, h->U, wt, (cd*)dens, h->stream));
+    a.kdata = h->kdata;
+    if (block) {
+        a.pSigAll = sig_all;
+        HIPCHK(dg::launch_sigma_all(a, h->stream));
+        HIPCHK(dg::launch_density_matrix_lr(a, h->W, h->Vd, h->ed, h->U, wt, (cd*)dens, h->stream));
+    } else {
+        HIPCHK(dg::launch_density_matrix(a, h->U, wt, (cd*)dens, h->stream));
+    }
     HIPCHK(hipStreamSynchronize(h->stream));
     return DMFTGPU_OK;
 }

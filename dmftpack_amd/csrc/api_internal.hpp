@@ -53,6 +53,7 @@ struct dmftgpu_handle_s {
     int schur_wg = 128;      // workgroup width of the Schur kernels (64 or 128)
     int schur_layout = 0;    // n_c <= 6: 0 one matrix per lane, 1 column per lane
     int upfold_chunk = 0;    // k per chunk of dmftgpu_upfold_density (0: scratch-bound)
+    int density_kernel = 0;  // dmftgpu_density_matrix: 0 block form where supported, 1 full inverse
     bool static_cache_on = true;  // reuse static weights for a repeated mu
     dgi::Workspace prep;     // prepare-time data
     dgi::Workspace ws;       // per-call scratch

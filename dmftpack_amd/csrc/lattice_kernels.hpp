@@ -85,6 +85,10 @@ hipError_t launch_lattice_plain(const LatArgs& a, int mode, hipStream_t s, int* 
 int plain_slots(int n, int mode);
 // densityMatDFT: dens [nk][n][n] from U, static weights wt [nk][n]
 hipError_t launch_density_matrix(const LatArgs& a, const cd* U, const double* wt, cd* dens, hipStream_t s);
+// densityMatDFT in block form (density_lr.hip): n <= 16, n_c <= 6, a.kdata set
+bool density_lr_supported(int n, int nc);
+hipError_t launch_density_matrix_lr(const LatArgs& a, const cd* W, const cd* Vd, const double* ed, const cd* U,
+                                    const double* wt, cd* dens, hipStream_t s);
 
 // Schur complement (FP64 or FP32 + refinement); wg = workgroup width of the
 // one-matrix-per-lane kernels (64 or 128); cols selects the column-per-lane

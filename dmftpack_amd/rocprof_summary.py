@@ -16,6 +16,10 @@
   python -m dmftpack_amd.rocprof_summary sq --run <dir> --method NAME [--json profiles/pmc_flops.json]
       executed FP64/FP32 VALU FLOPs per launch (SQ_INSTS_VALU_FLOPS_* x 64 lanes)
       and the instruction mix, from one --pmc pass of SQ counters.
+
+  python -m dmftpack_amd.rocprof_summary mfma --run <dir> [--command "..."]
+      FP64 MFMA and VALU FLOPs per launch and MFMA busy cycles, from one --pmc
+      pass of MFMA_COUNTERS.
 """
 import argparse
 import csv
@@ -40,7 +44,7 @@ def trace(db_path, command):
                      "from kernels group by name").fetchall()
     print("\n# resources (vgpr, agpr, scratch B/lane, lds B, grid x, grid y, wg)")
     for r in res:
-        if "dg::k_" in r[0]:
+        if is_ours(r[0]):
             print(f"{r[1]:>4} {r[2]:>4} {r[3]:>6} {r[4]:>7} {r[5]:>8} {r[6]:>5} {r[7]:>4}  {r[0][:110]}")
 
 
@@ -67,6 +71,11 @@ def read_counter(run_dir, counter):
     for (name, _, _), v in acc.items():
         out[name].append(v)
     return out
+
+
+def is_ours(name):
+    """Kernels of this library: dg::k_..., including those in anonymous namespaces."""
+    return re.search(r"(^|::|\s)k_\w+[(<]", name) is not None
 
 
 def lattice_mode(name):
@@ -166,7 +175,7 @@ def sq(argv):
           f"{'smem_Minst':>10}  kernel")
     out = {}
     for name in sorted(rows):
-        if "dg::k_" not in name:
+        if not is_ours(name):
             continue
         nd = len(dur[name])
         t = sum(dur[name].values()) / nd * 1e-9
@@ -187,13 +196,63 @@ def sq(argv):
             json.dump(db, fh, indent=1)
 
 
+MFMA_COUNTERS = ("SQ_INSTS_VALU_MFMA_MOPS_F64 SQ_INSTS_VALU_MFMA_F64 SQ_VALU_MFMA_BUSY_CYCLES SQ_INSTS_VALU_FLOPS_FP64 "
+                 "SQ_INSTS_VALU SQ_BUSY_CYCLES GRBM_GUI_ACTIVE")
+
+
+def mfma(argv):
+    """FP64 work of kernels that use the matrix cores, from one --pmc pass of
+    MFMA_COUNTERS: MFMA FLOPs = SQ_INSTS_VALU_MFMA_MOPS_F64 x 512 (one MOP is
+    512 FLOPs), VALU FLOPs = SQ_INSTS_VALU_FLOPS_FP64 x 64 lanes, both over the
+    launch duration under the profiler against the 78.6 TFLOP/s FP64 peak
+    (matrix and vector alike on MI355X); MFMA busy = SQ_VALU_MFMA_BUSY_CYCLES
+    over the launch's GRBM_GUI_ACTIVE cycles (that counter is summed over the
+    8 XCDs, so divided by 8) x 1024 SIMDs, which matches MFMA FLOPs / (time x
+    FP64 matrix peak) on the kernels measured here."""
+    ap = argparse.ArgumentParser(prog="rocprof_summary mfma")
+    ap.add_argument("--run", required=True)
+    ap.add_argument("--command", default="")
+    a = ap.parse_args(argv)
+    rows = defaultdict(lambda: defaultdict(float))
+    dur = defaultdict(dict)
+    for f in glob.glob(os.path.join(a.run, "**", "*results.db"), recursive=True):
+        db = sqlite3.connect(f)
+        for name, did, cn, v, d in db.execute("select kernel_name, dispatch_id, counter_name, value, duration "
+                                              "from counters_collection"):
+            rows[name][cn] += v
+            dur[name][(f, did)] = d
+    print(f"# FP64 matrix-core and vector work per launch from rocprofv3 --pmc {MFMA_COUNTERS}")
+    if a.command:
+        print(f"# command: {a.command}")
+    print("# mfma_GF = SQ_INSTS_VALU_MFMA_MOPS_F64 x 512, valu_GF = SQ_INSTS_VALU_FLOPS_FP64 x 64; frac of 78.6 "
+          "TFLOP/s over the launch time under the profiler; mfma_busy = SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE/8 "
+          "x 1024 SIMDs)")
+    print(f"{'launch_ms':>9} {'mfma_GF':>9} {'valu_GF':>9} {'TFLOP/s':>8} {'frac':>6} {'mfma_Minst':>10} "
+          f"{'mfma_busy':>9}  kernel")
+    for name in sorted(rows):
+        if not is_ours(name):
+            continue
+        nd = len(dur[name])
+        t = sum(dur[name].values()) / nd * 1e-9
+        c = {k: v / nd for k, v in rows[name].items()}
+        fm = c.get("SQ_INSTS_VALU_MFMA_MOPS_F64", 0.0) * 512
+        fv = c.get("SQ_INSTS_VALU_FLOPS_FP64", 0.0) * 64
+        rate = (fm + fv) / t / 1e12 if t > 0 else 0.0
+        gui = c.get("GRBM_GUI_ACTIVE", 0.0) / 8.0
+        busy = c.get("SQ_VALU_MFMA_BUSY_CYCLES", 0.0) / (gui * 1024) if gui > 0 else 0.0
+        print(f"{t * 1e3:>9.3f} {fm / 1e9:>9.1f} {fv / 1e9:>9.1f} {rate:>8.1f} {rate / 78.6:>6.3f} "
+              f"{c.get('SQ_INSTS_VALU_MFMA_F64', 0) / 1e6:>10.2f} {busy:>9.3f}  {name[:100]}")
+
+
 def main():
-    if len(sys.argv) < 2 or sys.argv[1] not in ("trace", "pmc", "sq"):
+    if len(sys.argv) < 2 or sys.argv[1] not in ("trace", "pmc", "sq", "mfma"):
         sys.exit(__doc__)
     if sys.argv[1] == "trace":
         trace(sys.argv[2], sys.argv[3] if len(sys.argv) > 3 else "")
     elif sys.argv[1] == "sq":
         sq(sys.argv[2:])
+    elif sys.argv[1] == "mfma":
+        mfma(sys.argv[2:])
     else:
         pmc(sys.argv[2:])
 

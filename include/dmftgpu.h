@@ -82,6 +82,9 @@ const char* dmftgpu_version(void);
                                            per lane (always used for n_c > 6) */
 #define DMFTGPU_OPT_UPFOLD_CHUNK 5      /* k points per chunk of dmftgpu_upfold_density (0 = as many as 512 MiB of
                                            scratch holds, the default) */
+#define DMFTGPU_OPT_DENSITY_KERNEL 6    /* dmftgpu_density_matrix: 0 = block form with the frequency sum on FP64
+                                           MFMA where it applies (n <= 16, n_corr <= 6; the default), 1 = one full
+                                           n x n inverse per (k, iw) always */
 int dmftgpu_set_option(dmftgpu_handle h, int option, int value);
 
 /* Instrumentation (no reference counterpart): with timing enabled, HIP events
@@ -157,8 +160,11 @@ int dmftgpu_greenftn_real(dmftgpu_handle h, int method, int n_e, double e0, doub
 int dmftgpu_greenftn_real_k(dmftgpu_handle h, int method, int n_e, double e0, double de, double eta,
                             const double* sigma_r, double mu, double* gcc_k, double* trace_k);
 
-/* Per-k density matrix densityMatDFT (GreenFtn.cpp:291-319): dens [n_k][n][n].
- * Full n x n inverse, DMFTGPU_PLAIN_F64 only. */
+/* Per-k density matrix densityMatDFT (GreenFtn.cpp:291-319): dens [n_k][n][n],
+ * FP64.  For n <= 16 and n_corr <= 6 the frequency sum is taken in block form
+ * (Schur complement of the correlated block, then the rank-n_corr frequency
+ * sum as one complex product per k on the matrix cores); otherwise one full
+ * n x n inverse per (k, iw).  DMFTGPU_OPT_DENSITY_KERNEL = 1 forces the latter. */
 int dmftgpu_density_matrix(dmftgpu_handle h, const double* sigma, double mu, double* dens);
 
 /* Kohn-Sham bands of the bound H(k) for the core-band term of GiwToNele

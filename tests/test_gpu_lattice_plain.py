@@ -147,16 +147,38 @@ def test_errors_are_reported(gpu):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["t2g", "dp14", "n_2", "n_9", "n_20", "n_27", "large32"])
-def test_density_matrix(gpu, name):
-    """densityMatDFT (GreenFtn.cpp:291-319), full n x n per k."""
+@pytest.mark.parametrize("kernel", [0, 1])
+@pytest.mark.parametrize("name", ["t2g", "dp14", "n_1", "n_2", "n_7", "n_9", "n_13", "n_16", "n_20", "n_27",
+                                  "large32"])
+def test_density_matrix(gpu, name, kernel):
+    """densityMatDFT (GreenFtn.cpp:291-319), full n x n per k: kernel 0 is the
+    block form (Schur complement + rank-n_corr frequency sum on FP64 MFMA) where
+    it applies (n <= 16, n_corr <= 6), kernel 1 one full inverse per (k, iw)."""
     Hk, sig, beta, ncl, mu = small_case(name)
     mom = O.asymto_moments(sig, beta)
+    print(f"case {name} kernel {kernel}: n={Hk.shape[1]} nc={sig.shape[1]} nk={Hk.shape[0]}", flush=True)
     ref = O.greenftn_w(Hk, sig, mom, mu, beta, n_cluster=ncl, density=True)
     lat = gpu.LatticeGF(Hk, sig.shape[1], sig.shape[0], beta, mom, n_cluster=ncl)
+    lat.set_option(gpu.capi.OPT_DENSITY_KERNEL, kernel)
     dm = lat.density_matrix(sig, mu)
     assert rel(dm, ref["density"]) <= TOL
     # its trace average is N(mu)
     N, _ = lat.density(sig, mu)
     assert abs(np.mean(np.trace(dm, axis1=1, axis2=2)).real - N) < 1e-11
+    lat.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["dp14", "n_9", "n_16"])
+def test_density_matrix_kernels_agree(gpu, name):
+    """The block form and the full inverse agree on a larger frequency sum."""
+    Hk, sig, beta, ncl, mu = small_case(name)
+    mom = O.asymto_moments(sig, beta)
+    lat = gpu.LatticeGF(Hk, sig.shape[1], sig.shape[0], beta, mom, n_cluster=ncl)
+    d0 = lat.density_matrix(sig, mu)
+    lat.set_option(gpu.capi.OPT_DENSITY_KERNEL, 1)
+    d1 = lat.density_matrix(sig, mu)
+    assert rel(d0, d1) <= TOL
+    # Hermitian per k, as (S + S^H)/beta + U diag(w) U^H is
+    assert np.max(np.abs(d0 - np.conj(np.transpose(d0, (0, 2, 1))))) <= 1e-13 * np.max(np.abs(d0))
     lat.close()
