@@ -323,13 +323,20 @@ def main():
     out.update(extra)
     print(json.dumps(out), flush=True)
     lat.close()
-    if args.calib_copy > 0:
-        src = torch.ones(args.calib_copy * (1 << 20) // 8, dtype=torch.float64, device=dev)
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        print(f"calibration copy: {args.calib_copy} MiB read + {args.calib_copy} MiB written per launch", flush=True)
+    calibration_copy(args, torch)
+
+
+def calibration_copy(args, torch):
+    """A plain device copy of --calib-copy MiB, three times, after the timed work:
+    the known-size kernel that scales HBM byte counters (FETCH_SIZE, WRITE_SIZE)."""
+    if args.calib_copy <= 0:
+        return
+    src = torch.ones(args.calib_copy * (1 << 20) // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(f"calibration copy: {args.calib_copy} MiB read + {args.calib_copy} MiB written per launch", flush=True)
 
 
 def plain_executed(rec, workload):
@@ -731,6 +738,7 @@ def bench_path(args, torch, capi, G, L):
     line.update(extra)
     print(json.dumps(line), flush=True)
     lat.close()
+    calibration_copy(args, torch)
 
 
 if __name__ == "__main__":

@@ -4,8 +4,8 @@
       per-kernel table (calls, total, average) and kernel resources from a
       --kernel-trace --stats SQLite result.
 
-  python -m dmftpack_amd.rocprof_summary pmc --fetch <dir> --write <dir> --copy-mib M --method NAME
-          [--json profiles/pmc_traffic.json] [--command "..."]
+  python -m dmftpack_amd.rocprof_summary pmc --fetch <dir> --write <dir> (--copy-mib M | --factors F W)
+          --method NAME [--json profiles/pmc_traffic.json] [--command "..."]
       HBM bytes per launch from two --pmc runs (FETCH_SIZE, then WRITE_SIZE;
       SQLite or csv output), calibrated by a plain copy of known size:
       the copy kernel (__amd_rocclr_copyBuffer, its largest dispatch) moves M MiB
@@ -104,7 +104,9 @@ def pmc(argv):
     ap = argparse.ArgumentParser(prog="rocprof_summary pmc")
     ap.add_argument("--fetch", required=True)
     ap.add_argument("--write", required=True)
-    ap.add_argument("--copy-mib", type=float, required=True)
+    ap.add_argument("--copy-mib", type=float, default=0.0)
+    ap.add_argument("--factors", type=float, nargs=2, default=None, metavar=("FETCH", "WRITE"),
+                    help="calibration factors of an earlier run on the same GPU type, for runs without a copy")
     ap.add_argument("--method", required=True)
     ap.add_argument("--json", default=None)
     ap.add_argument("--command", default="")
@@ -117,15 +119,20 @@ def pmc(argv):
         vals = [max(v) for k, v in d.items() if "copyBuffer" in k]
         return max(vals) if vals else None
 
-    cf, cw = copy_kib(fetch), copy_kib(write)
-    if not cf or not cw:
-        sys.exit("calibration copy not found in the runs")
-    ff, fw = copy_bytes / (cf * 1024.0), copy_bytes / (cw * 1024.0)
     print("# HBM traffic per launch from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (separate runs)")
     if a.command:
         print(f"# command: {a.command}")
-    print(f"# calibration: a {a.copy_mib:.0f} MiB copy reads FETCH_SIZE {cf:.0f} KiB (factor {ff:.4f}) and "
-          f"WRITE_SIZE {cw:.0f} KiB (factor {fw:.4f})")
+    if a.factors:
+        ff, fw = a.factors
+        print(f"# calibration: factors {ff:.4f} (FETCH_SIZE) and {fw:.4f} (WRITE_SIZE) of a calibration copy in an "
+              f"earlier run on the same GPU type")
+    else:
+        cf, cw = copy_kib(fetch), copy_kib(write)
+        if not cf or not cw or a.copy_mib <= 0:
+            sys.exit("calibration copy not found in the runs (or --copy-mib missing); give --factors")
+        ff, fw = copy_bytes / (cf * 1024.0), copy_bytes / (cw * 1024.0)
+        print(f"# calibration: a {a.copy_mib:.0f} MiB copy reads FETCH_SIZE {cf:.0f} KiB (factor {ff:.4f}) and "
+              f"WRITE_SIZE {cw:.0f} KiB (factor {fw:.4f})")
     print(f"{'dispatches':>10} {'fetch_MB':>10} {'write_MB':>10} {'total_MB':>10}  kernel")
     result = {}
     for name in sorted(set(fetch) | set(write)):
