@@ -7,7 +7,9 @@
 // but
 //   * the per-(k, iw) kernels of all systems with the same n_c run as ONE
 //     persistent launch over a static work list of (system, frequency tile,
-//     k chunk) items (k_lattice_schur_multi), one launch per (n_c, mode);
+//     k chunk) items, one launch per (n_c, mode): the one-matrix-per-lane
+//     kernels for n_c <= 6 (k_lattice_schur_multi), the column-per-lane
+//     kernels for 6 < n_c <= 16 (k_lattice_schur_cols_multi);
 //   * everything else (static eigen-decompositions, Sigma tables, chunk
 //     reductions, finalisation, loss cotangents) is device-side, so the whole
 //     step is captured once into a hipGraph and replayed with hipGraphLaunch;
@@ -90,6 +92,7 @@ inline int slot(int mode) { return mode == MODE_VJP ? 1 : 0; }
 
 struct Group {
     int nc;
+    bool cols = false;  // n_c > 6: column-per-lane kernels, one work list over every frequency
     std::vector<int> systems;
     int wg[2] = {64, 64};
     double target[2] = {0, 0};
@@ -205,7 +208,7 @@ int build_group(dmftgpu_batch_s* b, Group& g, int mode, int wg, double target)
             s.kpc_v = kpc;
         }
         LatArgs a = make_args(b, s, mode, chunks, kpc);
-        const bool split = s.nw_tot > s.nf;
+        const bool split = s.nw_tot > s.nf && !g.cols;
         if (split) {  // [0, nf) forward / adjoint, [nf, nw_tot) see Group::mode_hi
             LatArgs hi = a;
             a.nw_end = s.nf;
@@ -216,7 +219,8 @@ int build_group(dmftgpu_batch_s* b, Group& g, int mode, int wg, double target)
             args_hi.push_back(a);  // unused placeholder keeps the system index
         }
         args.push_back(a);
-        const int nwt = ((split ? s.nf : s.nw_tot) + wg - 1) / wg;
+        const int per_tile = g.cols ? dg::schur_slots(s.nc, mode, b->method == DMFTGPU_SCHUR_F32R, true) : wg;
+        const int nwt = ((split ? s.nf : s.nw_tot) + per_tile - 1) / per_tile;
         for (int c = 0; c < chunks; ++c)
             for (int t = 0; t < nwt; ++t) items.push_back(int4{(int)q, t, c, 0});
         if (split && !(mode == MODE_VJP && b->vjp)) {  // fused adjoint: the forward did these
@@ -277,6 +281,10 @@ int launch_group(dmftgpu_batch_s* b, Group& g, int mode)
     m.n_items = g.n_items[slot(mode)];
     m.grid = std::min(g.n_items[slot(mode)], 256 * 16);
     m.max_nd = g.max_nd;
+    if (g.cols) {  // every frequency in the one list (no upper-half work list)
+        HIPCHK(dg::launch_lattice_schur_cols_multi(proto, mode, fp32, h->stream, m));
+        return DMFTGPU_OK;
+    }
     HIPCHK(dg::launch_lattice_schur_multi(proto, mode, fp32, h->stream, g.wg[slot(mode)], m));
     if (g.n_items_hi[slot(mode)] > 0) {
         SchurMulti mh = m;
@@ -300,7 +308,7 @@ int issue_prepare(dmftgpu_batch_s* b)
             HIPCHK(dg::launch_herm_eig((const cd*)s.d.Hk, s.n, s.nc, s.nd, nullptr, 0, s.ed, s.Vd, s.nk, h->stream));
             HIPCHK(dg::launch_schur_prep((const cd*)s.d.Hk, s.Vd, s.n, s.nc, s.nk, s.W, h->stream));
         }
-        HIPCHK(dg::launch_schur_kdata((const cd*)s.d.Hk, s.W, s.ed, s.n, s.nc, s.nk, s.kdata, h->stream));
+        if (s.kdata) HIPCHK(dg::launch_schur_kdata((const cd*)s.d.Hk, s.W, s.ed, s.n, s.nc, s.nk, s.kdata, h->stream));
     }
     return DMFTGPU_OK;
 }
@@ -397,6 +405,7 @@ int tune_group(dmftgpu_batch_s* b, Group& g, int mode)
     double bt = targets[2];
     for (int wg : wgs)
         for (double t : targets) {
+            if (g.cols && wg != 64) continue;  // one width: a wavefront per workgroup
             int rc = build_group(b, g, mode, wg, t);
             if (rc) return rc;
             rc = launch_group(b, g, mode);  // warm-up
@@ -446,6 +455,7 @@ int tune_group_pair(dmftgpu_batch_s* b, Group& g)
             double bm = 1e300;
             int bw = 64;
             for (int wg : wgs) {
+                if (g.cols && wg != 64) continue;  // one width: a wavefront per workgroup
                 int rc = build_group(b, g, mode, wg, t);
                 if (rc) return rc;
                 rc = launch_group(b, g, mode);  // warm-up
@@ -528,11 +538,11 @@ static int batch_create_impl(dmftgpu_batch_s* b, int count, const dmftgpu_system
         Sys& s = b->sys[i];
         s.d = systems[i];
         const dmftgpu_lattice& L = s.d.lat;
-        if (L.n < 1 || L.n > 32 || L.n_corr < 1 || !dg::schur_supported(L.n, L.n_corr) ||
-            !dg::schur_multi_supported(L.n_corr) || L.n_k < 1 || L.n_freq < 1 ||
-            L.n_cluster < 1 || L.n_corr % L.n_cluster || !(L.beta > 0) || !s.d.Hk || !s.d.sigma || !s.d.moments ||
-            !s.d.gloc || !s.d.num_matrix || !s.d.n_total) {
-            return fail(h, DMFTGPU_ERR_ARG, "system %d: invalid descriptor or unsupported size (n_corr <= 6)", i);
+        if (L.n < 1 || L.n > 32 || L.n_corr < 1 || !dg::schur_supported(L.n, L.n_corr) || L.n_k < 1 ||
+            L.n_freq < 1 || L.n_cluster < 1 || L.n_corr % L.n_cluster || !(L.beta > 0) || !s.d.Hk || !s.d.sigma ||
+            !s.d.moments || !s.d.gloc || !s.d.num_matrix || !s.d.n_total) {
+            return fail(h, DMFTGPU_ERR_ARG,
+                        "system %d: invalid descriptor or unsupported size (n_corr <= 16, n <= 32)", i);
         }
         const bool v = s.d.gloc_target && s.d.sigma_bar && s.d.mu_bar;
         if (i == 0) b->vjp = v;
@@ -554,6 +564,7 @@ static int batch_create_impl(dmftgpu_batch_s* b, int count, const dmftgpu_system
             gidx[s.nc] = (int)b->groups.size();
             Group g;
             g.nc = s.nc;
+            g.cols = !dg::schur_multi_supported(s.nc);
             b->groups.push_back(g);
             it = gidx.find(s.nc);
         }
@@ -570,7 +581,7 @@ static int batch_create_impl(dmftgpu_batch_s* b, int count, const dmftgpu_system
         size_t t = 0;
         t += rnd(16 * 4 * s.m2) + rnd(8 * (size_t)s.nk * s.n) + rnd(16 * (size_t)s.nk * s.n * s.n);
         t += rnd(8 * (size_t)s.nk * ndd) + rnd(16 * (size_t)s.nk * ndd * ndd) + rnd(16 * (size_t)s.nk * s.nc * ndd);
-        t += rnd(16 * (size_t)s.nk * dg::schur_kdata_size(s.n, s.nc));
+        if (dg::schur_multi_supported(s.nc)) t += rnd(16 * (size_t)s.nk * dg::schur_kdata_size(s.n, s.nc));
         t += rnd(16 * (size_t)s.nw_tot * s.m2) + rnd(16 * (size_t)s.nf * s.m2) + rnd(8);
         t += 2 * rnd(8 * (size_t)s.nk * s.n) + rnd(8 * 2 * (size_t)s.st_chunks) + rnd(16 * (size_t)s.st_chunks * s.m2);
         t += rnd(16 * s.m2) + 2 * rnd(16 * (s.m2 + 3)) + rnd(16);
@@ -599,7 +610,8 @@ static int batch_create_impl(dmftgpu_batch_s* b, int count, const dmftgpu_system
         s.ed = (double*)take(8 * (size_t)s.nk * ndd);
         s.Vd = (cd*)take(16 * (size_t)s.nk * ndd * ndd);
         s.W = (cd*)take(16 * (size_t)s.nk * s.nc * ndd);
-        s.kdata = (cd*)take(16 * (size_t)s.nk * dg::schur_kdata_size(s.n, s.nc));
+        s.kdata = dg::schur_multi_supported(s.nc) ? (cd*)take(16 * (size_t)s.nk * dg::schur_kdata_size(s.n, s.nc))
+                                                  : nullptr;
         s.sig_all = (cd*)take(16 * (size_t)s.nw_tot * s.m2);
         s.gbar = (cd*)take(16 * (size_t)s.nf * s.m2);
         s.nbar = (double*)take(8);

@@ -8,4 +8,9 @@ hipError_t launch_cols_12(const LatArgs& a, int mode, bool fp32, const cd* sig_a
     return launch_cols_bucket<12>(a, mode, fp32, sig_all, s);
 }
 
+hipError_t launch_cols_multi_12(const LatArgs& proto, int mode, bool fp32, hipStream_t s, const SchurMulti& m)
+{
+    return launch_cols_bucket<12>(proto, mode, fp32, nullptr, s, &m);
+}
+
 }  // namespace dg

@@ -113,6 +113,9 @@ struct SchurMulti {
 };
 hipError_t launch_lattice_schur_multi(const LatArgs& proto, int mode, bool fp32, hipStream_t s, int wg,
                                       const SchurMulti& m);
+// the same for 6 < n_c <= 16: column-per-lane kernels (MODE_FWD, MODE_VJP over every frequency)
+hipError_t launch_lattice_schur_cols_multi(const LatArgs& proto, int mode, bool fp32, hipStream_t s,
+                                           const SchurMulti& m);
 // frequencies per workgroup of the Schur kernel for n_c (grid planning)
 int schur_slots(int nc, int mode, bool fp32, bool cols);
 bool schur_supported(int n, int nc);
@@ -136,6 +139,11 @@ hipError_t launch_cols_4(const LatArgs& a, int mode, bool fp32, const cd* sig_al
 hipError_t launch_cols_5(const LatArgs& a, int mode, bool fp32, const cd* sig_all, hipStream_t s);
 hipError_t launch_cols_6(const LatArgs& a, int mode, bool fp32, const cd* sig_all, hipStream_t s);
 hipError_t launch_cols_8(const LatArgs& a, int mode, bool fp32, const cd* sig_all, hipStream_t s);
+// persistent multi-lattice launches of the column-per-lane buckets for n_c > 6
+hipError_t launch_cols_multi_8(const LatArgs& proto, int mode, bool fp32, hipStream_t s, const SchurMulti& m);
+hipError_t launch_cols_multi_10(const LatArgs& proto, int mode, bool fp32, hipStream_t s, const SchurMulti& m);
+hipError_t launch_cols_multi_12(const LatArgs& proto, int mode, bool fp32, hipStream_t s, const SchurMulti& m);
+hipError_t launch_cols_multi_16(const LatArgs& proto, int mode, bool fp32, hipStream_t s, const SchurMulti& m);
 hipError_t launch_cols_10(const LatArgs& a, int mode, bool fp32, const cd* sig_all, hipStream_t s);
 hipError_t launch_cols_12(const LatArgs& a, int mode, bool fp32, const cd* sig_all, hipStream_t s);
 hipError_t launch_cols_16(const LatArgs& a, int mode, bool fp32, const cd* sig_all, hipStream_t s);

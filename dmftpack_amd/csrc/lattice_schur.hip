@@ -105,6 +105,18 @@ hipError_t launch_lattice_schur_multi(const LatArgs& proto, int mode, bool fp32,
     return dispatch_nc(proto, mode, fp32, nullptr, s, wg, &m);
 }
 
+hipError_t launch_lattice_schur_cols_multi(const LatArgs& proto, int mode, bool fp32, hipStream_t s,
+                                           const SchurMulti& m)
+{
+    switch (cols_bucket(proto.nc)) {
+        case 8: return launch_cols_multi_8(proto, mode, fp32, s, m);
+        case 10: return launch_cols_multi_10(proto, mode, fp32, s, m);
+        case 12: return launch_cols_multi_12(proto, mode, fp32, s, m);
+        case 16: return launch_cols_multi_16(proto, mode, fp32, s, m);
+        default: return hipErrorInvalidValue;
+    }
+}
+
 // W(k) = H_cd(k) V_d(k): W_ad = sum_e H(a, nc+e) V_d(e, d)
 __global__ void k_schur_prep(const cd* __restrict__ Hk, const cd* __restrict__ Vd, int n, int nc, int nk,
                              cd* __restrict__ W)

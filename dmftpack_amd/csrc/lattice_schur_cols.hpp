@@ -53,15 +53,16 @@ inline size_t cols_lds_bytes(int N, int mode, int prec, int, int)
     return sizeof(cd) * ((size_t)nwave * (spw + 1) * cols_slot_size(N, mode, prec));
 }
 
+// One (frequency tile, k chunk) work item of one lattice: the body of the
+// per-lattice kernel and of the persistent multi-lattice kernel below.
 template <int N, int MODE, int PREC, bool TRACE>
-__global__ __launch_bounds__(cols_wg(MODE, PREC)) void k_lattice_schur_cols(LatArgs a, const cd* __restrict__ sig_all)
+__device__ __forceinline__ void cols_item(const LatArgs& a, const cd* __restrict__ sig_all, int wtile, int chunk,
+                                          cd* lds)
 {
-    constexpr int WG = cols_wg(MODE, PREC);
     constexpr int SPW = 64 / N;
     constexpr int SLOTS = cols_slots(N, MODE, PREC);
     constexpr int SS = cols_slot_size(N, MODE, PREC);
     constexpr bool MATS = (PREC > 0 || MODE != MODE_FWD);
-    extern __shared__ cd lds[];
     const int n = a.n, nc = a.nc, nd = a.n - nc, m2 = nc * nc;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int sl = lane / N;  // == SPW for the spare lanes of a wavefront
@@ -75,10 +76,10 @@ __global__ __launch_bounds__(cols_wg(MODE, PREC)) void k_lattice_schur_cols(LatA
     cd* MA = gsh + 32;
     cd* MB = MA + N * N;
     cd* GB = MB + N * N;
-    const int nw = blockIdx.x * SLOTS + wave * SPW + (active ? sl : 0);
+    const int nw = wtile * SLOTS + wave * SPW + (active ? sl : 0);
     const bool wvalid = active && nw < a.nw_tot;
     const int nwc = nw < a.nw_tot ? nw : a.nw_tot - 1;
-    const int k0 = blockIdx.y * a.k_per_chunk;
+    const int k0 = chunk * a.k_per_chunk;
     const int k1 = min(a.nk, k0 + a.k_per_chunk);
     double mu, w;
     freq_point(a, nwc, mu, w);
@@ -311,7 +312,7 @@ __global__ __launch_bounds__(cols_wg(MODE, PREC)) void k_lattice_schur_cols(LatA
     my[2 * N + c] = muA;
     wave_sync();
     if (!wvalid) return;
-    const size_t ch = blockIdx.y;
+    const size_t ch = chunk;
     if (c == 0) {
         cd s1 = cd{0.0, 0.0}, s2 = cd{0.0, 0.0}, s3 = cd{0.0, 0.0};
         for (int l = 0; l < N; ++l) {
@@ -339,11 +340,44 @@ __global__ __launch_bounds__(cols_wg(MODE, PREC)) void k_lattice_schur_cols(LatA
 }
 
 template <int N, int MODE, int PREC, bool TRACE>
-hipError_t launch_cols_k(const LatArgs& a, const cd* sig_all, hipStream_t s)
+__global__ __launch_bounds__(cols_wg(MODE, PREC)) void k_lattice_schur_cols(LatArgs a, const cd* __restrict__ sig_all)
+{
+    extern __shared__ cd lds[];
+    cols_item<N, MODE, PREC, TRACE>(a, sig_all, blockIdx.x, blockIdx.y, lds);
+}
+
+// Persistent form over a work list spanning several lattices with n_c in this
+// bucket: workgroup b takes items b, b + gridDim.x, ... ({system, frequency
+// tile, k chunk}; static striding, no atomics, so every replay does the same
+// work in the same order).  Each system's tables come from its LatArgs.
+template <int N, int MODE, int PREC, bool TRACE>
+__global__ __launch_bounds__(cols_wg(MODE, PREC)) void k_lattice_schur_cols_multi(const LatArgs* __restrict__ args,
+                                                                                 const int4* __restrict__ items,
+                                                                                 int n_items)
+{
+    extern __shared__ cd lds[];
+    for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+        const int4 item = items[it];  // {system, wtile, chunk, -}
+        const LatArgs a = args[item.x];
+        cols_item<N, MODE, PREC, TRACE>(a, a.pSigAll, item.y, item.z, lds);
+        __syncthreads();
+    }
+}
+
+template <int N, int MODE, int PREC, bool TRACE>
+hipError_t launch_cols_k(const LatArgs& a, const cd* sig_all, hipStream_t s, const SchurMulti* m)
 {
     const int nd = a.n - a.nc;
     const size_t shm = cols_lds_bytes(N, MODE, PREC, a.nc, nd);
     constexpr int SLOTS = cols_slots(N, MODE, PREC);
+    if (m) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_lattice_schur_cols_multi<N, MODE, PREC, TRACE>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((k_lattice_schur_cols_multi<N, MODE, PREC, TRACE>), dim3(m->grid),
+                           dim3(cols_wg(MODE, PREC)), shm, s, m->args, m->items, m->n_items);
+        return hipGetLastError();
+    }
     hipError_t e = hipFuncSetAttribute((const void*)k_lattice_schur_cols<N, MODE, PREC, TRACE>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
     if (e != hipSuccess) return e;
@@ -354,22 +388,24 @@ hipError_t launch_cols_k(const LatArgs& a, const cd* sig_all, hipStream_t s)
 }
 
 template <int N, int PREC>
-hipError_t launch_cols_p(const LatArgs& a, int mode, const cd* sig_all, hipStream_t s)
+hipError_t launch_cols_p(const LatArgs& a, int mode, const cd* sig_all, hipStream_t s, const SchurMulti* m)
 {
     if (mode == MODE_FWD)
-        return a.want_trace ? launch_cols_k<N, MODE_FWD, PREC, true>(a, sig_all, s)
-                            : launch_cols_k<N, MODE_FWD, PREC, false>(a, sig_all, s);
-    if (mode == MODE_DENS) return launch_cols_k<N, MODE_DENS, PREC, false>(a, sig_all, s);
-    return launch_cols_k<N, MODE_VJP, PREC, false>(a, sig_all, s);
+        return a.want_trace ? launch_cols_k<N, MODE_FWD, PREC, true>(a, sig_all, s, m)
+                            : launch_cols_k<N, MODE_FWD, PREC, false>(a, sig_all, s, m);
+    if (mode == MODE_DENS) return launch_cols_k<N, MODE_DENS, PREC, false>(a, sig_all, s, m);
+    return launch_cols_k<N, MODE_VJP, PREC, false>(a, sig_all, s, m);
 }
 
 // entry of one bucket (one translation unit each); PREC 0 = FP64, 1 = FP32 +
-// a.refine (1 or 2) FP64-residual steps
+// a.refine (1 or 2) FP64-residual steps; with m, the persistent multi-lattice
+// launch over m's work list (a is then only a prototype: n_c, refine, want_trace)
 template <int N>
-hipError_t launch_cols_bucket(const LatArgs& a, int mode, bool fp32, const cd* sig_all, hipStream_t s)
+hipError_t launch_cols_bucket(const LatArgs& a, int mode, bool fp32, const cd* sig_all, hipStream_t s,
+                              const SchurMulti* m = nullptr)
 {
-    if (!fp32) return launch_cols_p<N, 0>(a, mode, sig_all, s);
-    return launch_cols_p<N, 1>(a, mode, sig_all, s);
+    if (!fp32) return launch_cols_p<N, 0>(a, mode, sig_all, s, m);
+    return launch_cols_p<N, 1>(a, mode, sig_all, s, m);
 }
 
 }  // namespace dg

@@ -232,8 +232,10 @@ int dmftgpu_gloc_loss_grad(dmftgpu_handle h, int method, const double* sigma, do
 
 /* ---- batched lattices ----------------------------------------------------------
  * No reference counterpart: DMFTpack handles one lattice per call.  Many
- * independent lattices (uneven n, n_corr <= 6, n_k) run as one replayable
- * step: per system GreenFtn_w (G_loc, NumMatrix, N) and, when a target is
+ * independent lattices (uneven n <= 32, n_corr <= 16, n_k) run as one
+ * replayable step, one persistent launch per n_corr group and mode (one
+ * (k, iw) matrix per lane for n_corr <= 6, one column per lane above): per
+ * system GreenFtn_w (G_loc, NumMatrix, N) and, when a target is
  * given, the VJP of L = sum_w ||G_loc - gloc_target||^2 + (N - n_target)^2
  * (G_loc cluster-block-diagonal as returned, as in dmftgpu_gloc_loss_grad).
  * All pointers in dmftgpu_system are DEVICE pointers; n_total and mu_bar
