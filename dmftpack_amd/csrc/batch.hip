@@ -581,7 +581,8 @@ static int batch_create_impl(dmftgpu_batch_s* b, int count, const dmftgpu_system
         size_t t = 0;
         t += rnd(16 * 4 * s.m2) + rnd(8 * (size_t)s.nk * s.n) + rnd(16 * (size_t)s.nk * s.n * s.n);
         t += rnd(8 * (size_t)s.nk * ndd) + rnd(16 * (size_t)s.nk * ndd * ndd) + rnd(16 * (size_t)s.nk * s.nc * ndd);
-        if (dg::schur_multi_supported(s.nc)) t += rnd(16 * (size_t)s.nk * dg::schur_kdata_size(s.n, s.nc));
+        if (dg::schur_multi_supported(s.nc))
+            t += rnd(16 * (size_t)s.nk * dg::schur_kdata_size(s.n, s.nc) + dg::kKdataPadBytes);
         t += rnd(16 * (size_t)s.nw_tot * s.m2) + rnd(16 * (size_t)s.nf * s.m2) + rnd(8);
         t += 2 * rnd(8 * (size_t)s.nk * s.n) + rnd(8 * 2 * (size_t)s.st_chunks) + rnd(16 * (size_t)s.st_chunks * s.m2);
         t += rnd(16 * s.m2) + 2 * rnd(16 * (s.m2 + 3)) + rnd(16);
@@ -610,8 +611,9 @@ static int batch_create_impl(dmftgpu_batch_s* b, int count, const dmftgpu_system
         s.ed = (double*)take(8 * (size_t)s.nk * ndd);
         s.Vd = (cd*)take(16 * (size_t)s.nk * ndd * ndd);
         s.W = (cd*)take(16 * (size_t)s.nk * s.nc * ndd);
-        s.kdata = dg::schur_multi_supported(s.nc) ? (cd*)take(16 * (size_t)s.nk * dg::schur_kdata_size(s.n, s.nc))
-                                                  : nullptr;
+        s.kdata = dg::schur_multi_supported(s.nc)
+                      ? (cd*)take(16 * (size_t)s.nk * dg::schur_kdata_size(s.n, s.nc) + dg::kKdataPadBytes)
+                      : nullptr;
         s.sig_all = (cd*)take(16 * (size_t)s.nw_tot * s.m2);
         s.gbar = (cd*)take(16 * (size_t)s.nf * s.m2);
         s.nbar = (double*)take(8);

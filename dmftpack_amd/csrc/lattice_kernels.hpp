@@ -25,6 +25,9 @@ constexpr int MODE_FWD_REAL = 5;
 // loads (s_load) whose results feed the VALU as SGPR operands.  Used for the
 // per-k data of the Schur kernels, which no kernel writes while they run.
 typedef __attribute__((address_space(4))) cd kconst_cd;
+// bytes of padding after the per-k blocks of the one-matrix-per-lane kernels
+// (their scalar-cache fill reads 3 KB from the start of a block)
+constexpr int kKdataPadBytes = 4096;
 typedef __attribute__((address_space(4))) double kconst_double;
 
 __device__ __forceinline__ cd ldc(const cd* p) { return *p; }

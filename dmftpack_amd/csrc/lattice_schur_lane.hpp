@@ -80,6 +80,70 @@ __device__ __forceinline__ void gj_inverse_f32(cf (&S)[NC][NC])
     }
 }
 
+// Scalar-cache fill of the next k's block before its d loops need it: one
+// dword from each of the first 48 64-byte lines (3 KB; the per-k blocks are
+// followed by kKdataPad bytes of padding, so the last block's fill stays in
+// the allocation), all into one throw-away SGPR, waited for inside the asm
+// block so that no load is in flight when it ends.  Without it every d
+// block's scalar loads wait on L2 in turn; with it they hit the scalar cache.
+__device__ __forceinline__ void kcache_fill(const void* base)
+{
+    unsigned t;
+    asm volatile(
+        "s_load_dword %0, %1, 0x0\n\t"
+        "s_load_dword %0, %1, 0x40\n\t"
+        "s_load_dword %0, %1, 0x80\n\t"
+        "s_load_dword %0, %1, 0xc0\n\t"
+        "s_load_dword %0, %1, 0x100\n\t"
+        "s_load_dword %0, %1, 0x140\n\t"
+        "s_load_dword %0, %1, 0x180\n\t"
+        "s_load_dword %0, %1, 0x1c0\n\t"
+        "s_load_dword %0, %1, 0x200\n\t"
+        "s_load_dword %0, %1, 0x240\n\t"
+        "s_load_dword %0, %1, 0x280\n\t"
+        "s_load_dword %0, %1, 0x2c0\n\t"
+        "s_load_dword %0, %1, 0x300\n\t"
+        "s_load_dword %0, %1, 0x340\n\t"
+        "s_load_dword %0, %1, 0x380\n\t"
+        "s_load_dword %0, %1, 0x3c0\n\t"
+        "s_load_dword %0, %1, 0x400\n\t"
+        "s_load_dword %0, %1, 0x440\n\t"
+        "s_load_dword %0, %1, 0x480\n\t"
+        "s_load_dword %0, %1, 0x4c0\n\t"
+        "s_load_dword %0, %1, 0x500\n\t"
+        "s_load_dword %0, %1, 0x540\n\t"
+        "s_load_dword %0, %1, 0x580\n\t"
+        "s_load_dword %0, %1, 0x5c0\n\t"
+        "s_load_dword %0, %1, 0x600\n\t"
+        "s_load_dword %0, %1, 0x640\n\t"
+        "s_load_dword %0, %1, 0x680\n\t"
+        "s_load_dword %0, %1, 0x6c0\n\t"
+        "s_load_dword %0, %1, 0x700\n\t"
+        "s_load_dword %0, %1, 0x740\n\t"
+        "s_load_dword %0, %1, 0x780\n\t"
+        "s_load_dword %0, %1, 0x7c0\n\t"
+        "s_load_dword %0, %1, 0x800\n\t"
+        "s_load_dword %0, %1, 0x840\n\t"
+        "s_load_dword %0, %1, 0x880\n\t"
+        "s_load_dword %0, %1, 0x8c0\n\t"
+        "s_load_dword %0, %1, 0x900\n\t"
+        "s_load_dword %0, %1, 0x940\n\t"
+        "s_load_dword %0, %1, 0x980\n\t"
+        "s_load_dword %0, %1, 0x9c0\n\t"
+        "s_load_dword %0, %1, 0xa00\n\t"
+        "s_load_dword %0, %1, 0xa40\n\t"
+        "s_load_dword %0, %1, 0xa80\n\t"
+        "s_load_dword %0, %1, 0xac0\n\t"
+        "s_load_dword %0, %1, 0xb00\n\t"
+        "s_load_dword %0, %1, 0xb40\n\t"
+        "s_load_dword %0, %1, 0xb80\n\t"
+        "s_load_dword %0, %1, 0xbc0\n\t"
+        "s_waitcnt lgkmcnt(0)"
+        : "=&s"(t)
+        : "s"(base));
+    asm volatile("" ::"s"(t));
+}
+
 // per-k block (k_schur_kdata): [Hcc: NC*NC cd][P: nd*NP cd][e_d: nd doubles]
 template <int NC>
 __host__ __device__ __forceinline__ int kblock_size(int nd)
@@ -157,6 +221,7 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
         // compiler treats as invariant: those loads stay scalar despite the
         // clobber.
         __asm__ __volatile__("" ::: "memory");
+        kcache_fill(a.kdata + (size_t)k * bs);
         const BT* B = (const BT*)(a.kdata + (size_t)k * bs);
         const BT* Hcc = B;
         const BT* P = B + M2;
