@@ -222,6 +222,21 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
         // clobber.
         __asm__ __volatile__("" ::: "memory");
         kcache_fill(a.kdata + (size_t)k * bs);
+        // Sigma's loads are issued here, all together, so that they complete under
+        // the d loop below instead of waiting in turn where S is combined; not in
+        // the FP64 forward kernels, whose G_cc sums already fill the registers
+        constexpr bool SIG_EARLY = F32R || !FWDLIKE;
+        cd sg[SIG_EARLY ? NC : 1][SIG_EARLY ? NC : 1];
+        if constexpr (SIG_EARLY) {
+#pragma unroll
+            for (int ib = 0; ib < NC; ++ib)
+#pragma unroll
+                for (int ia = 0; ia < NC; ++ia) sg[ia][ib] = sig[ia + ib * NC];
+        }
+        auto sig_at = [&](int ia, int ib) -> cd {
+            if constexpr (SIG_EARLY) return sg[ia][ib];
+            else return sig[ia + ib * NC];
+        };
         const BT* B = (const BT*)(a.kdata + (size_t)k * bs);
         const BT* Hcc = B;
         const BT* P = B + M2;
@@ -260,13 +275,13 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
         for (int ib = 0; ib < NC; ++ib)
 #pragma unroll
             for (int ia = 0; ia <= ib; ++ia) {
-                cd bab = csub(cneg(ldc(Hcc + ia + ib * NC)), sig[ia + ib * NC]);
+                cd bab = csub(cneg(ldc(Hcc + ia + ib * NC)), sig_at(ia, ib));
                 if (ia == ib) {
                     bab = cadd(bab, cd{mu, w});
                     G[ia][ia] = csub(bab, G[ia][ia]);  // - (R_aa + i J_aa)
                 } else {
                     const cd R = G[ia][ib], J = G[ib][ia];
-                    const cd bba = csub(cneg(ldc(Hcc + ib + ia * NC)), sig[ib + ia * NC]);
+                    const cd bba = csub(cneg(ldc(Hcc + ib + ia * NC)), sig_at(ib, ia));
                     G[ia][ib] = csub(bab, cd{R.x - J.y, R.y + J.x});   // - (R + i J)_ab
                     G[ib][ia] = csub(bba, cd{R.x + J.y, J.x - R.y});   // - (R + i J)_ba
                 }
@@ -425,6 +440,16 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
                 if (wvalid) trA = cadd(trA, t);
             }
         } else {
+            // the adjoint's cotangent, loaded all together here so that it arrives
+            // under the d loop below (it is k-invariant; keeping it across k would
+            // hold 4 NC^2 registers for the whole walk)
+            cd gbv[MODE == MODE_VJP ? NC : 1][MODE == MODE_VJP ? NC : 1];
+            if constexpr (MODE == MODE_VJP) {
+#pragma unroll
+                for (int j = 0; j < NC; ++j)
+#pragma unroll
+                    for (int i = 0; i < NC; ++i) gbv[i][j] = gb[i + j * NC];
+            }
             // ---- Y = sum_d g_d^2 P_d (X of the header), trace terms via M_d = w_d^H G w_d
             cd Y[NC][NC];
 #pragma unroll
@@ -541,7 +566,7 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
                 for (int j = 0; j < NC; ++j)
 #pragma unroll
                     for (int i = 0; i < NC; ++i) {
-                        const cd gv = gb[i + j * NC];
+                        const cd gv = gbv[i][j];
                         const double dg = i == j ? cn : 0.0;
                         V[i][j] = cd{fma(cn, Y[j][i].x, gv.x + dg), fma(-cn, Y[j][i].y, gv.y)};
                     }
