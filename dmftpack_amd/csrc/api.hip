@@ -221,7 +221,7 @@ extern "C" int dmftgpu_prepare(dmftgpu_handle h, const dmftgpu_lattice* L, const
     size_t o_U = cv.take<cd>((size_t)nk * n * n);
     size_t o_ed = cv.take<double>((size_t)nk * (nd > 0 ? nd : 1));
     size_t o_Vd = cv.take<cd>((size_t)nk * (nd > 0 ? nd * nd : 1));
-    size_t o_W = cv.take<cd>((size_t)nk * nc * (nd > 0 ? nd : 1));
+    size_t o_W = cv.take<cd>((size_t)nk * nc * (nd > 0 ? nd : 1) + dg::kKdataPadBytes / sizeof(cd));
     const bool lane_kernels = dg::schur_supported(n, nc) && dg::schur_multi_supported(nc);
     size_t o_kd = cv.take<cd>(lane_kernels ? (size_t)nk * dg::schur_kdata_size(n, nc) + dg::kKdataPadBytes / sizeof(cd)
                                           : 1);

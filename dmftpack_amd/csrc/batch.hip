@@ -580,7 +580,8 @@ static int batch_create_impl(dmftgpu_batch_s* b, int count, const dmftgpu_system
         const size_t C = s.max_chunks, ndd = std::max(1, s.nd);
         size_t t = 0;
         t += rnd(16 * 4 * s.m2) + rnd(8 * (size_t)s.nk * s.n) + rnd(16 * (size_t)s.nk * s.n * s.n);
-        t += rnd(8 * (size_t)s.nk * ndd) + rnd(16 * (size_t)s.nk * ndd * ndd) + rnd(16 * (size_t)s.nk * s.nc * ndd);
+        t += rnd(8 * (size_t)s.nk * ndd) + rnd(16 * (size_t)s.nk * ndd * ndd) +
+             rnd(16 * (size_t)s.nk * s.nc * ndd + dg::kKdataPadBytes);
         if (dg::schur_multi_supported(s.nc))
             t += rnd(16 * (size_t)s.nk * dg::schur_kdata_size(s.n, s.nc) + dg::kKdataPadBytes);
         t += rnd(16 * (size_t)s.nw_tot * s.m2) + rnd(16 * (size_t)s.nf * s.m2) + rnd(8);
@@ -610,7 +611,7 @@ static int batch_create_impl(dmftgpu_batch_s* b, int count, const dmftgpu_system
         s.U = (cd*)take(16 * (size_t)s.nk * s.n * s.n);
         s.ed = (double*)take(8 * (size_t)s.nk * ndd);
         s.Vd = (cd*)take(16 * (size_t)s.nk * ndd * ndd);
-        s.W = (cd*)take(16 * (size_t)s.nk * s.nc * ndd);
+        s.W = (cd*)take(16 * (size_t)s.nk * s.nc * ndd + dg::kKdataPadBytes);
         s.kdata = dg::schur_multi_supported(s.nc)
                       ? (cd*)take(16 * (size_t)s.nk * dg::schur_kdata_size(s.n, s.nc) + dg::kKdataPadBytes)
                       : nullptr;

@@ -26,8 +26,86 @@ constexpr int MODE_FWD_REAL = 5;
 // per-k data of the Schur kernels, which no kernel writes while they run.
 typedef __attribute__((address_space(4))) cd kconst_cd;
 // bytes of padding after the per-k blocks of the one-matrix-per-lane kernels
-// (their scalar-cache fill reads 3 KB from the start of a block)
+// and after W (the scalar-cache fills read up to 4 KB from a block's start)
 constexpr int kKdataPadBytes = 4096;
+
+// Scalar-cache fill of the first 4 KB from base: one dword from each 64-byte
+// line, all into one throw-away SGPR, waited for inside the asm block so that
+// no load is in flight when it ends.  The lines arrive together (one L2
+// round trip) and the scalar loads that follow hit the cache.
+__device__ __forceinline__ void kcache_fill64(const void* base)
+{
+    unsigned t;
+    asm volatile(
+        "s_load_dword %0, %1, 0x0\n\t"
+        "s_load_dword %0, %1, 0x40\n\t"
+        "s_load_dword %0, %1, 0x80\n\t"
+        "s_load_dword %0, %1, 0xc0\n\t"
+        "s_load_dword %0, %1, 0x100\n\t"
+        "s_load_dword %0, %1, 0x140\n\t"
+        "s_load_dword %0, %1, 0x180\n\t"
+        "s_load_dword %0, %1, 0x1c0\n\t"
+        "s_load_dword %0, %1, 0x200\n\t"
+        "s_load_dword %0, %1, 0x240\n\t"
+        "s_load_dword %0, %1, 0x280\n\t"
+        "s_load_dword %0, %1, 0x2c0\n\t"
+        "s_load_dword %0, %1, 0x300\n\t"
+        "s_load_dword %0, %1, 0x340\n\t"
+        "s_load_dword %0, %1, 0x380\n\t"
+        "s_load_dword %0, %1, 0x3c0\n\t"
+        "s_load_dword %0, %1, 0x400\n\t"
+        "s_load_dword %0, %1, 0x440\n\t"
+        "s_load_dword %0, %1, 0x480\n\t"
+        "s_load_dword %0, %1, 0x4c0\n\t"
+        "s_load_dword %0, %1, 0x500\n\t"
+        "s_load_dword %0, %1, 0x540\n\t"
+        "s_load_dword %0, %1, 0x580\n\t"
+        "s_load_dword %0, %1, 0x5c0\n\t"
+        "s_load_dword %0, %1, 0x600\n\t"
+        "s_load_dword %0, %1, 0x640\n\t"
+        "s_load_dword %0, %1, 0x680\n\t"
+        "s_load_dword %0, %1, 0x6c0\n\t"
+        "s_load_dword %0, %1, 0x700\n\t"
+        "s_load_dword %0, %1, 0x740\n\t"
+        "s_load_dword %0, %1, 0x780\n\t"
+        "s_load_dword %0, %1, 0x7c0\n\t"
+        "s_load_dword %0, %1, 0x800\n\t"
+        "s_load_dword %0, %1, 0x840\n\t"
+        "s_load_dword %0, %1, 0x880\n\t"
+        "s_load_dword %0, %1, 0x8c0\n\t"
+        "s_load_dword %0, %1, 0x900\n\t"
+        "s_load_dword %0, %1, 0x940\n\t"
+        "s_load_dword %0, %1, 0x980\n\t"
+        "s_load_dword %0, %1, 0x9c0\n\t"
+        "s_load_dword %0, %1, 0xa00\n\t"
+        "s_load_dword %0, %1, 0xa40\n\t"
+        "s_load_dword %0, %1, 0xa80\n\t"
+        "s_load_dword %0, %1, 0xac0\n\t"
+        "s_load_dword %0, %1, 0xb00\n\t"
+        "s_load_dword %0, %1, 0xb40\n\t"
+        "s_load_dword %0, %1, 0xb80\n\t"
+        "s_load_dword %0, %1, 0xbc0\n\t"
+        "s_load_dword %0, %1, 0xc00\n\t"
+        "s_load_dword %0, %1, 0xc40\n\t"
+        "s_load_dword %0, %1, 0xc80\n\t"
+        "s_load_dword %0, %1, 0xcc0\n\t"
+        "s_load_dword %0, %1, 0xd00\n\t"
+        "s_load_dword %0, %1, 0xd40\n\t"
+        "s_load_dword %0, %1, 0xd80\n\t"
+        "s_load_dword %0, %1, 0xdc0\n\t"
+        "s_load_dword %0, %1, 0xe00\n\t"
+        "s_load_dword %0, %1, 0xe40\n\t"
+        "s_load_dword %0, %1, 0xe80\n\t"
+        "s_load_dword %0, %1, 0xec0\n\t"
+        "s_load_dword %0, %1, 0xf00\n\t"
+        "s_load_dword %0, %1, 0xf40\n\t"
+        "s_load_dword %0, %1, 0xf80\n\t"
+        "s_load_dword %0, %1, 0xfc0\n\t"
+        "s_waitcnt lgkmcnt(0)"
+        : "=&s"(t)
+        : "s"(base));
+    asm volatile("" ::"s"(t));
+}
 typedef __attribute__((address_space(4))) double kconst_double;
 
 __device__ __forceinline__ cd ldc(const cd* p) { return *p; }

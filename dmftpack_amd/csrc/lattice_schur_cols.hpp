@@ -112,6 +112,23 @@ __device__ __forceinline__ void cols_item(const LatArgs& a, const cd* __restrict
         const kconst_cd* Hkk = (const kconst_cd*)(a.Hk + (size_t)k * n * n);  // H_cc(i, c) at i + c*n
         const kconst_cd* Wk = (const kconst_cd*)(a.W + (size_t)k * nc * nd);
         const kconst_double* ed = (const kconst_double*)(a.ed + (size_t)k * nd);
+        // This k's W and e_d into the caches at once: vector loads into L1 for
+        // the lane-dependent reads (W_cd, e_d), the scalar-cache fill for the
+        // wave-uniform ones (columns of W), one round trip per k instead of
+        // one per d
+        if (nd > 0) {
+            const cd* Wg = a.W + (size_t)k * nc * nd;
+            const int wn = nc * nd;  // complex entries of W(k)
+            double pf = 0.0;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int e = tid + r * 64;
+                if (e < wn) pf = fma(Wg[e].x, 0.0, pf);
+            }
+            if (tid < nd) pf = fma(a.ed[(size_t)k * nd + tid], 0.0, pf);
+            kcache_fill64(Wg);
+            asm volatile("" ::"v"(pf));
+        }
         // g_d = 1/(iw + mu - e_d) of this slot's frequency, shared by its lanes
         for (int d = c; d < nd; d += N) {
             const double x = mu - ed[d];
