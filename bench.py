@@ -227,9 +227,11 @@ def main():
         t_k, flop, pkey, units_k = t_f, f_fwd(n), "FWD_CALL", units
     if head == G.PLAIN_F64:
         kname = kname.replace("schur", "plain")
+    # HBM bytes per launch: FETCH_SIZE / WRITE_SIZE counters of the same kernel,
+    # taken on the dp14 workload only (profiles/pmc_traffic.json)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "pmc_traffic.json")
-    if os.path.exists(tpath):
+    if os.path.exists(tpath) and args.workload == "dp14":
         try:
             traffic = json.load(open(tpath)).get(args.method, {}).get(pkey)
         except Exception:
