@@ -73,8 +73,21 @@ def test_density_plain(gpu, name):
     lat.close()
 
 
+def check_eig(M, w, U):
+    """Ascending eigenvalues against numpy, residual M U - U diag(w), and U^H U = I."""
+    n = M.shape[0]
+    w_ref = np.linalg.eigvalsh(M)
+    scale = max(1.0, np.max(np.abs(w_ref)))
+    assert np.all(np.diff(w) >= 0)
+    assert np.max(np.abs(w - w_ref)) <= 1e-12 * scale
+    assert np.max(np.abs(M @ U - U * w)) <= 1e-12 * scale
+    assert np.max(np.abs(U.conj().T @ U - np.eye(n))) <= 1e-12
+
+
+# sizes 1..4, 5..8 and 9..16 run one matrix per 4-, 8- or 16-lane slot of a
+# wavefront; 17 and 32 the one-matrix-per-workgroup kernel
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["t2g", "dp14", "n_9", "large32"])
+@pytest.mark.parametrize("name", ["t2g", "dp14", "n_1", "n_2", "n_7", "n_9", "n_16", "n_17", "large32"])
 def test_static_eig(gpu, name):
     Hk, sig, beta, ncl, mu = small_case(name)
     mom = O.asymto_moments(sig, beta)
@@ -84,9 +97,31 @@ def test_static_eig(gpu, name):
     for k in range(Hk.shape[0]):
         M = Hk[k].copy()
         M[:nc, :nc] += mom[0]
-        w_ref = np.linalg.eigvalsh(M)
-        assert np.max(np.abs(ev[k] - w_ref)) <= 1e-12 * max(1.0, np.max(np.abs(w_ref)))
-        assert np.max(np.abs(M @ U[k] - U[k] * ev[k])) <= 1e-12 * max(1.0, np.max(np.abs(w_ref)))
+        check_eig(M, ev[k], U[k])
+    lat.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", [3, 8, 14, 20])
+def test_static_eig_degenerate(gpu, n):
+    """H(k) = Q diag(lambda) Q^H with repeated lambda, Q a random unitary per k:
+    the rotations must converge inside the degenerate subspaces too, and a
+    k point with H(k) already diagonal takes no sweep at all."""
+    rng = np.random.default_rng(n)
+    nk, nf, nc = 9, 8, 1
+    lam = np.repeat(np.linspace(-1.0, 1.0, (n + 2) // 3), 3)[:n]
+    Hk = np.empty((nk, n, n), complex)
+    for k in range(nk):
+        Q, _ = np.linalg.qr(rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)))
+        Hk[k] = (Q * lam) @ Q.conj().T
+        Hk[k] = 0.5 * (Hk[k] + Hk[k].conj().T)
+    Hk[0] = np.diag(lam)
+    mom = np.zeros((4, nc, nc), complex)
+    lat = gpu.LatticeGF(Hk, nc, nf, 20.0, mom, n_cluster=1)
+    ev, U = lat.static_eig()
+    for k in range(nk):
+        check_eig(Hk[k], ev[k], U[k])
+    assert np.array_equal(ev[0], np.sort(lam))
     lat.close()
 
 
