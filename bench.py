@@ -477,10 +477,38 @@ def _cpu_sample(fn, total_units, units_of, budget, probe):
     return units_of(size) / tc, size, tc
 
 
-def _flop_roofline(achieved_tflops, kernel, note, flop_per_unit, units):
-    return dict(bound="fp64_valu", achieved=achieved_tflops, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
-                frac=achieved_tflops / PEAK_FP64_TFLOPS, traffic=None, kernel=kernel, flop_per_unit=flop_per_unit,
-                units_per_launch=units, note=note)
+def _flop_roofline(achieved_tflops, kernel, note, flop_per_unit, units, plain_equivalent=False, executed=None):
+    """Roofline of a path's main kernel.  achieved_tflops is flop_per_unit x units
+    over the kernel time.  When flop_per_unit is the plain algorithm's 8n^3
+    (plain_equivalent) and the kernel executes less, that rate is a speed figure,
+    not a fraction of peak: it goes to its own field, and achieved / frac come
+    from counter-executed FLOPs (executed = (TFLOP/s, source)) or stay null."""
+    if not plain_equivalent:
+        return dict(bound="fp64_valu", achieved=achieved_tflops, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
+                    frac=achieved_tflops / PEAK_FP64_TFLOPS, traffic=None, kernel=kernel,
+                    flop_per_unit=flop_per_unit, units_per_launch=units, note=note)
+    ach = executed[0] if executed else None
+    return dict(bound="fp64_valu", achieved=ach, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
+                frac=None if ach is None else ach / PEAK_FP64_TFLOPS, traffic=None, kernel=kernel,
+                flop_source=executed[1] if executed else None,
+                plain_equivalent=dict(flop_per_unit=flop_per_unit, units_per_launch=units, tflops=achieved_tflops,
+                                      note=note), units_per_launch=units)
+
+
+def _density_lr_executed(t_gpu):
+    """Executed FP64 FLOPs of k_density_lr per launch (MFMA + VALU) from the
+    counter summary profiles/mfma_counters_density_matrix_dp14.txt, over t_gpu."""
+    path = os.path.join(ROOT, "profiles", "mfma_counters_density_matrix_dp14.txt")
+    try:
+        for line in open(path):
+            f = line.split()
+            if "k_density_lr" in line and len(f) > 3 and not line.startswith("#"):
+                gf = float(f[1]) + float(f[2])
+                return gf / 1e3 / t_gpu, ("profiles/mfma_counters_density_matrix_dp14.txt: SQ_INSTS_VALU_MFMA_MOPS_F64 "
+                                          "x 512 + SQ_INSTS_VALU_FLOPS_FP64 x 64 = %.1f GFLOP per launch" % gf)
+    except OSError:
+        pass
+    return None
 
 
 def bench_path(args, torch, capi, G, L):
@@ -526,7 +554,8 @@ def bench_path(args, torch, capi, G, L):
         ach = f_fwd(n) * units / t_gpu / 1e12
         roof = _flop_roofline(ach, "k_density_lr", "plain-equivalent 8n^3 FLOPs per (k,iw) x Nk*2N_freq / call time "
                               "(events); the block form executes far fewer (Schur complement + rank-n_corr product "
-                              "on FP64 MFMA): a speed figure, not a fraction of peak", f_fwd(n), units)
+                              "on FP64 MFMA): a speed figure, not a fraction of peak", f_fwd(n), units,
+                              plain_equivalent=True, executed=_density_lr_executed(t_gpu) if nk == 32768 else None)
         if not args.no_cpu:
             def run(kk):
                 O.greenftn_w(Hk[:kk], sig, mom, mu, beta, density=True, nthreads=cores)
@@ -560,7 +589,7 @@ def bench_path(args, torch, capi, G, L):
         ach = f_fwd(n) * units / (per[head]["gpu_ms"] / 1e3) / 1e12
         roof = _flop_roofline(ach, "k_lattice_schur_cols<FWD> (real axis)" if head != "plain_f64" else
                               "k_lattice_plain<FWD> (real axis)", "algorithmic 8n^3 FLOPs per (k,E) x Nk*n_E / call "
-                              "time (events)", f_fwd(n), units)
+                              "time (events)", f_fwd(n), units, plain_equivalent=head != "plain_f64")
         extra["methods"] = per
         extra["speedup_vs_plain_fp64"] = per[head]["value"] / per["plain_f64"]["value"]
         if not args.no_cpu:
@@ -709,7 +738,7 @@ def bench_path(args, torch, capi, G, L):
         metric, unit, value = "GreenFtn_w_adjust_mu (k,iw) units/s", "(k,w) units/s", units / tu
         ach = 8.0 * n ** 3 * units / tu_gpu / 1e12
         roof = _flop_roofline(ach, "k_lattice_schur<DENS>", "algorithmic 8n^3 FLOPs per (k,iw) per N(mu) evaluation "
-                              "x evaluations / call time (events)", f_fwd(n), units)
+                              "x evaluations / call time (events)", f_fwd(n), units, plain_equivalent=True)
         extra["moments_ms"] = tm * 1e3
         extra["hyb_delta_ms"] = th * 1e3
         extra["adjust_mu"] = dict(ms=tu * 1e3, n_eval=nev.value, mu=mu_out.value)

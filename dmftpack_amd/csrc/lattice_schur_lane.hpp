@@ -252,10 +252,22 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
         for (int j = 0; j < NC; ++j)
 #pragma unroll
             for (int i = 0; i < NC; ++i) G[i][j] = cd{0.0, 0.0};
-        for (int d = 0; d < nd; ++d) {
-            const double x = mu - ed[d];
+        // g_d = 1/(z + mu - e_d) one d ahead: its reciprocal chain is independent
+        // of the d step's FMAs, so the two interleave instead of the chain
+        // stalling the top of every step; e_{d+1} is read one step earlier still,
+        // with that step's P_d, so that the chain does not wait for it either
+        auto g_of = [&](double e) -> cd {
+            const double x = mu - e;
             const double r = rcp_pos(fma(x, x, w * w));
-            const double gr = x * r, gi = -w * r;
+            return cd{x * r, -w * r};
+        };
+        auto e_at = [&](int d) -> double { return (double)ed[d < nd ? d : nd - 1]; };
+        cd g_next = nd > 0 ? g_of(e_at(0)) : cd{0.0, 0.0};
+        double e_next = nd > 0 ? e_at(1) : 0.0;
+        for (int d = 0; d < nd; ++d) {
+            const double gr = g_next.x, gi = g_next.y;
+            g_next = g_of(e_next);
+            e_next = e_at(d + 2);
             const BT* Pd = P + d * NP;
 #pragma unroll
             for (int ib = 0; ib < NC; ++ib)
@@ -415,10 +427,12 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
                         G[ia][ib] = cd{x1.x + x2.x, x1.y - x2.y};
                         G[ib][ia] = cd{x1.y + x2.y, x2.x - x1.x};
                     }
+                g_next = nd > 0 ? g_of(e_at(0)) : cd{0.0, 0.0};
+                e_next = nd > 0 ? e_at(1) : 0.0;
                 for (int d = 0; d < nd; ++d) {
-                    const double x = mu - ed[d];
-                    const double r = rcp_pos(fma(x, x, w * w));
-                    const cd g = cd{x * r, -w * r};
+                    const cd g = g_next;
+                    g_next = g_of(e_next);
+                    e_next = e_at(d + 2);
                     const BT* Pd = P + d * NP;
                     cd Md = cd{0.0, 0.0};
 #pragma unroll
@@ -471,10 +485,12 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
                     G[ia][ib] = cd{x1.x + x2.x, x1.y - x2.y};
                     G[ib][ia] = cd{x1.y + x2.y, x2.x - x1.x};
                 }
+            g_next = nd > 0 ? g_of(e_at(0)) : cd{0.0, 0.0};
+            e_next = nd > 0 ? e_at(1) : 0.0;
             for (int d = 0; d < nd; ++d) {
-                const double x = mu - ed[d];
-                const double r = rcp_pos(fma(x, x, w * w));
-                const cd g = cd{x * r, -w * r};
+                const cd g = g_next;
+                g_next = g_of(e_next);
+                e_next = e_at(d + 2);
                 const cd g2 = cmul(g, g);
                 const BT* Pd = P + d * NP;
                 cd Md = cd{0.0, 0.0};
