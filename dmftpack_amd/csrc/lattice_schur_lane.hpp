@@ -152,6 +152,20 @@ __host__ __device__ __forceinline__ int kblock_size(int nd)
     return NC * NC + nd * NP + (nd + 1) / 2;
 }
 
+// Per-lane sums in LDS: an LDS add without return (ds_add_f64) instead of a
+// read, add and write.  Nothing waits for the old value, so a sum does not
+// stall the wave; each lane owns its entries, so the adds land in program
+// order and the result is the one of the plain read-add-write.
+__device__ __forceinline__ void lds_add(double* p, double v)
+{
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void lds_add(cd* p, cd v)
+{
+    lds_add(&p->x, v.x);
+    lds_add(&p->y, v.y);
+}
+
 // One (frequency tile, k chunk) work item.  The per-k block is read from
 // a.kdata through the constant address space: its addresses are wave-uniform,
 // so the loads become scalar loads and the products against it take SGPR
@@ -385,8 +399,8 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
                     const double h0 = i == j ? G[i][i].x : G[i][j].x + G[j][i].x;
                     const double h1 = G[i][j].y - G[j][i].y;
                     if (ACC_LDS) {
-                        sAccH[q * WG + lane] += h0;
-                        if (i != j) sAccH[(q + 1) * WG + lane] += h1;
+                        lds_add(&sAccH[q * WG + lane], h0);
+                        if (i != j) lds_add(&sAccH[(q + 1) * WG + lane], h1);
                     } else {
                         accH[q] += h0;
                         if (i != j) accH[q + 1] += h1;
@@ -400,10 +414,7 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
 #pragma unroll
                 for (int j = 0; j < NC; ++j)
 #pragma unroll
-                    for (int i = 0; i < NC; ++i) {
-                        cd* p = &sAcc[(i + j * NC) * WG + lane];
-                        *p = cadd(*p, G[i][j]);
-                    }
+                    for (int i = 0; i < NC; ++i) lds_add(&sAcc[(i + j * NC) * WG + lane], G[i][j]);
             } else {
 #pragma unroll
                 for (int i = 0; i < (ACC_LDS || HERM ? 1 : NC); ++i)
@@ -602,8 +613,7 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
                         cd s = cd{0.0, 0.0};
 #pragma unroll
                         for (int i = 0; i < NC; ++i) s = cfmac(G[i][aa], t[i], s);
-                        cd* p = &sAcc[(aa + b * NC) * WG + lane];
-                        *p = cadd(*p, s);
+                        lds_add(&sAcc[(aa + b * NC) * WG + lane], s);
                         const cd sp = aa == b ? cadd(Y[aa][b], cd{1.0, 0.0}) : Y[aa][b];
                         ms = cfmac(s, sp, ms);  // S'_ab conj(C_ab)
                     }
