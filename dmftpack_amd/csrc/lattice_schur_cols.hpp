@@ -119,13 +119,14 @@ __device__ __forceinline__ void cols_item(const LatArgs& a, const cd* __restrict
         if (nd > 0) {
             const cd* Wg = a.W + (size_t)k * nc * nd;
             const int wn = nc * nd;  // complex entries of W(k)
+            // indices clamped, not branched on: the loads issue together
             double pf = 0.0;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int e = tid + r * 64;
-                if (e < wn) pf = fma(Wg[e].x, 0.0, pf);
+                pf = fma(Wg[e < wn ? e : wn - 1].x, 0.0, pf);
             }
-            if (tid < nd) pf = fma(a.ed[(size_t)k * nd + tid], 0.0, pf);
+            pf = fma(a.ed[(size_t)k * nd + (tid < nd ? tid : nd - 1)], 0.0, pf);
             kcache_fill64(Wg);
             asm volatile("" ::"v"(pf));
         }
@@ -140,11 +141,18 @@ __device__ __forceinline__ void cols_item(const LatArgs& a, const cd* __restrict
         // ---- column c of S (identity column for padding lanes)
         cd col[N];
         cd y[MODE == MODE_FWD ? 1 : N];  // column c of Y = sum_d g_d^2 w_d w_d^H
+        // (the loads are unconditional, rows clamped in range, and pinned by an
+        // empty asm so that the compiler cannot sink each into a branch of its
+        // own: they issue together instead of one round trip per entry)
+        cd hcol[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) hcol[i] = ldc(Hkk + (i < nc ? i : 0) + (size_t)cc * n);
+#pragma unroll
+        for (int i = 0; i < N; ++i) asm volatile("" ::"v"(hcol[i].x), "v"(hcol[i].y));
 #pragma unroll
         for (int i = 0; i < N; ++i) {
-            cd v = (i == c) ? cd{1.0, 0.0} : cd{0.0, 0.0};
-            if (live) v = (i < nc) ? csub(E[i], ldc(Hkk + i + (size_t)cc * n)) : cd{0.0, 0.0};
-            col[i] = v;
+            const cd pad = (i == c) ? cd{1.0, 0.0} : cd{0.0, 0.0};
+            col[i] = live ? ((i < nc) ? csub(E[i], hcol[i]) : cd{0.0, 0.0}) : pad;
         }
         if constexpr (MODE != MODE_FWD) {
 #pragma unroll
@@ -186,18 +194,32 @@ __device__ __forceinline__ void cols_item(const LatArgs& a, const cd* __restrict
             for (int it = 0; it < steps; ++it) {
                 wave_sync();  // keeps the S and X0 reads inside the step (no hoisting into registers)
                 cf r[N];  // e_c - S x_c in FP64, stored in FP32
+                // each row of S (then of X0) is read from LDS as a whole before
+                // its products, pinned by empty asm statements: left to itself the
+                // compiler issues one read, waits for it and uses it, one LDS
+                // round trip per entry
 #pragma unroll
                 for (int i = 0; i < N; ++i) {
+                    cd srow[N];
+#pragma unroll
+                    for (int l = 0; l < N; ++l) srow[l] = MA[i + l * N];
+#pragma unroll
+                    for (int l = 0; l < N; ++l) asm volatile("" ::"v"(srow[l].x), "v"(srow[l].y));
                     cd s = (i == c) ? cd{1.0, 0.0} : cd{0.0, 0.0};
 #pragma unroll
-                    for (int l = 0; l < N; ++l) s = cfms(MA[i + l * N], col[l], s);
+                    for (int l = 0; l < N; ++l) s = cfms(srow[l], col[l], s);
                     r[i] = cf{(float)s.x, (float)s.y};
                 }
 #pragma unroll
                 for (int i = 0; i < N; ++i) {
+                    cf xrow[N];
+#pragma unroll
+                    for (int l = 0; l < N; ++l) xrow[l] = X0[i + l * N];
+#pragma unroll
+                    for (int l = 0; l < N; ++l) asm volatile("" ::"v"(xrow[l].x), "v"(xrow[l].y));
                     cf t = cf{0.f, 0.f};
 #pragma unroll
-                    for (int l = 0; l < N; ++l) t = ffma(X0[i + l * N], r[l], t);
+                    for (int l = 0; l < N; ++l) t = ffma(xrow[l], r[l], t);
                     col[i] = cadd(col[i], cd{(double)t.x, (double)t.y});
                 }
             }
