@@ -34,6 +34,17 @@
 
 namespace dg {
 
+// Smallest n_c the bucket of width N serves (lattice_schur.hip, cols_bucket):
+// rows i below it are inside every served block, so in the d loops the tests
+// i < n_c on them fold away at compile time and the W loads issue
+// unconditionally (a load under a uniform branch waits on its own).  The
+// adjoint's LDS products keep their tests: unconditional, their reads would
+// all be hoisted and spill.
+__host__ __device__ constexpr int cols_nc_min(int N)
+{
+    return N <= 4 ? 1 : N <= 6 ? N : N == 8 ? 7 : N == 10 ? 9 : N == 12 ? 11 : 13;
+}
+
 // one wavefront per workgroup: the per-k data comes from global memory, so
 // wavefronts share nothing but the cache
 __host__ __device__ constexpr int cols_wg(int, int) { return 64; }
@@ -64,6 +75,8 @@ __device__ __forceinline__ void cols_item(const LatArgs& a, const cd* __restrict
     constexpr int SS = cols_slot_size(N, MODE, PREC);
     constexpr bool MATS = (PREC > 0 || MODE != MODE_FWD);
     const int n = a.n, nc = a.nc, nd = a.n - nc, m2 = nc * nc;
+    constexpr int NMIN = cols_nc_min(N);
+    auto inb = [nc](int i) { return i < NMIN || i < nc; };  // row / column i inside the n_c block
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int sl = lane / N;  // == SPW for the spare lanes of a wavefront
     const int c = lane - sl * N;
@@ -164,12 +177,12 @@ __device__ __forceinline__ void cols_item(const LatArgs& a, const cd* __restrict
             const cd u = live ? cmulc(g, ldc(Wd + cc)) : cd{0.0, 0.0};  // g_d conj(W_cd)
 #pragma unroll
             for (int i = 0; i < N; ++i)
-                if (i < nc) col[i] = cfms(ldc(Wd + i), u, col[i]);
+                if (inb(i)) col[i] = cfms(ldc(Wd + i), u, col[i]);
             if constexpr (MODE != MODE_FWD) {
                 const cd u2 = cmul(g, u);
 #pragma unroll
                 for (int i = 0; i < N; ++i)
-                    if (i < nc) y[i] = cfma(ldc(Wd + i), u2, y[i]);
+                    if (inb(i)) y[i] = cfma(ldc(Wd + i), u2, y[i]);
             }
         }
 
@@ -242,7 +255,7 @@ __device__ __forceinline__ void cols_item(const LatArgs& a, const cd* __restrict
                     cd v = cd{0.0, 0.0};
 #pragma unroll
                     for (int i = 0; i < N; ++i)
-                        if (i < nc) v = cfmac(ldc(Wd + i), col[i], v);
+                        if (inb(i)) v = cfmac(ldc(Wd + i), col[i], v);
                     const cd m = live ? cmul(ldc(Wd + cc), v) : cd{0.0, 0.0};
                     t = cfma(cmul(g, g), m, t);
                     if (c == 0) t = cadd(t, g);
@@ -275,7 +288,7 @@ __device__ __forceinline__ void cols_item(const LatArgs& a, const cd* __restrict
                 cd v = cd{0.0, 0.0};
 #pragma unroll
                 for (int i = 0; i < N; ++i)
-                    if (i < nc) v = cfmac(ldc(Wd + i), col[i], v);
+                    if (inb(i)) v = cfmac(ldc(Wd + i), col[i], v);
                 const cd m = live ? cmul(ldc(Wd + cc), v) : cd{0.0, 0.0};
                 tg = cfma(g2, m, tg);
                 t2 = cfma(cscale(cmul(g2, g), 2.0), m, t2);
@@ -406,6 +419,7 @@ __global__ __launch_bounds__(cols_wg(MODE, PREC)) void k_lattice_schur_cols_mult
 template <int N, int MODE, int PREC, bool TRACE>
 hipError_t launch_cols_k(const LatArgs& a, const cd* sig_all, hipStream_t s, const SchurMulti* m)
 {
+    if (a.nc < cols_nc_min(N) || a.nc > N) return hipErrorInvalidValue;  // the kernels assume this bucket
     const int nd = a.n - a.nc;
     const size_t shm = cols_lds_bytes(N, MODE, PREC, a.nc, nd);
     constexpr int SLOTS = cols_slots(N, MODE, PREC);
