@@ -88,6 +88,20 @@ __device__ __forceinline__ void wave_sync()
 
 __device__ __forceinline__ cd ld(const cd* p) { return *p; }
 
+// Per-lane sums in LDS: an LDS add without return (ds_add_f64) instead of a
+// read, add and write.  Nothing waits for the old value, so a sum does not
+// stall the wave; each lane owns its entries, so the adds land in program
+// order and the result is the one of the plain read-add-write.
+__device__ __forceinline__ void lds_add(double* p, double v)
+{
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void lds_add(cd* p, cd v)
+{
+    lds_add(&p->x, v.x);
+    lds_add(&p->y, v.y);
+}
+
 // Matsubara frequency w_n = pi(2n+1)/beta (GreenFtn.cpp:219).
 __device__ __forceinline__ double matsubara(int n, double beta) { return kPi * (2.0 * n + 1.0) / beta; }
 

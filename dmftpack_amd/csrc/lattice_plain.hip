@@ -149,7 +149,7 @@ __global__ __launch_bounds__(WG) void k_lattice_plain(LatArgs a)
             if (c < nc) {
 #pragma unroll
                 for (int i = 0; i < N; ++i)
-                    if (i < nc) sAcc[slot * m2 + i + c * nc] = cadd(sAcc[slot * m2 + i + c * nc], col[i]);
+                    if (i < nc) lds_add(&sAcc[slot * m2 + i + c * nc], col[i]);
             }
         } else {
             wave_sync();

@@ -152,20 +152,6 @@ __host__ __device__ __forceinline__ int kblock_size(int nd)
     return NC * NC + nd * NP + (nd + 1) / 2;
 }
 
-// Per-lane sums in LDS: an LDS add without return (ds_add_f64) instead of a
-// read, add and write.  Nothing waits for the old value, so a sum does not
-// stall the wave; each lane owns its entries, so the adds land in program
-// order and the result is the one of the plain read-add-write.
-__device__ __forceinline__ void lds_add(double* p, double v)
-{
-    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-__device__ __forceinline__ void lds_add(cd* p, cd v)
-{
-    lds_add(&p->x, v.x);
-    lds_add(&p->y, v.y);
-}
-
 // One (frequency tile, k chunk) work item.  The per-k block is read from
 // a.kdata through the constant address space: its addresses are wave-uniform,
 // so the loads become scalar loads and the products against it take SGPR
