@@ -46,7 +46,19 @@ __global__ __launch_bounds__(64) void k_herm_eig(const cd* __restrict__ X, int l
     // padding index are skipped.
     const int M = m + (m & 1);
     const int np = M / 2;
-    double* rc = (double*)(Vs + m * m);  // per pair: c, s, phase (x, y) and p, q
+    double* rc = (double*)(Vs + m * m);  // per pair: c, s, phase (x, y)
+    int* pqt = (int*)(rc + 4 * 16);      // per pair: p, q (q = -1: pair skipped)
+    // Each lane's work items of a pass (pair k, row or column i) are the same
+    // every round: their indices are formed once here instead of by an
+    // integer division per item per round.  np * m <= 16 * 32 = 8 * 64.
+    const int nitem = np * m;
+    int it_k[8], it_i[8];
+#pragma unroll
+    for (int it = 0; it < 8; ++it) {
+        const int t = lane + 64 * it;
+        it_k[it] = t < nitem ? t / m : -1;
+        it_i[it] = t < nitem ? t % m : 0;
+    }
     for (int sweep = 0; sweep < 60; ++sweep) {
         double off2 = 0.0;
         for (int e = lane; e < m * m; e += 64) {
@@ -81,22 +93,24 @@ __global__ __launch_bounds__(64) void k_herm_eig(const cd* __restrict__ X, int l
                         sn = t * c;
                     }
                 }
-                rc[6 * lane] = c;
-                rc[6 * lane + 1] = sn;
-                rc[6 * lane + 2] = phx;
-                rc[6 * lane + 3] = phy;
-                rc[6 * lane + 4] = (double)p;
-                rc[6 * lane + 5] = (double)(q < m ? q : -1);
+                rc[4 * lane] = c;
+                rc[4 * lane + 1] = sn;
+                rc[4 * lane + 2] = phx;
+                rc[4 * lane + 3] = phy;
+                pqt[2 * lane] = p;
+                pqt[2 * lane + 1] = q < m ? q : -1;
             }
             __syncthreads();
             // columns p, q of A and V: A <- A J
-            for (int t = lane; t < np * m; t += 64) {
-                const int k = t / m, i = t % m;
-                const int q = (int)rc[6 * k + 5];
+#pragma unroll
+            for (int it = 0; it < 8; ++it) {
+                const int k = it_k[it], i = it_i[it];
+                if (k < 0) continue;
+                const int q = pqt[2 * k + 1];
                 if (q < 0) continue;
-                const int p = (int)rc[6 * k + 4];
-                const double c = rc[6 * k], sn = rc[6 * k + 1];
-                const cd phc = cd{rc[6 * k + 2], -rc[6 * k + 3]};
+                const int p = pqt[2 * k];
+                const double c = rc[4 * k], sn = rc[4 * k + 1];
+                const cd phc = cd{rc[4 * k + 2], -rc[4 * k + 3]};
                 cd xp = A[i + p * m], xq = A[i + q * m];
                 A[i + p * m] = csub(cscale(xp, c), cscale(cmul(phc, xq), sn));
                 A[i + q * m] = cadd(cscale(xp, sn), cscale(cmul(phc, xq), c));
@@ -106,13 +120,15 @@ __global__ __launch_bounds__(64) void k_herm_eig(const cd* __restrict__ X, int l
             }
             __syncthreads();
             // rows p, q of A: A <- J^H A
-            for (int t = lane; t < np * m; t += 64) {
-                const int k = t / m, j = t % m;
-                const int q = (int)rc[6 * k + 5];
+#pragma unroll
+            for (int it = 0; it < 8; ++it) {
+                const int k = it_k[it], j = it_i[it];
+                if (k < 0) continue;
+                const int q = pqt[2 * k + 1];
                 if (q < 0) continue;
-                const int p = (int)rc[6 * k + 4];
-                const double c = rc[6 * k], sn = rc[6 * k + 1];
-                const cd ph = cd{rc[6 * k + 2], rc[6 * k + 3]};
+                const int p = pqt[2 * k];
+                const double c = rc[4 * k], sn = rc[4 * k + 1];
+                const cd ph = cd{rc[4 * k + 2], rc[4 * k + 3]};
                 cd xp = A[p + j * m], xq = A[q + j * m];
                 cd yp = csub(cscale(xp, c), cscale(cmul(ph, xq), sn));
                 cd yq = cadd(cscale(xp, sn), cscale(cmul(ph, xq), c));
@@ -377,7 +393,7 @@ hipError_t launch_herm_eig(const cd* X, int ldx, int off, int m, const cd* add, 
     if (m <= 8) return launch_slots<8>(X, ldx, off, m, add, nadd, w, V, batch, s);
     if (m <= 16) return launch_slots<16>(X, ldx, off, m, add, nadd, w, V, batch, s);
     if (m > 32) return hipErrorInvalidValue;  // the pair table holds 16 rotations
-    size_t shm = sizeof(cd) * 2 * (size_t)m * m + sizeof(double) * 6 * 17;
+    size_t shm = sizeof(cd) * 2 * (size_t)m * m + sizeof(double) * 4 * 16 + sizeof(int) * 2 * 16;
     hipLaunchKernelGGL(k_herm_eig, dim3(batch), dim3(64), shm, s, X, ldx, off, m, add, nadd, w, V, batch);
     return hipGetLastError();
 }
