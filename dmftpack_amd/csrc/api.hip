@@ -170,7 +170,8 @@ extern "C" int dmftgpu_set_option(dmftgpu_handle h, int option, int value)
             h->target_wg = value;
             return DMFTGPU_OK;
         case DMFTGPU_OPT_SCHUR_WG:
-            if (value != 64 && value != 128) return fail(h, DMFTGPU_ERR_ARG, "Schur workgroup width %d not 64/128", value);
+            if (value != 0 && value != 64 && value != 128)
+                return fail(h, DMFTGPU_ERR_ARG, "Schur workgroup width %d not 0/64/128", value);
             h->schur_wg = value;
             return DMFTGPU_OK;
         case DMFTGPU_OPT_SCHUR_LAYOUT:
@@ -546,7 +547,10 @@ int run_lattice(dmftgpu_handle h, int method, int mode, const double* sigma, dou
     if (method == DMFTGPU_PLAIN_F64) {
         HIPCHK(dg::launch_lattice_plain(a, mode, h->stream, nullptr));
     } else {
-        HIPCHK(dg::launch_lattice_schur(a, mode, method == DMFTGPU_SCHUR_F32R, h->stream, h->schur_wg,
+        // 0: the forward keeps two workgroups' worth of waves per SIMD at 64, the adjoint
+        // (more registers, fewer waves) shares each W_d / P_d read over 128 frequencies
+        const int wg = h->schur_wg ? h->schur_wg : (mode == dg::MODE_VJP ? 128 : 64);
+        HIPCHK(dg::launch_lattice_schur(a, mode, method == DMFTGPU_SCHUR_F32R, h->stream, wg,
                                         h->schur_layout == 1, fused_tr2 != nullptr));
     }
     if (h->timing) {

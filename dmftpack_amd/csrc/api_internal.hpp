@@ -50,7 +50,7 @@ struct dmftgpu_handle_s {
     double static_mu = 0.0;
     dgi::Workspace static_cache;  // [m2 cd][2 doubles]
     int refine_steps = 2;    // FP64-residual refinement steps of the FP32 Schur solve
-    int schur_wg = 128;      // workgroup width of the Schur kernels (64 or 128)
+    int schur_wg = 0;        // workgroup width of the Schur kernels (64 or 128; 0 = 64 forward, 128 adjoint)
     int schur_layout = 0;    // n_c <= 6: 0 one matrix per lane, 1 column per lane
     int upfold_chunk = 0;    // k per chunk of dmftgpu_upfold_density (0: scratch-bound)
     int density_kernel = 0;  // dmftgpu_density_matrix: 0 block form where supported, 1 full inverse
