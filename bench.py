@@ -73,6 +73,8 @@ def parse():
     ap.add_argument("--wg", type=int, default=0, choices=[0, 64, 128],
                     help="frequencies per workgroup of the Schur kernels (DMFTGPU_OPT_SCHUR_WG; "
                          "0 = 64 for the forward, 128 for the adjoint)")
+    ap.add_argument("--target", type=int, default=0,
+                    help="work items per lattice launch, sets the k chunking (DMFTGPU_OPT_TARGET_WORKGROUPS; 0 = default)")
     ap.add_argument("--layout", type=int, default=0, choices=[0, 1],
                     help="Schur kernels for n_corr <= 6: 0 one (k,iw) matrix per lane, 1 one column per lane")
     ap.add_argument("--plain-steps", type=int, default=2)
@@ -133,6 +135,7 @@ def main():
     lat.set_option(capi.OPT_REFINE_STEPS, args.refine)
     lat.set_option(capi.OPT_SCHUR_LAYOUT, args.layout)
     lat.set_option(capi.OPT_SCHUR_WG, args.wg)
+    lat.set_option(capi.OPT_TARGET_WORKGROUPS, args.target)
     s_dev = lat.sigma_dev(sig)
     # target G_loc from a seeded perturbation of Sigma (plain FP64, untimed)
     s_tgt = lat.sigma_dev(L.perturbed(sig, seed=7))
@@ -318,6 +321,7 @@ def main():
         "config": {"workload": WORKLOADS[args.workload],
                    "n": n, "n_corr": nc, "n_k": nk, "n_freq": nf, "beta": beta, "method": args.method,
                    "refine_steps": args.refine, "schur_layout": args.layout, "schur_wg": args.wg or "64 fwd / 128 vjp",
+                   "target_workgroups": args.target or "default",
                    "units_per_step": units,
                    "step": "tail-moment fit of Sigma + static eig for the tail + GreenFtn_w forward (G_loc, NumMatrix, "
                            "N) + loss cotangents + VJP (dL/dSigma, dL/dmu): dmftgpu_moments, dmftgpu_update_moments, "
