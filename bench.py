@@ -71,8 +71,9 @@ def parse():
     ap.add_argument("--n-freq", type=int, default=None)
     ap.add_argument("--refine", type=int, default=1)
     ap.add_argument("--wg", type=int, default=0, choices=[0, 64, 128],
-                    help="frequencies per workgroup of the Schur kernels (DMFTGPU_OPT_SCHUR_WG; "
-                         "0 = 64 for the forward, 128 for the adjoint)")
+                    help="frequencies per workgroup of the Schur kernels (DMFTGPU_OPT_SCHUR_WG; 0 = the default "
+                         "of each launch: adjoint 128, FP32R forward below N_freq as two wavefronts splitting the k "
+                         "walk over 64 frequencies, the others 64)")
     ap.add_argument("--target", type=int, default=0,
                     help="work items per lattice launch, sets the k chunking (DMFTGPU_OPT_TARGET_WORKGROUPS; 0 = default)")
     ap.add_argument("--layout", type=int, default=0, choices=[0, 1],
@@ -320,7 +321,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": WORKLOADS[args.workload],
                    "n": n, "n_corr": nc, "n_k": nk, "n_freq": nf, "beta": beta, "method": args.method,
-                   "refine_steps": args.refine, "schur_layout": args.layout, "schur_wg": args.wg or "64 fwd / 128 vjp",
+                   "refine_steps": args.refine, "schur_layout": args.layout, "schur_wg": args.wg or "default (adjoint 128, forward 64, FP32R forward below N_freq k-split)",
                    "target_workgroups": args.target or "default",
                    "units_per_step": units,
                    "step": "tail-moment fit of Sigma + static eig for the tail + GreenFtn_w forward (G_loc, NumMatrix, "

@@ -547,10 +547,8 @@ int run_lattice(dmftgpu_handle h, int method, int mode, const double* sigma, dou
     if (method == DMFTGPU_PLAIN_F64) {
         HIPCHK(dg::launch_lattice_plain(a, mode, h->stream, nullptr));
     } else {
-        // 0: the forward keeps two workgroups' worth of waves per SIMD at 64, the adjoint
-        // (more registers, fewer waves) shares each W_d / P_d read over 128 frequencies
-        const int wg = h->schur_wg ? h->schur_wg : (mode == dg::MODE_VJP ? 128 : 64);
-        HIPCHK(dg::launch_lattice_schur(a, mode, method == DMFTGPU_SCHUR_F32R, h->stream, wg,
+        // schur_wg 0: each launch at its default width (launch_lattice_schur)
+        HIPCHK(dg::launch_lattice_schur(a, mode, method == DMFTGPU_SCHUR_F32R, h->stream, h->schur_wg,
                                         h->schur_layout == 1, fused_tr2 != nullptr));
     }
     if (h->timing) {

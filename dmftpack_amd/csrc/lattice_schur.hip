@@ -61,9 +61,18 @@ bool schur_can_fuse(int nc, bool cols, bool real_axis) { return nc >= 1 && nc <=
 hipError_t launch_lattice_schur(const LatArgs& a, int mode, bool fp32, hipStream_t s, int wg, bool cols, bool fused)
 {
     if (fused && !schur_can_fuse(a.nc, cols, a.real_axis)) return hipErrorInvalidValue;
+    // wg = 0: the adjoint call at 128 frequencies per workgroup (its one wavefront
+    // per SIMD gains from a second sharing each per-k scalar fill), the FP32R
+    // forward over the first n_freq frequencies at n_c >= 5 as two wavefronts
+    // splitting the k walk over 64 frequencies (its LDS accumulators, n_c^2 x 1 KB
+    // per 64 frequencies, then allow the 8 wavefronts per CU that its registers
+    // allow, against 6 at n_c = 5 and 4 at n_c = 6), the other launches at 64
+    const int wg_call = wg ? wg : mode == MODE_VJP ? 128 : 64;
+    const int wg_lo = wg ? wg : mode == MODE_FWD && fp32 && a.nc >= 5 ? SCHUR_WG_KSPLIT : wg_call;
     hipError_t e = launch_sigma_all(a, s);
     if (e != hipSuccess) return e;
-    if (a.real_axis && mode == MODE_FWD && a.nc <= 6 && !cols) return dispatch_nc(a, MODE_FWD_REAL, fp32, a.pSigAll, s, wg, nullptr);
+    if (a.real_axis && mode == MODE_FWD && a.nc <= 6 && !cols)
+        return dispatch_nc(a, MODE_FWD_REAL, fp32, a.pSigAll, s, wg_call, nullptr);
     if (a.nc > 6 || cols || a.real_axis) {  // the per-lane kernels take the real axis in the forward only
         switch (cols_bucket(a.nc)) {
             case 4: return launch_cols_4(a, mode, fp32, a.pSigAll, s);
@@ -81,10 +90,10 @@ hipError_t launch_lattice_schur(const LatArgs& a, int mode, bool fp32, hipStream
         LatArgs lo = a, hi = a;
         lo.nw_end = a.n_freq;
         hi.nw_begin = a.n_freq;
-        e = dispatch_nc(lo, MODE_FWD, fp32, a.pSigAll, s, wg, nullptr);
+        e = dispatch_nc(lo, MODE_FWD, fp32, a.pSigAll, s, wg_lo, nullptr);
         if (e != hipSuccess) return e;
         // fused: the adjoint that follows needs tr G^2 there too (a.pTr2)
-        return dispatch_nc(hi, fused ? MODE_DENS_HERM : MODE_FWD_HERM, fp32, a.pSigAll, s, wg, nullptr);
+        return dispatch_nc(hi, fused ? MODE_DENS_HERM : MODE_FWD_HERM, fp32, a.pSigAll, s, wg_call, nullptr);
     }
     if (mode == MODE_VJP && a.nw_tot > a.n_freq) {
         // Frequencies beyond n_freq contribute only tr G and tr G^2 (dN/dmu):
@@ -92,11 +101,11 @@ hipError_t launch_lattice_schur(const LatArgs& a, int mode, bool fp32, hipStream
         LatArgs lo = a, hi = a;
         lo.nw_end = a.n_freq;
         hi.nw_begin = a.n_freq;
-        e = dispatch_nc(lo, MODE_VJP, fp32, a.pSigAll, s, wg, nullptr);
+        e = dispatch_nc(lo, MODE_VJP, fp32, a.pSigAll, s, wg_call, nullptr);
         if (e != hipSuccess || fused) return e;  // fused: the forward wrote tr G^2 beyond n_freq
-        return dispatch_nc(hi, MODE_DENS, fp32, a.pSigAll, s, wg, nullptr);
+        return dispatch_nc(hi, MODE_DENS, fp32, a.pSigAll, s, wg_call, nullptr);
     }
-    return dispatch_nc(a, mode, fp32, a.pSigAll, s, wg, nullptr);
+    return dispatch_nc(a, mode, fp32, a.pSigAll, s, wg_lo, nullptr);
 }
 
 hipError_t launch_lattice_schur_multi(const LatArgs& proto, int mode, bool fp32, hipStream_t s, int wg,

@@ -156,7 +156,10 @@ __host__ __device__ __forceinline__ int kblock_size(int nd)
 // a.kdata through the constant address space: its addresses are wave-uniform,
 // so the loads become scalar loads and the products against it take SGPR
 // operands, with no LDS traffic and no barriers in the k loop.
-template <int NC, int MODE, bool F32R, int WG, bool TRACE, int REFINE>
+// KS = 2 (the FP32R forward only): the WG lanes are KS wavefronts over the same
+// WG / KS frequencies, each walking every KS-th k of the chunk and adding into
+// one set of LDS accumulators, which then costs a wavefront half the LDS.
+template <int NC, int MODE, bool F32R, int WG, bool TRACE, int REFINE, int KS = 1>
 __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restrict__ sig_all, int wtile, int chunk,
                                            cd* lds)
 {
@@ -168,13 +171,20 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
     constexpr bool FWDLIKE = MODE == MODE_FWD || MODE == MODE_FWD_HERM || MODE == MODE_FWD_REAL;
     // accumulators that do not fit next to the working matrices live in LDS
     constexpr bool ACC_LDS = (MODE == MODE_VJP) || (MODE == MODE_DENS_HERM) || (FWDLIKE && F32R);
+    constexpr int FW = WG / KS;       // frequencies per workgroup
+    static_assert(KS == 1 || (KS == 2 && MODE == MODE_FWD && ACC_LDS && FW % 64 == 0),
+                  "k split: LDS-accumulating forward, whole wavefronts per part");
     const int nd = a.n - NC;
     const int bs = kblock_size<NC>(nd);
-    cd* sAcc = lds;                   // [M2][WG] complex, lane-contiguous
-    double* sAccH = (double*)lds;     // HERM: [M2][WG] reals
-    const int lane = threadIdx.x;
+    cd* sAcc = lds;                   // [M2][FW] complex, frequency-contiguous
+    double* sAccH = (double*)lds;     // HERM: [M2][FW] reals
+    cd* sTr = lds + M2 * FW;          // KS > 1: [FW] sums of the tr G shares
+    const int lane = KS == 1 ? (int)threadIdx.x : (int)threadIdx.x % FW;  // this lane's frequency in the tile
+    // which k of every KS: wave-uniform (FW is a multiple of 64), so that the k
+    // walk and the per-k block addresses stay scalar
+    const int part = KS == 1 ? 0 : __builtin_amdgcn_readfirstlane((int)threadIdx.x / FW);
     const int nw_hi = a.nw_end ? a.nw_end : a.nw_tot;
-    const int nw = a.nw_begin + wtile * WG + lane;
+    const int nw = a.nw_begin + wtile * FW + lane;
     const bool wvalid = nw < nw_hi;
     const int nwc = wvalid ? nw : nw_hi - 1;  // clamp for loads
     const int k0 = chunk * a.k_per_chunk;
@@ -199,13 +209,18 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
     double accH[HERM && !ACC_LDS ? M2 : 1];
     if (HERM) {
         if (ACC_LDS) {
-            for (int e = 0; e < M2; ++e) sAccH[e * WG + lane] = 0.0;
+            for (int e = 0; e < M2; ++e) sAccH[e * FW + lane] = 0.0;
         } else {
 #pragma unroll
             for (int e = 0; e < (HERM && !ACC_LDS ? M2 : 1); ++e) accH[e] = 0.0;
         }
     } else if (ACC_LDS) {
-        for (int e = 0; e < M2; ++e) sAcc[e * WG + lane] = cd{0.0, 0.0};
+        // (KS > 1: every part clears the same entries, before the barrier below)
+        for (int e = 0; e < M2; ++e) sAcc[e * FW + lane] = cd{0.0, 0.0};
+        if constexpr (KS > 1) {
+            sTr[lane] = cd{0.0, 0.0};
+            __syncthreads();
+        }
     } else {
 #pragma unroll
         for (int i = 0; i < (ACC_LDS ? 1 : NC); ++i)
@@ -214,7 +229,11 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
     }
     cd trA = cd{0.0, 0.0}, tr2A = cd{0.0, 0.0}, muA = cd{0.0, 0.0};
 
-    for (int k = k0; k < k1; ++k) {
+    // KS > 1: both parts take the same number of steps (the barriers below); a
+    // part past the chunk's end repeats its last k and adds nothing
+    for (int kb = k0; kb < k1; kb += KS) {
+        const int k = KS == 1 ? kb : min(kb + part, k1 - 1);
+        const bool kin = KS == 1 || kb + part < k1;
         // Sigma(iw) is re-read from the cache every k rather than hoisted out
         // of the loop, where it would hold 4*NC*NC registers for its duration.
         // The k block is read through the constant address space, which the
@@ -385,8 +404,8 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
                     const double h0 = i == j ? G[i][i].x : G[i][j].x + G[j][i].x;
                     const double h1 = G[i][j].y - G[j][i].y;
                     if (ACC_LDS) {
-                        lds_add(&sAccH[q * WG + lane], h0);
-                        if (i != j) lds_add(&sAccH[(q + 1) * WG + lane], h1);
+                        lds_add(&sAccH[q * FW + lane], h0);
+                        if (i != j) lds_add(&sAccH[(q + 1) * FW + lane], h1);
                     } else {
                         accH[q] += h0;
                         if (i != j) accH[q + 1] += h1;
@@ -397,10 +416,22 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
         if (FWDLIKE) {
             if constexpr (HERM) {
             } else if (ACC_LDS) {
+                auto add_g = [&]() {
 #pragma unroll
-                for (int j = 0; j < NC; ++j)
+                    for (int j = 0; j < NC; ++j)
 #pragma unroll
-                    for (int i = 0; i < NC; ++i) lds_add(&sAcc[(i + j * NC) * WG + lane], G[i][j]);
+                        for (int i = 0; i < NC; ++i) lds_add(&sAcc[(i + j * NC) * FW + lane], G[i][j]);
+                };
+                if constexpr (KS > 1) {
+                    // the parts add in turn, part 0 first, so that every sum has one
+                    // order and the result is the same on every run
+                    __syncthreads();
+                    if (part == 0) add_g();
+                    __syncthreads();
+                    if (part == 1 && kin) add_g();
+                } else {
+                    add_g();
+                }
             } else {
 #pragma unroll
                 for (int i = 0; i < (ACC_LDS || HERM ? 1 : NC); ++i)
@@ -448,7 +479,7 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
                         }
                     t = cadd(t, cadd(g, cmul(cmul(g, g), Md)));
                 }
-                if (wvalid) trA = cadd(trA, t);
+                if (wvalid && kin) trA = cadd(trA, t);
             }
         } else {
             // the adjoint's cotangent, loaded all together here so that it arrives
@@ -599,7 +630,7 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
                         cd s = cd{0.0, 0.0};
 #pragma unroll
                         for (int i = 0; i < NC; ++i) s = cfmac(G[i][aa], t[i], s);
-                        lds_add(&sAcc[(aa + b * NC) * WG + lane], s);
+                        lds_add(&sAcc[(aa + b * NC) * FW + lane], s);
                         const cd sp = aa == b ? cadd(Y[aa][b], cd{1.0, 0.0}) : Y[aa][b];
                         ms = cfmac(s, sp, ms);  // S'_ab conj(C_ab)
                     }
@@ -609,6 +640,12 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
         }
     }
 
+    if constexpr (KS > 1) {  // the parts' tr G shares and G_cc sums meet in LDS; part 0 writes them
+        lds_add(&sTr[lane], trA);
+        __syncthreads();
+        if (part != 0) return;
+        trA = sTr[lane];
+    }
     if (!wvalid) return;
     const size_t ch = chunk;
     if (HERM) {
@@ -621,12 +658,12 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
         for (int j = 0; j < NC; ++j)
 #pragma unroll
             for (int i = 0; i <= j; ++i) {
-                const double h0 = ACC_LDS ? sAccH[q * WG + lane] : accH[q];
+                const double h0 = ACC_LDS ? sAccH[q * FW + lane] : accH[q];
                 if (i == j) {
                     out[i + i * NC] = cd{h0, 0.0};
                     q += 1;
                 } else {
-                    const double h1 = ACC_LDS ? sAccH[(q + 1) * WG + lane] : accH[q + 1];
+                    const double h1 = ACC_LDS ? sAccH[(q + 1) * FW + lane] : accH[q + 1];
                     out[i + j * NC] = cd{0.5 * h0, 0.5 * h1};
                     out[j + i * NC] = cd{0.5 * h0, -0.5 * h1};
                     q += 2;
@@ -635,7 +672,7 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
     } else if (MODE == MODE_FWD || MODE == MODE_FWD_REAL) {
         a.pTr[ch * a.nw_tot + nw] = trA;
         if (ACC_LDS) {
-            for (int e = 0; e < M2; ++e) a.pG[(ch * a.nw_tot + nw) * M2 + e] = sAcc[e * WG + lane];
+            for (int e = 0; e < M2; ++e) a.pG[(ch * a.nw_tot + nw) * M2 + e] = sAcc[e * FW + lane];
         } else {
 #pragma unroll
             for (int i = 0; i < (ACC_LDS || HERM ? 1 : NC); ++i)
@@ -648,7 +685,7 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
         a.pTr2[ch * a.nw_tot + nw] = tr2A;
         if (MODE == MODE_VJP && vjp_here) {
             a.pMu[ch * a.n_freq + nw] = muA;
-            for (int e = 0; e < M2; ++e) a.pSb[(ch * a.n_freq + nw) * M2 + e] = sAcc[e * WG + lane];
+            for (int e = 0; e < M2; ++e) a.pSb[(ch * a.n_freq + nw) * M2 + e] = sAcc[e * FW + lane];
         }
     }
 }
@@ -659,12 +696,12 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
 template <int NC, int MODE, int REFINE>
 constexpr int schur_waves() { return (NC == 5 && MODE != MODE_VJP && REFINE <= 1) ? 2 : 1; }
 
-template <int NC, int MODE, bool F32R, int WG, bool TRACE, int REFINE>
+template <int NC, int MODE, bool F32R, int WG, bool TRACE, int REFINE, int KS = 1>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(schur_waves<NC, MODE, REFINE>())))
 void k_lattice_schur(LatArgs a, const cd* __restrict__ sig_all)
 {
     extern __shared__ cd lds[];
-    schur_item<NC, MODE, F32R, WG, TRACE, REFINE>(a, sig_all, blockIdx.x, blockIdx.y, lds);
+    schur_item<NC, MODE, F32R, WG, TRACE, REFINE, KS>(a, sig_all, blockIdx.x, blockIdx.y, lds);
 }
 
 // Persistent form over a work list spanning several lattices of the same n_c:
@@ -685,19 +722,20 @@ void k_lattice_schur_multi(const LatArgs* __restrict__ args,
     }
 }
 
-template <int NC, int MODE, bool F32R, int WG>
+template <int NC, int MODE, bool F32R, int WG, int KS = 1>
 constexpr size_t schur_shm()
 {
     if constexpr (MODE == MODE_FWD_HERM) return F32R ? sizeof(double) * (size_t)NC * NC * WG : 0;
     if constexpr (MODE == MODE_DENS_HERM) return sizeof(double) * (size_t)NC * NC * WG;
     constexpr bool ACC_LDS = (MODE == MODE_VJP) || ((MODE == MODE_FWD || MODE == MODE_FWD_REAL) && F32R);
-    return ACC_LDS ? sizeof(cd) * (size_t)NC * NC * WG : 0;
+    // KS > 1: one accumulator set per frequency, plus the tr G sums
+    return ACC_LDS ? sizeof(cd) * (size_t)(NC * NC + (KS > 1 ? 1 : 0)) * (WG / KS) : 0;
 }
 
-template <int NC, int MODE, bool F32R, bool TRACE, int REFINE, int WG>
+template <int NC, int MODE, bool F32R, bool TRACE, int REFINE, int WG, int KS = 1>
 hipError_t launch_nc_w(const LatArgs& a, const cd* sig_all, hipStream_t s, const SchurMulti* multi)
 {
-    constexpr size_t shm = schur_shm<NC, MODE, F32R, WG>();
+    constexpr size_t shm = schur_shm<NC, MODE, F32R, WG, KS>();
     if (multi) {
         hipError_t e = hipFuncSetAttribute((const void*)k_lattice_schur_multi<NC, MODE, F32R, WG, TRACE, REFINE>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
@@ -706,12 +744,13 @@ hipError_t launch_nc_w(const LatArgs& a, const cd* sig_all, hipStream_t s, const
                            s, multi->args, multi->items, multi->n_items);
         return hipGetLastError();
     }
+    constexpr int FW = WG / KS;  // frequencies per workgroup
     const int span = (a.nw_end ? a.nw_end : a.nw_tot) - a.nw_begin;
-    dim3 grid((span + WG - 1) / WG, a.n_chunks);
-    hipError_t e = hipFuncSetAttribute((const void*)k_lattice_schur<NC, MODE, F32R, WG, TRACE, REFINE>,
+    dim3 grid((span + FW - 1) / FW, a.n_chunks);
+    hipError_t e = hipFuncSetAttribute((const void*)k_lattice_schur<NC, MODE, F32R, WG, TRACE, REFINE, KS>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_lattice_schur<NC, MODE, F32R, WG, TRACE, REFINE>), grid, dim3(WG), shm, s, a, sig_all);
+    hipLaunchKernelGGL((k_lattice_schur<NC, MODE, F32R, WG, TRACE, REFINE, KS>), grid, dim3(WG), shm, s, a, sig_all);
     return hipGetLastError();
 }
 
@@ -722,6 +761,9 @@ hipError_t launch_nc_r(const LatArgs& a, const cd* sig_all, hipStream_t s, int w
         (void)wg;
         return launch_nc_w<NC, MODE, F32R, TRACE, REFINE, 128>(a, sig_all, s, m);
     } else {
+        if constexpr (MODE == MODE_FWD && F32R && NC >= 5) {  // two wavefronts over 64 frequencies, single lattice
+            if (wg == SCHUR_WG_KSPLIT && !m) return launch_nc_w<NC, MODE, F32R, TRACE, REFINE, 128, 2>(a, sig_all, s, m);
+        }
         if (wg == 128) return launch_nc_w<NC, MODE, F32R, TRACE, REFINE, 128>(a, sig_all, s, m);
         return launch_nc_w<NC, MODE, F32R, TRACE, REFINE, 64>(a, sig_all, s, m);
     }

@@ -78,7 +78,9 @@ const char* dmftgpu_version(void);
                                            32768 for the one-matrix-per-lane Schur kernels, keeping >= 64 k per
                                            chunk and >= 8192 items; 8192 for the others) */
 #define DMFTGPU_OPT_SCHUR_WG 3          /* frequencies per workgroup of the Schur kernels (64 or 128; default 0 =
-                                           64 for the forward, 128 for the adjoint) */
+                                           the adjoint at 128, the FP32R forward over the first N_freq
+                                           frequencies at n_corr >= 5 as two wavefronts over 64 frequencies
+                                           splitting the k walk, the others at 64) */
 #define DMFTGPU_OPT_SCHUR_LAYOUT 4      /* n_c <= 6: 0 = one (k,iw) matrix per lane (default), 1 = one column
                                            per lane (always used for n_c > 6) */
 #define DMFTGPU_OPT_UPFOLD_CHUNK 5      /* k points per chunk of dmftgpu_upfold_density (0 = as many as 512 MiB of

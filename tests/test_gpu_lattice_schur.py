@@ -237,3 +237,33 @@ def test_k_chunking_does_not_change_results(gpu, method):
         assert abs(r["n_total"] - out[0]["n_total"]) <= 1e-12
         assert rel(r["sigma_bar"], out[0]["sigma_bar"]) <= 1e-12
         assert abs(r["mu_bar"] - out[0]["mu_bar"]) <= 1e-12 * max(1.0, abs(out[0]["mu_bar"]))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("target", [0, 7, 126])
+def test_ksplit_forward_matches_one_wavefront(gpu, target):
+    """The FP32R forward over the first N_freq frequencies at n_c >= 5 runs by
+    default as two wavefronts splitting each k chunk (every other k) into one set
+    of LDS sums, added in turn; an explicit width of 64 runs it as one wavefront.
+    The G_cc sums are then taken in the same k order, so G_loc and NumMatrix agree
+    to rounding and repeat exactly; N differs only in the order of its tr G sums.
+    Targets: default chunking, one chunk of 125 k (odd), 3 k per chunk (odd, and
+    a last chunk of 2)."""
+    w = L.workload("dp14", seed=5, nk_side=5, n_freq=96)
+    Hk, sig, beta = w["Hk"], w["sigma"], w["beta"]
+    lat = gpu.LatticeGF(Hk, 5, 96, beta, w["exact_moments"])
+    lat.set_option(gpu.capi.OPT_TARGET_WORKGROUPS, target)
+    split = lat.greenftn_w(sig, 0.2, method=gpu.SCHUR_F32R)
+    again = lat.greenftn_w(sig, 0.2, method=gpu.SCHUR_F32R)
+    lat.set_option(gpu.capi.OPT_SCHUR_WG, 64)
+    one = lat.greenftn_w(sig, 0.2, method=gpu.SCHUR_F32R)
+    lat.set_option(gpu.capi.OPT_SCHUR_WG, 0)
+    ref = lat.greenftn_w(sig, 0.2, method=gpu.SCHUR_F64)
+    lat.close()
+    print(f"target {target}: G_loc bitwise equal to one wavefront: {np.array_equal(split['gloc'], one['gloc'])}",
+          flush=True)
+    assert np.array_equal(split["gloc"], again["gloc"]) and split["n_total"] == again["n_total"]
+    assert rel(split["gloc"], one["gloc"]) <= 1e-14
+    assert rel(split["num_matrix"], one["num_matrix"]) <= 1e-14
+    assert abs(split["n_total"] - one["n_total"]) <= 1e-13 * max(1.0, abs(one["n_total"]))
+    assert np.max(np.abs(split["gloc"] - ref["gloc"])) <= 1e-9
