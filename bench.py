@@ -321,7 +321,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": WORKLOADS[args.workload],
                    "n": n, "n_corr": nc, "n_k": nk, "n_freq": nf, "beta": beta, "method": args.method,
-                   "refine_steps": args.refine, "schur_layout": args.layout, "schur_wg": args.wg or "default (adjoint 128, forward 64, FP32R forward below N_freq k-split)",
+                   "refine_steps": args.refine, "schur_layout": args.layout, "schur_wg": args.wg or "default (adjoint 128, forward 64, FP32R forward below N_freq in k pairs)",
                    "target_workgroups": args.target or "default",
                    "units_per_step": units,
                    "step": "tail-moment fit of Sigma + static eig for the tail + GreenFtn_w forward (G_loc, NumMatrix, "

@@ -174,7 +174,7 @@ hipError_t launch_density_matrix_lr(const LatArgs& a, const cd* W, const cd* Vd,
 // Schur complement (FP64 or FP32 + refinement); wg = workgroup width of the
 // one-matrix-per-lane kernels: 64 or 128 frequencies, or 0 for the default of
 // each launch (the FP32R forward over the first n_freq frequencies in
-// SCHUR_WG_KSPLIT form, the adjoint at 128, the others at 64); cols selects the
+// SCHUR_WG_KPAIR form, the adjoint at 128, the others at 64); cols selects the
 // column-per-lane kernels for n_c <= 6 (always used above 6)
 // fused (one-matrix-per-lane kernels, schur_can_fuse): a forward that also
 // writes tr G^2 (a.pTr2) beyond n_freq, and the adjoint at the same (Sigma, mu)
@@ -182,7 +182,7 @@ hipError_t launch_density_matrix_lr(const LatArgs& a, const cd* W, const cd* Vd,
 // wg value of the FP32R forward kernel as two wavefronts over 64 frequencies,
 // each walking every other k of the chunk into shared LDS accumulators (half
 // the LDS per wavefront of the 64-lane form; single-lattice launches only)
-constexpr int SCHUR_WG_KSPLIT = 1;
+constexpr int SCHUR_WG_KPAIR = 1;
 hipError_t launch_lattice_schur(const LatArgs& a, int mode, bool fp32, hipStream_t s, int wg = 64, bool cols = false,
                                 bool fused = false);
 bool schur_can_fuse(int nc, bool cols, bool real_axis);

@@ -68,7 +68,7 @@ hipError_t launch_lattice_schur(const LatArgs& a, int mode, bool fp32, hipStream
     // per 64 frequencies, then allow the 8 wavefronts per CU that its registers
     // allow, against 6 at n_c = 5 and 4 at n_c = 6), the other launches at 64
     const int wg_call = wg ? wg : mode == MODE_VJP ? 128 : 64;
-    const int wg_lo = wg ? wg : mode == MODE_FWD && fp32 && a.nc >= 5 ? SCHUR_WG_KSPLIT : wg_call;
+    const int wg_lo = wg ? wg : mode == MODE_FWD && fp32 && a.nc >= 5 ? SCHUR_WG_KPAIR : wg_call;
     hipError_t e = launch_sigma_all(a, s);
     if (e != hipSuccess) return e;
     if (a.real_axis && mode == MODE_FWD && a.nc <= 6 && !cols)

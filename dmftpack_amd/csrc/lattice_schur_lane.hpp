@@ -173,7 +173,7 @@ __device__ __forceinline__ void schur_item(const LatArgs& a, const cd* __restric
     constexpr bool ACC_LDS = (MODE == MODE_VJP) || (MODE == MODE_DENS_HERM) || (FWDLIKE && F32R);
     constexpr int FW = WG / KS;       // frequencies per workgroup
     static_assert(KS == 1 || (KS == 2 && MODE == MODE_FWD && ACC_LDS && FW % 64 == 0),
-                  "k split: LDS-accumulating forward, whole wavefronts per part");
+                  "k pairs: LDS-accumulating forward, whole wavefronts per part");
     const int nd = a.n - NC;
     const int bs = kblock_size<NC>(nd);
     cd* sAcc = lds;                   // [M2][FW] complex, frequency-contiguous
@@ -762,7 +762,7 @@ hipError_t launch_nc_r(const LatArgs& a, const cd* sig_all, hipStream_t s, int w
         return launch_nc_w<NC, MODE, F32R, TRACE, REFINE, 128>(a, sig_all, s, m);
     } else {
         if constexpr (MODE == MODE_FWD && F32R && NC >= 5) {  // two wavefronts over 64 frequencies, single lattice
-            if (wg == SCHUR_WG_KSPLIT && !m) return launch_nc_w<NC, MODE, F32R, TRACE, REFINE, 128, 2>(a, sig_all, s, m);
+            if (wg == SCHUR_WG_KPAIR && !m) return launch_nc_w<NC, MODE, F32R, TRACE, REFINE, 128, 2>(a, sig_all, s, m);
         }
         if (wg == 128) return launch_nc_w<NC, MODE, F32R, TRACE, REFINE, 128>(a, sig_all, s, m);
         return launch_nc_w<NC, MODE, F32R, TRACE, REFINE, 64>(a, sig_all, s, m);

@@ -241,7 +241,7 @@ def test_k_chunking_does_not_change_results(gpu, method):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("target", [0, 7, 126])
-def test_ksplit_forward_matches_one_wavefront(gpu, target):
+def test_kpair_forward_matches_one_wavefront(gpu, target):
     """The FP32R forward over the first N_freq frequencies at n_c >= 5 runs by
     default as two wavefronts splitting each k chunk (every other k) into one set
     of LDS sums, added in turn; an explicit width of 64 runs it as one wavefront.
