@@ -49,10 +49,13 @@ __host__ __device__ constexpr int cols_nc_min(int N)
 // wavefronts share nothing but the cache
 __host__ __device__ constexpr int cols_wg(int, int) { return 64; }
 
-// per-slot LDS (cd units): bc 2N (GJ broadcast), gsh 32 (g_d), MA N*N, MB N*N, GB N*N
+// per-slot LDS (cd units): bc 2N (GJ broadcast), gsh 32 (g_d), MA N*N, MB N*N, GB N*N.
+// The FP32R forward keeps only X0 (FP32) in MB: N*N/2 cd, so that at N = 10 a
+// wavefront needs 22.6 KB instead of 28.2 KB and 7 fit in a CU's LDS instead of 5.
 __host__ __device__ constexpr int cols_slot_size(int N, int mode, int prec)
 {
-    return 2 * N + 32 + ((prec > 0 || mode != MODE_FWD) ? 2 * N * N : 0) + (mode == MODE_VJP ? N * N : 0);
+    const int mb = mode == MODE_FWD ? (prec > 0 ? (N * N + 1) / 2 : 0) : N * N;
+    return 2 * N + 32 + ((prec > 0 || mode != MODE_FWD) ? N * N + mb : 0) + (mode == MODE_VJP ? N * N : 0);
 }
 
 __host__ __device__ constexpr int cols_slots(int N, int mode, int prec) { return cols_wg(mode, prec) / 64 * (64 / N); }
