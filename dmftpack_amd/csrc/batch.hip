@@ -219,12 +219,14 @@ int build_group(dmftgpu_batch_s* b, Group& g, int mode, int wg, double target)
             args_hi.push_back(a);  // unused placeholder keeps the system index
         }
         args.push_back(a);
-        const int per_tile = g.cols ? dg::schur_slots(s.nc, mode, b->method == DMFTGPU_SCHUR_F32R, true) : wg;
+        // (the k-pair form covers 64 frequencies per item; the upper launch runs at 64)
+        const int wf = wg == dg::SCHUR_WG_KPAIR ? 64 : wg;
+        const int per_tile = g.cols ? dg::schur_slots(s.nc, mode, b->method == DMFTGPU_SCHUR_F32R, true) : wf;
         const int nwt = ((split ? s.nf : s.nw_tot) + per_tile - 1) / per_tile;
         for (int c = 0; c < chunks; ++c)
             for (int t = 0; t < nwt; ++t) items.push_back(int4{(int)q, t, c, 0});
         if (split && !(mode == MODE_VJP && b->vjp)) {  // fused adjoint: the forward did these
-            const int nwh = (s.nw_tot - s.nf + wg - 1) / wg;
+            const int nwh = (s.nw_tot - s.nf + wf - 1) / wf;
             for (int c = 0; c < chunks; ++c)
                 for (int t = 0; t < nwh; ++t) items_hi.push_back(int4{(int)q, t, c, 0});
         }
@@ -292,7 +294,8 @@ int launch_group(dmftgpu_batch_s* b, Group& g, int mode)
         mh.items = g.ditems_hi[slot(mode)];
         mh.n_items = g.n_items_hi[slot(mode)];
         mh.grid = std::min(mh.n_items, 256 * 16);
-        HIPCHK(dg::launch_lattice_schur_multi(proto, g.mode_hi[slot(mode)], fp32, h->stream, g.wg[slot(mode)], mh));
+        const int wg_hi = g.wg[slot(mode)] == dg::SCHUR_WG_KPAIR ? 64 : g.wg[slot(mode)];
+        HIPCHK(dg::launch_lattice_schur_multi(proto, g.mode_hi[slot(mode)], fp32, h->stream, wg_hi, mh));
     }
     return DMFTGPU_OK;
 }
@@ -383,11 +386,17 @@ int issue_step(dmftgpu_batch_s* b)
     return DMFTGPU_OK;
 }
 
+// the k-pair form exists for the FP32R forward of the one-matrix-per-lane kernels at n_c >= 5
+bool kpair_ok(const dmftgpu_batch_s* b, const Group& g, int mode)
+{
+    return mode == MODE_FWD && b->method == DMFTGPU_SCHUR_F32R && !g.cols && g.nc >= 5;
+}
+
 // times one mode's group launch for each candidate and keeps the fastest
 int tune_group(dmftgpu_batch_s* b, Group& g, int mode)
 {
     dmftgpu_handle h = b->h;
-    const int wgs[2] = {64, 128};
+    const int wgs[3] = {64, 128, dg::SCHUR_WG_KPAIR};
     const double targets[4] = {6.25e4, 2.5e5, 1.0e6, 4.0e6};
     struct Events {  // destroyed on every return path
         hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -406,6 +415,7 @@ int tune_group(dmftgpu_batch_s* b, Group& g, int mode)
     for (int wg : wgs)
         for (double t : targets) {
             if (g.cols && wg != 64) continue;  // one width: a wavefront per workgroup
+            if (wg == dg::SCHUR_WG_KPAIR && !kpair_ok(b, g, mode)) continue;
             int rc = build_group(b, g, mode, wg, t);
             if (rc) return rc;
             rc = launch_group(b, g, mode);  // warm-up
@@ -433,7 +443,7 @@ int tune_group(dmftgpu_batch_s* b, Group& g, int mode)
 int tune_group_pair(dmftgpu_batch_s* b, Group& g)
 {
     dmftgpu_handle h = b->h;
-    const int wgs[2] = {64, 128};
+    const int wgs[3] = {64, 128, dg::SCHUR_WG_KPAIR};
     const double targets[4] = {6.25e4, 2.5e5, 1.0e6, 4.0e6};
     struct Events {  // destroyed on every return path
         hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -456,6 +466,7 @@ int tune_group_pair(dmftgpu_batch_s* b, Group& g)
             int bw = 64;
             for (int wg : wgs) {
                 if (g.cols && wg != 64) continue;  // one width: a wavefront per workgroup
+                if (wg == dg::SCHUR_WG_KPAIR && !kpair_ok(b, g, mode)) continue;
                 int rc = build_group(b, g, mode, wg, t);
                 if (rc) return rc;
                 rc = launch_group(b, g, mode);  // warm-up
@@ -645,7 +656,8 @@ static int batch_create_impl(dmftgpu_batch_s* b, int count, const dmftgpu_system
     for (Group& g : b->groups) {
         for (int mode : {MODE_FWD, MODE_VJP}) {
             if (mode == MODE_VJP && !b->vjp) continue;
-            rc = (flags & DMFTGPU_BATCH_TUNE) ? 0 : build_group(b, g, mode, 64, 1.0e6);
+            const int wg0 = kpair_ok(b, g, mode) ? dg::SCHUR_WG_KPAIR : 64;  // as the single-lattice default
+            rc = (flags & DMFTGPU_BATCH_TUNE) ? 0 : build_group(b, g, mode, wg0, 1.0e6);
             if (rc) return rc;
         }
     }

@@ -179,9 +179,10 @@ hipError_t launch_density_matrix_lr(const LatArgs& a, const cd* W, const cd* Vd,
 // fused (one-matrix-per-lane kernels, schur_can_fuse): a forward that also
 // writes tr G^2 (a.pTr2) beyond n_freq, and the adjoint at the same (Sigma, mu)
 // that then covers the first n_freq frequencies only (same k chunking).
-// wg value of the FP32R forward kernel as two wavefronts over 64 frequencies,
-// each walking every other k of the chunk into shared LDS accumulators (half
-// the LDS per wavefront of the 64-lane form; single-lattice launches only)
+// wg value of the FP32R forward kernel (n_c >= 5) as two wavefronts over 64
+// frequencies, each walking every other k of the chunk into shared LDS
+// accumulators (half the LDS per wavefront of the 64-lane form); its work items
+// cover 64 frequencies
 constexpr int SCHUR_WG_KPAIR = 1;
 hipError_t launch_lattice_schur(const LatArgs& a, int mode, bool fp32, hipStream_t s, int wg = 64, bool cols = false,
                                 bool fused = false);

@@ -708,7 +708,7 @@ void k_lattice_schur(LatArgs a, const cd* __restrict__ sig_all)
 // workgroup b takes items b, b + gridDim.x, ... (static striding, no atomics,
 // so every replay does the same work in the same order).  Each system's per-k
 // blocks come from its LatArgs (kdata), read through the constant address space.
-template <int NC, int MODE, bool F32R, int WG, bool TRACE, int REFINE>
+template <int NC, int MODE, bool F32R, int WG, bool TRACE, int REFINE, int KS = 1>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(schur_waves<NC, MODE, REFINE>())))
 void k_lattice_schur_multi(const LatArgs* __restrict__ args,
                                                             const int4* __restrict__ items, int n_items)
@@ -717,7 +717,7 @@ void k_lattice_schur_multi(const LatArgs* __restrict__ args,
     for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
         const int4 item = items[it];  // {system, wtile, chunk, -}
         const LatArgs a = args[item.x];
-        schur_item<NC, MODE, F32R, WG, TRACE, REFINE>(a, a.pSigAll, item.y, item.z, lds);
+        schur_item<NC, MODE, F32R, WG, TRACE, REFINE, KS>(a, a.pSigAll, item.y, item.z, lds);
         __syncthreads();
     }
 }
@@ -737,11 +737,11 @@ hipError_t launch_nc_w(const LatArgs& a, const cd* sig_all, hipStream_t s, const
 {
     constexpr size_t shm = schur_shm<NC, MODE, F32R, WG, KS>();
     if (multi) {
-        hipError_t e = hipFuncSetAttribute((const void*)k_lattice_schur_multi<NC, MODE, F32R, WG, TRACE, REFINE>,
+        hipError_t e = hipFuncSetAttribute((const void*)k_lattice_schur_multi<NC, MODE, F32R, WG, TRACE, REFINE, KS>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL((k_lattice_schur_multi<NC, MODE, F32R, WG, TRACE, REFINE>), dim3(multi->grid), dim3(WG), shm,
-                           s, multi->args, multi->items, multi->n_items);
+        hipLaunchKernelGGL((k_lattice_schur_multi<NC, MODE, F32R, WG, TRACE, REFINE, KS>), dim3(multi->grid), dim3(WG),
+                           shm, s, multi->args, multi->items, multi->n_items);
         return hipGetLastError();
     }
     constexpr int FW = WG / KS;  // frequencies per workgroup
@@ -761,8 +761,8 @@ hipError_t launch_nc_r(const LatArgs& a, const cd* sig_all, hipStream_t s, int w
         (void)wg;
         return launch_nc_w<NC, MODE, F32R, TRACE, REFINE, 128>(a, sig_all, s, m);
     } else {
-        if constexpr (MODE == MODE_FWD && F32R && NC >= 5) {  // two wavefronts over 64 frequencies, single lattice
-            if (wg == SCHUR_WG_KPAIR && !m) return launch_nc_w<NC, MODE, F32R, TRACE, REFINE, 128, 2>(a, sig_all, s, m);
+        if constexpr (MODE == MODE_FWD && F32R && NC >= 5) {  // two wavefronts over 64 frequencies
+            if (wg == SCHUR_WG_KPAIR) return launch_nc_w<NC, MODE, F32R, TRACE, REFINE, 128, 2>(a, sig_all, s, m);
         }
         if (wg == 128) return launch_nc_w<NC, MODE, F32R, TRACE, REFINE, 128>(a, sig_all, s, m);
         return launch_nc_w<NC, MODE, F32R, TRACE, REFINE, 64>(a, sig_all, s, m);

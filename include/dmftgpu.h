@@ -267,7 +267,8 @@ int dmftgpu_batch_create(dmftgpu_handle h, int count, const dmftgpu_system* syst
                          dmftgpu_batch* out);
 int dmftgpu_batch_run(dmftgpu_batch b);
 /* Tuning result of n_corr group `group` (returns the number of groups when out of range):
- * wg[2], target[2], ms[2] per mode (forward, adjoint). */
+ * wg[2], target[2], ms[2] per mode (forward, adjoint); wg = 1 is the FP32R forward's k-pair form
+ * (two wavefronts over 64 frequencies, each taking every other k of a chunk). */
 int dmftgpu_batch_tuning(dmftgpu_batch b, int group, int* nc, int* systems, int* wg, double* target, double* ms);
 int dmftgpu_batch_destroy(dmftgpu_batch b);
 
