@@ -275,52 +275,82 @@ __global__ void k_axis_phases(const double* __restrict__ kphase, int kc, int rma
 }
 
 // k_dm_r with the phase taken apart per axis, exp(+i phi.R) = ex[R1] ey[R2] ez[R3]:
-// a tile of kt k values stages the three matrix elements and the k's axis
-// tables in LDS; two complex products per (R, k) instead of a sincos.
+// a tile of kt k values stages the matrix elements and the k's axis tables in
+// LDS; two complex products per (R, k) instead of a sincos.  A work item is a
+// run of up to 256 consecutive entries of the R list over up to DMR_EM matrix
+// elements (entry q: element slot eslot[q] of the item), so that a workgroup's
+// threads are all busy when each element has few R (27 for R in {-1,0,1}^3).
+// blockIdx.y takes segment y of the chunk's k (segments of kseg k, so that the
+// launch has enough workgroups to fill the GPU); each thread sums its k of the
+// segment in order and stores the sum to part[y][x][entry] for k_dm_r_merge.
+constexpr int DMR_EM = 16;
+struct RItemE {
+    int q0, q1, eb, ne;  // entries [q0, q1), elements ielem[eb .. eb + ne)
+};
 __global__ __launch_bounds__(256) void k_dm_r_axes(const cd* __restrict__ Y0, const cd* __restrict__ Y1,
                                                    const cd* __restrict__ Y2, const cd* __restrict__ tab, int rmax,
-                                                   int kc, int kt, size_t oo, const RItem* __restrict__ items,
-                                                   const int4* __restrict__ rs, int n_r, double inv_nk,
-                                                   cd* __restrict__ dmr)
+                                                   int kc, int kt, size_t oo, const RItemE* __restrict__ items,
+                                                   const int* __restrict__ ielem, const int* __restrict__ eslot,
+                                                   const int4* __restrict__ rs, int n_r, int kseg,
+                                                   cd* __restrict__ part)
 {
     extern __shared__ cd sm[];
+    __shared__ int se[DMR_EM];
     const int nr = 2 * rmax + 1, per = 3 * nr;
-    cd* y = sm;              // [3][kt]
-    cd* ph = sm + 3 * kt;    // [kt][3][nr]
-    const RItem it = items[blockIdx.x];
+    cd* y = sm;                        // [3][DMR_EM][kt]
+    cd* ph = sm + 3 * DMR_EM * kt;     // [kt][3][nr]
+    const RItemE it = items[blockIdx.x];
     const int t = threadIdx.x;
-    const int r = it.r0 + t;
-    const bool live = r < it.r1;
-    const int4 R = rs[live ? r : it.r0];
+    const int q = it.q0 + t;
+    const bool live = q < it.q1;
+    const int4 R = rs[live ? q : it.q0];
+    const int j = eslot[live ? q : it.q0];
+    if (t < it.ne) se[t] = ielem[it.eb + t];
     const int ox = R.x + rmax, oy = nr + R.y + rmax, oz = 2 * nr + R.z + rmax;
+    const cd* y0 = y + (size_t)j * kt;
+    const cd* y1 = y + (size_t)(DMR_EM + j) * kt;
+    const cd* y2 = y + (size_t)(2 * DMR_EM + j) * kt;
     cd a0 = cd{0.0, 0.0}, a1 = a0, a2 = a0;
-    for (int k0 = 0; k0 < kc; k0 += kt) {
-        const int kn = min(kt, kc - k0);
-        for (int q = t; q < kn; q += 256) {
-            const size_t o = (size_t)(k0 + q) * oo + it.e;
-            y[q] = Y0[o];
-            y[kt + q] = Y1[o];
-            y[2 * kt + q] = Y2[o];
+    __syncthreads();
+    const int kb = blockIdx.y * kseg, ke = min(kc, kb + kseg);
+    for (int k0 = kb; k0 < ke; k0 += kt) {
+        const int kn = min(kt, ke - k0);
+        for (int v = t; v < kn * it.ne; v += 256) {  // the item's elements of one k are neighbours in memory
+            const int qk = v / it.ne, jj = v - qk * it.ne;
+            const size_t o = (size_t)(k0 + qk) * oo + se[jj];
+            y[(size_t)jj * kt + qk] = Y0[o];
+            y[(size_t)(DMR_EM + jj) * kt + qk] = Y1[o];
+            y[(size_t)(2 * DMR_EM + jj) * kt + qk] = Y2[o];
         }
-        for (int q = t; q < kn * per; q += 256) ph[q] = tab[(size_t)k0 * per + q];
+        for (int v = t; v < kn * per; v += 256) ph[v] = tab[(size_t)k0 * per + v];
         __syncthreads();
-        for (int q = 0; q < kn; ++q) {
-            const cd* p = ph + q * per;
+        for (int qk = 0; qk < kn; ++qk) {
+            const cd* p = ph + qk * per;
             const cd z = cmul(cmul(p[ox], p[oy]), p[oz]);
-            a0 = cfma(y[q], z, a0);
-            a1 = cfma(y[kt + q], z, a1);
-            a2 = cfma(y[2 * kt + q], z, a2);
+            a0 = cfma(y0[qk], z, a0);
+            a1 = cfma(y1[qk], z, a1);
+            a2 = cfma(y2[qk], z, a2);
         }
         __syncthreads();
     }
     if (!live) return;
-    const int o = R.w;
-    cd* d0 = dmr + o;
-    cd* d1 = dmr + (size_t)n_r + o;
-    cd* d2 = dmr + 2 * (size_t)n_r + o;
-    *d0 = cadd(*d0, cscale(a0, inv_nk));
-    *d1 = cadd(*d1, cscale(a1, inv_nk));
-    *d2 = cadd(*d2, cscale(a2, inv_nk));
+    cd* pp = part + (size_t)blockIdx.y * 3 * n_r + q;
+    pp[0] = a0;
+    pp[n_r] = a1;
+    pp[2 * (size_t)n_r] = a2;
+}
+
+// dmr[x][R.w] += (1/n_k_global) sum over the segments, in order, of part[y][x][entry]
+__global__ void k_dm_r_merge(const cd* __restrict__ part, int nseg, const int4* __restrict__ rs, int n_r,
+                             double inv_nk, cd* __restrict__ dmr)
+{
+    const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= 3 * (size_t)n_r) return;
+    const int x = (int)(gid / n_r), q = (int)(gid % n_r);
+    cd a = part[gid];
+    for (int y = 1; y < nseg; ++y) a = cadd(a, part[(size_t)y * 3 * n_r + gid]);
+    cd* d = dmr + (size_t)x * n_r + rs[q].w;
+    *d = cadd(*d, cscale(a, inv_nk));
 }
 
 __global__ void k_zero(cd* __restrict__ p, size_t count)
@@ -384,12 +414,41 @@ extern "C" int dmftgpu_upfold_density(dmftgpu_handle h, int n, int n_orb, int n_
         for (int a = q; a < q1; a += 256) items.push_back(RItem{(int)e, a, std::min(q1, a + 256), 0});
         q = q1;
     }
+    // the axes kernel's items: runs of <= 256 entries over <= DMR_EM elements
+    std::vector<RItemE> items_e;
+    std::vector<int> ielem, eslot(std::max(n_r, 1), 0);
+    {
+        RItemE cur{0, 0, 0, 0};
+        bool open = false;
+        for (int q = 0; q < n_r;) {
+            const size_t e = elem(order[q]);
+            int q1 = q;
+            while (q1 < n_r && elem(order[q1]) == e) ++q1;
+            for (int a = q; a < q1;) {
+                if (!open || cur.q1 - cur.q0 == 256 || cur.ne == DMR_EM) {
+                    if (open) items_e.push_back(cur);
+                    cur = RItemE{a, a, (int)ielem.size(), 0};
+                    open = true;
+                }
+                const int take = std::min(q1 - a, 256 - (cur.q1 - cur.q0));
+                ielem.push_back((int)e);
+                for (int b = a; b < a + take; ++b) eslot[b] = cur.ne;
+                cur.ne += 1;
+                cur.q1 += take;
+                a += take;
+            }
+            q = q1;
+        }
+        if (open) items_e.push_back(cur);
+    }
     int rmax = 0;  // the R sums take the per-axis phase tables when |R_a| <= 16
     for (int r = 0; r < n_r; ++r)
         for (int c = 0; c < 3; ++c) rmax = std::max(rmax, std::abs(r_idx[5 * (size_t)r + c]));
     const bool axes = n_r > 0 && rmax <= 16;
     const int nrr = 2 * rmax + 1;
-    const int kt = std::max(16, std::min(256, (48 * 1024 / (int)sizeof(cd)) / (3 + 3 * nrr)));
+    const int kt = std::max(16, std::min(256, (48 * 1024 / (int)sizeof(cd)) / (3 * DMR_EM + 3 * nrr)));
+    // k segments of the R sums: about 1024 workgroups per launch, each segment >= 2 tiles
+    const int nseg_max = axes ? std::max(1, std::min(16, (1024 + (int)items_e.size() - 1) / (int)items_e.size())) : 1;
     for (int q = 0; q < n_r; ++q) {
         const int r = order[q];
         rs[4 * (size_t)q] = r_idx[5 * (size_t)r];
@@ -405,16 +464,25 @@ extern "C" int dmftgpu_upfold_density(dmftgpu_handle h, int n, int n_orb, int n_
     const size_t b_inv = align256(inv.size() * sizeof(int));
     const size_t b_rs = align256(rs.size() * sizeof(int));
     const size_t b_items = align256(std::max<size_t>(items.size(), 1) * sizeof(RItem));
+    const size_t b_items_e = align256(std::max<size_t>(items_e.size(), 1) * sizeof(RItemE));
+    const size_t b_ielem = align256(std::max<size_t>(ielem.size(), 1) * sizeof(int));
+    const size_t b_eslot = align256(eslot.size() * sizeof(int));
+    const size_t b_part = axes ? align256((size_t)nseg_max * 3 * n_r * sizeof(cd)) : 0;
+    const size_t b_lists = b_items + b_items_e + b_ielem + b_eslot + b_part;
     const size_t b_flag = 256;
     const size_t b_chunk = align256(per_k * kc_max);
     hipError_t e;
-    char* base = (char*)h->ws.get(b_inv + b_rs + b_items + b_flag + b_chunk, &e);
+    char* base = (char*)h->ws.get(b_inv + b_rs + b_lists + b_flag + b_chunk, &e);
     if (!base) return fail(h, DMFTGPU_ERR_HIP, "workspace: %s", hipGetErrorString(e));
     int* d_inv = (int*)base;
     int4* d_rs = (int4*)(base + b_inv);
     RItem* d_items = (RItem*)(base + b_inv + b_rs);
-    int* d_flag = (int*)(base + b_inv + b_rs + b_items);
-    cd* W1 = (cd*)(base + b_inv + b_rs + b_items + b_flag);
+    RItemE* d_items_e = (RItemE*)(base + b_inv + b_rs + b_items);
+    int* d_ielem = (int*)(base + b_inv + b_rs + b_items + b_items_e);
+    int* d_eslot = (int*)(base + b_inv + b_rs + b_items + b_items_e + b_ielem);
+    cd* d_part = (cd*)(base + b_inv + b_rs + b_items + b_items_e + b_ielem + b_eslot);
+    int* d_flag = (int*)(base + b_inv + b_rs + b_lists);
+    cd* W1 = (cd*)(base + b_inv + b_rs + b_lists + b_flag);
     cd* W2 = W1 + oo * kc_max;
     cd* Eb = W2 + oo * kc_max;
     cd* Dw = Eb + oo * kc_max;
@@ -426,6 +494,9 @@ extern "C" int dmftgpu_upfold_density(dmftgpu_handle h, int n, int n_orb, int n_
     if (n_r > 0) {
         HIPCHK(hipMemcpyAsync(d_rs, rs.data(), rs.size() * sizeof(int), hipMemcpyHostToDevice, s));
         HIPCHK(hipMemcpyAsync(d_items, items.data(), items.size() * sizeof(RItem), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d_items_e, items_e.data(), items_e.size() * sizeof(RItemE), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d_ielem, ielem.data(), ielem.size() * sizeof(int), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d_eslot, eslot.data(), eslot.size() * sizeof(int), hipMemcpyHostToDevice, s));
         const size_t cnt = 3 * (size_t)n_r;
         hipLaunchKernelGGL(k_zero, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, (cd*)dmr, cnt);
         HIPCHK(hipGetLastError());
@@ -467,9 +538,16 @@ extern "C" int dmftgpu_upfold_density(dmftgpu_handle h, int n, int n_orb, int n_
             hipLaunchKernelGGL(k_axis_phases, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s,
                                kphase + 3 * (size_t)k0, kc, rmax, Ptab);
             HIPCHK(hipGetLastError());
-            const size_t shm = sizeof(cd) * ((size_t)3 * kt + (size_t)kt * 3 * nrr);
-            hipLaunchKernelGGL(k_dm_r_axes, dim3((unsigned)items.size()), dim3(256), shm, s, DD, DDir, (const cd*)D,
-                               (const cd*)Ptab, rmax, kc, kt, oo, d_items, d_rs, n_r, inv_nk, (cd*)dmr);
+            const size_t shm = sizeof(cd) * ((size_t)3 * DMR_EM * kt + (size_t)kt * 3 * nrr);
+            const int nseg = std::max(1, std::min(nseg_max, kc / (2 * kt)));
+            const int kseg = (kc + nseg - 1) / nseg;
+            hipLaunchKernelGGL(k_dm_r_axes, dim3((unsigned)items_e.size(), (unsigned)nseg), dim3(256), shm, s, DD,
+                               DDir, (const cd*)D, (const cd*)Ptab, rmax, kc, kt, oo, d_items_e, d_ielem, d_eslot,
+                               d_rs, n_r, kseg, d_part);
+            HIPCHK(hipGetLastError());
+            const size_t cnt = 3 * (size_t)n_r;
+            hipLaunchKernelGGL(k_dm_r_merge, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, (const cd*)d_part,
+                               nseg, d_rs, n_r, inv_nk, (cd*)dmr);
             HIPCHK(hipGetLastError());
         } else if (n_r > 0) {
             hipLaunchKernelGGL(k_dm_r, dim3((unsigned)items.size()), dim3(256), 0, s, DD, DDir, (const cd*)D,

@@ -44,6 +44,34 @@ def test_upfold_vs_oracle(gpu, n, n_orb, side, overlap):
         assert rel(dmr[x], dmr_r[x]) <= 1e-12, x
 
 
+def uneven_r_idx(n_orb, seed):
+    """An R list of uneven length per matrix element, in shuffled order: element
+    (0, 0) with all 343 R of |R_a| <= 3 (more than one 256-entry work item of the
+    R sums), the others with 1 to 5 random R of |R_a| <= 2 (up to 16 elements share
+    a work item)."""
+    rng = np.random.default_rng(seed)
+    rows = [(a, b, d, 0, 0) for a in range(-3, 4) for b in range(-3, 4) for d in range(-3, 4)]
+    for i0 in range(n_orb):
+        for m0 in range(n_orb):
+            if (i0, m0) != (0, 0):
+                for _ in range(1 + (i0 * n_orb + m0) % 5):
+                    rows.append((*(int(x) for x in rng.integers(-2, 3, 3)), i0, m0))
+    rows = np.array(rows, dtype=np.int32)
+    return rows[rng.permutation(len(rows))]
+
+
+@pytest.mark.gpu
+def test_upfold_uneven_r_lists(gpu):
+    G, h = gpu
+    c = L.upfold_case(3, 7, 3, 41)
+    c["r_idx"] = uneven_r_idx(7, 5)
+    print(f"upfold uneven R lists: n_r={len(c['r_idx'])}", flush=True)
+    (D, dmr), (Dr, dmr_r) = both(h, c)
+    assert rel(D, Dr) <= 1e-12
+    for x in range(3):
+        assert rel(dmr[x], dmr_r[x]) <= 1e-12, x
+
+
 @pytest.mark.gpu
 def test_upfold_round_trip_identity(gpu):
     """Identity bases on an 8^3 mesh: the R sums of H(k) built on the GPU from a
