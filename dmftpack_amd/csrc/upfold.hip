@@ -113,24 +113,50 @@ __global__ __launch_bounds__(256) void k_gj_inverse(int N, cd* A, int* singular)
         M = mlds;
         __syncthreads();
     }
+    // the elimination's (row, column) walk: thread t keeps row t % N and takes every
+    // (256 / N)-th column from t / N (N <= 256), no division per element
+    const int cpp = N <= 256 ? 256 / N : 0;
+    const int ti = N <= 256 ? t % N : 0, tc = N <= 256 ? t / N : 0;
     for (int j = 0; j < N; ++j) {
-        double best = -1.0;
-        int bi = j;
-        for (int i = j + t; i < N; i += 256) {
-            const cd a = M[i + (size_t)j * N];
-            const double m = fma(a.x, a.x, a.y * a.y);
-            if (m > best) { best = m; bi = i; }
-        }
-        rmax[t] = best;
-        ridx[t] = bi;
-        __syncthreads();
-        for (int w = 128; w > 0; w >>= 1) {
-            if (t < w) {
-                const double o = rmax[t + w];
-                const int oi = ridx[t + w];
-                if (o > rmax[t] || (o == rmax[t] && oi < ridx[t])) { rmax[t] = o; ridx[t] = oi; }
+        if (N <= 64) {  // every candidate row in wavefront 0: a wavefront reduction, one barrier
+            if (t < 64) {
+                double best = -1.0;
+                int bi = j;
+                if (j + t < N) {
+                    const cd a = M[j + t + (size_t)j * N];
+                    best = fma(a.x, a.x, a.y * a.y);
+                    bi = j + t;
+                }
+                for (int off = 32; off > 0; off >>= 1) {
+                    const double o = __shfl_xor(best, off);
+                    const int oi = __shfl_xor(bi, off);
+                    if (o > best || (o == best && oi < bi)) { best = o; bi = oi; }
+                }
+                if (t == 0) {
+                    rmax[0] = best;
+                    ridx[0] = bi;
+                }
             }
             __syncthreads();
+        } else {
+            double best = -1.0;
+            int bi = j;
+            for (int i = j + t; i < N; i += 256) {
+                const cd a = M[i + (size_t)j * N];
+                const double m = fma(a.x, a.x, a.y * a.y);
+                if (m > best) { best = m; bi = i; }
+            }
+            rmax[t] = best;
+            ridx[t] = bi;
+            __syncthreads();
+            for (int w = 128; w > 0; w >>= 1) {
+                if (t < w) {
+                    const double o = rmax[t + w];
+                    const int oi = ridx[t + w];
+                    if (o > rmax[t] || (o == rmax[t] && oi < ridx[t])) { rmax[t] = o; ridx[t] = oi; }
+                }
+                __syncthreads();
+            }
         }
         const int p = ridx[0];
         const bool zero = !(rmax[0] > 0.0);
@@ -151,11 +177,22 @@ __global__ __launch_bounds__(256) void k_gj_inverse(int N, cd* A, int* singular)
         __syncthreads();
         for (int c = t; c < N; c += 256) M[j + (size_t)c * N] = (c == j) ? r : cmul(M[j + (size_t)c * N], r);
         __syncthreads();
-        for (size_t e = t; e < (size_t)N * N; e += 256) {
-            const int i = (int)(e % N), c = (int)(e / N);
-            if (i == j) continue;
-            const cd base = (c == j) ? cd{0.0, 0.0} : M[e];
-            M[e] = cfms(col[i], M[j + (size_t)c * N], base);
+        if (N <= 256) {
+            if (ti != j && tc < cpp) {
+                const cd ci = col[ti];
+                for (int c = tc; c < N; c += cpp) {
+                    const size_t e = ti + (size_t)c * N;
+                    const cd base = (c == j) ? cd{0.0, 0.0} : M[e];
+                    M[e] = cfms(ci, M[j + (size_t)c * N], base);
+                }
+            }
+        } else {
+            for (size_t e = t; e < (size_t)N * N; e += 256) {
+                const int i = (int)(e % N), c = (int)(e / N);
+                if (i == j) continue;
+                const cd base = (c == j) ? cd{0.0, 0.0} : M[e];
+                M[e] = cfms(col[i], M[j + (size_t)c * N], base);
+            }
         }
         __syncthreads();
     }
