@@ -96,27 +96,28 @@ hipError_t zgemm_sq(int N, int batch, const cd* A, const cd* B, cd* C, hipStream
 // on the columns at the end.  One workgroup per matrix; the matrix stays in
 // global memory (the workgroup's barriers order its own loads and stores).
 // A zero pivot sets *singular.  IN_LDS: the matrix is staged in LDS for the
-// elimination (N <= 64), the same steps in the same order.
-template <bool IN_LDS>
-__global__ __launch_bounds__(256) void k_gj_inverse(int N, cd* A, int* singular)
+// elimination (N <= 64, one wavefront of BS = 64 per matrix), the same steps in
+// the same order.
+template <bool IN_LDS, int BS>
+__global__ __launch_bounds__(BS) void k_gj_inverse(int N, cd* A, int* singular)
 {
-    __shared__ cd col[UPF_MAX_ORB];
-    __shared__ int piv[UPF_MAX_ORB];
-    __shared__ double rmax[256];
-    __shared__ int ridx[256];
+    __shared__ cd col[IN_LDS ? 64 : UPF_MAX_ORB];  // IN_LDS: N <= 64
+    __shared__ int piv[IN_LDS ? 64 : UPF_MAX_ORB];
+    __shared__ double rmax[BS];
+    __shared__ int ridx[BS];
     extern __shared__ cd mlds[];
     const int t = threadIdx.x;
     cd* G = A + (size_t)blockIdx.x * N * N;
     cd* M = G;
     if constexpr (IN_LDS) {
-        for (int e = t; e < N * N; e += 256) mlds[e] = G[e];
+        for (int e = t; e < N * N; e += BS) mlds[e] = G[e];
         M = mlds;
         __syncthreads();
     }
     // the elimination's (row, column) walk: thread t keeps row t % N and takes every
-    // (256 / N)-th column from t / N (N <= 256), no division per element
-    const int cpp = N <= 256 ? 256 / N : 0;
-    const int ti = N <= 256 ? t % N : 0, tc = N <= 256 ? t / N : 0;
+    // (BS / N)-th column from t / N (N <= BS), no division per element
+    const int cpp = N <= BS ? BS / N : 0;
+    const int ti = N <= BS ? t % N : 0, tc = N <= BS ? t / N : 0;
     for (int j = 0; j < N; ++j) {
         if (N <= 64) {  // every candidate row in wavefront 0: a wavefront reduction, one barrier
             if (t < 64) {
@@ -141,7 +142,7 @@ __global__ __launch_bounds__(256) void k_gj_inverse(int N, cd* A, int* singular)
         } else {
             double best = -1.0;
             int bi = j;
-            for (int i = j + t; i < N; i += 256) {
+            for (int i = j + t; i < N; i += BS) {
                 const cd a = M[i + (size_t)j * N];
                 const double m = fma(a.x, a.x, a.y * a.y);
                 if (m > best) { best = m; bi = i; }
@@ -149,7 +150,7 @@ __global__ __launch_bounds__(256) void k_gj_inverse(int N, cd* A, int* singular)
             rmax[t] = best;
             ridx[t] = bi;
             __syncthreads();
-            for (int w = 128; w > 0; w >>= 1) {
+            for (int w = BS / 2; w > 0; w >>= 1) {
                 if (t < w) {
                     const double o = rmax[t + w];
                     const int oi = ridx[t + w];
@@ -165,7 +166,7 @@ __global__ __launch_bounds__(256) void k_gj_inverse(int N, cd* A, int* singular)
             if (zero) *singular = 1;
         }
         if (p != j)
-            for (int c = t; c < N; c += 256) {
+            for (int c = t; c < N; c += BS) {
                 const cd a = M[j + (size_t)c * N];
                 M[j + (size_t)c * N] = M[p + (size_t)c * N];
                 M[p + (size_t)c * N] = a;
@@ -173,11 +174,11 @@ __global__ __launch_bounds__(256) void k_gj_inverse(int N, cd* A, int* singular)
         __syncthreads();
         const cd d = M[j + (size_t)j * N];
         const cd r = zero ? cd{0.0, 0.0} : crcp(d);
-        for (int i = t; i < N; i += 256) col[i] = M[i + (size_t)j * N];
+        for (int i = t; i < N; i += BS) col[i] = M[i + (size_t)j * N];
         __syncthreads();
-        for (int c = t; c < N; c += 256) M[j + (size_t)c * N] = (c == j) ? r : cmul(M[j + (size_t)c * N], r);
+        for (int c = t; c < N; c += BS) M[j + (size_t)c * N] = (c == j) ? r : cmul(M[j + (size_t)c * N], r);
         __syncthreads();
-        if (N <= 256) {
+        if (N <= BS) {
             if (ti != j && tc < cpp) {
                 const cd ci = col[ti];
                 for (int c = tc; c < N; c += cpp) {
@@ -187,7 +188,7 @@ __global__ __launch_bounds__(256) void k_gj_inverse(int N, cd* A, int* singular)
                 }
             }
         } else {
-            for (size_t e = t; e < (size_t)N * N; e += 256) {
+            for (size_t e = t; e < (size_t)N * N; e += BS) {
                 const int i = (int)(e % N), c = (int)(e / N);
                 if (i == j) continue;
                 const cd base = (c == j) ? cd{0.0, 0.0} : M[e];
@@ -199,7 +200,7 @@ __global__ __launch_bounds__(256) void k_gj_inverse(int N, cd* A, int* singular)
     for (int j = N - 1; j >= 0; --j) {
         const int p = piv[j];
         if (p != j)
-            for (int i = t; i < N; i += 256) {
+            for (int i = t; i < N; i += BS) {
                 const cd a = M[i + (size_t)j * N];
                 M[i + (size_t)j * N] = M[i + (size_t)p * N];
                 M[i + (size_t)p * N] = a;
@@ -207,19 +208,19 @@ __global__ __launch_bounds__(256) void k_gj_inverse(int N, cd* A, int* singular)
         __syncthreads();
     }
     if constexpr (IN_LDS)
-        for (int e = t; e < N * N; e += 256) G[e] = mlds[e];
+        for (int e = t; e < N * N; e += BS) G[e] = mlds[e];
 }
 
 hipError_t gj_inverse(int N, int batch, cd* A, int* singular, hipStream_t s)
 {
-    if (N <= 64) {  // up to 64 KiB of matrix next to the 13 KiB of static arrays
+    if (N <= 64) {  // up to 64 KiB of matrix next to the static arrays; one wavefront per matrix
         const size_t shm = sizeof(cd) * (size_t)N * N;
-        const hipError_t e = hipFuncSetAttribute((const void*)k_gj_inverse<true>,
+        const hipError_t e = hipFuncSetAttribute((const void*)k_gj_inverse<true, 64>,
                                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_gj_inverse<true>, dim3((unsigned)batch), dim3(256), shm, s, N, A, singular);
+        hipLaunchKernelGGL((k_gj_inverse<true, 64>), dim3((unsigned)batch), dim3(64), shm, s, N, A, singular);
     } else
-        hipLaunchKernelGGL(k_gj_inverse<false>, dim3((unsigned)batch), dim3(256), 0, s, N, A, singular);
+        hipLaunchKernelGGL((k_gj_inverse<false, 256>), dim3((unsigned)batch), dim3(256), 0, s, N, A, singular);
     return hipGetLastError();
 }
 
